@@ -1,0 +1,170 @@
+"""Benchmark: two-stage nss_hip.BM3D on 1920x1080 YUV420PS frames, sigma 10.
+
+One step = one frame through both stages (basic + Wiener), all three planes,
+plain FP32, one frame per launch sequence, input already resident in HBM.
+Prints one JSON line. Per-kernel times come from the library's own event
+tracing on the launch stream; the CPU baseline is the oracle run
+frame-parallel on the host cores over a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "vapoursynth-nssfactory_amd"), os.path.join(ROOT, "oracle")]
+
+METRIC = "1080p 2-stage BM3D Mpix/s, plain FP32 vs MFMA form at ≥60dB; p50/p99 latency"
+W, H, FMT, SIGMA = 1920, 1080, "YUV420PS", 10
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def window_candidates(width, height, block, step, rng):
+    """Sum over the reference grid of the clamped window size minus one
+    (the SSD candidates C_ref of each reference)."""
+    import numpy as np
+    nx = (width - block + step - 1) // step + 1
+    ny = (height - block + step - 1) // step + 1
+    xs = np.minimum(np.arange(nx) * step, width - block)
+    ys = np.minimum(np.arange(ny) * step, height - block)
+    wx = np.minimum(xs + rng, width - block) - np.maximum(xs - rng, 0) + 1
+    wy = np.minimum(ys + rng, height - block) - np.maximum(ys - rng, 0) + 1
+    return int(np.sum(wy[:, None] * wx[None, :] - 1)), nx * ny
+
+
+def stage_work(params, dims, basic):
+    """Algorithmic work of one stage over the frame's planes (one launch of
+    each kernel): SSD flops, transform flops, aggregation + finish bytes."""
+    ssd = tr = agg = 0
+    for i, (h, w) in enumerate(dims):
+        b = params.block_size_basic[i] if basic else params.block_size[i]
+        g = params.group_size_basic[i] if basic else params.group_size[i]
+        s = params.block_step_basic[i] if basic else params.block_step[i]
+        cands, nref = window_candidates(w, h, b, s, params.bm_range[i])
+        ssd += 3 * b * b * cands
+        per_dir = g * 4 * b ** 3 + 2 * g * g * b * b
+        tr += nref * per_dir * (2 if basic else 3)
+        agg += nref * g * b * b * 20 + w * h * 16  # k <= G; G at these sizes (window >> G)
+    return ssd, tr, agg
+
+
+def cpu_baseline(noisy, params, threads):
+    """Oracle (fp32 port of nss.BM3D) on the host: one frame per thread, as
+    VapourSynth runs frames in parallel. Returns (Mpix/s, oracle output)."""
+    import oracle
+    outs = [None] * threads
+
+    def work(i):
+        outs[i] = oracle.bm3d_frame(noisy, params)
+
+    t0 = time.perf_counter()
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    dt = time.perf_counter() - t0
+    return threads * W * H / dt / 1e6, outs[0], dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; each frame is independent, so N GPUs would each run this "
+              "workload on their own frames (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import nss_hip
+    import synth
+
+    print(f"# generating {FMT} {W}x{H} sigma {SIGMA}", file=sys.stderr, flush=True)
+    clean, noisy = synth.noisy_frame(FMT, W, H, SIGMA, seed=0)
+    dev = [torch.from_numpy(p).cuda() for p in noisy]
+    inst = nss_hip.BM3D(FMT, W, H, sigma=SIGMA, final=1)
+    out = [torch.empty_like(p) for p in dev]
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        inst(dev, out=out)
+    torch.cuda.synchronize()
+    inst.set_tracing(True)
+    inst.kernel_times(reset=True)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        inst(dev, out=out)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = inst.kernel_times(reset=True)
+    inst.set_tracing(False)
+    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
+    ms_per_step = wall / args.steps * 1e3
+    value = W * H * args.steps / wall / 1e6
+    got = [o.cpu().numpy() for o in out]
+    print(f"# {ms_per_step:.3f} ms/frame, kernels {kt}", file=sys.stderr, flush=True)
+
+    dims = [tuple(p.shape) for p in noisy]
+    sw_b, tr_b, ag_b = stage_work(inst.params, dims, True)
+    sw_f, tr_f, ag_f = stage_work(inst.params, dims, False)
+    kernels = {}
+    for kind, (n, ms) in kt.items():
+        if n:
+            kernels[kind] = {"launches": n, "avg_us": ms / n * 1e3}
+    # Per launch of a kind: one stage of the frame (basic, then Wiener, alternating).
+    work = {"match": ((sw_b + sw_f) / 2, "TFLOP/s", PEAK_FP32_TFLOPS),
+            "filter": ((tr_b + tr_f) / 2, "TFLOP/s", PEAK_FP32_TFLOPS),
+            "finish": (W * H * 1.5 * 16 / 1, "GB/s", PEAK_HBM_GBS)}
+    for kind, (amount, unit, peak) in work.items():
+        if kind in kernels:
+            avg_s = kernels[kind]["avg_us"] * 1e-6
+            ach = amount / avg_s / (1e12 if unit == "TFLOP/s" else 1e9)
+            kernels[kind].update({"achieved": ach, "unit": unit, "peak": peak, "frac": ach / peak,
+                                  "algorithmic_per_launch": amount})
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    dk = kernels[dom]
+    roofline = {"kernel": dom, "bound": "hbm" if dk.get("unit") == "GB/s" else "mfma",
+                "achieved": dk.get("achieved"), "peak": dk.get("peak"), "unit": dk.get("unit"),
+                "frac": dk.get("frac"), "traffic": None}
+
+    res = {"metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "two-stage BM3D (final=1, sigma 10, B8 s8 G8 r9) on one 1920x1080 YUV420PS "
+                                  "frame per step, plain FP32, one frame per launch sequence",
+                      "format": FMT, "width": W, "height": H, "sigma": SIGMA, "final": 1, "form": "plain-fp32"},
+           "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+           "roofline": roofline, "kernels": kernels,
+           "psnr_vs_clean": [round(float(10 * np.log10(1 / np.mean((g.astype(np.float64) - c) ** 2))), 3)
+                             for g, c in zip(got, clean)]}
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"# cpu baseline: oracle on {threads} threads, one frame each", file=sys.stderr, flush=True)
+        import oracle
+        oracle.lib()
+        cpu_value, want, dt = cpu_baseline(noisy, inst.params, threads)
+        import oracle as o
+        res["psnr_vs_cpu_min_plane"] = round(min(o.psnr(g, w_) for g, w_ in zip(got, want)), 3)
+        res["cpu_baseline"] = {"value": round(cpu_value, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                               "sample": f"{threads} copies of the same 1920x1080 YUV420PS frame, one per thread, "
+                                         f"two stages ({dt:.1f} s wall)"}
+    inst.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

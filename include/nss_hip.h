@@ -1,0 +1,178 @@
+/*
+ * nss_hip: MI355X-native BM3D behind a C ABI.
+ *
+ * The entry points replace, for the BM3D path only, the reference plugin's
+ * interface (AmusementClub/vapoursynth-nssfactory):
+ *   - argument resolution of nss.BM3D: parse_bm3d and its helpers
+ *     (src/frontend/bm3d_args.cpp:140-170), the numeric/budget checks of
+ *     checked_create (src/frontend/validate.hpp:30-50) and the temporal
+ *     arguments (src/frontend/temporal_args.cpp:11-59);
+ *   - the per-frame work of the filter node: bm3dGetFrame / rollingGetFrame
+ *     (src/host/filter_bm3d.cpp:359-494, :985-1072), which the VapourSynth
+ *     glue calls once per frame;
+ *   - the block matcher spatial_match (include/nss/cpu_api.hpp:146-170) as a
+ *     batched device entry point, for match-list parity checks.
+ *
+ * Only plain C types cross the boundary. Pixel buffers given to the
+ * process/match calls are DEVICE pointers (HIP), 32-bit float, with strides
+ * in elements; work is queued on the given HIP stream (NULL = the default
+ * stream) and is complete when that stream is.
+ *
+ * Status codes mirror nss::KernelStatus (include/nss/contracts.hpp:18).
+ * On failure nss_hip_last_error() returns the message of the calling thread's
+ * last failed call (argument errors read "<ns>.BM3D: <message>", as
+ * src/frontend/args.hpp:13-15 formats them).
+ */
+#ifndef NSS_HIP_H
+#define NSS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSS_HIP_OK 0
+#define NSS_HIP_BYPASS 1
+#define NSS_HIP_INVALID_INPUT 2
+#define NSS_HIP_NUMERICAL_FAILURE 3
+#define NSS_HIP_UNSUPPORTED 4
+#define NSS_HIP_RESOURCE_FAILURE 5
+
+/* Colour families and sample types: VapourSynth API4 values (cfGray = 1,
+ * cfRGB = 2, cfYUV = 3; stInteger = 0, stFloat = 1). */
+#define NSS_HIP_CF_GRAY 1
+#define NSS_HIP_CF_RGB 2
+#define NSS_HIP_CF_YUV 3
+
+typedef struct nss_hip_video_info {
+    int color_family;
+    int sample_type;
+    int bits_per_sample;
+    int sub_sampling_w;
+    int sub_sampling_h;
+    int num_planes;
+    int width; /* 0 = variable */
+    int height;
+    int num_frames;
+} nss_hip_video_info;
+
+/* One argument of the call, as a VSMap entry holds it. count = number of
+ * elements (0 or a missing key = not given). type: 'i' int64 array, 'f'
+ * double array, 's' data (utf-8), 'c' clip (the clip itself travels
+ * separately; only its video info is read, from clip_info). */
+typedef struct nss_hip_arg {
+    const char* key;
+    char type;
+    int count;
+    const int64_t* ints;
+    const double* floats;
+    const char* data;
+    const nss_hip_video_info* clip_info;
+} nss_hip_arg;
+
+/* Resolved nss.BM3D arguments (include/nss/params/bm3d.hpp:15-31 Bm3dParams,
+ * include/nss/params/temporal.hpp RollingParams). sigma and sigma_basic are
+ * EFFECTIVE values: (user * (1/255)) * 0.75, 0 = plane copied. */
+typedef struct nss_hip_bm3d_params {
+    float sigma[3];
+    int block_size[3];
+    int group_size[3];
+    int block_step[3];
+    int bm_range[3];
+    int ps_num[3];
+    int ps_range[3];
+    int radius;
+    int chroma;
+    int final_stage;
+    float sigma_basic[3];
+    int block_size_basic[3];
+    int group_size_basic[3];
+    int block_step_basic[3];
+    int ps_num_basic[3];
+    int has_ref;        /* a ref clip was given (Wiener stage with that ref) */
+    int temporal_mode;  /* 0 legacy, 1 rolling (radius > 0 only) */
+    int rolling_chunk;
+    int rolling_cache_chunks;
+    int rolling_cache_limit;
+    int rolling_cache_adaptive;
+    int memory_limit_mb; /* 0 = not given */
+    int device_id;       /* backend tail device_id:int:opt (default 0) */
+    int num_streams;     /* backend tail num_streams:int:opt (default 1) */
+} nss_hip_bm3d_params;
+
+typedef struct nss_hip_bm3d nss_hip_bm3d;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* nss_hip_last_error(void);
+
+/* Library and device description (replaces the CUDA plugin's Version /
+ * Backend functions, src/cuda/plugin.cpp:35-71): writes a NUL-terminated
+ * text of at most len bytes. */
+int nss_hip_backend_info(int device, char* text, int len);
+
+/* Creation-time argument resolution of nss.BM3D. ns is the namespace used in
+ * error texts ("nss_hip"). ref_info is NULL unless a ref clip was given. On
+ * NSS_HIP_INVALID_INPUT, nss_hip_last_error() holds the exact message.
+ * Replaces parse_bm3d (src/frontend/bm3d_args.cpp:140-170) with
+ * validate_numeric_args and the memory_limit_mb check of checked_create
+ * (src/frontend/validate.cpp:13-46, validate.hpp:33-45) and
+ * parse_temporal_mode (src/frontend/temporal_args.cpp:11-59). */
+int nss_hip_bm3d_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* ref_info,
+                       const nss_hip_arg* args, int nargs, const char* ns, nss_hip_bm3d_params* out);
+
+/* Output format of the node: the clip's, or for legacy temporal output
+ * (radius > 0, temporal_mode "legacy") 2r+1 num/den slice pairs stacked
+ * vertically (src/host/filter_bm3d.cpp:1155-1157). */
+int nss_hip_bm3d_output_info(const nss_hip_bm3d_params* params, const nss_hip_video_info* clip_info,
+                             nss_hip_video_info* out);
+
+/* Filter instance for frames of one fixed format and size on one device
+ * (the node creation of src/host/filter_bm3d.cpp:1143-1163). Device buffers
+ * are sized here, from the parameters. */
+int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device,
+                        nss_hip_bm3d** out);
+
+/* Number of temporal slots the process call reads per plane: 2*radius + 1. */
+int nss_hip_bm3d_slots(const nss_hip_bm3d* handle);
+
+/* Filter one frame: the work of bm3dGetFrame for the spatial case (radius 0),
+ * both stages when final=1 (the basic estimate stays on the device).
+ * src[p] / ref[p] / dst[p]: device pointers of plane p (ref may be NULL, or
+ * hold NULLs, when no ref clip was given), strides in floats. For radius > 0
+ * the temporal slots of every plane are given as src[slot * 3 + p] (slot t is
+ * frame n + clamp(t - radius, -n, frames - 1 - n)) with the frame's index n
+ * and the clip length; the output is the finished ("rolling") frame.
+ * Asynchronous on stream. */
+int nss_hip_bm3d_process(nss_hip_bm3d* handle, const float* const* src, const int* src_stride,
+                         const float* const* ref, const int* ref_stride, float* const* dst, const int* dst_stride,
+                         int frame, int frame_count, void* stream);
+
+/* Kernel tracing (the role of the reference's NVTX ranges group.match /
+ * group.filter / group.aggregate, src/cuda/runtime/nvtx.hpp:15-32): while on,
+ * the instance records a HIP event pair around each of its launches on the
+ * launch stream. nss_hip_bm3d_kernel_times waits for those events and
+ * reports, per kernel kind (0 clear, 1 match, 2 filter + aggregate,
+ * 3 finish), the launch count and total milliseconds since the last reset;
+ * reset != 0 starts a new window. */
+#define NSS_HIP_KERNEL_KINDS 4
+int nss_hip_bm3d_set_tracing(nss_hip_bm3d* handle, int on);
+int nss_hip_bm3d_kernel_times(nss_hip_bm3d* handle, int* launches, double* total_ms, int reset);
+
+/* Frees the instance's device buffers (synchronises its work first). */
+int nss_hip_bm3d_destroy(nss_hip_bm3d* handle);
+
+/* Batched block matching over one plane's reference grid (spatial_match for
+ * every reference, include/nss/cpu_api.hpp:146-170; grid of
+ * src/host/batch_runner.hpp:17-31). plane: device pointer. out_rec: device
+ * int32 [nref][group][4] = (x, y, t, bits of the float SSD), reference
+ * first, then ascending (distance, y, x); out_count: device int32 [nref].
+ * nref = nx * ny with nx = (width - block + step - 1) / step + 1, ny alike. */
+int nss_hip_spatial_match(const float* plane, int width, int height, int stride, int block, int step, int bm_range,
+                          int group, int* out_rec, int* out_count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NSS_HIP_H */

@@ -1,0 +1,282 @@
+"""CPU oracle for the BM3D hot path: ctypes wrapper over ``bm3d_oracle.cpp``.
+
+TEST INFRASTRUCTURE ONLY. Only the tests, the smoke check and the benchmark's
+CPU-baseline leg use this module, as the checker. The product library never
+imports it.
+
+Frame-level semantics follow the reference CPU plugin:
+- per plane, sigma == 0 copies the plane (src/host/filter_bm3d.cpp:402-412);
+- ``final=1`` is the basic (hard-threshold) stage with the ``*_basic``
+  values, then the Wiener stage with that estimate as ``ref``
+  (src/frontend/bm3d_args.cpp:186-221 bm3d_two_nodes);
+- matching runs on ``ref`` when one is given (src/host/filter_bm3d.cpp:351,
+  process_plane_batched passes refs as the match planes);
+- radius > 0 ("rolling"): each centre frame's 2r+1 num/den slices are added
+  into their target frames in ascending centre order, then finished
+  (src/host/filter_bm3d.cpp:730-851, src/frontend/temporal.hpp:11-36).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_bm3d.so")
+_lib = None
+
+MAX_GROUP = 64
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with g++ (-O3, no fast-math)."""
+    src = os.path.join(_HERE, "bm3d_oracle.cpp")
+    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle_bm3d.so"])
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER
+        f32p, i32p = P(ctypes.c_float), P(ctypes.c_int)
+        fpp = P(P(ctypes.c_float))
+        ci, cf = ctypes.c_int, ctypes.c_float
+        L.orc_spatial_match.argtypes = [f32p, ci, ci, ci, ci, ci, ci, ci, ci, ci, i32p, f32p]
+        L.orc_spatial_match.restype = ci
+        L.orc_predictive_match.argtypes = [fpp, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci,
+                                           i32p, f32p]
+        L.orc_predictive_match.restype = ci
+        L.orc_filter_group.argtypes = [f32p, ci, ci, ci, cf, ci, f32p, ci]
+        L.orc_filter_group.restype = cf
+        L.orc_bm3d_plane.argtypes = [fpp, fpp, fpp, ci, ci, ci, ci, ci, f32p, ci, f32p, f32p, cf,
+                                     ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, i32p]
+        L.orc_bm3d_plane.restype = ci
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _ptrs(arrs):
+    t = ctypes.POINTER(ctypes.c_float) * len(arrs)
+    return t(*[_fp(a) for a in arrs])
+
+
+def effective_sigma(user: float) -> float:
+    """include/nss/contracts.hpp:27-29: (sigma * (1/255)) * 0.75 in float."""
+    return float(np.float32(np.float32(user) * np.float32(1.0 / 255.0)) * np.float32(0.75))
+
+
+def spatial_match(plane, bx, by, block, bm_range, group, fp64=False):
+    """Match list of one reference: list of (x, y, t, dist), reference first."""
+    plane = np.ascontiguousarray(plane, np.float32)
+    h, w = plane.shape
+    xyt = np.zeros((MAX_GROUP, 3), np.int32)
+    dist = np.zeros(MAX_GROUP, np.float32)
+    n = lib().orc_spatial_match(_fp(plane), w, w, h, bx, by, block, bm_range, group, int(fp64), _ip(xyt), _fp(dist))
+    return [(int(xyt[i, 0]), int(xyt[i, 1]), int(xyt[i, 2]), float(dist[i])) for i in range(n)]
+
+
+def predictive_match(planes, bx, by, t0, block, group, bm_range, ps_num, ps_range, radius,
+                     valid_begin=0, valid_end=-1, fp64=False):
+    planes = [np.ascontiguousarray(p, np.float32) for p in planes]
+    h, w = planes[0].shape
+    xyt = np.zeros((MAX_GROUP, 3), np.int32)
+    dist = np.zeros(MAX_GROUP, np.float32)
+    n = lib().orc_predictive_match(_ptrs(planes), w, len(planes), w, h, bx, by, t0, block, group, bm_range, ps_num,
+                                   ps_range, radius, valid_begin, valid_end, int(fp64), _ip(xyt), _fp(dist))
+    return [(int(xyt[i, 0]), int(xyt[i, 1]), int(xyt[i, 2]), float(dist[i])) for i in range(n)]
+
+
+def filter_group(cube, k, sigma, wiener=False, ref_cube=None, fp64=False):
+    """Filter a (group, block, block) cube in place semantics; returns (cube, weight)."""
+    c = np.ascontiguousarray(cube, np.float32).copy()
+    g, b, _ = c.shape
+    r = np.ascontiguousarray(ref_cube if ref_cube is not None else np.zeros_like(c), np.float32).copy()
+    w = lib().orc_filter_group(_fp(c), g, k, b, sigma, int(wiener), _fp(r), int(fp64))
+    return c, float(w)
+
+
+def raster_grid(width, height, block, step):
+    """Reference positions (src/host/batch_runner.hpp:17-31)."""
+    xs = [min(x, width - block) for x in range(0, width - block + step, step)]
+    ys = [min(y, height - block) for y in range(0, height - block + step, step)]
+    return [(x, y) for y in ys for x in xs]
+
+
+@dataclass
+class PlaneArgs:
+    """One plane's resolved arguments for one stage (effective sigma)."""
+    sigma: float
+    block: int = 8
+    group: int = 8
+    step: int = 8
+    bm_range: int = 9
+    ps_num: int = 2
+    ps_range: int = 4
+
+
+def bm3d_plane(src, sigma, block=8, group=8, step=8, bm_range=9, ps_num=2, ps_range=4, wref=None,
+               match=None, fp64=False, want_matches=False):
+    """One spatial plane of one stage. src/wref/match: 2-D float32 arrays.
+
+    Matching runs on ``match`` (defaults to ``wref`` when given, else ``src``).
+    Returns the finished plane (and the match dump when asked)."""
+    return bm3d_plane_temporal([src], 0, sigma, block, group, step, bm_range, ps_num, ps_range, 0,
+                               wrefs=None if wref is None else [wref],
+                               matches=None if match is None else [match], center=0, frame_count=1,
+                               fp64=fp64, want_matches=want_matches)
+
+
+def bm3d_plane_temporal(srcs, t0, sigma, block, group, step, bm_range, ps_num, ps_range, radius, wrefs=None,
+                        matches=None, center=0, frame_count=1, fp64=False, want_matches=False, want_slices=False):
+    srcs = [np.ascontiguousarray(s, np.float32) for s in srcs]
+    h, w = srcs[0].shape
+    wiener = wrefs is not None
+    wr = [np.ascontiguousarray(s, np.float32) for s in wrefs] if wiener else srcs
+    mt = [np.ascontiguousarray(s, np.float32) for s in matches] if matches is not None else (wr if wiener else srcs)
+    slices = 2 * radius + 1
+    dst = np.zeros((h, w), np.float32)
+    num = np.zeros((slices, h, w), np.float32) if want_slices else None
+    den = np.zeros((slices, h, w), np.float32) if want_slices else None
+    nref = len(raster_grid(w, h, block, step))
+    dump = np.zeros((nref, 1 + 3 * MAX_GROUP), np.int32) if want_matches else None
+    rc = lib().orc_bm3d_plane(_ptrs(mt), _ptrs(srcs), _ptrs(wr), len(srcs), t0, w, h, w, _fp(dst), w,
+                              _fp(num) if want_slices else None, _fp(den) if want_slices else None,
+                              sigma, block, group, step, bm_range, ps_num, ps_range, radius, int(wiener),
+                              center, frame_count, int(fp64), _ip(dump) if want_matches else None)
+    if rc < 0:
+        raise RuntimeError("oracle: matching failed for an active group")
+    out = [dst]
+    if want_matches:
+        out.append(dump)
+    if want_slices:
+        out += [num, den]
+    return out[0] if len(out) == 1 else tuple(out)
+
+
+def _stage_args(params, plane, basic):
+    p = params
+    if basic:
+        return dict(sigma=p.sigma_basic[plane], block=p.block_size_basic[plane], group=p.group_size_basic[plane],
+                    step=p.block_step_basic[plane], bm_range=p.bm_range[plane], ps_num=p.ps_num_basic[plane],
+                    ps_range=p.ps_range[plane])
+    return dict(sigma=p.sigma[plane], block=p.block_size[plane], group=p.group_size[plane],
+                step=p.block_step[plane], bm_range=p.bm_range[plane], ps_num=p.ps_num[plane],
+                ps_range=p.ps_range[plane])
+
+
+def _is_final(params):
+    """final=1 (named final_stage in the C ABI's parameter struct)."""
+    return bool(getattr(params, "final_stage", getattr(params, "final", False)))
+
+
+def bm3d_frame(planes, params, ref=None, fp64=False):
+    """Spatial (radius 0) nss.BM3D on one frame: list of 2-D planes in, list out.
+
+    ``params`` carries resolved per-plane values (effective sigma) with the
+    attribute names of Bm3dParams (include/nss/params/bm3d.hpp:15-31)."""
+    if params.radius != 0:
+        raise ValueError("bm3d_frame is the spatial path; use bm3d_clip for radius > 0")
+    if _is_final(params):
+        basic = [np.array(p, np.float32, copy=True) if params.sigma_basic[i] == 0 else
+                 bm3d_plane(p, fp64=fp64, **_stage_args(params, i, True)) for i, p in enumerate(planes)]
+        ref = basic
+    out = []
+    for i, p in enumerate(planes):
+        if params.sigma[i] == 0:
+            out.append(np.array(p, np.float32, copy=True))
+            continue
+        a = _stage_args(params, i, False)
+        out.append(bm3d_plane(p, wref=None if ref is None else ref[i], fp64=fp64, **a))
+    return out
+
+
+def _temporal_stage(frames, plane, params, basic, wref_frames, fp64):
+    """Rolling temporal stage of one plane over a whole clip (list of 2-D)."""
+    n_frames = len(frames)
+    r = params.radius
+    a = _stage_args(params, plane, basic)
+    if a["sigma"] == 0:
+        return [np.array(f, np.float32, copy=True) for f in frames]
+    h, w = frames[0].shape
+    acc_num = [np.zeros((h, w), np.float64 if fp64 else np.float32) for _ in range(n_frames)]
+    acc_den = [np.zeros((h, w), np.float64 if fp64 else np.float32) for _ in range(n_frames)]
+    for c in range(n_frames):
+        slot_frames = [c + min(max(t - r, -c), n_frames - 1 - c) for t in range(2 * r + 1)]
+        srcs = [frames[f] for f in slot_frames]
+        wr = [wref_frames[f] for f in slot_frames] if wref_frames is not None else None
+        _, num, den = bm3d_plane_temporal(srcs, r, a["sigma"], a["block"], a["group"], a["step"], a["bm_range"],
+                                          a["ps_num"], a["ps_range"], r, wrefs=wr, center=c, frame_count=n_frames,
+                                          fp64=fp64, want_slices=True)
+        for t in range(2 * r + 1):
+            target = c + t - r
+            if 0 <= target < n_frames:
+                acc_num[target] += num[t]
+                acc_den[target] += den[t]
+    out = []
+    for n in range(n_frames):
+        d = acc_den[n]
+        src = frames[n].astype(acc_num[n].dtype)
+        res = np.where(d > 1e-12, acc_num[n] / np.where(d > 1e-12, d, 1), src)
+        out.append(res.astype(np.float32))
+    return out
+
+
+def bm3d_clip(clip, params, fp64=False):
+    """nss.BM3D over a clip: list of frames, each a list of 2-D planes."""
+    if params.radius == 0:
+        return [bm3d_frame(f, params, fp64=fp64) for f in clip]
+    nplanes = len(clip[0])
+    out = [[None] * nplanes for _ in clip]
+    for p in range(nplanes):
+        frames = [f[p] for f in clip]
+        wref = None
+        if _is_final(params):
+            wref = _temporal_stage(frames, p, params, True, None, fp64)
+        res = _temporal_stage(frames, p, params, False, wref, fp64)
+        for n, r in enumerate(res):
+            out[n][p] = r
+    return out
+
+
+@dataclass
+class Params:
+    """Mirror of Bm3dParams with the reference defaults (effective sigma)."""
+    sigma: list = field(default_factory=lambda: [effective_sigma(3)] * 3)
+    block_size: list = field(default_factory=lambda: [8] * 3)
+    group_size: list = field(default_factory=lambda: [8] * 3)
+    block_step: list = field(default_factory=lambda: [8] * 3)
+    bm_range: list = field(default_factory=lambda: [9] * 3)
+    ps_num: list = field(default_factory=lambda: [2] * 3)
+    ps_range: list = field(default_factory=lambda: [4] * 3)
+    radius: int = 0
+    chroma: bool = False
+    final: bool = False
+    sigma_basic: list = field(default_factory=lambda: [0.0] * 3)
+    block_size_basic: list = field(default_factory=lambda: [0] * 3)
+    group_size_basic: list = field(default_factory=lambda: [0] * 3)
+    block_step_basic: list = field(default_factory=lambda: [0] * 3)
+    ps_num_basic: list = field(default_factory=lambda: [0] * 3)
+
+
+def psnr(a, b, peak=1.0):
+    """10 log10(peak^2 / MSE) in float64 (tests/cuda/test_cuda_bm3d_live.py:204-206)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    mse = float(np.mean((a - b) ** 2))
+    return float("inf") if mse == 0 else 10.0 * np.log10(peak * peak / mse)

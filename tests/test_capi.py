@@ -1,0 +1,45 @@
+"""The C-ABI library loads and exports every entry point include/nss_hip.h
+declares; host-only calls behave without a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "nss_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nss_hip_\w+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    import nss_hip
+    lib = nss_hip.lib()
+    names = _declared()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(nss_hip.EXPORTS) == set(names)
+
+
+def test_parse_defaults_and_error_text():
+    import nss_hip
+    p = nss_hip.parse("YUV420PS", 1920, 1080, sigma=10, final=1)
+    d = p.as_dict()
+    assert d["block_size"] == [8, 8, 8] and d["group_size"] == [8, 8, 8] and d["block_step"] == [8, 8, 8]
+    assert d["bm_range"] == [9, 9, 9] and d["final_stage"] == 1
+    assert d["sigma"][0] == pytest.approx(10 / 255 * 0.75, rel=1e-6)
+    with pytest.raises(nss_hip.Error, match=r"^nss_hip\.BM3D: block_size must be one of 1, 2, 4, 8, 12, 16, 32$"):
+        nss_hip.parse("GRAYS", 48, 40, block_size=3)
+    assert nss_hip.last_error().startswith("nss_hip.BM3D: block_size")
+
+
+def test_null_arguments_are_errors():
+    import nss_hip
+    lib = nss_hip.lib()
+    assert lib.nss_hip_bm3d_create(None, None, 0, None) == nss_hip.INVALID_INPUT
+    assert lib.nss_hip_bm3d_destroy(None) == nss_hip.OK
+    assert lib.nss_hip_bm3d_set_tracing(None, 1) == nss_hip.INVALID_INPUT

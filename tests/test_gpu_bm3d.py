@@ -1,0 +1,191 @@
+"""Parity of the HIP path with the CPU oracle, through the C ABI.
+
+Gates (the reference's own, tests/data/cuda_tolerances_v1.json and
+tests/cuda/test_cuda_match.cpp:49-76):
+- match lists: same count; an entry may differ only as a near tie
+  |a - b| <= 2e-5 max(1, |a|, |b|), and on exact-tie (piecewise-constant)
+  images not at all; distances agree within the same bound;
+- output: PSNR >= 60 dB per plane against the oracle, and run-to-run
+  bitwise identity.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import closed_form as cf
+
+pytestmark = pytest.mark.gpu
+
+MIN_PSNR = 60.0
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _near_tie(a, b):
+    return abs(a - b) <= 2e-5 * max(1.0, abs(a), abs(b))
+
+
+def _test_plane(width, height, seed, flat):
+    """tests/cuda/test_cuda_match.cpp:33-46: sinusoid + noise, or exact-tie
+    blocks of 0 / 0.25 / 0.5."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:height, 0:width]
+    if flat:
+        return ((((x // 8) + (y // 8) + seed) % 3) * 0.25).astype(np.float32)
+    v = 0.5 + 0.3 * np.sin(x * 0.21 + seed) * np.cos(y * 0.17)
+    return (v + rng.normal(0, 0.05, (height, width))).astype(np.float32)
+
+
+def _compare_matches(gpu, oracle, plane, block, step, rng_, group, flat, label):
+    import torch
+    rec, cnt = gpu.spatial_match(_dev(plane), block=block, step=step, bm_range=rng_, group=group)
+    rec = _host(rec)
+    cnt = _host(cnt)
+    h, w = plane.shape
+    grid = oracle.raster_grid(w, h, block, step)
+    assert len(grid) == rec.shape[0]
+    flips = 0
+    for i, (bx, by) in enumerate(grid):
+        want = oracle.spatial_match(plane, bx, by, block, rng_, group)
+        assert cnt[i] == len(want), (label, i, cnt[i], len(want))
+        for j, (x, y, t, d) in enumerate(want):
+            gx, gy, gt, gd = rec[i, j]
+            gdist = float(np.array(gd, np.int32).view(np.float32))
+            if (gx, gy) != (x, y):
+                flips += 1
+                assert not flat, (label, i, j, (x, y), (gx, gy))
+                assert _near_tie(d, gdist), (label, i, j, d, gdist)
+            else:
+                assert _near_tie(d, gdist), (label, i, j, d, gdist)
+    return flips
+
+
+@pytest.mark.parametrize("flat", [False, True])
+@pytest.mark.parametrize("block,group,step,rng_", [(8, 16, 3, 16), (8, 8, 8, 9), (4, 16, 2, 7), (12, 8, 6, 5),
+                                                   (16, 4, 8, 6), (2, 32, 1, 3), (1, 8, 1, 2), (32, 2, 16, 4)])
+def test_match_lists(gpu, oracle, block, group, step, rng_, flat):
+    print(f"match b{block} g{group} s{step} r{rng_} flat={flat}", flush=True)
+    plane = _test_plane(77, 61, 7 + block + group, flat)
+    flips = _compare_matches(gpu, oracle, plane, block, step, rng_, group, flat, f"b{block}g{group}")
+    print(f"near-tie flips {flips}", flush=True)
+
+
+def test_match_lists_config0(gpu, oracle):
+    """256x256 gray, sigma 5, block 8, step 3, group 16, range 16."""
+    import synth
+    _, noisy = synth.noisy_frame("GRAYS", 256, 256, 5, seed=0)
+    flips = _compare_matches(gpu, oracle, noisy[0], 8, 3, 16, 16, False, "config0")
+    print(f"config0 near-tie flips {flips}", flush=True)
+
+
+def _run(gpu, fmt, planes, ref=None, **kw):
+    h, w = planes[0].shape
+    inst = gpu.BM3D(fmt, w, h, ref=ref is not None, **kw)
+    out = inst([_dev(p) for p in planes], ref=None if ref is None else [_dev(p) for p in ref])
+    res = [_host(o) for o in out]
+    inst.close()
+    return res
+
+
+def _oracle_frame(oracle, gpu, fmt, planes, ref=None, **kw):
+    h, w = planes[0].shape
+    p = gpu.parse(fmt, w, h, ref=ref is not None, **kw)
+    return oracle.bm3d_frame(planes, p, ref=ref)
+
+
+def test_config0_one_stage(gpu, oracle):
+    import synth
+    _, noisy = synth.noisy_frame("GRAYS", 256, 256, 5, seed=0)
+    kw = dict(sigma=5, block_size=8, block_step=3, group_size=16, bm_range=16)
+    got = _run(gpu, "GRAYS", noisy, **kw)
+    want = _oracle_frame(oracle, gpu, "GRAYS", noisy, **kw)
+    p = oracle.psnr(got[0], want[0])
+    print(f"config0 PSNR vs oracle {p:.2f} dB", flush=True)
+    assert p >= MIN_PSNR
+
+
+@pytest.mark.parametrize("fmt", ["YUV420PS", "RGBS", "GRAYS"])
+def test_two_stage(gpu, oracle, fmt):
+    import synth
+    _, noisy = synth.noisy_frame(fmt, 160, 120, 10, seed=1)
+    kw = dict(sigma=10, final=1)
+    got = _run(gpu, fmt, noisy, **kw)
+    want = _oracle_frame(oracle, gpu, fmt, noisy, **kw)
+    for i, (g, w_) in enumerate(zip(got, want)):
+        p = oracle.psnr(g, w_)
+        print(f"{fmt} plane {i} PSNR vs oracle {p:.2f} dB", flush=True)
+        assert p >= MIN_PSNR
+
+
+@pytest.mark.parametrize("block", [1, 2, 4, 8, 12, 16, 32])
+def test_shapes_hard_and_wiener(gpu, oracle, block):
+    """tests/cuda/test_cuda_bm3d_live.py:33-50: every block x group, basic and
+    Wiener (ref given), on a small seeded sinusoid + noise plane."""
+    import synth
+    h, w = 48, 56
+    clean = synth.clean_image(w, h, seed=block)
+    noisy = synth.add_noise(clean, 20, seed=block)
+    for group in (1, 2, 4, 8, 16, 32, 64):
+        if block == 32 and group >= 32:
+            continue  # beyond the on-chip group cube (documented as unsupported)
+        step = max(1, block // 2)
+        for wiener in (False, True):
+            kw = dict(sigma=20, block_size=block, group_size=group, block_step=step, bm_range=4)
+            ref = [clean] if wiener else None
+            got = _run(gpu, "GRAYS", [noisy], ref=ref, **kw)
+            want = _oracle_frame(oracle, gpu, "GRAYS", [noisy], ref=ref, **kw)
+            p = oracle.psnr(got[0], want[0])
+            assert p >= MIN_PSNR, (block, group, wiener, p)
+    print(f"block {block} ok", flush=True)
+
+
+@pytest.mark.parametrize("block", [1, 2, 4, 8, 12, 16, 32])
+def test_single_block_closed_form(gpu, oracle, block):
+    """tests/test_bm3d_semantics.py:161-173 on the GPU: one block, one match."""
+    rng = np.random.default_rng(59)
+    p = rng.uniform(-.02, .03, (block, block)).astype(np.float32)
+    ref = (p * np.float32(.8)).astype(np.float32)
+    for group in (1, 2, 4, 8, 16, 32, 64):
+        if block == 32 and group >= 32:
+            continue
+        for user in (.5, 3, 20, 40):
+            for wiener in (False, True):
+                got = _run(gpu, "GRAYS", [p], ref=[ref] if wiener else None, sigma=user, block_size=block,
+                           block_step=block, group_size=group, bm_range=1)
+                want = cf.single_block(p.astype(np.float64), ref.astype(np.float64), group,
+                                       oracle.effective_sigma(user), wiener)
+                assert np.max(np.abs(got[0] - want)) <= 3e-5, (block, group, user, wiener)
+
+
+def test_constant_image_identity(gpu):
+    img = np.full((64, 80), .3, np.float32)
+    got = _run(gpu, "GRAYS", [img], sigma=10, block_step=4)
+    np.testing.assert_allclose(got[0], img, atol=1e-6)
+
+
+def test_sigma_zero_copies(gpu):
+    import synth
+    _, noisy = synth.noisy_frame("YUV420PS", 64, 48, 10, seed=2)
+    got = _run(gpu, "YUV420PS", noisy, sigma=[0, 5, 0])
+    assert np.array_equal(got[0], noisy[0]) and np.array_equal(got[2], noisy[2])
+    assert not np.array_equal(got[1], noisy[1])
+
+
+def test_run_to_run_identity(gpu):
+    """Fixed-point aggregation: two runs are bitwise identical."""
+    import synth
+    _, noisy = synth.noisy_frame("YUV420PS", 320, 240, 10, seed=4)
+    a = _run(gpu, "YUV420PS", noisy, sigma=10, final=1)
+    b = _run(gpu, "YUV420PS", noisy, sigma=10, final=1)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

@@ -1,0 +1,94 @@
+// Launch descriptors shared by the HIP kernels and the host driver.
+//
+// One launch serves up to kMaxJobs planes (of one frame, or of several frames
+// in a batch) that share a block size: the grid is the concatenation of each
+// job's workgroups, and a workgroup finds its job by scanning wg_begin. The
+// tables travel as kernel arguments (no descriptor copies on the stream).
+#pragma once
+
+#include <stdint.h>
+
+namespace nsship {
+
+inline constexpr int kMaxJobs = 8;
+inline constexpr int kMaxGroup = 64;   // include/nss/params.hpp:39 kBmMaxGroup
+inline constexpr int kMaxRadius = 16;  // include/nss/params.hpp:44 kBmMaxRadius
+inline constexpr int kMaxSlots = 2 * kMaxRadius + 1;
+
+// A match record: x, y, temporal slot, and the distance's bits (the float
+// SSD; non-finite distances collapse to +inf bits, so they rank equal and
+// fall back to the position order).
+struct MatchRec {
+    int x, y, t;
+    uint32_t dist;
+};
+
+// Reference grid of one plane: positions (i*step, j*step) clamped to
+// (width - block, height - block) (src/gpuhost/geometry.hpp:17-26).
+struct Grid {
+    int nx, ny, step;
+};
+
+struct MatchJob {
+    const float* plane;  // the plane searched (the reference's own frame)
+    int width, height, stride;
+    Grid grid;
+    int range;   // bm_range
+    int group;   // configured group size; the record stride
+    int t0;      // slot recorded for spatial matches
+    int wg_begin;
+    MatchRec* out;   // [nref][group]
+    int* counts;     // [nref]
+};
+
+struct MatchTable {
+    int n;
+    MatchJob job[kMaxJobs];
+};
+
+struct FilterJob {
+    const float* src;   // filtered plane (noisy)
+    const float* wref;  // Wiener reference plane (the basic estimate); null in the hard stage
+    int width, height, stride, wref_stride;
+    int nref;
+    int group;
+    float sigma;        // effective sigma
+    int wg_begin;
+    const MatchRec* matches;
+    const int* counts;
+    unsigned long long* num;  // fixed-point numerator, 2^32 units, pitch = width
+    unsigned long long* den;  // fixed-point weight at each patch's top-left corner
+};
+
+struct FilterTable {
+    int n;
+    FilterJob job[kMaxJobs];
+};
+
+struct FinishJob {
+    const unsigned long long* num;
+    const unsigned long long* den;
+    const float* src;   // fallback where no patch landed
+    float* dst;
+    int width, height, sstride, dstride;
+    int tiles_x;
+    int wg_begin;
+};
+
+struct FinishTable {
+    int n;
+    FinishJob job[kMaxJobs];
+};
+
+struct ZeroJob {
+    unsigned long long* p;
+    long long count;
+    int wg_begin;
+};
+
+struct ZeroTable {
+    int n;
+    ZeroJob job[2 * kMaxJobs];
+};
+
+}  // namespace nsship

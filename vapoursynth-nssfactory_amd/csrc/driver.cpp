@@ -1,0 +1,487 @@
+// Host driver behind the C ABI: filter instances, per-frame stage
+// orchestration and the batched matcher entry point.
+//
+// One frame (src/host/filter_bm3d.cpp:359-494 bm3dGetFrame): for every stage
+// (basic then Wiener when final=1, src/frontend/bm3d_args.cpp:186-221), planes
+// with sigma 0 are copied; the others are grouped by block size and each
+// group runs four launches on the caller's stream: clear num/den, match,
+// filter+aggregate, finish. The basic estimate stays on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/nss_hip.h"
+#include "bm3d_jobs.hpp"
+
+namespace nsship {
+
+hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, bool stage, hipStream_t s);
+hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s);
+hipError_t launch_finish(int block, const FinishTable& t, int total_wg, hipStream_t s);
+hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s);
+hipError_t upload_dct_tables(const float* host, size_t count);
+size_t match_lds_bytes(int block, int range, bool stage);
+size_t filter_lds_bytes(int block, int group, bool wiener);
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(NSS_HIP_RESOURCE_FAILURE, std::string("nss_hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceGuard(int device) {
+        err = hipGetDevice(&prev);
+        if (err == hipSuccess && prev != device) err = hipSetDevice(device);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+constexpr int kLdsLimit = 160 * 1024;  // LDS per workgroup on gfx950
+
+// Per device, once: the DCT matrices in constant memory.
+std::mutex g_init_mutex;
+bool g_dct_ready[64] = {};
+
+int ensure_tables(int device) {
+    std::lock_guard<std::mutex> lock(g_init_mutex);
+    if (device < 0 || device >= 64) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: device index out of range");
+    if (g_dct_ready[device]) return NSS_HIP_OK;
+    std::vector<float> t;
+    const double pi = 3.14159265358979323846264338327950288;
+    for (int n : {2, 4, 8, 12, 16, 32})
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < n; ++i)
+                t.push_back(static_cast<float>(std::sqrt((k == 0 ? 1.0 : 2.0) / n) * std::cos(pi * (i + 0.5) * k / n)));
+    const hipError_t e = upload_dct_tables(t.data(), t.size());
+    if (e != hipSuccess) return hip_fail(e, "DCT table upload");
+    g_dct_ready[device] = true;
+    return NSS_HIP_OK;
+}
+
+Grid make_grid(int width, int height, int block, int step) {
+    Grid g{0, 0, step};
+    if (width < block || height < block || step < 1) return g;
+    g.nx = (width - block + step - 1) / step + 1;
+    g.ny = (height - block + step - 1) / step + 1;
+    return g;
+}
+
+struct StagePlane {
+    float sigma;
+    int block, group, step, range, ps_num, ps_range;
+};
+
+struct Stage {
+    StagePlane plane[3];
+    bool wiener;
+};
+
+}  // namespace nsship
+
+using namespace nsship;
+
+// Event pairs around launches while tracing is on (one vector per kind).
+struct Tracer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[NSS_HIP_KERNEL_KINDS];
+
+    hipEvent_t take() {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void reset() {
+        used = 0;
+        for (auto& v : spans) v.clear();
+    }
+    ~Tracer() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct nss_hip_bm3d {
+    nss_hip_bm3d_params p{};
+    Tracer tracer;
+    nss_hip_video_info vi{};
+    int device = 0;
+    int nplanes = 0;
+    int pw[3]{}, ph[3]{};
+    Stage stages[2]{};
+    int nstages = 0;
+    unsigned long long* num[3]{};
+    unsigned long long* den[3]{};
+    MatchRec* matches[3]{};
+    int* counts[3]{};
+    float* basic[3]{};
+};
+
+namespace {
+
+bool allowed_block(int b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 12 || b == 16 || b == 32; }
+bool allowed_group(int g) { return g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64; }
+
+int check_stage_plane(const StagePlane& s, int w, int h, bool wiener) {
+    if (!allowed_block(s.block) || !allowed_group(s.group) || s.block > w || s.block > h || s.step < 1 ||
+        s.step > s.block || s.range < 1 || s.range > 64)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip: invalid resolved BM3D parameters");
+    if (filter_lds_bytes(s.block, s.group, wiener) > static_cast<size_t>(kLdsLimit))
+        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: block_size " + std::to_string(s.block) + " with group_size " +
+                                             std::to_string(s.group) + " exceeds the on-chip group cube");
+    return NSS_HIP_OK;
+}
+
+void free_buffers(nss_hip_bm3d* h) {
+    for (int i = 0; i < 3; ++i) {
+        if (h->num[i]) (void)hipFree(h->num[i]);
+        if (h->den[i]) (void)hipFree(h->den[i]);
+        if (h->matches[i]) (void)hipFree(h->matches[i]);
+        if (h->counts[i]) (void)hipFree(h->counts[i]);
+        if (h->basic[i]) (void)hipFree(h->basic[i]);
+        h->num[i] = h->den[i] = nullptr;
+        h->matches[i] = nullptr;
+        h->counts[i] = nullptr;
+        h->basic[i] = nullptr;
+    }
+}
+
+// One stage over the given planes: match on `match`, filter `src` (Wiener
+// reference `wref` when the stage is Wiener), finish into `dst`.
+int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const int* src_stride,
+              const float* const* match, const int* match_stride, const float* const* wref, const int* wref_stride,
+              float* const* dst, const int* dst_stride, hipStream_t s) {
+    bool active[3] = {false, false, false};
+    for (int i = 0; i < h->nplanes; ++i) {
+        if (st.plane[i].sigma == 0.f) {
+            const hipError_t e =
+                hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, src[i],
+                                 static_cast<size_t>(src_stride[i]) * 4, static_cast<size_t>(h->pw[i]) * 4, h->ph[i],
+                                 hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) return hip_fail(e, "plane copy");
+        } else {
+            active[i] = true;
+        }
+    }
+    for (int b : {1, 2, 4, 8, 12, 16, 32}) {
+        ZeroTable zt{};
+        MatchTable mt{};
+        FilterTable ft{};
+        FinishTable fn{};
+        int zwg = 0, mwg = 0, fwg = 0, nwg = 0;
+        size_t mlds = 0, flds = 0;
+        bool stage_lds = true;
+        for (int i = 0; i < h->nplanes; ++i) {
+            if (!active[i] || st.plane[i].block != b) continue;
+            const StagePlane& sp = st.plane[i];
+            const int w = h->pw[i], hh = h->ph[i];
+            const Grid g = make_grid(w, hh, b, sp.step);
+            const int nref = g.nx * g.ny;
+            const long long npx = static_cast<long long>(w) * hh;
+            for (unsigned long long* buf : {h->num[i], h->den[i]}) {
+                ZeroJob& z = zt.job[zt.n++];
+                z.p = buf;
+                z.count = npx;
+                z.wg_begin = zwg;
+                zwg += static_cast<int>((npx + 1023) / 1024);
+            }
+            MatchJob& m = mt.job[mt.n++];
+            m.plane = match[i];
+            m.width = w;
+            m.height = hh;
+            m.stride = match_stride[i];
+            m.grid = g;
+            m.range = sp.range;
+            m.group = sp.group;
+            m.t0 = 0;
+            m.wg_begin = mwg;
+            m.out = h->matches[i];
+            m.counts = h->counts[i];
+            mwg += nref;
+            mlds = std::max(mlds, match_lds_bytes(b, sp.range, true));
+            FilterJob& f = ft.job[ft.n++];
+            f.src = src[i];
+            f.wref = st.wiener ? wref[i] : nullptr;
+            f.width = w;
+            f.height = hh;
+            f.stride = src_stride[i];
+            f.wref_stride = st.wiener ? wref_stride[i] : 0;
+            f.nref = nref;
+            f.group = sp.group;
+            f.sigma = sp.sigma;
+            f.wg_begin = fwg;
+            f.matches = h->matches[i];
+            f.counts = h->counts[i];
+            f.num = h->num[i];
+            f.den = h->den[i];
+            fwg += nref;
+            flds = std::max(flds, filter_lds_bytes(b, sp.group, st.wiener));
+            FinishJob& n = fn.job[fn.n++];
+            n.num = h->num[i];
+            n.den = h->den[i];
+            n.src = src[i];
+            n.dst = dst[i];
+            n.width = w;
+            n.height = hh;
+            n.sstride = src_stride[i];
+            n.dstride = dst_stride[i];
+            n.tiles_x = (w + 15) / 16;
+            n.wg_begin = nwg;
+            nwg += n.tiles_x * ((hh + 15) / 16);
+        }
+        if (mt.n == 0) continue;
+        if (mlds > static_cast<size_t>(kLdsLimit)) {
+            stage_lds = false;
+            mlds = 0;
+            for (int j = 0; j < mt.n; ++j) mlds = std::max(mlds, match_lds_bytes(b, mt.job[j].range, false));
+        }
+        auto traced = [&](int kind, auto&& launch) -> hipError_t {
+            hipEvent_t a = nullptr, z = nullptr;
+            if (h->tracer.on) {
+                a = h->tracer.take();
+                z = h->tracer.take();
+                if (a) (void)hipEventRecord(a, s);
+            }
+            const hipError_t err = launch();
+            if (a && z) {
+                (void)hipEventRecord(z, s);
+                h->tracer.spans[kind].emplace_back(a, z);
+            }
+            return err;
+        };
+        hipError_t e = traced(0, [&] { return launch_zero(zt, zwg, s); });
+        if (e != hipSuccess) return hip_fail(e, "zero launch");
+        e = traced(1, [&] { return launch_match(b, mt, mwg, mlds, stage_lds, s); });
+        if (e != hipSuccess) return hip_fail(e, "match launch");
+        e = traced(2, [&] { return launch_filter(b, ft, fwg, flds, st.wiener, s); });
+        if (e != hipSuccess) return hip_fail(e, "filter launch");
+        e = traced(3, [&] { return launch_finish(b, fn, nwg, s); });
+        if (e != hipSuccess) return hip_fail(e, "finish launch");
+    }
+    return NSS_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nss_hip_last_error(void) { return g_last_error.c_str(); }
+
+int nss_hip_backend_info(int device, char* text, int len) {
+    if (!text || len < 1) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: no output buffer");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return hip_fail(e, "device count");
+    if (device < 0 || device >= count) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: device index out of range");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return hip_fail(e, "device properties");
+    std::snprintf(text, static_cast<size_t>(len), "nss_hip 1.0 device=%d name=%s arch=%s cus=%d hbm_bytes=%zu",
+                  device, prop.name, prop.gcnArchName, prop.multiProcessorCount, prop.totalGlobalMem);
+    return NSS_HIP_OK;
+}
+
+int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device,
+                        nss_hip_bm3d** out) {
+    if (!params || !info || !out) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    *out = nullptr;
+    if (info->sample_type != 1 || info->bits_per_sample != 32 || info->width <= 0 || info->height <= 0 ||
+        info->num_planes < 1 || info->num_planes > 3)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.BM3D: constant Gray/YUV/RGB 32-bit float required");
+    if (params->radius != 0)
+        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: radius > 0 is not available in this build");
+    if (params->chroma) return fail(NSS_HIP_UNSUPPORTED, "nss_hip: chroma=1 is not available in this build");
+    DeviceGuard guard(device);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
+    int rc = ensure_tables(device);
+    if (rc != NSS_HIP_OK) return rc;
+
+    auto* h = new (std::nothrow) nss_hip_bm3d();
+    if (!h) return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: out of host memory");
+    h->p = *params;
+    h->vi = *info;
+    h->device = device;
+    h->nplanes = info->num_planes;
+    for (int i = 0; i < h->nplanes; ++i) {
+        h->pw[i] = i ? info->width >> info->sub_sampling_w : info->width;
+        h->ph[i] = i ? info->height >> info->sub_sampling_h : info->height;
+    }
+    const nss_hip_bm3d_params& p = *params;
+    if (p.final_stage) {
+        Stage& b = h->stages[h->nstages++];
+        b.wiener = false;
+        for (int i = 0; i < 3; ++i)
+            b.plane[i] = StagePlane{p.sigma_basic[i], p.block_size_basic[i], p.group_size_basic[i],
+                                    p.block_step_basic[i], p.bm_range[i], p.ps_num_basic[i], p.ps_range[i]};
+    }
+    {
+        Stage& f = h->stages[h->nstages++];
+        f.wiener = p.final_stage || p.has_ref;
+        for (int i = 0; i < 3; ++i)
+            f.plane[i] = StagePlane{p.sigma[i], p.block_size[i], p.group_size[i], p.block_step[i], p.bm_range[i],
+                                    p.ps_num[i], p.ps_range[i]};
+    }
+    for (int s = 0; s < h->nstages; ++s)
+        for (int i = 0; i < h->nplanes; ++i) {
+            if (h->stages[s].plane[i].sigma == 0.f) continue;
+            rc = check_stage_plane(h->stages[s].plane[i], h->pw[i], h->ph[i], h->stages[s].wiener);
+            if (rc != NSS_HIP_OK) {
+                delete h;
+                return rc;
+            }
+        }
+    for (int i = 0; i < h->nplanes; ++i) {
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        size_t recs = 0, nref_max = 0;
+        for (int s = 0; s < h->nstages; ++s) {
+            const StagePlane& sp = h->stages[s].plane[i];
+            if (sp.sigma == 0.f) continue;
+            const Grid g = make_grid(h->pw[i], h->ph[i], sp.block, sp.step);
+            const size_t nref = static_cast<size_t>(g.nx) * g.ny;
+            recs = std::max(recs, nref * sp.group);
+            nref_max = std::max(nref_max, nref);
+        }
+        hipError_t e = hipSuccess;
+        if (recs) {
+            e = hipMalloc(&h->num[i], npx * 8);
+            if (e == hipSuccess) e = hipMalloc(&h->den[i], npx * 8);
+            if (e == hipSuccess) e = hipMalloc(&h->matches[i], recs * sizeof(MatchRec));
+            if (e == hipSuccess) e = hipMalloc(&h->counts[i], nref_max * sizeof(int));
+        }
+        if (e == hipSuccess && p.final_stage) e = hipMalloc(&h->basic[i], npx * 4);
+        if (e != hipSuccess) {
+            free_buffers(h);
+            delete h;
+            return hip_fail(e, "device allocation");
+        }
+    }
+    *out = h;
+    return NSS_HIP_OK;
+}
+
+int nss_hip_bm3d_slots(const nss_hip_bm3d* handle) { return handle ? 2 * handle->p.radius + 1 : 0; }
+
+int nss_hip_bm3d_process(nss_hip_bm3d* h, const float* const* src, const int* src_stride, const float* const* ref,
+                         const int* ref_stride, float* const* dst, const int* dst_stride, int frame, int frame_count,
+                         void* stream) {
+    (void)frame;
+    (void)frame_count;
+    if (!h || !src || !src_stride || !dst || !dst_stride) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    for (int i = 0; i < h->nplanes; ++i)
+        if (!src[i] || !dst[i] || src_stride[i] < h->pw[i] || dst_stride[i] < h->pw[i])
+            return fail(NSS_HIP_INVALID_INPUT, "nss_hip: missing plane or stride below width");
+    if (h->p.has_ref && !h->p.final_stage) {
+        if (!ref || !ref_stride) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: ref planes required");
+        for (int i = 0; i < h->nplanes; ++i)
+            if (!ref[i] || ref_stride[i] < h->pw[i])
+                return fail(NSS_HIP_INVALID_INPUT, "nss_hip: missing ref plane or stride below width");
+    }
+    DeviceGuard guard(h->device);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->p.final_stage) {
+        float* basic[3] = {h->basic[0], h->basic[1], h->basic[2]};
+        int bstride[3] = {h->pw[0], h->pw[1], h->pw[2]};
+        int rc = run_stage(h, h->stages[0], src, src_stride, src, src_stride, nullptr, nullptr, basic, bstride, s);
+        if (rc != NSS_HIP_OK) return rc;
+        const float* cb[3] = {basic[0], basic[1], basic[2]};
+        return run_stage(h, h->stages[1], src, src_stride, cb, bstride, cb, bstride, dst, dst_stride, s);
+    }
+    if (h->stages[0].wiener) return run_stage(h, h->stages[0], src, src_stride, ref, ref_stride, ref, ref_stride, dst,
+                                              dst_stride, s);
+    return run_stage(h, h->stages[0], src, src_stride, src, src_stride, nullptr, nullptr, dst, dst_stride, s);
+}
+
+int nss_hip_bm3d_set_tracing(nss_hip_bm3d* h, int on) {
+    if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null handle");
+    h->tracer.on = on != 0;
+    return NSS_HIP_OK;
+}
+
+int nss_hip_bm3d_kernel_times(nss_hip_bm3d* h, int* launches, double* total_ms, int reset) {
+    if (!h || !launches || !total_ms) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    DeviceGuard guard(h->device);
+    for (int k = 0; k < NSS_HIP_KERNEL_KINDS; ++k) {
+        launches[k] = static_cast<int>(h->tracer.spans[k].size());
+        double sum = 0;
+        for (auto& span : h->tracer.spans[k]) {
+            hipError_t e = hipEventSynchronize(span.second);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, span.first, span.second);
+            if (e != hipSuccess) return hip_fail(e, "event timing");
+            sum += ms;
+        }
+        total_ms[k] = sum;
+    }
+    if (reset) h->tracer.reset();
+    return NSS_HIP_OK;
+}
+
+int nss_hip_bm3d_destroy(nss_hip_bm3d* h) {
+    if (!h) return NSS_HIP_OK;
+    {
+        DeviceGuard guard(h->device);
+        (void)hipDeviceSynchronize();
+        free_buffers(h);
+    }
+    delete h;
+    return NSS_HIP_OK;
+}
+
+int nss_hip_spatial_match(const float* plane, int width, int height, int stride, int block, int step, int bm_range,
+                          int group, int* out_rec, int* out_count, void* stream) {
+    if (!plane || !out_rec || !out_count) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    if (!allowed_block(block) || group < 1 || group > kMaxGroup || width < block || height < block || step < 1 ||
+        stride < width || bm_range < 0 || bm_range > 64)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip: invalid match arguments");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "get device");
+    MatchTable mt{};
+    MatchJob& m = mt.job[mt.n++];
+    m.plane = plane;
+    m.width = width;
+    m.height = height;
+    m.stride = stride;
+    m.grid = make_grid(width, height, block, step);
+    m.range = bm_range;
+    m.group = group;
+    m.t0 = 0;
+    m.wg_begin = 0;
+    m.out = reinterpret_cast<MatchRec*>(out_rec);
+    m.counts = out_count;
+    const int nref = m.grid.nx * m.grid.ny;
+    if (nref == 0) return NSS_HIP_OK;
+    size_t lds = match_lds_bytes(block, bm_range, true);
+    bool stage = true;
+    if (lds > static_cast<size_t>(kLdsLimit)) {
+        stage = false;
+        lds = match_lds_bytes(block, bm_range, false);
+    }
+    e = launch_match(block, mt, nref, lds, stage, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "match launch");
+    return NSS_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,300 @@
+"""nss_hip: Python host layer over the C ABI of libnss_hip.so.
+
+Mirrors the reference plugin's ``nss.BM3D`` interface for callers without
+VapourSynth: the same argument names, defaults, per-plane short-array
+inheritance and error texts (resolved by the C++ parser behind
+``nss_hip_bm3d_parse``), applied to frames held as device tensors.
+
+    inst = BM3D("YUV420PS", 1920, 1080, sigma=10, final=1)
+    out_planes = inst(planes)          # list of torch.cuda float32 2-D tensors
+
+PyTorch only provides device memory and the stream; every computation runs in
+the HIP kernels of libnss_hip.so. A missing library is an error: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnss_hip.so")
+NAMESPACE = "nss_hip"
+
+# The reference signature (src/frontend/bm3d_args.hpp:8-12) plus the device
+# backend tail (src/cuda/runtime/context.hpp:17).
+BM3D_SIGNATURE = (
+    "clip:vnode;ref:vnode:opt;sigma:float[]:opt;block_size:int[]:opt;group_size:int[]:opt;"
+    "block_step:int[]:opt;bm_range:int[]:opt;radius:int:opt;ps_num:int[]:opt;ps_range:int[]:opt;chroma:int:opt;"
+    "final:int:opt;sigma_basic:float[]:opt;block_size_basic:int[]:opt;group_size_basic:int[]:opt;"
+    "temporal_mode:data:opt;rolling_chunk:int:opt;rolling_cache_chunks:int:opt;rolling_cache_limit:int:opt;"
+    "memory_limit_mb:int:opt;"
+    "device_id:int:opt;num_streams:int:opt;"
+)
+
+OK, BYPASS, INVALID_INPUT, NUMERICAL_FAILURE, UNSUPPORTED, RESOURCE_FAILURE = range(6)
+
+# name: (color family, sub-sampling w, h, planes); VapourSynth API4 values.
+FORMATS = {
+    "GRAYS": (1, 0, 0, 1),
+    "RGBS": (2, 0, 0, 3),
+    "YUV420PS": (3, 1, 1, 3),
+    "YUV422PS": (3, 1, 0, 3),
+    "YUV444PS": (3, 0, 0, 3),
+}
+FORMAT_NAMES = {"GRAYS": "GrayS", "RGBS": "RGBS", "YUV420PS": "YUV420PS", "YUV422PS": "YUV422PS",
+                "YUV444PS": "YUV444PS"}
+
+
+class Error(Exception):
+    """Raised with the library's message (the text nss.BM3D would report)."""
+
+
+class VideoInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("color_family", "sample_type", "bits_per_sample", "sub_sampling_w",
+                                            "sub_sampling_h", "num_planes", "width", "height", "num_frames")]
+
+
+class Arg(ctypes.Structure):
+    _fields_ = [("key", ctypes.c_char_p), ("type", ctypes.c_char), ("count", ctypes.c_int),
+                ("ints", ctypes.POINTER(ctypes.c_int64)), ("floats", ctypes.POINTER(ctypes.c_double)),
+                ("data", ctypes.c_char_p), ("clip_info", ctypes.POINTER(VideoInfo))]
+
+
+def _arr(t, n):
+    return t * n
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("sigma", _arr(ctypes.c_float, 3)), ("block_size", _arr(ctypes.c_int, 3)),
+                ("group_size", _arr(ctypes.c_int, 3)), ("block_step", _arr(ctypes.c_int, 3)),
+                ("bm_range", _arr(ctypes.c_int, 3)), ("ps_num", _arr(ctypes.c_int, 3)),
+                ("ps_range", _arr(ctypes.c_int, 3)), ("radius", ctypes.c_int), ("chroma", ctypes.c_int),
+                ("final_stage", ctypes.c_int), ("sigma_basic", _arr(ctypes.c_float, 3)),
+                ("block_size_basic", _arr(ctypes.c_int, 3)), ("group_size_basic", _arr(ctypes.c_int, 3)),
+                ("block_step_basic", _arr(ctypes.c_int, 3)), ("ps_num_basic", _arr(ctypes.c_int, 3)),
+                ("has_ref", ctypes.c_int), ("temporal_mode", ctypes.c_int), ("rolling_chunk", ctypes.c_int),
+                ("rolling_cache_chunks", ctypes.c_int), ("rolling_cache_limit", ctypes.c_int),
+                ("rolling_cache_adaptive", ctypes.c_int), ("memory_limit_mb", ctypes.c_int),
+                ("device_id", ctypes.c_int), ("num_streams", ctypes.c_int)]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+_lib = None
+
+EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "nss_hip_bm3d_output_info",
+           "nss_hip_bm3d_create", "nss_hip_bm3d_slots", "nss_hip_bm3d_process", "nss_hip_bm3d_destroy",
+           "nss_hip_spatial_match", "nss_hip_bm3d_set_tracing", "nss_hip_bm3d_kernel_times")
+KERNEL_KINDS = ("clear", "match", "filter", "finish")
+
+
+def lib():
+    """The loaded libnss_hip.so; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp, ci = ctypes.c_void_p, ctypes.c_int
+        L.nss_hip_last_error.restype = ctypes.c_char_p
+        L.nss_hip_last_error.argtypes = []
+        L.nss_hip_backend_info.argtypes = [ci, ctypes.c_char_p, ci]
+        L.nss_hip_bm3d_parse.argtypes = [P(VideoInfo), P(VideoInfo), P(Arg), ci, ctypes.c_char_p, P(Params)]
+        L.nss_hip_bm3d_output_info.argtypes = [P(Params), P(VideoInfo), P(VideoInfo)]
+        L.nss_hip_bm3d_create.argtypes = [P(Params), P(VideoInfo), ci, P(vp)]
+        L.nss_hip_bm3d_slots.argtypes = [vp]
+        L.nss_hip_bm3d_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), ci, ci, vp]
+        L.nss_hip_bm3d_destroy.argtypes = [vp]
+        L.nss_hip_spatial_match.argtypes = [vp, ci, ci, ci, ci, ci, ci, ci, vp, vp, vp]
+        L.nss_hip_bm3d_set_tracing.argtypes = [vp, ci]
+        L.nss_hip_bm3d_kernel_times.argtypes = [vp, P(ci), P(ctypes.c_double), ci]
+        for name in EXPORTS[1:]:
+            getattr(L, name).restype = ci
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().nss_hip_last_error().decode()
+
+
+def _check(rc):
+    if rc != OK:
+        raise Error(last_error())
+
+
+def video_info(fmt: str, width: int, height: int, frames: int = 1) -> VideoInfo:
+    cf, sw, sh, np_ = FORMATS[fmt]
+    return VideoInfo(cf, 1, 32, sw, sh, np_, width, height, frames)
+
+
+def _signature_types():
+    types = {}
+    for item in filter(None, BM3D_SIGNATURE.split(";")):
+        name, kind, *_ = item.split(":")
+        types[name] = kind.rstrip("[]")
+    return types
+
+
+_TYPES = _signature_types()
+
+
+def _build_args(kwargs):
+    """kwargs -> Arg array, values converted to the signature's type (as
+    VapourSynth's invoke does); the buffers are kept alive by the return."""
+    keep = []
+    args = []
+    for key, value in kwargs.items():
+        if value is None:
+            continue
+        kind = _TYPES.get(key)
+        if kind is None:
+            raise Error(f"{NAMESPACE}.BM3D: no argument named {key}")
+        vals = list(value) if isinstance(value, (list, tuple)) else [value]
+        if kind == "int":
+            buf = (ctypes.c_int64 * len(vals))(*[int(v) for v in vals])
+            keep.append(buf)
+            args.append(Arg(key.encode(), b"i", len(vals), buf, None, None, None))
+        elif kind == "float":
+            buf = (ctypes.c_double * len(vals))(*[float(v) for v in vals])
+            keep.append(buf)
+            args.append(Arg(key.encode(), b"f", len(vals), None, buf, None, None))
+        elif kind == "data":
+            data = str(vals[0]).encode()
+            keep.append(data)
+            args.append(Arg(key.encode(), b"s", 1, None, None, data, None))
+        else:
+            raise Error(f"{NAMESPACE}.BM3D: argument {key} is a clip")
+    arr = (Arg * max(1, len(args)))(*args)
+    return arr, len(args), keep
+
+
+def parse(fmt, width, height, frames=1, ref=False, info=None, ref_info=None, **kwargs) -> Params:
+    """Resolve nss.BM3D arguments; raises Error with the reference's text."""
+    vi = info if info is not None else video_info(fmt, width, height, frames)
+    rvi = ref_info if ref_info is not None else (vi if ref else None)
+    arr, n, keep = _build_args(kwargs)
+    out = Params()
+    rc = lib().nss_hip_bm3d_parse(ctypes.byref(vi), ctypes.byref(rvi) if rvi is not None else None, arr, n,
+                                  NAMESPACE.encode(), ctypes.byref(out))
+    del keep
+    _check(rc)
+    return out
+
+
+def output_info(params: Params, info: VideoInfo) -> VideoInfo:
+    out = VideoInfo()
+    _check(lib().nss_hip_bm3d_output_info(ctypes.byref(params), ctypes.byref(info), ctypes.byref(out)))
+    return out
+
+
+def backend_info(device: int = 0) -> str:
+    buf = ctypes.create_string_buffer(512)
+    _check(lib().nss_hip_backend_info(device, buf, 512))
+    return buf.value.decode()
+
+
+def _stream_handle(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class BM3D:
+    """One filter instance (fixed format and size) on one device.
+
+    Calling it filters a frame: ``planes`` is a list of 2-D float32 CUDA
+    tensors (one per plane); ``ref`` likewise when the instance was made with
+    ``ref=True``. Returns new output tensors (or writes ``out`` when given).
+    Work is queued on the current torch stream.
+    """
+
+    def __init__(self, fmt, width, height, frames=1, ref=False, **kwargs):
+        self.fmt = fmt
+        self.info = video_info(fmt, width, height, frames)
+        self.params = parse(fmt, width, height, frames, ref=ref, info=self.info, **kwargs)
+        self.out_info = output_info(self.params, self.info)
+        self.device = self.params.device_id
+        h = ctypes.c_void_p()
+        _check(lib().nss_hip_bm3d_create(ctypes.byref(self.params), ctypes.byref(self.info), self.device,
+                                         ctypes.byref(h)))
+        self._h = h
+        cf, sw, sh, n = FORMATS[fmt]
+        self.plane_dims = [(height >> (sh if i else 0), width >> (sw if i else 0)) for i in range(n)]
+
+    def set_tracing(self, on=True):
+        _check(lib().nss_hip_bm3d_set_tracing(self._h, int(on)))
+
+    def kernel_times(self, reset=True):
+        """{kind: (launches, total_ms)} since the last reset (waits for the events)."""
+        n = (ctypes.c_int * 4)()
+        ms = (ctypes.c_double * 4)()
+        _check(lib().nss_hip_bm3d_kernel_times(self._h, n, ms, int(reset)))
+        return {k: (n[i], ms[i]) for i, k in enumerate(KERNEL_KINDS)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nss_hip_bm3d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __call__(self, planes, ref=None, out=None, frame=0, frame_count=1, stream=None):
+        import torch
+        n = len(self.plane_dims)
+        slots = lib().nss_hip_bm3d_slots(self._h)
+        if len(planes) != n * slots:
+            raise Error(f"{NAMESPACE}.BM3D: expected {n * slots} planes, got {len(planes)}")
+        for i, p in enumerate(planes):
+            if p.dtype != torch.float32 or not p.is_cuda or p.dim() != 2 or p.stride(1) != 1:
+                raise Error(f"{NAMESPACE}.BM3D: plane {i} must be a row-contiguous 2-D float32 CUDA tensor")
+            if tuple(p.shape) != self.plane_dims[i % n]:
+                raise Error(f"{NAMESPACE}.BM3D: plane {i} has shape {tuple(p.shape)}, expected {self.plane_dims[i % n]}")
+        if out is None:
+            out = [torch.empty(self.plane_dims[i], dtype=torch.float32, device=planes[i].device) for i in range(n)]
+        VP = ctypes.c_void_p * len(planes)
+        IP = ctypes.c_int * len(planes)
+        src = VP(*[p.data_ptr() for p in planes])
+        sst = IP(*[p.stride(0) for p in planes])
+        if ref is not None:
+            rp = VP(*[p.data_ptr() for p in ref])
+            rst = IP(*[p.stride(0) for p in ref])
+        else:
+            rp, rst = None, None
+        dp = (ctypes.c_void_p * n)(*[p.data_ptr() for p in out])
+        dst = (ctypes.c_int * n)(*[p.stride(0) for p in out])
+        _check(lib().nss_hip_bm3d_process(self._h, src, sst, rp, rst, dp, dst, frame, frame_count,
+                                          _stream_handle(stream)))
+        return out
+
+
+def grid_size(width, height, block, step):
+    """(nx, ny) of the reference grid (src/host/batch_runner.hpp:17-31)."""
+    if width < block or height < block or step < 1:
+        return 0, 0
+    return (width - block + step - 1) // step + 1, (height - block + step - 1) // step + 1
+
+
+def spatial_match(plane, block=8, step=8, bm_range=9, group=8, stream=None):
+    """Batched matcher over one plane's grid. plane: 2-D float32 CUDA tensor.
+    Returns (rec int32 [nref, group, 4] = x, y, t, dist bits; count int32 [nref])."""
+    import torch
+    h, w = plane.shape
+    nx, ny = grid_size(w, h, block, step)
+    rec = torch.zeros((nx * ny, group, 4), dtype=torch.int32, device=plane.device)
+    cnt = torch.zeros((nx * ny,), dtype=torch.int32, device=plane.device)
+    _check(lib().nss_hip_spatial_match(ctypes.c_void_p(plane.data_ptr()), w, h, plane.stride(0), block, step,
+                                       bm_range, group, ctypes.c_void_p(rec.data_ptr()),
+                                       ctypes.c_void_p(cnt.data_ptr()), _stream_handle(stream)))
+    return rec, cnt
