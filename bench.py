@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one elementwise copy that reads and writes MIB MiB: the known-size kernel "
+                         "that scales rocprof's FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; each frame is independent, so N GPUs would each run this "
@@ -89,6 +92,13 @@ def main():
     import nss_hip
     import synth
 
+    if args.copy_calibration:
+        n = args.copy_calibration * 1024 * 1024 // 4
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     print(f"# generating {FMT} {W}x{H} sigma {SIGMA}", file=sys.stderr, flush=True)
     clean, noisy = synth.noisy_frame(FMT, W, H, SIGMA, seed=0)
     dev = [torch.from_numpy(p).cuda() for p in noisy]

@@ -72,10 +72,13 @@ def _compare_matches(gpu, oracle, plane, block, step, rng_, group, flat, label):
 
 @pytest.mark.parametrize("flat", [False, True])
 @pytest.mark.parametrize("block,group,step,rng_", [(8, 16, 3, 16), (8, 8, 8, 9), (4, 16, 2, 7), (12, 8, 6, 5),
-                                                   (16, 4, 8, 6), (2, 32, 1, 3), (1, 8, 1, 2), (32, 2, 16, 4)])
+                                                   (16, 4, 8, 6), (2, 32, 1, 3), (1, 8, 1, 2), (32, 2, 16, 4),
+                                                   (8, 64, 3, 30), (16, 8, 8, 20), (32, 4, 16, 64)])
 def test_match_lists(gpu, oracle, block, group, step, rng_, flat):
+    """Covers the run matcher (B <= 12, small windows), the wave matcher with
+    the window in LDS, and with the window read from global (B 32, range 64)."""
     print(f"match b{block} g{group} s{step} r{rng_} flat={flat}", flush=True)
-    plane = _test_plane(77, 61, 7 + block + group, flat)
+    plane = _test_plane(77 if rng_ < 30 else 150, 61 if rng_ < 30 else 140, 7 + block + group, flat)
     flips = _compare_matches(gpu, oracle, plane, block, step, rng_, group, flat, f"b{block}g{group}")
     print(f"near-tie flips {flips}", flush=True)
 
@@ -147,6 +150,24 @@ def test_shapes_hard_and_wiener(gpu, oracle, block):
             p = oracle.psnr(got[0], want[0])
             assert p >= MIN_PSNR, (block, group, wiener, p)
     print(f"block {block} ok", flush=True)
+
+
+@pytest.mark.parametrize("block,group,rng_", [(32, 8, 40), (16, 32, 48), (8, 8, 60)])
+def test_large_windows(gpu, oracle, block, group, rng_):
+    """Windows too large for tile aggregation in LDS take the frame-sized
+    atomic sums (and the wave matcher); same gates."""
+    import synth
+    h, w = 120, 136
+    clean = synth.clean_image(w, h, seed=block)
+    noisy = synth.add_noise(clean, 20, seed=block + 1)
+    for wiener in (False, True):
+        kw = dict(sigma=20, block_size=block, group_size=group, block_step=block // 2, bm_range=rng_)
+        ref = [clean] if wiener else None
+        got = _run(gpu, "GRAYS", [noisy], ref=ref, **kw)
+        want = _oracle_frame(oracle, gpu, "GRAYS", [noisy], ref=ref, **kw)
+        p = oracle.psnr(got[0], want[0])
+        print(f"large window b{block} g{group} r{rng_} wiener={wiener}: {p:.2f} dB", flush=True)
+        assert p >= MIN_PSNR
 
 
 @pytest.mark.parametrize("block", [1, 2, 4, 8, 12, 16, 32])

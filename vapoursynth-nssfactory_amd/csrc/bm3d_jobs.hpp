@@ -14,6 +14,7 @@ inline constexpr int kMaxJobs = 8;
 inline constexpr int kMaxGroup = 64;   // include/nss/params.hpp:39 kBmMaxGroup
 inline constexpr int kMaxRadius = 16;  // include/nss/params.hpp:44 kBmMaxRadius
 inline constexpr int kMaxSlots = 2 * kMaxRadius + 1;
+inline constexpr int kRunRefs = 8;     // references per workgroup of the run matcher
 
 // A match record: x, y, temporal slot, and the distance's bits (the float
 // SSD; non-finite distances collapse to +inf bits, so they rank equal and
@@ -46,18 +47,45 @@ struct MatchTable {
     MatchJob job[kMaxJobs];
 };
 
+// Tiles of the reference grid for LDS aggregation: tile (tx, ty) holds the
+// references i in [tx*tw, tx*tw + tw), j in [ty*th, ty*th + th). Its sums
+// cover the rectangle anchored at (min(tx*tw*step, width - block) - range,
+// min(ty*th*step, height - block) - range) of aw x ah pixels, which holds
+// every patch its references can match.
+struct TileGeom {
+    int tw, th;        // references per tile
+    int ntx, nty;      // tiles
+    int aw, ah;        // accumulation rectangle
+    __host__ __device__ int ax0(int tx, int width, int block, int step, int range) const {
+        const int x = tx * tw * step;
+        return (x < width - block ? x : width - block) - range;
+    }
+    __host__ __device__ int ay0(int ty, int height, int block, int step, int range) const {
+        const int y = ty * th * step;
+        return (y < height - block ? y : height - block) - range;
+    }
+};
+
 struct FilterJob {
     const float* src;   // filtered plane (noisy)
     const float* wref;  // Wiener reference plane (the basic estimate); null in the hard stage
     int width, height, stride, wref_stride;
+    Grid grid;
+    int range;
     int nref;
     int group;
     float sigma;        // effective sigma
     int wg_begin;
     const MatchRec* matches;
     const int* counts;
-    unsigned long long* num;  // fixed-point numerator, 2^32 units, pitch = width
-    unsigned long long* den;  // fixed-point weight at each patch's top-left corner
+    // Frame-sized fixed-point sums (the atomic path): numerator at every
+    // pixel, weight at each patch's top-left corner, 2^32 units.
+    unsigned long long* num;
+    unsigned long long* den;
+    // Per-tile partial sums (the tile path): [tile][ah][aw] floats each.
+    TileGeom tiles;
+    float* part_num;
+    float* part_den;
 };
 
 struct FilterTable {
@@ -66,12 +94,16 @@ struct FilterTable {
 };
 
 struct FinishJob {
-    const unsigned long long* num;
+    const unsigned long long* num;  // atomic path
     const unsigned long long* den;
+    const float* part_num;          // tile path
+    const float* part_den;
+    TileGeom tiles;
+    int block, step, range;
     const float* src;   // fallback where no patch landed
     float* dst;
     int width, height, sstride, dstride;
-    int tiles_x;
+    int tiles_x;        // 16 x 16 output tiles per row
     int wg_begin;
 };
 

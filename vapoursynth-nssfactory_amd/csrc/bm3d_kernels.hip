@@ -1,21 +1,32 @@
 // BM3D kernels for CDNA4 (gfx950): block matching, group filter with
 // weighted fixed-point aggregation, and finish.
 //
-//   match_kernel<B, STAGE>   one wave per reference: the clamped search
-//                            window (plus the block overhang) staged in LDS,
-//                            SSD of every candidate in FP32, then the G-1
-//                            smallest keys (dist, y, x) by rounds of a
-//                            wave-wide 64-bit min.
-//   filter_kernel<B, WIENER> one wave per group: gather the k matched patches
-//                            (zero-padded to the configured group size) into
-//                            an LDS cube, 2-D DCT per patch, orthonormal Haar
-//                            along the group, hard threshold or Wiener
-//                            shrinkage, inverse, weight, and scatter-add into
-//                            int64 fixed-point num/den planes (order-free,
-//                            so results are run-to-run identical).
-//   finish_kernel<B>         den is kept at each patch's top-left corner; the
-//                            finish box-sums it over B x B and writes
-//                            num/den, or the source where nothing landed.
+// Main path (the shapes whose working sets fit the LDS budget):
+//   match_run_kernel<B>        one workgroup per run of 8 references on one
+//                              grid row; the union of their search windows is
+//                              staged in LDS once; each half-wave owns one
+//                              reference (patch in registers) and each lane a
+//                              4 x 4 tile of candidates; then the G-1 smallest
+//                              keys (dist, y, x) by rounds of a half-wave min.
+//   filter_tile_kernel<B, W>   one workgroup per tile of references; each wave
+//                              filters whole groups (LDS cube: 2-D DCT, Haar
+//                              along the group, hard threshold or Wiener,
+//                              inverse) and adds them into the tile's int64
+//                              fixed-point sums in LDS (ds_add_u64: exact and
+//                              order-free); the tile then writes its num and
+//                              box-summed den as float partials.
+//   finish_tiles_kernel        per pixel, the partials of the (at most a few)
+//                              tiles covering it, summed in a fixed order, then
+//                              num / den or the source.
+// General path (any legal shape):
+//   match_wave_kernel<B, S>    one wave per reference, window in LDS (or read
+//                              from global when it does not fit).
+//   filter_atomic_kernel<B, W> one wave per group, device-scope int64 atomics
+//                              into frame-sized sums (den at the corner).
+//   finish_corner_kernel<B>    box-sums the corner weights, divides.
+// Workgroups are dealt round-robin over the 8 XCDs; the main-path kernels
+// remap their block index so that each XCD works on a contiguous part of the
+// frame (its L2 then holds the overlapping windows and patches).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,11 +46,30 @@ __device__ constexpr int dct_offset() {
 
 inline constexpr uint32_t kTaken = 0xffffffffu;
 inline constexpr uint32_t kNonFinite = 0x7f800000u;
+inline constexpr int kXcds = 8;
 
-__device__ inline unsigned long long wave_min_u64(unsigned long long v) {
+// Block index such that each XCD (blockIdx % 8 on dispatch) covers a
+// contiguous range of logical blocks; a bijection on [0, gridDim.x).
+__device__ inline int xcd_block() {
+    const int nb = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    const int q = nb / kXcds, rem = nb % kXcds;
+    const int xcd = b % kXcds, slot = b / kXcds;
+    return xcd < rem ? xcd * (q + 1) + slot : rem * (q + 1) + (xcd - rem) * q + slot;
+}
+
+// Orders the calling wave's LDS accesses (one wave's DS operations execute
+// in issue order; this keeps the compiler from moving accesses across).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int WIDTH>
+__device__ inline unsigned long long group_min_u64(unsigned long long v) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
+    for (int off = WIDTH / 2; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, WIDTH);
         v = o < v ? o : v;
     }
     return v;
@@ -58,25 +88,146 @@ __device__ inline float wave_sum_f32(float v) {
 }
 
 template <typename Table>
-__device__ inline int find_job(const Table& tab) {
+__device__ inline int find_job(const Table& tab, int block) {
     int j = 0;
-    while (j + 1 < tab.n && static_cast<int>(blockIdx.x) >= tab.job[j + 1].wg_begin) ++j;
+    while (j + 1 < tab.n && block >= tab.job[j + 1].wg_begin) ++j;
     return j;
 }
 
+__device__ inline uint32_t dist_key(float d) {
+    const uint32_t k = __float_as_uint(d);
+    return (k & kNonFinite) == kNonFinite ? kNonFinite : k;
+}
+
+__device__ inline int ref_x(const Grid& g, int i, int width, int block) { return min(i * g.step, width - block); }
+__device__ inline int ref_y(const Grid& g, int j, int height, int block) { return min(j * g.step, height - block); }
+
 // ---------------------------------------------------------------------------
-// Block matching (src/cpu/bm/matcher.hpp:247-283 collect_spatial_coords;
-// order src/cpu/bm/matcher.hpp:27-57 match_less).
+// Block matching (src/cpu/bm/matcher.hpp:247-283 collect_spatial_coords:
+// window [c - r, c + r] clamped to the plane, raster scan, self excluded,
+// reference first; order src/cpu/bm/matcher.hpp:27-57 match_less).
 // ---------------------------------------------------------------------------
+
+// Run matcher: references i0 .. i0+7 of grid row j share one LDS region.
+template <int B>
+__global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
+    extern __shared__ __align__(16) float lds[];
+    const int lb = xcd_block();
+    const MatchJob& jb = tab.job[find_job(tab, lb)];
+    const int run = lb - jb.wg_begin;
+    const int rpr = (jb.grid.nx + kRunRefs - 1) / kRunRefs;
+    const int row = run / rpr, i0 = (run - row * rpr) * kRunRefs;
+    const int nrun = min(kRunRefs, jb.grid.nx - i0);
+    const int r = jb.range;
+    const int cy = ref_y(jb.grid, row, jb.height, B);
+    const int top = max(cy - r, 0), bottom = min(cy + r, jb.height - B);
+    const int rx0 = max(ref_x(jb.grid, i0, jb.width, B) - r, 0);
+    const int rx1 = min(ref_x(jb.grid, i0 + nrun - 1, jb.width, B) + r, jb.width - B) + B - 1;
+    // 3 more rows and columns than any window needs: a lane's 4 x 4 tile of
+    // candidates may overhang the window (those candidates are discarded).
+    const int RW = rx1 - rx0 + 1 + 3, RH = bottom - top + B + 3;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int yy = wave; yy < RH; yy += 4) {
+        const int gy = top + yy;
+        const float* src = jb.plane + static_cast<size_t>(min(gy, jb.height - 1)) * jb.stride;
+        for (int xx = lane; xx < RW; xx += 64) {
+            const int gx = rx0 + xx;
+            lds[yy * RW + xx] = (gy < jb.height && gx < jb.width) ? src[gx] : 0.f;
+        }
+    }
+    const int wmax = 2 * r + 1;
+    uint32_t* dist = reinterpret_cast<uint32_t*>(lds + ((RW * RH + 3) & ~3));
+    __syncthreads();
+
+    const int half = threadIdx.x >> 5, hl = threadIdx.x & 31;
+    const int cx = ref_x(jb.grid, i0 + half, jb.width, B);
+    const int left = max(cx - r, 0), right = min(cx + r, jb.width - B);
+    const int ww = right - left + 1, wh = bottom - top + 1;
+    const int ncand = ww * wh;
+    uint32_t* dk = dist + half * wmax * wmax;
+    if (half < nrun) {
+        const int ox = left - rx0;
+        const float* rp = lds + (cy - top) * RW + (cx - rx0);
+        const int tw = (ww + 3) >> 2, th = (wh + 3) >> 2;
+        const int self = (cy - top) * ww + (cx - left);
+        for (int t = hl; t < tw * th; t += 32) {
+            const int ty = t / tw, tx = t - ty * tw;
+            const int dy0 = ty * 4, dx0 = tx * 4;
+            float acc[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+            const float* base = lds + dy0 * RW + ox + dx0;
+            // Block row py of the reference against block row py of the 4 x 4
+            // candidates: region rows dy0 + py + i, columns dx0 .. dx0 + B + 2.
+            for (int py = 0; py < B; ++py) {
+                float rrow[B];
+#pragma unroll
+                for (int px = 0; px < B; ++px) rrow[px] = rp[py * RW + px];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float* rowp = base + (py + i) * RW;
+                    float v[B + 3];
+#pragma unroll
+                    for (int c = 0; c < B + 3; ++c) v[c] = rowp[c];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+#pragma unroll
+                        for (int px = 0; px < B; ++px) {
+                            const float d = rrow[px] - v[j + px];
+                            acc[i][j] = fmaf(d, d, acc[i][j]);
+                        }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int dy = dy0 + i, dx = dx0 + j;
+                    if (dy < wh && dx < ww) {
+                        const int c = dy * ww + dx;
+                        dk[c] = c == self ? kTaken : dist_key(acc[i][j]);
+                    }
+                }
+        }
+        wave_sync();
+        const int K = min(jb.group - 1, ncand - 1);
+        const int ref_id = row * jb.grid.nx + i0 + half;
+        MatchRec* out = jb.out + static_cast<size_t>(ref_id) * jb.group;
+        if (hl == 0) {
+            out[0] = MatchRec{cx, cy, jb.t0, 0u};
+            jb.counts[ref_id] = 1 + K;
+        }
+        for (int k = 0; k < K; ++k) {
+            unsigned long long best = ~0ull;
+            for (int c = hl; c < ncand; c += 32) {
+                const uint32_t d = dk[c];
+                const unsigned long long key = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
+                best = (d != kTaken && key < best) ? key : best;
+            }
+            best = group_min_u64<32>(best);
+            const int c = static_cast<int>(static_cast<uint32_t>(best));
+            if (hl == (c & 31)) {
+                dk[c] = kTaken;
+                const int dy = c / ww, dx = c - dy * ww;
+                out[1 + k] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(best >> 32)};
+            }
+            wave_sync();
+        }
+    }
+}
+
+// General matcher: one wave per reference.
 template <int B, bool STAGE>
-__global__ __launch_bounds__(64) void match_kernel(MatchTable tab) {
+__global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
-    const MatchJob& jb = tab.job[find_job(tab)];
+    const MatchJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
     const int ref = static_cast<int>(blockIdx.x) - jb.wg_begin;
     if (ref >= jb.grid.nx * jb.grid.ny) return;
-    const int cx = min((ref % jb.grid.nx) * jb.grid.step, jb.width - B);
-    const int cy = min((ref / jb.grid.nx) * jb.grid.step, jb.height - B);
+    const int cx = ref_x(jb.grid, ref % jb.grid.nx, jb.width, B);
+    const int cy = ref_y(jb.grid, ref / jb.grid.nx, jb.height, B);
     const int r = jb.range;
     const int top = max(cy - r, 0), bottom = min(cy + r, jb.height - B);
     const int left = max(cx - r, 0), right = min(cx + r, jb.width - B);
@@ -116,9 +267,7 @@ __global__ __launch_bounds__(64) void match_kernel(MatchTable tab) {
                 acc = fmaf(d, d, acc);
             }
         }
-        uint32_t key = __float_as_uint(acc);
-        if ((key & kNonFinite) == kNonFinite) key = kNonFinite;
-        dk[c] = c == self ? kTaken : key;
+        dk[c] = c == self ? kTaken : dist_key(acc);
     }
     __syncthreads();
 
@@ -132,7 +281,7 @@ __global__ __launch_bounds__(64) void match_kernel(MatchTable tab) {
             const unsigned long long key = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
             best = (d != kTaken && key < best) ? key : best;
         }
-        best = wave_min_u64(best);
+        best = group_min_u64<64>(best);
         const int c = static_cast<int>(static_cast<uint32_t>(best));
         if (lane == (c & 63)) {
             dk[c] = kTaken;
@@ -145,9 +294,7 @@ __global__ __launch_bounds__(64) void match_kernel(MatchTable tab) {
 }
 
 // ---------------------------------------------------------------------------
-// Group filter (src/cpu/bm/dct8.cpp:926-1133 Bm3dFilterGroup) and
-// aggregation (src/cpu/bm/pack.cpp:42-86 UnpackPatch; fixed point as in the
-// design of src/cuda/common/fixed_accumulate.cuh:18-35, den at the corner).
+// Group filter (src/cpu/bm/dct8.cpp:926-1133 Bm3dFilterGroup).
 // ---------------------------------------------------------------------------
 
 // Factor of position j of the in-place orthonormal Haar of n samples
@@ -186,14 +333,14 @@ __device__ inline void haar_inverse(float* base, int n, int stride) {
         }
 }
 
-// DCT of the lines of every patch of a cube (patch stride ps floats).
-// ROWS: lines along x (sample stride 1); else along y (sample stride B).
+// DCT of the lines of every patch of a cube (patch stride ps floats), by the
+// 64 lanes of one wave. ROWS: lines along x (sample stride 1); else along y.
 template <int B, bool ROWS>
-__device__ inline void dct_lines(float* cube, int group, int ps, bool inverse) {
+__device__ inline void dct_lines(float* cube, int group, int ps, bool inverse, int lane) {
     if constexpr (B > 1) {
         const float* m = c_dct + dct_offset<B>();
         const int lines = group * B;
-        for (int L = threadIdx.x; L < lines; L += 64) {
+        for (int L = lane; L < lines; L += 64) {
             const int p = L / B, q = L - p * B;
             float* base = cube + p * ps + (ROWS ? q * B : q);
             constexpr int ss = ROWS ? 1 : B;
@@ -223,24 +370,16 @@ __device__ inline void dct_lines(float* cube, int group, int ps, bool inverse) {
     }
 }
 
-__device__ inline unsigned long long to_fixed(float v) {
-    return static_cast<unsigned long long>(__float2ll_rn(v * 4294967296.0f));
-}
-
+// One group by one wave: gather the k matched patches (zero beyond k, up to
+// the configured group size), forward 3-D transform, shrink, inverse. The
+// filtered patches stay in `cube` (patch stride PS); returns the aggregation
+// weight, the same in every lane.
 template <int B, bool WIENER>
-__global__ __launch_bounds__(64) void filter_kernel(FilterTable tab) {
-    extern __shared__ __align__(16) float cube[];
+__device__ inline float filter_group_wave(const FilterJob& jb, const MatchRec* rec, int k, float* cube, float* rc,
+                                          int lane) {
     constexpr int A = B * B;
     constexpr int PS = A + 1;
-    const int lane = threadIdx.x;
-    const FilterJob& jb = tab.job[find_job(tab)];
-    const int g = static_cast<int>(blockIdx.x) - jb.wg_begin;
-    if (g >= jb.nref) return;
     const int G = jb.group;
-    const int k = jb.counts[g];
-    const MatchRec* rec = jb.matches + static_cast<size_t>(g) * G;
-    float* rc = cube + G * PS;
-
     for (int j = 0; j < G; ++j) {
         int mx = 0, my = 0;
         if (j < k) {
@@ -258,13 +397,13 @@ __global__ __launch_bounds__(64) void filter_kernel(FilterTable tab) {
             if constexpr (WIENER) rc[j * PS + e] = w;
         }
     }
-    __syncthreads();
-    dct_lines<B, true>(cube, G, PS, false);
-    if constexpr (WIENER) dct_lines<B, true>(rc, G, PS, false);
-    __syncthreads();
-    dct_lines<B, false>(cube, G, PS, false);
-    if constexpr (WIENER) dct_lines<B, false>(rc, G, PS, false);
-    __syncthreads();
+    wave_sync();
+    dct_lines<B, true>(cube, G, PS, false, lane);
+    if constexpr (WIENER) dct_lines<B, true>(rc, G, PS, false, lane);
+    wave_sync();
+    dct_lines<B, false>(cube, G, PS, false, lane);
+    if constexpr (WIENER) dct_lines<B, false>(rc, G, PS, false, lane);
+    wave_sync();
 
     int kept = 0;
     float w2 = 0.f;
@@ -295,18 +434,387 @@ __global__ __launch_bounds__(64) void filter_kernel(FilterTable tab) {
         }
         haar_inverse(cube + f, G, PS);
     }
-    __syncthreads();
-    dct_lines<B, false>(cube, G, PS, true);
-    __syncthreads();
-    dct_lines<B, true>(cube, G, PS, true);
+    wave_sync();
+    dct_lines<B, false>(cube, G, PS, true, lane);
+    wave_sync();
+    dct_lines<B, true>(cube, G, PS, true, lane);
+    wave_sync();
+    if constexpr (WIENER) {
+        return 1.f / fmaxf(wave_sum_f32(w2), 1e-12f);
+    } else {
+        return 1.f / static_cast<float>(max(wave_sum_i32(kept), 1));
+    }
+}
+
+__device__ inline unsigned long long to_fixed(float v) {
+    return static_cast<unsigned long long>(__float2ll_rn(v * 4294967296.0f));
+}
+
+__device__ inline void lds_add(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Tile path: sums of a tile of groups in LDS, then float partials.
+// LDS: num [ah][aw] int64 | den corners [ah-B+1][aw-B+1] int64 | per-wave
+// cubes (reused after filtering for the den box sum).
+template <int B, bool WIENER>
+__global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
+    extern __shared__ __align__(16) unsigned long long acc[];
+    constexpr int A = B * B;
+    constexpr int PS = A + 1;
+    const int lb = xcd_block();
+    const FilterJob& jb = tab.job[find_job(tab, lb)];
+    const TileGeom& tg = jb.tiles;
+    const int tile = lb - jb.wg_begin;
+    const int tx = tile % tg.ntx, ty = tile / tg.ntx;
+    const int aw = tg.aw, ah = tg.ah;
+    const int cw = aw - B + 1, ch = ah - B + 1;
+    const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
+    const int ay0 = tg.ay0(ty, jb.height, B, jb.grid.step, jb.range);
+    unsigned long long* num = acc;
+    unsigned long long* corner = acc + aw * ah;
+    float* scratch = reinterpret_cast<float*>(corner + cw * ch);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int e = threadIdx.x; e < aw * ah + cw * ch; e += 256) acc[e] = 0ull;
     __syncthreads();
 
-    float weight;
-    if constexpr (WIENER) {
-        weight = 1.f / fmaxf(wave_sum_f32(w2), 1e-12f);
-    } else {
-        weight = 1.f / static_cast<float>(max(wave_sum_i32(kept), 1));
+    const int G = jb.group;
+    const int cube_floats = G * PS * (WIENER ? 2 : 1);
+    float* cube = scratch + wave * cube_floats;
+    float* rc = cube + G * PS;
+    const int i_begin = tx * tg.tw, j_begin = ty * tg.th;
+    const int ni = min(tg.tw, jb.grid.nx - i_begin), nj = min(tg.th, jb.grid.ny - j_begin);
+    for (int g = wave; g < ni * nj; g += 4) {
+        const int gj = g / ni, gi = g - gj * ni;
+        const int rid = (j_begin + gj) * jb.grid.nx + i_begin + gi;
+        const int k = jb.counts[rid];
+        const MatchRec* rec = jb.matches + static_cast<size_t>(rid) * G;
+        const float w = filter_group_wave<B, WIENER>(jb, rec, k, cube, rc, lane);
+        const unsigned long long wfix = to_fixed(w);
+        for (int j = 0; j < k; ++j) {
+            const int lx = rec[j].x - ax0, ly = rec[j].y - ay0;
+            for (int e = lane; e < A; e += 64) {
+                const int py = e / B, px = e - (e / B) * B;
+                lds_add(num + (ly + py) * aw + lx + px, to_fixed(w * cube[j * PS + e]));
+            }
+            if (lane == 0) lds_add(corner + ly * cw + lx, wfix);
+        }
+        wave_sync();
     }
+    __syncthreads();
+
+    // den at each pixel: the B x B box sum of the corner weights ending there.
+    // Horizontal pass over the corner rows into scratch, then vertical.
+    for (int e = threadIdx.x; e < ch * aw; e += 256) {
+        const int y = e / aw, x = e - (e / aw) * aw;
+        unsigned long long s = 0;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cx = x - i;
+            if (cx >= 0 && cx < cw) s += corner[y * cw + cx];
+        }
+        scratch[e] = static_cast<float>(static_cast<double>(static_cast<long long>(s)) * (1.0 / 4294967296.0));
+    }
+    __syncthreads();
+    float* pn = jb.part_num + static_cast<size_t>(tile) * aw * ah;
+    float* pd = jb.part_den + static_cast<size_t>(tile) * aw * ah;
+    for (int e = threadIdx.x; e < aw * ah; e += 256) {
+        const int y = e / aw, x = e - (e / aw) * aw;
+        float d = 0.f;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cy = y - i;
+            if (cy >= 0 && cy < ch) d += scratch[cy * aw + x];
+        }
+        pn[e] = static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
+        pd[e] = d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block-8 group filter in registers (G in {8, 16, 32}). One wave per group;
+// three layouts of the G x 8 x 8 cube over the 64 lanes, with LDS transposes
+// between them (per-wave buffer T, address g*72 + row*9 + col):
+//   rows    lane (gb, y): rows y of patches g = gb + 8m, 8 samples over x
+//   columns lane (gb, kx): columns kx of the same patches, 8 samples over y
+//   group   lane (ky, kx): coefficient (ky, kx) of all G patches
+// Row DCT, column DCT and the group-axis Haar + shrink are then lane-local.
+// ---------------------------------------------------------------------------
+inline constexpr int kTs = 72;  // patch stride in the transpose buffer
+inline constexpr int kTr = 9;   // row stride
+
+template <bool INVERSE>
+__device__ inline void dct8(float v[8]) {
+    const float* m = c_dct + dct_offset<8>();
+    float o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc = fmaf(INVERSE ? m[i * 8 + k] : m[k * 8 + i], v[i], acc);
+        o[k] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = o[i];
+}
+
+// rows (or columns) layout -> the transposed layout within each patch.
+template <int M>
+__device__ inline void transpose_rc(float (&v)[M][8], float* T, int lane) {
+    const int gb = lane >> 3, q = lane & 7;
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) T[(gb + 8 * m) * kTs + q * kTr + i] = v[m][i];
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[m][i] = T[(gb + 8 * m) * kTs + i * kTr + q];
+    wave_sync();
+}
+
+// columns layout (lane (gb, kx) holds ky = 0..7) -> group layout.
+template <int G>
+__device__ inline void columns_to_group(float (&v)[G / 8][8], float (&c)[G], float* T, int lane) {
+    const int gb = lane >> 3, kx = lane & 7;
+#pragma unroll
+    for (int m = 0; m < G / 8; ++m)
+#pragma unroll
+        for (int ky = 0; ky < 8; ++ky) T[(gb + 8 * m) * kTs + ky * kTr + kx] = v[m][ky];
+    wave_sync();
+    const int ky = lane >> 3;
+#pragma unroll
+    for (int g = 0; g < G; ++g) c[g] = T[g * kTs + ky * kTr + kx];
+    wave_sync();
+}
+
+template <int G>
+__device__ inline void group_to_columns(float (&c)[G], float (&v)[G / 8][8], float* T, int lane) {
+    const int ky = lane >> 3, kx = lane & 7;
+#pragma unroll
+    for (int g = 0; g < G; ++g) T[g * kTs + ky * kTr + kx] = c[g];
+    wave_sync();
+    const int gb = lane >> 3;
+#pragma unroll
+    for (int m = 0; m < G / 8; ++m)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[m][k] = T[(gb + 8 * m) * kTs + k * kTr + kx];
+    wave_sync();
+}
+
+template <int G>
+__device__ inline void haar_forward_reg(float (&c)[G]) {
+#pragma unroll
+    for (int h = 1; h < G; h <<= 1)
+#pragma unroll
+        for (int i = 0; i < G; i += 2 * h) {
+            const float a = c[i], b = c[i + h];
+            c[i] = a + b;
+            c[i + h] = a - b;
+        }
+#pragma unroll
+    for (int j = 0; j < G; ++j) c[j] *= haar_scale(G, j);
+}
+
+template <int G>
+__device__ inline void haar_inverse_reg(float (&c)[G]) {
+#pragma unroll
+    for (int j = 0; j < G; ++j) c[j] *= haar_scale(G, j);
+#pragma unroll
+    for (int h = G >> 1; h >= 1; h >>= 1)
+#pragma unroll
+        for (int i = 0; i < G; i += 2 * h) {
+            const float a = c[i], b = c[i + h];
+            c[i] = a + b;
+            c[i + h] = a - b;
+        }
+}
+
+// Forward: rows (spatial) -> group-layout 3-D coefficients.
+template <int G>
+__device__ inline void forward_b8(float (&v)[G / 8][8], float (&c)[G], float* T, int lane) {
+#pragma unroll
+    for (int m = 0; m < G / 8; ++m) dct8<false>(v[m]);
+    transpose_rc<G / 8>(v, T, lane);
+#pragma unroll
+    for (int m = 0; m < G / 8; ++m) dct8<false>(v[m]);
+    columns_to_group<G>(v, c, T, lane);
+    haar_forward_reg<G>(c);
+}
+
+// Gathers the rows this lane owns (zero beyond k).
+template <int G>
+__device__ inline void gather_rows_b8(const float* plane, int stride, const MatchRec* rec, int k, int lane,
+                                      float (&v)[G / 8][8]) {
+    const int gb = lane >> 3, y = lane & 7;
+#pragma unroll
+    for (int m = 0; m < G / 8; ++m) {
+        const int g = gb + 8 * m;
+        if (g < k) {
+            const float* p = plane + static_cast<size_t>(rec[g].y + y) * stride + rec[g].x;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) v[m][x] = p[x];
+        } else {
+#pragma unroll
+            for (int x = 0; x < 8; ++x) v[m][x] = 0.f;
+        }
+    }
+}
+
+// Tile kernel for block 8: LDS = num [ah][aw] int64 | corner [ch][cw] int64 |
+// match records of the tile | per-wave transpose buffers (reused for the
+// den box sum).
+template <int G, bool WIENER>
+__global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
+    extern __shared__ __align__(16) unsigned long long acc[];
+    constexpr int B = 8;
+    constexpr int M = G / 8;
+    const int lb = xcd_block();
+    const FilterJob& jb = tab.job[find_job(tab, lb)];
+    const TileGeom& tg = jb.tiles;
+    const int tile = lb - jb.wg_begin;
+    const int tx = tile % tg.ntx, ty = tile / tg.ntx;
+    const int aw = tg.aw, ah = tg.ah;
+    const int cw = aw - B + 1, ch = ah - B + 1;
+    const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
+    const int ay0 = tg.ay0(ty, jb.height, B, jb.grid.step, jb.range);
+    const int i_begin = tx * tg.tw, j_begin = ty * tg.th;
+    const int ni = min(tg.tw, jb.grid.nx - i_begin), nj = min(tg.th, jb.grid.ny - j_begin);
+    const int ngroups = ni * nj;
+    unsigned long long* num = acc;
+    unsigned long long* corner = acc + aw * ah;
+    MatchRec* recs = reinterpret_cast<MatchRec*>(corner + cw * ch);
+    int* cnt = reinterpret_cast<int*>(recs + tg.tw * tg.th * G);
+    float* scratch = reinterpret_cast<float*>(cnt + ((tg.tw * tg.th + 3) & ~3));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* T = scratch + wave * (G * kTs);
+    for (int e = threadIdx.x; e < aw * ah + cw * ch; e += 256) acc[e] = 0ull;
+    // The tile's match records and counts, row by row of the grid.
+    for (int e = threadIdx.x; e < ngroups * G; e += 256) {
+        const int g = e / G, j = e - g * G;
+        const int gj = g / ni, gi = g - gj * ni;
+        const int rid = (j_begin + gj) * jb.grid.nx + i_begin + gi;
+        recs[e] = jb.matches[static_cast<size_t>(rid) * G + j];
+    }
+    for (int g = threadIdx.x; g < ngroups; g += 256) {
+        const int gj = g / ni, gi = g - gj * ni;
+        cnt[g] = jb.counts[(j_begin + gj) * jb.grid.nx + i_begin + gi];
+    }
+    __syncthreads();
+
+    const float thr = 2.7f * jb.sigma;  // include/nss/params.hpp:46 kBmHardLambda
+    const float sig2 = jb.sigma * jb.sigma;
+    float v[M][8], vr[M][8];
+    int g = wave;
+    if (g < ngroups) {
+        gather_rows_b8<G>(jb.src, jb.stride, recs + g * G, cnt[g], lane, v);
+        if constexpr (WIENER) gather_rows_b8<G>(jb.wref, jb.wref_stride, recs + g * G, cnt[g], lane, vr);
+    }
+    for (; g < ngroups; g += 4) {
+        const int k = cnt[g];
+        const MatchRec* rec = recs + g * G;
+        float c[G];
+        float r[G];
+        forward_b8<G>(v, c, T, lane);
+        if constexpr (WIENER) forward_b8<G>(vr, r, T, lane);
+        // Prefetch the next group's patches while this one is filtered.
+        const int gn = g + 4;
+        if (gn < ngroups) {
+            gather_rows_b8<G>(jb.src, jb.stride, recs + gn * G, cnt[gn], lane, v);
+            if constexpr (WIENER) gather_rows_b8<G>(jb.wref, jb.wref_stride, recs + gn * G, cnt[gn], lane, vr);
+        }
+        int kept = 0;
+        float w2 = 0.f;
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const bool dc = (lane == 0 && j == 0);
+            if constexpr (WIENER) {
+                float w = 1.f;
+                if (!dc) {
+                    const float r2 = r[j] * r[j];
+                    const float den = r2 + sig2;
+                    w = den > 0.f ? r2 / den : 0.f;
+                }
+                c[j] *= w;
+                w2 = fmaf(w, w, w2);
+            } else {
+                if (!dc && fabsf(c[j]) < thr)
+                    c[j] = 0.f;
+                else
+                    ++kept;
+            }
+        }
+        haar_inverse_reg<G>(c);
+        float u[M][8];
+        group_to_columns<G>(c, u, T, lane);
+#pragma unroll
+        for (int m = 0; m < M; ++m) dct8<true>(u[m]);
+        transpose_rc<M>(u, T, lane);
+#pragma unroll
+        for (int m = 0; m < M; ++m) dct8<true>(u[m]);
+        float w;
+        if constexpr (WIENER) {
+            w = 1.f / fmaxf(wave_sum_f32(w2), 1e-12f);
+        } else {
+            w = 1.f / static_cast<float>(max(wave_sum_i32(kept), 1));
+        }
+        const int gb = lane >> 3, y = lane & 7;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int p = gb + 8 * m;
+            if (p < k) {
+                const int lx = rec[p].x - ax0, ly = rec[p].y - ay0;
+                unsigned long long* row = num + (ly + y) * aw + lx;
+#pragma unroll
+                for (int x = 0; x < 8; ++x) lds_add(row + x, to_fixed(w * u[m][x]));
+                if (y == 0) lds_add(corner + ly * cw + lx, to_fixed(w));
+            }
+        }
+    }
+    __syncthreads();
+
+    for (int e = threadIdx.x; e < ch * aw; e += 256) {
+        const int yy = e / aw, xx = e - (e / aw) * aw;
+        unsigned long long s = 0;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cx = xx - i;
+            if (cx >= 0 && cx < cw) s += corner[yy * cw + cx];
+        }
+        scratch[e] = static_cast<float>(static_cast<double>(static_cast<long long>(s)) * (1.0 / 4294967296.0));
+    }
+    __syncthreads();
+    float* pn = jb.part_num + static_cast<size_t>(tile) * aw * ah;
+    float* pd = jb.part_den + static_cast<size_t>(tile) * aw * ah;
+    for (int e = threadIdx.x; e < aw * ah; e += 256) {
+        const int yy = e / aw, xx = e - (e / aw) * aw;
+        float d = 0.f;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cy = yy - i;
+            if (cy >= 0 && cy < ch) d += scratch[cy * aw + xx];
+        }
+        pn[e] = static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
+        pd[e] = d;
+    }
+}
+
+// Atomic path: one wave per group, frame-sized fixed-point sums.
+template <int B, bool WIENER>
+__global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
+    extern __shared__ __align__(16) float cube[];
+    constexpr int A = B * B;
+    constexpr int PS = A + 1;
+    const int lane = threadIdx.x;
+    const FilterJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
+    const int g = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    if (g >= jb.nref) return;
+    const int G = jb.group;
+    const int k = jb.counts[g];
+    const MatchRec* rec = jb.matches + static_cast<size_t>(g) * G;
+    float* rc = cube + G * PS;
+    const float weight = filter_group_wave<B, WIENER>(jb, rec, k, cube, rc, lane);
     const unsigned long long wfix = to_fixed(weight);
     for (int j = 0; j < k; ++j) {
         const int mx = rec[j].x, my = rec[j].y;
@@ -319,16 +827,46 @@ __global__ __launch_bounds__(64) void filter_kernel(FilterTable tab) {
 }
 
 // ---------------------------------------------------------------------------
-// Finish (src/cpu/bm/agg.cpp:17-42 AggFinish): dst = den > 0 ? num/den : src,
-// den being the B x B box sum of the corner weights.
+// Finish (src/cpu/bm/agg.cpp:17-42 AggFinish): dst = den > 1e-12 ? num/den : src.
 // ---------------------------------------------------------------------------
+
+// Tile path: per pixel, the partials of every tile whose rectangle covers it.
+__global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
+    const FinishJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
+    const int t = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    const int x = (t % jb.tiles_x) * 16 + (threadIdx.x & 15);
+    const int y = (t / jb.tiles_x) * 16 + (threadIdx.x >> 4);
+    if (x >= jb.width || y >= jb.height) return;
+    const TileGeom& tg = jb.tiles;
+    const int B = jb.block, s = jb.step, r = jb.range;
+    int txh = min(tg.ntx - 1, (x + r) / (tg.tw * s));
+    while (txh + 1 < tg.ntx && tg.ax0(txh + 1, jb.width, B, s, r) <= x) ++txh;
+    int tyh = min(tg.nty - 1, (y + r) / (tg.th * s));
+    while (tyh + 1 < tg.nty && tg.ay0(tyh + 1, jb.height, B, s, r) <= y) ++tyh;
+    float n = 0.f, d = 0.f;
+    const size_t area = static_cast<size_t>(tg.aw) * tg.ah;
+    for (int ty = tyh; ty >= 0; --ty) {
+        const int oy = y - tg.ay0(ty, jb.height, B, s, r);
+        if (oy >= tg.ah) break;
+        for (int tx = txh; tx >= 0; --tx) {
+            const int ox = x - tg.ax0(tx, jb.width, B, s, r);
+            if (ox >= tg.aw) break;
+            const size_t o = (static_cast<size_t>(ty) * tg.ntx + tx) * area + static_cast<size_t>(oy) * tg.aw + ox;
+            n += jb.part_num[o];
+            d += jb.part_den[o];
+        }
+    }
+    jb.dst[static_cast<size_t>(y) * jb.dstride + x] = d > 1e-12f ? n / d : jb.src[static_cast<size_t>(y) * jb.sstride + x];
+}
+
+// Atomic path: den kept at each patch's top-left corner, box-summed here.
 template <int B>
-__global__ __launch_bounds__(256) void finish_kernel(FinishTable tab) {
+__global__ __launch_bounds__(256) void finish_corner_kernel(FinishTable tab) {
     constexpr int T = 16;
     constexpr int TS = T + B - 1;
     __shared__ unsigned long long corner[TS * TS];
     __shared__ unsigned long long colsum[T * TS];
-    const FinishJob& jb = tab.job[find_job(tab)];
+    const FinishJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
     const int tile = static_cast<int>(blockIdx.x) - jb.wg_begin;
     const int x0 = (tile % jb.tiles_x) * T, y0 = (tile / jb.tiles_x) * T;
     const int tid = threadIdx.x;
@@ -363,7 +901,7 @@ __global__ __launch_bounds__(256) void finish_kernel(FinishTable tab) {
 }
 
 __global__ __launch_bounds__(256) void zero_kernel(ZeroTable tab) {
-    const ZeroJob& jb = tab.job[find_job(tab)];
+    const ZeroJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
     const long long base = static_cast<long long>(static_cast<int>(blockIdx.x) - jb.wg_begin) * 1024;
     for (int i = threadIdx.x; i < 1024; i += 256) {
         const long long o = base + i;
@@ -372,10 +910,10 @@ __global__ __launch_bounds__(256) void zero_kernel(ZeroTable tab) {
 }
 
 // ---------------------------------------------------------------------------
-// Host-side launchers.
+// Host-side sizing and launchers.
 // ---------------------------------------------------------------------------
 
-size_t match_lds_bytes(int block, int range, bool stage) {
+size_t match_wave_lds_bytes(int block, int range, bool stage) {
     const int w = 2 * range + 1;
     const size_t cand = static_cast<size_t>(w) * w * 4;
     if (!stage) return cand;
@@ -383,77 +921,144 @@ size_t match_lds_bytes(int block, int range, bool stage) {
     return ((reg + 3) & ~static_cast<size_t>(3)) * 4 + cand;
 }
 
-size_t filter_lds_bytes(int block, int group, bool wiener) {
+size_t match_run_lds_bytes(int block, int step, int range) {
+    const size_t w = 2 * static_cast<size_t>(range) + 1;
+    const size_t rw = static_cast<size_t>(kRunRefs - 1) * step + 2 * range + block + 3;
+    const size_t rh = 2 * static_cast<size_t>(range) + block + 3;
+    return (((rw * rh) + 3) & ~static_cast<size_t>(3)) * 4 + kRunRefs * w * w * 4;
+}
+
+size_t filter_atomic_lds_bytes(int block, int group, bool wiener) {
     return static_cast<size_t>(group) * (block * block + 1) * 4 * (wiener ? 2 : 1);
 }
 
+size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t) {
+    const size_t aw = t.aw, ah = t.ah;
+    const size_t acc = aw * ah * 8 + (aw - 7) * (ah - 7) * 8;
+    const size_t recs = static_cast<size_t>(t.tw) * t.th * group * sizeof(MatchRec) +
+                        ((static_cast<size_t>(t.tw) * t.th + 3) & ~static_cast<size_t>(3)) * 4;
+    const size_t tbuf = 4 * static_cast<size_t>(group) * kTs * 4;
+    const size_t boxsum = (ah - 7) * aw * 4;
+    return acc + recs + (tbuf > boxsum ? tbuf : boxsum);
+}
+
+size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t) {
+    const size_t aw = t.aw, ah = t.ah;
+    const size_t acc = aw * ah * 8 + (aw - block + 1) * (ah - block + 1) * 8;
+    const size_t cubes = 4 * filter_atomic_lds_bytes(block, group, wiener);
+    const size_t boxsum = (ah - block + 1) * aw * 4;
+    return acc + (cubes > boxsum ? cubes : boxsum);
+}
+
+template <typename K>
+static void allow_lds(K kernel, size_t lds) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds));
+}
+
 template <int B>
-static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, bool stage, hipStream_t s) {
-    if (stage) {
-        if (lds > 65536)
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&match_kernel<B, true>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        match_kernel<B, true><<<total_wg, 64, lds, s>>>(t);
+static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
+    if (kind == 2) {
+        if constexpr (B <= 12) {
+            allow_lds(&match_run_kernel<B>, lds);
+            match_run_kernel<B><<<total_wg, 256, lds, s>>>(t);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    } else if (kind == 1) {
+        allow_lds(&match_wave_kernel<B, true>, lds);
+        match_wave_kernel<B, true><<<total_wg, 64, lds, s>>>(t);
     } else {
-        if (lds > 65536)
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&match_kernel<B, false>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        match_kernel<B, false><<<total_wg, 64, lds, s>>>(t);
+        allow_lds(&match_wave_kernel<B, false>, lds);
+        match_wave_kernel<B, false><<<total_wg, 64, lds, s>>>(t);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, bool stage, hipStream_t s) {
+// kind: 2 run matcher, 1 wave matcher with the window in LDS, 0 from global.
+hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
     switch (block) {
-        case 1: return launch_match_b<1>(t, total_wg, lds, stage, s);
-        case 2: return launch_match_b<2>(t, total_wg, lds, stage, s);
-        case 4: return launch_match_b<4>(t, total_wg, lds, stage, s);
-        case 8: return launch_match_b<8>(t, total_wg, lds, stage, s);
-        case 12: return launch_match_b<12>(t, total_wg, lds, stage, s);
-        case 16: return launch_match_b<16>(t, total_wg, lds, stage, s);
-        case 32: return launch_match_b<32>(t, total_wg, lds, stage, s);
+        case 1: return launch_match_b<1>(t, total_wg, lds, kind, s);
+        case 2: return launch_match_b<2>(t, total_wg, lds, kind, s);
+        case 4: return launch_match_b<4>(t, total_wg, lds, kind, s);
+        case 8: return launch_match_b<8>(t, total_wg, lds, kind, s);
+        case 12: return launch_match_b<12>(t, total_wg, lds, kind, s);
+        case 16: return launch_match_b<16>(t, total_wg, lds, kind, s);
+        case 32: return launch_match_b<32>(t, total_wg, lds, kind, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <int B>
-static hipError_t launch_filter_b(const FilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
+static hipError_t launch_filter_b(const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
+                                  hipStream_t s) {
+    if (tiles) {
+        if (wiener) {
+            allow_lds(&filter_tile_kernel<B, true>, lds);
+            filter_tile_kernel<B, true><<<total_wg, 256, lds, s>>>(t);
+        } else {
+            allow_lds(&filter_tile_kernel<B, false>, lds);
+            filter_tile_kernel<B, false><<<total_wg, 256, lds, s>>>(t);
+        }
+    } else if (wiener) {
+        allow_lds(&filter_atomic_kernel<B, true>, lds);
+        filter_atomic_kernel<B, true><<<total_wg, 64, lds, s>>>(t);
+    } else {
+        allow_lds(&filter_atomic_kernel<B, false>, lds);
+        filter_atomic_kernel<B, false><<<total_wg, 64, lds, s>>>(t);
+    }
+    return hipGetLastError();
+}
+
+template <int G>
+static hipError_t launch_filter_b8(const FilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
     if (wiener) {
-        if (lds > 65536)
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&filter_kernel<B, true>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        filter_kernel<B, true><<<total_wg, 64, lds, s>>>(t);
+        allow_lds(&filter_tile_b8_kernel<G, true>, lds);
+        filter_tile_b8_kernel<G, true><<<total_wg, 256, lds, s>>>(t);
     } else {
-        if (lds > 65536)
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&filter_kernel<B, false>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        filter_kernel<B, false><<<total_wg, 64, lds, s>>>(t);
+        allow_lds(&filter_tile_b8_kernel<G, false>, lds);
+        filter_tile_b8_kernel<G, false><<<total_wg, 256, lds, s>>>(t);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
+// fast_group: 8, 16 or 32 selects the block-8 register kernel (tile path).
+hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
+                         int fast_group, hipStream_t s) {
+    if (tiles && block == 8 && fast_group) {
+        switch (fast_group) {
+            case 8: return launch_filter_b8<8>(t, total_wg, lds, wiener, s);
+            case 16: return launch_filter_b8<16>(t, total_wg, lds, wiener, s);
+            case 32: return launch_filter_b8<32>(t, total_wg, lds, wiener, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (block) {
-        case 1: return launch_filter_b<1>(t, total_wg, lds, wiener, s);
-        case 2: return launch_filter_b<2>(t, total_wg, lds, wiener, s);
-        case 4: return launch_filter_b<4>(t, total_wg, lds, wiener, s);
-        case 8: return launch_filter_b<8>(t, total_wg, lds, wiener, s);
-        case 12: return launch_filter_b<12>(t, total_wg, lds, wiener, s);
-        case 16: return launch_filter_b<16>(t, total_wg, lds, wiener, s);
-        case 32: return launch_filter_b<32>(t, total_wg, lds, wiener, s);
+        case 1: return launch_filter_b<1>(t, total_wg, lds, wiener, tiles, s);
+        case 2: return launch_filter_b<2>(t, total_wg, lds, wiener, tiles, s);
+        case 4: return launch_filter_b<4>(t, total_wg, lds, wiener, tiles, s);
+        case 8: return launch_filter_b<8>(t, total_wg, lds, wiener, tiles, s);
+        case 12: return launch_filter_b<12>(t, total_wg, lds, wiener, tiles, s);
+        case 16: return launch_filter_b<16>(t, total_wg, lds, wiener, tiles, s);
+        case 32: return launch_filter_b<32>(t, total_wg, lds, wiener, tiles, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_finish(int block, const FinishTable& t, int total_wg, hipStream_t s) {
+hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s) {
+    if (tiles) {
+        finish_tiles_kernel<<<total_wg, 256, 0, s>>>(t);
+        return hipGetLastError();
+    }
     switch (block) {
-        case 1: finish_kernel<1><<<total_wg, 256, 0, s>>>(t); break;
-        case 2: finish_kernel<2><<<total_wg, 256, 0, s>>>(t); break;
-        case 4: finish_kernel<4><<<total_wg, 256, 0, s>>>(t); break;
-        case 8: finish_kernel<8><<<total_wg, 256, 0, s>>>(t); break;
-        case 12: finish_kernel<12><<<total_wg, 256, 0, s>>>(t); break;
-        case 16: finish_kernel<16><<<total_wg, 256, 0, s>>>(t); break;
-        case 32: finish_kernel<32><<<total_wg, 256, 0, s>>>(t); break;
+        case 1: finish_corner_kernel<1><<<total_wg, 256, 0, s>>>(t); break;
+        case 2: finish_corner_kernel<2><<<total_wg, 256, 0, s>>>(t); break;
+        case 4: finish_corner_kernel<4><<<total_wg, 256, 0, s>>>(t); break;
+        case 8: finish_corner_kernel<8><<<total_wg, 256, 0, s>>>(t); break;
+        case 12: finish_corner_kernel<12><<<total_wg, 256, 0, s>>>(t); break;
+        case 16: finish_corner_kernel<16><<<total_wg, 256, 0, s>>>(t); break;
+        case 32: finish_corner_kernel<32><<<total_wg, 256, 0, s>>>(t); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

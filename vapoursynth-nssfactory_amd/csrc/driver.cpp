@@ -3,9 +3,11 @@
 //
 // One frame (src/host/filter_bm3d.cpp:359-494 bm3dGetFrame): for every stage
 // (basic then Wiener when final=1, src/frontend/bm3d_args.cpp:186-221), planes
-// with sigma 0 are copied; the others are grouped by block size and each
-// group runs four launches on the caller's stream: clear num/den, match,
-// filter+aggregate, finish. The basic estimate stays on the device.
+// with sigma 0 are copied; the others are grouped by block size, and each
+// group runs on the caller's stream: match, filter + aggregate, finish (plus
+// a clear of the frame-sized sums on the atomic path). The launch plan of
+// every (stage, block size) is fixed at creation. The basic estimate stays
+// on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,13 +24,17 @@
 
 namespace nsship {
 
-hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, bool stage, hipStream_t s);
-hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s);
-hipError_t launch_finish(int block, const FinishTable& t, int total_wg, hipStream_t s);
+hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s);
+hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
+                         int fast_group, hipStream_t s);
+hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s);
 hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s);
 hipError_t upload_dct_tables(const float* host, size_t count);
-size_t match_lds_bytes(int block, int range, bool stage);
-size_t filter_lds_bytes(int block, int group, bool wiener);
+size_t match_wave_lds_bytes(int block, int range, bool stage);
+size_t match_run_lds_bytes(int block, int step, int range);
+size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
+size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);
+size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t);
 
 thread_local std::string g_last_error;
 
@@ -54,7 +60,11 @@ struct DeviceGuard {
     }
 };
 
-constexpr int kLdsLimit = 160 * 1024;  // LDS per workgroup on gfx950
+constexpr size_t kLdsLimit = 160 * 1024;  // LDS per workgroup on gfx950
+// Tile aggregation aims at two workgroups per CU; a one-reference tile may
+// take the whole LDS.
+constexpr size_t kTileLdsBudget = 76 * 1024;
+constexpr size_t kRunLdsBudget = 64 * 1024;
 
 // Per device, once: the DCT matrices in constant memory.
 std::mutex g_init_mutex;
@@ -84,15 +94,44 @@ Grid make_grid(int width, int height, int block, int step) {
     return g;
 }
 
+TileGeom make_tiles(const Grid& g, int tw, int th, int block, int range) {
+    TileGeom t{};
+    t.tw = tw;
+    t.th = th;
+    t.ntx = (g.nx + tw - 1) / tw;
+    t.nty = (g.ny + th - 1) / th;
+    t.aw = (tw - 1) * g.step + 2 * range + block;
+    t.ah = (th - 1) * g.step + 2 * range + block;
+    return t;
+}
+
 struct StagePlane {
     float sigma;
     int block, group, step, range, ps_num, ps_range;
 };
 
+// Launch plan of one block-size bucket of a stage.
+struct Bucket {
+    int block = 0;
+    int planes[3]{};
+    int nplanes = 0;
+    int match_kind = 0;  // 2 run, 1 wave + LDS window, 0 wave + global window
+    size_t match_lds = 0;
+    bool tiles = false;  // tile aggregation, else frame-sized atomics
+    TileGeom geom[3]{};  // per plane of the bucket (tile path)
+    size_t filter_lds = 0;
+    int fast_group = 0;  // block 8 with one group size in {8, 16, 32}: the register kernel
+};
+
 struct Stage {
     StagePlane plane[3];
     bool wiener;
+    Bucket bucket[7];
+    int nbuckets;
 };
+
+bool allowed_block(int b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 12 || b == 16 || b == 32; }
+bool allowed_group(int g) { return g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64; }
 
 }  // namespace nsship
 
@@ -124,15 +163,17 @@ struct Tracer {
 
 struct nss_hip_bm3d {
     nss_hip_bm3d_params p{};
-    Tracer tracer;
     nss_hip_video_info vi{};
+    Tracer tracer;
     int device = 0;
     int nplanes = 0;
     int pw[3]{}, ph[3]{};
     Stage stages[2]{};
     int nstages = 0;
-    unsigned long long* num[3]{};
+    unsigned long long* num[3]{};  // atomic path sums
     unsigned long long* den[3]{};
+    float* part[3]{};              // tile path partials: num then den
+    size_t part_floats[3]{};       // per half (num or den)
     MatchRec* matches[3]{};
     int* counts[3]{};
     float* basic[3]{};
@@ -140,71 +181,143 @@ struct nss_hip_bm3d {
 
 namespace {
 
-bool allowed_block(int b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 12 || b == 16 || b == 32; }
-bool allowed_group(int g) { return g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64; }
-
-int check_stage_plane(const StagePlane& s, int w, int h, bool wiener) {
+int check_stage_plane(const StagePlane& s, int w, int h) {
     if (!allowed_block(s.block) || !allowed_group(s.group) || s.block > w || s.block > h || s.step < 1 ||
         s.step > s.block || s.range < 1 || s.range > 64)
         return fail(NSS_HIP_INVALID_INPUT, "nss_hip: invalid resolved BM3D parameters");
-    if (filter_lds_bytes(s.block, s.group, wiener) > static_cast<size_t>(kLdsLimit))
-        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: block_size " + std::to_string(s.block) + " with group_size " +
-                                             std::to_string(s.group) + " exceeds the on-chip group cube");
+    return NSS_HIP_OK;
+}
+
+// Chooses the kernels of every (stage, block size) bucket.
+int plan_stage(nss_hip_bm3d* h, Stage& st) {
+    st.nbuckets = 0;
+    for (int b : {1, 2, 4, 8, 12, 16, 32}) {
+        Bucket bk;
+        bk.block = b;
+        for (int i = 0; i < h->nplanes; ++i)
+            if (st.plane[i].sigma != 0.f && st.plane[i].block == b) bk.planes[bk.nplanes++] = i;
+        if (!bk.nplanes) continue;
+        size_t run_lds = 0, wave_lds = 0;
+        for (int k = 0; k < bk.nplanes; ++k) {
+            const StagePlane& sp = st.plane[bk.planes[k]];
+            run_lds = std::max(run_lds, match_run_lds_bytes(b, sp.step, sp.range));
+            wave_lds = std::max(wave_lds, match_wave_lds_bytes(b, sp.range, true));
+        }
+        if (b <= 12 && run_lds <= kRunLdsBudget) {
+            bk.match_kind = 2;
+            bk.match_lds = run_lds;
+        } else if (wave_lds <= kLdsLimit) {
+            bk.match_kind = 1;
+            bk.match_lds = wave_lds;
+        } else {
+            bk.match_kind = 0;
+            bk.match_lds = 0;
+            for (int k = 0; k < bk.nplanes; ++k)
+                bk.match_lds = std::max(bk.match_lds, match_wave_lds_bytes(b, st.plane[bk.planes[k]].range, false));
+        }
+        int fast = 0;
+        if (b == 8) {
+            fast = st.plane[bk.planes[0]].group;
+            for (int k = 1; k < bk.nplanes; ++k)
+                if (st.plane[bk.planes[k]].group != fast) fast = 0;
+            if (fast != 8 && fast != 16 && fast != 32) fast = 0;
+        }
+        static const int shapes[][2] = {{8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}};
+        for (const auto& sh : shapes) {
+            size_t lds = 0;
+            TileGeom g[3]{};
+            for (int k = 0; k < bk.nplanes; ++k) {
+                const int i = bk.planes[k];
+                const StagePlane& sp = st.plane[i];
+                g[k] = make_tiles(make_grid(h->pw[i], h->ph[i], b, sp.step), sh[0], sh[1], b, sp.range);
+                lds = std::max(lds, fast ? filter_tile_b8_lds_bytes(sp.group, g[k])
+                                         : filter_tile_lds_bytes(b, sp.group, st.wiener, g[k]));
+            }
+            const size_t budget = (sh[0] == 1 && sh[1] == 1) ? kLdsLimit : kTileLdsBudget;
+            if (lds <= budget) {
+                bk.tiles = true;
+                bk.fast_group = fast;
+                bk.filter_lds = lds;
+                for (int k = 0; k < bk.nplanes; ++k) bk.geom[k] = g[k];
+                break;
+            }
+        }
+        if (!bk.tiles) {
+            for (int k = 0; k < bk.nplanes; ++k)
+                bk.filter_lds =
+                    std::max(bk.filter_lds, filter_atomic_lds_bytes(b, st.plane[bk.planes[k]].group, st.wiener));
+            if (bk.filter_lds > kLdsLimit)
+                return fail(NSS_HIP_UNSUPPORTED, "nss_hip: block_size " + std::to_string(b) +
+                                                     " with this group_size exceeds the on-chip group cube");
+        }
+        st.bucket[st.nbuckets++] = bk;
+    }
     return NSS_HIP_OK;
 }
 
 void free_buffers(nss_hip_bm3d* h) {
     for (int i = 0; i < 3; ++i) {
-        if (h->num[i]) (void)hipFree(h->num[i]);
-        if (h->den[i]) (void)hipFree(h->den[i]);
-        if (h->matches[i]) (void)hipFree(h->matches[i]);
-        if (h->counts[i]) (void)hipFree(h->counts[i]);
-        if (h->basic[i]) (void)hipFree(h->basic[i]);
+        for (void* p : {static_cast<void*>(h->num[i]), static_cast<void*>(h->den[i]), static_cast<void*>(h->part[i]),
+                        static_cast<void*>(h->matches[i]), static_cast<void*>(h->counts[i]),
+                        static_cast<void*>(h->basic[i])})
+            if (p) (void)hipFree(p);
         h->num[i] = h->den[i] = nullptr;
+        h->part[i] = nullptr;
         h->matches[i] = nullptr;
         h->counts[i] = nullptr;
         h->basic[i] = nullptr;
     }
 }
 
-// One stage over the given planes: match on `match`, filter `src` (Wiener
-// reference `wref` when the stage is Wiener), finish into `dst`.
+// One stage: match on `match`, filter `src` (Wiener reference `wref` when the
+// stage is Wiener), finish into `dst`.
 int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const int* src_stride,
               const float* const* match, const int* match_stride, const float* const* wref, const int* wref_stride,
               float* const* dst, const int* dst_stride, hipStream_t s) {
-    bool active[3] = {false, false, false};
     for (int i = 0; i < h->nplanes; ++i) {
-        if (st.plane[i].sigma == 0.f) {
-            const hipError_t e =
-                hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, src[i],
-                                 static_cast<size_t>(src_stride[i]) * 4, static_cast<size_t>(h->pw[i]) * 4, h->ph[i],
-                                 hipMemcpyDeviceToDevice, s);
-            if (e != hipSuccess) return hip_fail(e, "plane copy");
-        } else {
-            active[i] = true;
-        }
+        if (st.plane[i].sigma != 0.f) continue;
+        const hipError_t e = hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, src[i],
+                                              static_cast<size_t>(src_stride[i]) * 4, static_cast<size_t>(h->pw[i]) * 4,
+                                              h->ph[i], hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "plane copy");
     }
-    for (int b : {1, 2, 4, 8, 12, 16, 32}) {
+    auto traced = [&](int kind, auto&& launch) -> hipError_t {
+        hipEvent_t a = nullptr, z = nullptr;
+        if (h->tracer.on) {
+            a = h->tracer.take();
+            z = h->tracer.take();
+            if (a) (void)hipEventRecord(a, s);
+        }
+        const hipError_t err = launch();
+        if (a && z) {
+            (void)hipEventRecord(z, s);
+            h->tracer.spans[kind].emplace_back(a, z);
+        }
+        return err;
+    };
+    for (int bi = 0; bi < st.nbuckets; ++bi) {
+        const Bucket& bk = st.bucket[bi];
+        const int b = bk.block;
         ZeroTable zt{};
         MatchTable mt{};
         FilterTable ft{};
         FinishTable fn{};
         int zwg = 0, mwg = 0, fwg = 0, nwg = 0;
-        size_t mlds = 0, flds = 0;
-        bool stage_lds = true;
-        for (int i = 0; i < h->nplanes; ++i) {
-            if (!active[i] || st.plane[i].block != b) continue;
+        for (int k = 0; k < bk.nplanes; ++k) {
+            const int i = bk.planes[k];
             const StagePlane& sp = st.plane[i];
             const int w = h->pw[i], hh = h->ph[i];
             const Grid g = make_grid(w, hh, b, sp.step);
             const int nref = g.nx * g.ny;
-            const long long npx = static_cast<long long>(w) * hh;
-            for (unsigned long long* buf : {h->num[i], h->den[i]}) {
-                ZeroJob& z = zt.job[zt.n++];
-                z.p = buf;
-                z.count = npx;
-                z.wg_begin = zwg;
-                zwg += static_cast<int>((npx + 1023) / 1024);
+            if (!bk.tiles) {
+                const long long npx = static_cast<long long>(w) * hh;
+                for (unsigned long long* buf : {h->num[i], h->den[i]}) {
+                    ZeroJob& z = zt.job[zt.n++];
+                    z.p = buf;
+                    z.count = npx;
+                    z.wg_begin = zwg;
+                    zwg += static_cast<int>((npx + 1023) / 1024);
+                }
             }
             MatchJob& m = mt.job[mt.n++];
             m.plane = match[i];
@@ -218,8 +331,8 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
             m.wg_begin = mwg;
             m.out = h->matches[i];
             m.counts = h->counts[i];
-            mwg += nref;
-            mlds = std::max(mlds, match_lds_bytes(b, sp.range, true));
+            mwg += bk.match_kind == 2 ? g.ny * ((g.nx + kRunRefs - 1) / kRunRefs) : nref;
+
             FilterJob& f = ft.job[ft.n++];
             f.src = src[i];
             f.wref = st.wiener ? wref[i] : nullptr;
@@ -227,6 +340,8 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
             f.height = hh;
             f.stride = src_stride[i];
             f.wref_stride = st.wiener ? wref_stride[i] : 0;
+            f.grid = g;
+            f.range = sp.range;
             f.nref = nref;
             f.group = sp.group;
             f.sigma = sp.sigma;
@@ -235,11 +350,20 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
             f.counts = h->counts[i];
             f.num = h->num[i];
             f.den = h->den[i];
-            fwg += nref;
-            flds = std::max(flds, filter_lds_bytes(b, sp.group, st.wiener));
+            f.tiles = bk.geom[k];
+            f.part_num = h->part[i];
+            f.part_den = h->part[i] ? h->part[i] + h->part_floats[i] : nullptr;
+            fwg += bk.tiles ? bk.geom[k].ntx * bk.geom[k].nty : nref;
+
             FinishJob& n = fn.job[fn.n++];
             n.num = h->num[i];
             n.den = h->den[i];
+            n.part_num = f.part_num;
+            n.part_den = f.part_den;
+            n.tiles = bk.geom[k];
+            n.block = b;
+            n.step = sp.step;
+            n.range = sp.range;
             n.src = src[i];
             n.dst = dst[i];
             n.width = w;
@@ -250,33 +374,16 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
             n.wg_begin = nwg;
             nwg += n.tiles_x * ((hh + 15) / 16);
         }
-        if (mt.n == 0) continue;
-        if (mlds > static_cast<size_t>(kLdsLimit)) {
-            stage_lds = false;
-            mlds = 0;
-            for (int j = 0; j < mt.n; ++j) mlds = std::max(mlds, match_lds_bytes(b, mt.job[j].range, false));
+        hipError_t e = hipSuccess;
+        if (!bk.tiles) {
+            e = traced(0, [&] { return launch_zero(zt, zwg, s); });
+            if (e != hipSuccess) return hip_fail(e, "clear launch");
         }
-        auto traced = [&](int kind, auto&& launch) -> hipError_t {
-            hipEvent_t a = nullptr, z = nullptr;
-            if (h->tracer.on) {
-                a = h->tracer.take();
-                z = h->tracer.take();
-                if (a) (void)hipEventRecord(a, s);
-            }
-            const hipError_t err = launch();
-            if (a && z) {
-                (void)hipEventRecord(z, s);
-                h->tracer.spans[kind].emplace_back(a, z);
-            }
-            return err;
-        };
-        hipError_t e = traced(0, [&] { return launch_zero(zt, zwg, s); });
-        if (e != hipSuccess) return hip_fail(e, "zero launch");
-        e = traced(1, [&] { return launch_match(b, mt, mwg, mlds, stage_lds, s); });
+        e = traced(1, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, s); });
         if (e != hipSuccess) return hip_fail(e, "match launch");
-        e = traced(2, [&] { return launch_filter(b, ft, fwg, flds, st.wiener, s); });
+        e = traced(2, [&] { return launch_filter(b, ft, fwg, bk.filter_lds, st.wiener, bk.tiles, bk.fast_group, s); });
         if (e != hipSuccess) return hip_fail(e, "filter launch");
-        e = traced(3, [&] { return launch_finish(b, fn, nwg, s); });
+        e = traced(3, [&] { return launch_finish(b, fn, nwg, bk.tiles, s); });
         if (e != hipSuccess) return hip_fail(e, "finish launch");
     }
     return NSS_HIP_OK;
@@ -342,32 +449,56 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
             f.plane[i] = StagePlane{p.sigma[i], p.block_size[i], p.group_size[i], p.block_step[i], p.bm_range[i],
                                     p.ps_num[i], p.ps_range[i]};
     }
-    for (int s = 0; s < h->nstages; ++s)
+    for (int s = 0; s < h->nstages; ++s) {
         for (int i = 0; i < h->nplanes; ++i) {
             if (h->stages[s].plane[i].sigma == 0.f) continue;
-            rc = check_stage_plane(h->stages[s].plane[i], h->pw[i], h->ph[i], h->stages[s].wiener);
+            rc = check_stage_plane(h->stages[s].plane[i], h->pw[i], h->ph[i]);
             if (rc != NSS_HIP_OK) {
                 delete h;
                 return rc;
             }
         }
+        rc = plan_stage(h, h->stages[s]);
+        if (rc != NSS_HIP_OK) {
+            delete h;
+            return rc;
+        }
+    }
     for (int i = 0; i < h->nplanes; ++i) {
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        size_t recs = 0, nref_max = 0;
+        size_t recs = 0, nref_max = 0, part = 0;
+        bool atomic = false;
         for (int s = 0; s < h->nstages; ++s) {
-            const StagePlane& sp = h->stages[s].plane[i];
+            const Stage& st = h->stages[s];
+            const StagePlane& sp = st.plane[i];
             if (sp.sigma == 0.f) continue;
             const Grid g = make_grid(h->pw[i], h->ph[i], sp.block, sp.step);
             const size_t nref = static_cast<size_t>(g.nx) * g.ny;
             recs = std::max(recs, nref * sp.group);
             nref_max = std::max(nref_max, nref);
+            for (int bi = 0; bi < st.nbuckets; ++bi)
+                for (int k = 0; k < st.bucket[bi].nplanes; ++k) {
+                    if (st.bucket[bi].planes[k] != i) continue;
+                    if (st.bucket[bi].tiles) {
+                        const TileGeom& tg = st.bucket[bi].geom[k];
+                        part = std::max(part, static_cast<size_t>(tg.ntx) * tg.nty * tg.aw * tg.ah);
+                    } else {
+                        atomic = true;
+                    }
+                }
         }
         hipError_t e = hipSuccess;
         if (recs) {
+            e = hipMalloc(&h->matches[i], recs * sizeof(MatchRec));
+            if (e == hipSuccess) e = hipMalloc(&h->counts[i], nref_max * sizeof(int));
+        }
+        if (e == hipSuccess && atomic) {
             e = hipMalloc(&h->num[i], npx * 8);
             if (e == hipSuccess) e = hipMalloc(&h->den[i], npx * 8);
-            if (e == hipSuccess) e = hipMalloc(&h->matches[i], recs * sizeof(MatchRec));
-            if (e == hipSuccess) e = hipMalloc(&h->counts[i], nref_max * sizeof(int));
+        }
+        if (e == hipSuccess && part) {
+            h->part_floats[i] = part;
+            e = hipMalloc(&h->part[i], 2 * part * sizeof(float));
         }
         if (e == hipSuccess && p.final_stage) e = hipMalloc(&h->basic[i], npx * 4);
         if (e != hipSuccess) {
@@ -408,8 +539,8 @@ int nss_hip_bm3d_process(nss_hip_bm3d* h, const float* const* src, const int* sr
         const float* cb[3] = {basic[0], basic[1], basic[2]};
         return run_stage(h, h->stages[1], src, src_stride, cb, bstride, cb, bstride, dst, dst_stride, s);
     }
-    if (h->stages[0].wiener) return run_stage(h, h->stages[0], src, src_stride, ref, ref_stride, ref, ref_stride, dst,
-                                              dst_stride, s);
+    if (h->stages[0].wiener)
+        return run_stage(h, h->stages[0], src, src_stride, ref, ref_stride, ref, ref_stride, dst, dst_stride, s);
     return run_stage(h, h->stages[0], src, src_stride, src, src_stride, nullptr, nullptr, dst, dst_stride, s);
 }
 
@@ -455,9 +586,6 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
     if (!allowed_block(block) || group < 1 || group > kMaxGroup || width < block || height < block || step < 1 ||
         stride < width || bm_range < 0 || bm_range > 64)
         return fail(NSS_HIP_INVALID_INPUT, "nss_hip: invalid match arguments");
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return hip_fail(e, "get device");
     MatchTable mt{};
     MatchJob& m = mt.job[mt.n++];
     m.plane = plane;
@@ -473,13 +601,23 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
     m.counts = out_count;
     const int nref = m.grid.nx * m.grid.ny;
     if (nref == 0) return NSS_HIP_OK;
-    size_t lds = match_lds_bytes(block, bm_range, true);
-    bool stage = true;
-    if (lds > static_cast<size_t>(kLdsLimit)) {
-        stage = false;
-        lds = match_lds_bytes(block, bm_range, false);
+    int kind;
+    size_t lds;
+    int wg;
+    if (block <= 12 && match_run_lds_bytes(block, step, bm_range) <= kRunLdsBudget) {
+        kind = 2;
+        lds = match_run_lds_bytes(block, step, bm_range);
+        wg = m.grid.ny * ((m.grid.nx + kRunRefs - 1) / kRunRefs);
+    } else if (match_wave_lds_bytes(block, bm_range, true) <= kLdsLimit) {
+        kind = 1;
+        lds = match_wave_lds_bytes(block, bm_range, true);
+        wg = nref;
+    } else {
+        kind = 0;
+        lds = match_wave_lds_bytes(block, bm_range, false);
+        wg = nref;
     }
-    e = launch_match(block, mt, nref, lds, stage, static_cast<hipStream_t>(stream));
+    const hipError_t e = launch_match(block, mt, wg, lds, kind, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "match launch");
     return NSS_HIP_OK;
 }
