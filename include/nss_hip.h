@@ -136,17 +136,28 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
 /* Number of temporal slots the process call reads per plane: 2*radius + 1. */
 int nss_hip_bm3d_slots(const nss_hip_bm3d* handle);
 
-/* Filter one frame: the work of bm3dGetFrame for the spatial case (radius 0),
- * both stages when final=1 (the basic estimate stays on the device).
- * src[p] / ref[p] / dst[p]: device pointers of plane p (ref may be NULL, or
- * hold NULLs, when no ref clip was given), strides in floats. For radius > 0
- * the temporal slots of every plane are given as src[slot * 3 + p] (slot t is
- * frame n + clamp(t - radius, -n, frames - 1 - n)) with the frame's index n
- * and the clip length; the output is the finished ("rolling") frame.
- * Asynchronous on stream. */
+/* Filter one frame of a spatial instance (radius 0): the work of
+ * bm3dGetFrame, both stages when final=1 (the basic estimate stays on the
+ * device). src[p] / ref[p] / dst[p]: device pointers of plane p (ref may be
+ * NULL when no ref clip was given), strides in floats. frame / frame_count are
+ * informational. Asynchronous on stream. */
 int nss_hip_bm3d_process(nss_hip_bm3d* handle, const float* const* src, const int* src_stride,
                          const float* const* ref, const int* ref_stride, float* const* dst, const int* dst_stride,
                          int frame, int frame_count, void* stream);
+
+/* Temporal instances (radius > 0, temporal_mode rolling): the rolling
+ * computation of rollingGetFrame (src/host/filter_bm3d.cpp:730-851,
+ * :985-1072) with all state on the device. Push the clip's frames in order
+ * (device pointers, copied into the instance's device ring; ref planes when a
+ * ref clip was given); each push runs the centres it makes ready. Pop returns
+ * finished frames in order: NSS_HIP_OK with *frame set, or NSS_HIP_BYPASS when
+ * none is ready. Pop after every push (a push fails with
+ * NSS_HIP_RESOURCE_FAILURE while too many finished frames wait). After the
+ * last frame (num_frames of the create call) every remaining frame becomes
+ * ready. Use one stream for an instance's calls (or order them). */
+int nss_hip_bm3d_push(nss_hip_bm3d* handle, const float* const* src, const int* src_stride, const float* const* ref,
+                      const int* ref_stride, void* stream);
+int nss_hip_bm3d_pop(nss_hip_bm3d* handle, float* const* dst, const int* dst_stride, int* frame, void* stream);
 
 /* Kernel tracing (the role of the reference's NVTX ranges group.match /
  * group.filter / group.aggregate, src/cuda/runtime/nvtx.hpp:15-32): while on,

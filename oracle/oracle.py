@@ -185,73 +185,85 @@ def _is_final(params):
     return bool(getattr(params, "final_stage", getattr(params, "final", False)))
 
 
+def _stage_frame(planes, params, basic, wref, match, fp64):
+    """One spatial stage over a frame's planes. With chroma (CBM3D) every
+    plane is matched on plane 0 of ``match`` (src/host/filter_bm3d.cpp:451-486)."""
+    chroma = bool(getattr(params, "chroma", False))
+    out = []
+    for i, p in enumerate(planes):
+        a = _stage_args(params, i, basic)
+        if a["sigma"] == 0:
+            out.append(np.array(p, np.float32, copy=True))
+            continue
+        out.append(bm3d_plane(p, wref=None if wref is None else wref[i], match=match[0 if chroma else i],
+                              fp64=fp64, **a))
+    return out
+
+
 def bm3d_frame(planes, params, ref=None, fp64=False):
     """Spatial (radius 0) nss.BM3D on one frame: list of 2-D planes in, list out.
 
     ``params`` carries resolved per-plane values (effective sigma) with the
-    attribute names of Bm3dParams (include/nss/params/bm3d.hpp:15-31)."""
+    attribute names of Bm3dParams (include/nss/params/bm3d.hpp:15-31); ``ref``
+    is the planes of the ref clip's frame when one is given."""
     if params.radius != 0:
         raise ValueError("bm3d_frame is the spatial path; use bm3d_clip for radius > 0")
     if _is_final(params):
-        basic = [np.array(p, np.float32, copy=True) if params.sigma_basic[i] == 0 else
-                 bm3d_plane(p, fp64=fp64, **_stage_args(params, i, True)) for i, p in enumerate(planes)]
-        ref = basic
-    out = []
-    for i, p in enumerate(planes):
-        if params.sigma[i] == 0:
-            out.append(np.array(p, np.float32, copy=True))
-            continue
-        a = _stage_args(params, i, False)
-        out.append(bm3d_plane(p, wref=None if ref is None else ref[i], fp64=fp64, **a))
-    return out
+        ref = _stage_frame(planes, params, True, None, planes, fp64)
+    return _stage_frame(planes, params, False, ref, ref if ref is not None else planes, fp64)
 
 
-def _temporal_stage(frames, plane, params, basic, wref_frames, fp64):
-    """Rolling temporal stage of one plane over a whole clip (list of 2-D)."""
-    n_frames = len(frames)
+def _temporal_stage(clip, params, basic, wref_clip, match_clip, fp64):
+    """Rolling temporal stage over a whole clip ([frame][plane] 2-D arrays)."""
+    n_frames = len(clip)
     r = params.radius
-    a = _stage_args(params, plane, basic)
-    if a["sigma"] == 0:
-        return [np.array(f, np.float32, copy=True) for f in frames]
-    h, w = frames[0].shape
-    acc_num = [np.zeros((h, w), np.float64 if fp64 else np.float32) for _ in range(n_frames)]
-    acc_den = [np.zeros((h, w), np.float64 if fp64 else np.float32) for _ in range(n_frames)]
-    for c in range(n_frames):
-        slot_frames = [c + min(max(t - r, -c), n_frames - 1 - c) for t in range(2 * r + 1)]
-        srcs = [frames[f] for f in slot_frames]
-        wr = [wref_frames[f] for f in slot_frames] if wref_frames is not None else None
-        _, num, den = bm3d_plane_temporal(srcs, r, a["sigma"], a["block"], a["group"], a["step"], a["bm_range"],
-                                          a["ps_num"], a["ps_range"], r, wrefs=wr, center=c, frame_count=n_frames,
-                                          fp64=fp64, want_slices=True)
-        for t in range(2 * r + 1):
-            target = c + t - r
-            if 0 <= target < n_frames:
-                acc_num[target] += num[t]
-                acc_den[target] += den[t]
-    out = []
-    for n in range(n_frames):
-        d = acc_den[n]
-        src = frames[n].astype(acc_num[n].dtype)
-        res = np.where(d > 1e-12, acc_num[n] / np.where(d > 1e-12, d, 1), src)
-        out.append(res.astype(np.float32))
-    return out
-
-
-def bm3d_clip(clip, params, fp64=False):
-    """nss.BM3D over a clip: list of frames, each a list of 2-D planes."""
-    if params.radius == 0:
-        return [bm3d_frame(f, params, fp64=fp64) for f in clip]
-    nplanes = len(clip[0])
-    out = [[None] * nplanes for _ in clip]
-    for p in range(nplanes):
+    chroma = bool(getattr(params, "chroma", False))
+    out = [[None] * len(clip[0]) for _ in clip]
+    for p in range(len(clip[0])):
+        a = _stage_args(params, p, basic)
         frames = [f[p] for f in clip]
-        wref = None
-        if _is_final(params):
-            wref = _temporal_stage(frames, p, params, True, None, fp64)
-        res = _temporal_stage(frames, p, params, False, wref, fp64)
-        for n, r in enumerate(res):
-            out[n][p] = r
+        if a["sigma"] == 0:
+            for n in range(n_frames):
+                out[n][p] = np.array(frames[n], np.float32, copy=True)
+            continue
+        mplane = 0 if chroma else p
+        h, w = frames[0].shape
+        dt = np.float64 if fp64 else np.float32
+        acc_num = [np.zeros((h, w), dt) for _ in range(n_frames)]
+        acc_den = [np.zeros((h, w), dt) for _ in range(n_frames)]
+        for c in range(n_frames):
+            slot_frames = [c + min(max(t - r, -c), n_frames - 1 - c) for t in range(2 * r + 1)]
+            srcs = [frames[f] for f in slot_frames]
+            wr = [wref_clip[f][p] for f in slot_frames] if wref_clip is not None else None
+            mt = [match_clip[f][mplane] for f in slot_frames]
+            _, num, den = bm3d_plane_temporal(srcs, r, a["sigma"], a["block"], a["group"], a["step"], a["bm_range"],
+                                              a["ps_num"], a["ps_range"], r, wrefs=wr, matches=mt, center=c,
+                                              frame_count=n_frames, fp64=fp64, want_slices=True)
+            for t in range(2 * r + 1):
+                target = c + t - r
+                if 0 <= target < n_frames:
+                    acc_num[target] += num[t]
+                    acc_den[target] += den[t]
+        for n in range(n_frames):
+            d = acc_den[n]
+            src = frames[n].astype(dt)
+            res = np.where(d > 1e-12, acc_num[n] / np.where(d > 1e-12, d, 1), src)
+            out[n][p] = res.astype(np.float32)
     return out
+
+
+def bm3d_clip(clip, params, ref_clip=None, fp64=False):
+    """nss.BM3D over a clip ([frame][plane] 2-D arrays); radius > 0 gives the
+    rolling (finished) frames."""
+    if params.radius == 0:
+        return [bm3d_frame(f, params, ref=None if ref_clip is None else ref_clip[n], fp64=fp64)
+                for n, f in enumerate(clip)]
+    if _is_final(params):
+        basic = _temporal_stage(clip, params, True, None, clip, fp64)
+        return _temporal_stage(clip, params, False, basic, basic, fp64)
+    if ref_clip is not None:
+        return _temporal_stage(clip, params, False, ref_clip, ref_clip, fp64)
+    return _temporal_stage(clip, params, False, None, clip, fp64)
 
 
 @dataclass

@@ -1,0 +1,156 @@
+"""Temporal (radius > 0, rolling) and CBM3D (chroma=1) on the HIP path
+against the oracle, through the C ABI (push/pop for temporal).
+
+Gates as in tests/test_gpu_bm3d.py: >= 60 dB per plane against the oracle
+(tests/data/cuda_tolerances_v1.json), bitwise run-to-run identity, and the
+reference's own invariants (one-frame clip at maximum radius = spatial;
+CBM3D plane 0 = the separate filter's plane 0).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MIN_PSNR = 60.0
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _host(planes):
+    import torch
+    torch.cuda.synchronize()
+    return [p.cpu().numpy() for p in planes]
+
+
+def _gpu_clip(gpu, fmt, clip, ref_clip=None, **kw):
+    h, w = clip[0][0].shape
+    inst = gpu.BM3D(fmt, w, h, len(clip), ref=ref_clip is not None, **kw)
+    out = inst.filter_clip([[_dev(p) for p in f] for f in clip],
+                           None if ref_clip is None else [[_dev(p) for p in f] for f in ref_clip])
+    res = [_host(f) for f in out]
+    inst.close()
+    assert len(res) == len(clip)
+    return res
+
+
+def _oracle_clip(oracle, gpu, fmt, clip, ref_clip=None, **kw):
+    h, w = clip[0][0].shape
+    p = gpu.parse(fmt, w, h, len(clip), ref=ref_clip is not None, **kw)
+    return oracle.bm3d_clip(clip, p, ref_clip=ref_clip)
+
+
+def _check(oracle, got, want, label):
+    for n, (gf, wf) in enumerate(zip(got, want)):
+        for i, (g, w_) in enumerate(zip(gf, wf)):
+            p = oracle.psnr(g, w_)
+            assert p >= MIN_PSNR, (label, n, i, p)
+
+
+def _moving_clip(fmt, w, h, frames, sigma, seed=0):
+    import synth
+    _, noisy = synth.shifted_sequence(fmt, w, h, frames, sigma, seed=seed, max_step=2)
+    return noisy
+
+
+@pytest.mark.parametrize("radius", [1, 2])
+@pytest.mark.parametrize("final", [0, 1])
+def test_temporal_rolling(gpu, oracle, radius, final):
+    clip = _moving_clip("YUV420PS", 64, 48, 7, 10, seed=radius)
+    kw = dict(sigma=10, radius=radius, final=final)
+    got = _gpu_clip(gpu, "YUV420PS", clip, **kw)
+    want = _oracle_clip(oracle, gpu, "YUV420PS", clip, **kw)
+    print(f"temporal r{radius} final={final}: min PSNR vs oracle "
+          f"{min(oracle.psnr(g, w_) for gf, wf in zip(got, want) for g, w_ in zip(gf, wf)):.2f} dB", flush=True)
+    _check(oracle, got, want, f"r{radius} final{final}")
+
+
+def test_temporal_shapes_and_ref(gpu, oracle):
+    """Other block / group / predictive settings, and a ref clip (Wiener)."""
+    import synth
+    clip = _moving_clip("GRAYS", 56, 44, 5, 20, seed=3)
+    clean = [[synth.clean_image(56, 44, seed=3)] for _ in range(5)]
+    for kw in (dict(block_size=4, block_step=2, group_size=16, ps_num=3, ps_range=2),
+               dict(block_size=8, block_step=4, group_size=32, bm_range=6, ps_range=5),
+               dict(block_size=16, block_step=8, group_size=4, bm_range=4)):
+        kw = dict(kw, sigma=20, radius=1)
+        _check(oracle, _gpu_clip(gpu, "GRAYS", clip, **kw), _oracle_clip(oracle, gpu, "GRAYS", clip, **kw), str(kw))
+        got = _gpu_clip(gpu, "GRAYS", clip, ref_clip=clean, **kw)
+        want = _oracle_clip(oracle, gpu, "GRAYS", clip, ref_clip=clean, **kw)
+        _check(oracle, got, want, "ref " + str(kw))
+    print("temporal shapes ok", flush=True)
+
+
+def test_one_frame_at_max_radius_equals_spatial(gpu):
+    """tests/test_bm3d_semantics.py:125-135 on the GPU."""
+    rng = np.random.default_rng(73)
+    noisy = list(rng.uniform(.1, .8, (3, 12, 12)).astype(np.float32))
+    kw = dict(sigma=3, block_size=4, group_size=8, block_step=2, bm_range=2)
+    inst = gpu.BM3D("RGBS", 12, 12, 1, **kw)
+    spatial = _host(inst([_dev(p) for p in noisy]))
+    inst.close()
+    temporal = _gpu_clip(gpu, "RGBS", [noisy], radius=16, **kw)[0]
+    for a, b in zip(spatial, temporal):
+        np.testing.assert_allclose(a, b, atol=1e-5)
+
+
+def test_temporal_run_to_run_identity(gpu):
+    clip = _moving_clip("YUV420PS", 64, 48, 5, 10, seed=8)
+    a = _gpu_clip(gpu, "YUV420PS", clip, sigma=10, radius=1, final=1)
+    b = _gpu_clip(gpu, "YUV420PS", clip, sigma=10, radius=1, final=1)
+    for fa, fb in zip(a, b):
+        for x, y in zip(fa, fb):
+            assert np.array_equal(x, y)
+
+
+def test_push_pop_protocol(gpu):
+    """Frames come out in order; pushing past the clip, or leaving too many
+    finished frames unpopped, is an error (never a silent change)."""
+    import torch
+    clip = _moving_clip("GRAYS", 32, 32, 4, 10, seed=2)
+    inst = gpu.BM3D("GRAYS", 32, 32, 4, sigma=10, radius=1)
+    order = []
+    for f in clip:
+        inst.push([_dev(p) for p in f])
+        while (r := inst.pop()) is not None:
+            order.append(r[0])
+    assert order == [0, 1, 2, 3]
+    with pytest.raises(gpu.Error, match="already pushed"):
+        inst.push([_dev(p) for p in clip[0]])
+    inst.close()
+    inst = gpu.BM3D("GRAYS", 32, 32, 40, sigma=10, radius=1)
+    with pytest.raises(gpu.Error, match="pop the finished frames"):
+        for _ in range(40):
+            inst.push([_dev(p) for p in clip[0]])
+    inst.close()
+    torch.cuda.synchronize()
+
+
+def test_chroma_spatial(gpu, oracle):
+    import synth
+    _, noisy = synth.noisy_frame("YUV444PS", 72, 56, 10, seed=6)
+    for kw in (dict(sigma=10, chroma=1), dict(sigma=[10, 5, 0], chroma=1, final=1),
+               dict(sigma=[0, 10, 10], chroma=1, block_step=4, group_size=16)):
+        inst = gpu.BM3D("YUV444PS", 72, 56, **kw)
+        got = _host(inst([_dev(p) for p in noisy]))
+        inst.close()
+        want = oracle.bm3d_frame(noisy, gpu.parse("YUV444PS", 72, 56, **kw))
+        for i, (g, w_) in enumerate(zip(got, want)):
+            assert oracle.psnr(g, w_) >= MIN_PSNR, (kw, i)
+    # Plane 0 of CBM3D is the plain filter's plane 0 (tests/test_bm3d_chroma.py).
+    a = gpu.BM3D("YUV444PS", 72, 56, sigma=10, chroma=1)
+    b = gpu.BM3D("YUV444PS", 72, 56, sigma=10)
+    pa = _host(a([_dev(p) for p in noisy]))
+    pb = _host(b([_dev(p) for p in noisy]))
+    assert np.array_equal(pa[0], pb[0])
+    a.close()
+    b.close()
+
+
+def test_chroma_temporal(gpu, oracle):
+    clip = _moving_clip("YUV444PS", 48, 40, 5, 10, seed=4)
+    kw = dict(sigma=10, chroma=1, radius=1, final=1)
+    _check(oracle, _gpu_clip(gpu, "YUV444PS", clip, **kw), _oracle_clip(oracle, gpu, "YUV444PS", clip, **kw),
+           "chroma temporal")

@@ -194,3 +194,54 @@ def test_fp32_fp64_agree(oracle):
     b = oracle.bm3d_plane(noisy[0], s, block=8, group=16, step=3, bm_range=8, fp64=True)
     assert oracle.psnr(a, b) > 80
     assert oracle.psnr(a, clean[0]) > oracle.psnr(noisy[0], clean[0]) + 1.5
+
+
+def test_one_frame_temporal_equals_spatial(oracle):
+    """tests/test_bm3d_semantics.py:125-135: a one-frame clip at maximum
+    radius has no other frame to search, so the rolling result is the
+    spatial one (hard and Wiener)."""
+    import nss_hip
+    rng = np.random.default_rng(73)
+    noisy = rng.uniform(.1, .8, (3, 12, 12)).astype(np.float32)
+    for wiener in (False, True):
+        kw = dict(sigma=3, block_size=4, group_size=8, block_step=2, bm_range=2)
+        ref = [[p for p in noisy]] if wiener else None
+        ps = nss_hip.parse("RGBS", 12, 12, 1, ref=wiener, **kw)
+        spatial = oracle.bm3d_clip([list(noisy)], ps, ref_clip=ref)
+        pt = nss_hip.parse("RGBS", 12, 12, 1, ref=wiener, radius=16, **kw)
+        temporal = oracle.bm3d_clip([list(noisy)], pt, ref_clip=ref)
+        for a, b in zip(spatial[0], temporal[0]):
+            np.testing.assert_allclose(a, b, atol=1e-5)
+
+
+def test_chroma_plane0_is_the_separate_filter(oracle):
+    """tests/test_bm3d_chroma.py: with chroma=1 the groups are matched on
+    plane 0, so plane 0 equals the plain filter's plane 0; planes 1, 2 are
+    filtered with plane 0's groups."""
+    import nss_hip
+    import synth
+    _, noisy = synth.noisy_frame("YUV444PS", 48, 40, 10, seed=5)
+    pc = nss_hip.parse("YUV444PS", 48, 40, sigma=10, chroma=1)
+    pp = nss_hip.parse("YUV444PS", 48, 40, sigma=10)
+    c = oracle.bm3d_frame(noisy, pc)
+    p = oracle.bm3d_frame(noisy, pp)
+    assert np.array_equal(c[0], p[0])
+    assert not np.array_equal(c[1], p[1])
+    want1 = oracle.bm3d_plane(noisy[1], pc.sigma[1], block=8, group=8, step=8, bm_range=9, match=noisy[0])
+    assert np.array_equal(c[1], want1)
+
+
+def test_static_clip_temporal_denoises_more(oracle):
+    """A static clip gives the temporal search exact copies of every patch
+    in the neighbouring frames' noise: radius 1 beats the spatial filter."""
+    import nss_hip
+    import synth
+    clean = synth.clean_image(48, 40, seed=9)
+    clip = [[synth.add_noise(clean, 10, seed=100 + f)] for f in range(4)]
+    p0 = nss_hip.parse("GRAYS", 48, 40, 4, sigma=10)
+    p1 = nss_hip.parse("GRAYS", 48, 40, 4, sigma=10, radius=1)
+    s = oracle.bm3d_clip(clip, p0)
+    t = oracle.bm3d_clip(clip, p1)
+    ps = np.mean([oracle.psnr(f[0], clean) for f in s])
+    pt = np.mean([oracle.psnr(f[0], clean) for f in t])
+    assert pt > ps + 0.5, (ps, pt)

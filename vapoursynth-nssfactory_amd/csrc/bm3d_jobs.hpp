@@ -112,6 +112,74 @@ struct FinishTable {
     FinishJob job[kMaxJobs];
 };
 
+// Frames of a clip held on the device: frame f at frames + (f % ring) * frame_stride.
+struct FrameRing {
+    const float* frames;
+    long long frame_stride;
+    int ring;
+    __host__ __device__ const float* frame(int f) const {
+        return frames + static_cast<long long>(f % ring) * frame_stride;
+    }
+};
+
+// Temporal slots of centre frame c: slot t is frame
+// c + clamp(t - radius, -c, nframes - 1 - c) (src/frontend/temporal.hpp:19-21);
+// slots outside [max(0, radius - c), radius + min(radius + 1, nframes - c))
+// repeat an edge frame and take no part in matching
+// (src/host/filter_bm3d.cpp:96-97).
+struct Slots {
+    int center, nframes, radius;
+    __host__ __device__ int frame(int t) const {
+        int d = t - radius;
+        if (d < -center) d = -center;
+        if (d > nframes - 1 - center) d = nframes - 1 - center;
+        return center + d;
+    }
+    __host__ __device__ int valid_begin() const { return radius - center > 0 ? radius - center : 0; }
+    __host__ __device__ int valid_end() const {
+        return radius + (radius + 1 < nframes - center ? radius + 1 : nframes - center);
+    }
+};
+
+// Predictive temporal search of one plane, after the spatial search on the
+// centre frame (src/cpu/bm/matcher.hpp:319-373 collect_temporal).
+struct PredJob {
+    FrameRing match;  // frames searched
+    int width, height, stride;
+    int nref;
+    int group, ps_num, ps_range;
+    Slots slots;
+    int wg_begin;
+    MatchRec* recs;   // in: spatial matches (t = radius); out: the temporal group
+    int* counts;
+};
+
+struct PredTable {
+    int n;
+    PredJob job[kMaxJobs];
+};
+
+// Temporal group filter: patches from their slot's frame, sums into the
+// accumulator of the frame they came from (a ring of frame-sized int64 sums).
+struct TFilterJob {
+    FrameRing src, wref;
+    int width, height, stride, wref_stride;
+    int nref, group;
+    float sigma;
+    Slots slots;
+    int wg_begin;
+    const MatchRec* matches;
+    const int* counts;
+    unsigned long long* num;  // frame f's sums at num + (f % acc_ring) * width * height
+    unsigned long long* den;
+    int acc_ring;
+};
+
+struct TFilterTable {
+    int n;
+    TFilterJob job[kMaxJobs];
+};
+
 struct ZeroJob {
     unsigned long long* p;
     long long count;

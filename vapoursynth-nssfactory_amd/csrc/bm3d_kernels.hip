@@ -370,34 +370,58 @@ __device__ inline void dct_lines(float* cube, int group, int ps, bool inverse, i
     }
 }
 
-// One group by one wave: gather the k matched patches (zero beyond k, up to
-// the configured group size), forward 3-D transform, shrink, inverse. The
-// filtered patches stay in `cube` (patch stride PS); returns the aggregation
-// weight, the same in every lane.
-template <int B, bool WIENER>
-__device__ inline float filter_group_wave(const FilterJob& jb, const MatchRec* rec, int k, float* cube, float* rc,
-                                          int lane) {
+// Gathers the k matched patches of a group into an LDS cube (patch stride
+// B*B + 1), zero beyond k up to the configured group size G. src_of(t) and
+// wref_of(t) give the plane of temporal slot t (the record's t).
+template <int B, bool WIENER, typename SrcOf, typename WrefOf>
+__device__ inline void gather_group_wave(const MatchRec* rec, int k, int G, int stride, int wref_stride, SrcOf src_of,
+                                         WrefOf wref_of, float* cube, float* rc, int lane) {
     constexpr int A = B * B;
     constexpr int PS = A + 1;
-    const int G = jb.group;
     for (int j = 0; j < G; ++j) {
-        int mx = 0, my = 0;
+        int mx = 0, my = 0, mt = 0;
         if (j < k) {
             mx = rec[j].x;
             my = rec[j].y;
+            mt = rec[j].t;
         }
+        const float* sp = j < k ? src_of(mt) : nullptr;
+        const float* wp = (WIENER && j < k) ? wref_of(mt) : nullptr;
         for (int e = lane; e < A; e += 64) {
             const int py = e / B, px = e - (e / B) * B;
             float v = 0.f, w = 0.f;
             if (j < k) {
-                v = jb.src[static_cast<size_t>(my + py) * jb.stride + mx + px];
-                if constexpr (WIENER) w = jb.wref[static_cast<size_t>(my + py) * jb.wref_stride + mx + px];
+                v = sp[static_cast<size_t>(my + py) * stride + mx + px];
+                if constexpr (WIENER) w = wp[static_cast<size_t>(my + py) * wref_stride + mx + px];
             }
             cube[j * PS + e] = v;
             if constexpr (WIENER) rc[j * PS + e] = w;
         }
     }
     wave_sync();
+}
+
+// One group by one wave: gather (spatial: every patch from the job's own
+// planes), then the transform. Returns the aggregation weight.
+template <int B, bool WIENER>
+__device__ inline float transform_group_wave(int G, float sigma, float* cube, float* rc, int lane);
+
+template <int B, bool WIENER>
+__device__ inline float filter_group_wave(const FilterJob& jb, const MatchRec* rec, int k, float* cube, float* rc,
+                                          int lane) {
+    gather_group_wave<B, WIENER>(
+        rec, k, jb.group, jb.stride, jb.wref_stride, [&](int) { return jb.src; }, [&](int) { return jb.wref; }, cube,
+        rc, lane);
+    return transform_group_wave<B, WIENER>(jb.group, jb.sigma, cube, rc, lane);
+}
+
+// Forward 3-D transform of the cube (and of the Wiener reference cube),
+// shrink, inverse; the filtered patches stay in `cube`. Returns the
+// aggregation weight, the same in every lane.
+template <int B, bool WIENER>
+__device__ inline float transform_group_wave(int G, float sigma, float* cube, float* rc, int lane) {
+    constexpr int A = B * B;
+    constexpr int PS = A + 1;
     dct_lines<B, true>(cube, G, PS, false, lane);
     if constexpr (WIENER) dct_lines<B, true>(rc, G, PS, false, lane);
     wave_sync();
@@ -407,8 +431,8 @@ __device__ inline float filter_group_wave(const FilterJob& jb, const MatchRec* r
 
     int kept = 0;
     float w2 = 0.f;
-    const float thr = 2.7f * jb.sigma;  // include/nss/params.hpp:46 kBmHardLambda
-    const float sig2 = jb.sigma * jb.sigma;
+    const float thr = 2.7f * sigma;  // include/nss/params.hpp:46 kBmHardLambda
+    const float sig2 = sigma * sigma;
     for (int f = lane; f < A; f += 64) {
         haar_forward(cube + f, G, PS);
         if constexpr (WIENER) haar_forward(rc + f, G, PS);
@@ -827,6 +851,191 @@ __global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
 }
 
 // ---------------------------------------------------------------------------
+// Temporal: predictive search (src/cpu/bm/matcher.hpp:319-373) and the
+// group filter with per-slot sources and accumulators.
+// ---------------------------------------------------------------------------
+inline constexpr int kPredChunk = 2048;  // candidates of a layer evaluated per LDS pass
+
+__device__ inline unsigned long long pack_pos(int t, int y, int x) {
+    return (static_cast<unsigned long long>(t) << 26) | (static_cast<unsigned long long>(y) << 13) |
+           static_cast<unsigned long long>(x);
+}
+
+// One wave per reference. The spatial matches (slot radius) seed the search:
+// per direction, layer dt scans the union of (2 ps_range + 1)^2 windows
+// around the previous layer's winners (exact membership over the union's
+// bounding box, raster order), keeps its min(ps_num, G) smallest (dist, y, x),
+// and those feed the global G-1 smallest (dist, t, y, x) with the spatial
+// matches. Layer 1's centres are the first ps_num spatial matches.
+// LDS: reference patch | layer chunk keys | best, next best | centres | merged.
+template <int B>
+__global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const PredJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
+    const int ref = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    if (ref >= jb.nref) return;
+    const int G = jb.group;
+    const Slots& sl = jb.slots;
+    const int t0 = sl.radius;
+    MatchRec* rec = jb.recs + static_cast<size_t>(ref) * G;
+    const int n = jb.counts[ref];
+    const int cx = rec[0].x, cy = rec[0].y;
+    float* refp = reinterpret_cast<float*>(smem);
+    uint32_t* cand = reinterpret_cast<uint32_t*>(refp + ((B * B + 3) & ~3));
+    unsigned long long* best = reinterpret_cast<unsigned long long*>(cand + kPredChunk);
+    unsigned long long* nbest = best + kMaxGroup;
+    int* centers = reinterpret_cast<int*>(nbest + kMaxGroup);
+    unsigned long long* merged = reinterpret_cast<unsigned long long*>(centers + kMaxGroup);
+    {
+        const float* p0 = jb.match.frame(sl.frame(t0));
+        for (int e = lane; e < B * B; e += 64) refp[e] = p0[static_cast<size_t>(cy + e / B) * jb.stride + cx + e % B];
+    }
+    for (int i = lane; i < n - 1; i += 64)
+        merged[i] = (static_cast<unsigned long long>(rec[1 + i].dist) << 32) | pack_pos(t0, rec[1 + i].y, rec[1 + i].x);
+    int nm = n - 1;
+    const int vb = sl.valid_begin(), ve = sl.valid_end();
+    const int kl = min(jb.ps_num, G);
+    const int seeds = min(n, jb.ps_num);
+    const int ps = jb.ps_range;
+    const int maxx = jb.width - B, maxy = jb.height - B;
+    __syncthreads();
+    for (int sign = -1; sign <= 1; sign += 2) {
+        if (lane < seeds) centers[lane] = (rec[lane].y << 13) | rec[lane].x;
+        int nc = seeds;
+        __syncthreads();
+        for (int dt = 1; dt <= sl.radius; ++dt) {
+            const int t = t0 + sign * dt;
+            if (t < vb || t >= ve) break;
+            const float* pt = jb.match.frame(sl.frame(t));
+            int ymin = jb.height, ymax = -1, xmin = jb.width, xmax = -1;
+            for (int i = 0; i < nc; ++i) {
+                const int ccy = centers[i] >> 13, ccx = centers[i] & 8191;
+                ymin = min(ymin, max(0, ccy - ps));
+                ymax = max(ymax, min(maxy, ccy + ps));
+                xmin = min(xmin, max(0, ccx - ps));
+                xmax = max(xmax, min(maxx, ccx + ps));
+            }
+            const int bw = xmax - xmin + 1;
+            const int cells = bw * (ymax - ymin + 1);
+            int nb = 0;
+            for (int base = 0; base < cells; base += kPredChunk) {
+                const int cnt = min(kPredChunk, cells - base);
+                for (int c = lane; c < cnt; c += 64) {
+                    const int cell = base + c;
+                    const int y = ymin + cell / bw, x = xmin + cell % bw;
+                    bool member = false;
+                    for (int i = 0; i < nc; ++i) {
+                        const int ccy = centers[i] >> 13, ccx = centers[i] & 8191;
+                        member |= abs(y - ccy) <= ps && x >= max(0, ccx - ps) && x <= min(maxx, ccx + ps);
+                    }
+                    uint32_t key = kTaken;
+                    if (member) {
+                        float acc = 0.f;
+                        for (int py = 0; py < B; ++py) {
+                            const float* row = pt + static_cast<size_t>(y + py) * jb.stride + x;
+#pragma unroll
+                            for (int px = 0; px < B; ++px) {
+                                const float d = refp[py * B + px] - row[px];
+                                acc = fmaf(d, d, acc);
+                            }
+                        }
+                        key = dist_key(acc);
+                    }
+                    cand[c] = key;
+                }
+                __syncthreads();
+                // The chunk's candidates and the running best: the kl smallest.
+                int nn = 0;
+                for (int k = 0; k < kl; ++k) {
+                    unsigned long long m = ~0ull;
+                    for (int c = lane; c < cnt; c += 64) {
+                        if (cand[c] == kTaken) continue;
+                        const int cell = base + c;
+                        const unsigned long long key = (static_cast<unsigned long long>(cand[c]) << 32) |
+                                                       pack_pos(0, ymin + cell / bw, xmin + cell % bw);
+                        m = key < m ? key : m;
+                    }
+                    for (int i = lane; i < nb; i += 64) m = best[i] < m ? best[i] : m;
+                    m = group_min_u64<64>(m);
+                    if (m == ~0ull) break;
+                    const int my = static_cast<int>((m >> 13) & 8191), mx = static_cast<int>(m & 8191);
+                    const int mc = (my - ymin) * bw + (mx - xmin) - base;
+                    if (mc >= 0 && mc < cnt && (mc & 63) == lane && cand[mc] != kTaken &&
+                        cand[mc] == static_cast<uint32_t>(m >> 32))
+                        cand[mc] = kTaken;
+                    for (int i = lane; i < nb; i += 64)
+                        if (best[i] == m) best[i] = ~0ull;
+                    if (lane == 0) nbest[nn] = m;
+                    ++nn;
+                    __syncthreads();
+                }
+                for (int i = lane; i < nn; i += 64) best[i] = nbest[i];
+                nb = nn;
+                __syncthreads();
+            }
+            // Layer winners (ascending): into the merged list, and the next centres.
+            for (int i = lane; i < nb; i += 64) {
+                const unsigned long long b = best[i];
+                merged[nm + i] = (b & 0xffffffff00000000ull) | pack_pos(t, static_cast<int>((b >> 13) & 8191),
+                                                                       static_cast<int>(b & 8191));
+                centers[i] = static_cast<int>(b & 0x3ffffff);
+            }
+            nm += nb;
+            nc = nb;
+            __syncthreads();
+            if (nc == 0) break;
+        }
+    }
+    const int K = min(G - 1, nm);
+    for (int k = 0; k < K; ++k) {
+        unsigned long long m = ~0ull;
+        for (int i = lane; i < nm; i += 64) m = merged[i] < m ? merged[i] : m;
+        m = group_min_u64<64>(m);
+        for (int i = lane; i < nm; i += 64)
+            if (merged[i] == m) merged[i] = ~0ull;
+        if (lane == 0)
+            rec[1 + k] = MatchRec{static_cast<int>(m & 8191), static_cast<int>((m >> 13) & 8191),
+                                  static_cast<int>((m >> 26) & 63), static_cast<uint32_t>(m >> 32)};
+        __syncthreads();
+    }
+    if (lane == 0) jb.counts[ref] = 1 + K;
+}
+
+// One wave per group: patches from their slots' frames, sums into the
+// accumulators of those frames.
+template <int B, bool WIENER>
+__global__ __launch_bounds__(64) void filter_temporal_kernel(TFilterTable tab) {
+    extern __shared__ __align__(16) float cube[];
+    constexpr int A = B * B;
+    constexpr int PS = A + 1;
+    const int lane = threadIdx.x;
+    const TFilterJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
+    const int g = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    if (g >= jb.nref) return;
+    const int G = jb.group;
+    const int k = jb.counts[g];
+    const MatchRec* rec = jb.matches + static_cast<size_t>(g) * G;
+    float* rc = cube + G * PS;
+    gather_group_wave<B, WIENER>(
+        rec, k, G, jb.stride, jb.wref_stride, [&](int t) { return jb.src.frame(jb.slots.frame(t)); },
+        [&](int t) { return jb.wref.frame(jb.slots.frame(t)); }, cube, rc, lane);
+    const float weight = transform_group_wave<B, WIENER>(G, jb.sigma, cube, rc, lane);
+    const unsigned long long wfix = to_fixed(weight);
+    const size_t plane = static_cast<size_t>(jb.width) * jb.height;
+    for (int j = 0; j < k; ++j) {
+        const int mx = rec[j].x, my = rec[j].y;
+        const size_t slot = static_cast<size_t>(jb.slots.frame(rec[j].t) % jb.acc_ring) * plane;
+        unsigned long long* num = jb.num + slot;
+        for (int e = lane; e < A; e += 64) {
+            const int py = e / B, px = e - (e / B) * B;
+            atomicAdd(num + static_cast<size_t>(my + py) * jb.width + mx + px, to_fixed(weight * cube[j * PS + e]));
+        }
+        if (lane == 0) atomicAdd(jb.den + slot + static_cast<size_t>(my) * jb.width + mx, wfix);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Finish (src/cpu/bm/agg.cpp:17-42 AggFinish): dst = den > 1e-12 ? num/den : src.
 // ---------------------------------------------------------------------------
 
@@ -1067,6 +1276,53 @@ hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool til
 hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s) {
     zero_kernel<<<total_wg, 256, 0, s>>>(t);
     return hipGetLastError();
+}
+
+size_t predictive_lds_bytes(int block, int group, int ps_num, int radius) {
+    const size_t kl = static_cast<size_t>(ps_num < group ? ps_num : group);
+    const size_t merged = static_cast<size_t>(group - 1) + 2 * static_cast<size_t>(radius) * kl;
+    return ((static_cast<size_t>(block) * block + 3) & ~static_cast<size_t>(3)) * 4 + kPredChunk * 4 +
+           2 * kMaxGroup * 8 + kMaxGroup * 4 + merged * 8;
+}
+
+hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s) {
+    switch (block) {
+        case 1: allow_lds(&predictive_kernel<1>, lds); predictive_kernel<1><<<total_wg, 64, lds, s>>>(t); break;
+        case 2: allow_lds(&predictive_kernel<2>, lds); predictive_kernel<2><<<total_wg, 64, lds, s>>>(t); break;
+        case 4: allow_lds(&predictive_kernel<4>, lds); predictive_kernel<4><<<total_wg, 64, lds, s>>>(t); break;
+        case 8: allow_lds(&predictive_kernel<8>, lds); predictive_kernel<8><<<total_wg, 64, lds, s>>>(t); break;
+        case 12: allow_lds(&predictive_kernel<12>, lds); predictive_kernel<12><<<total_wg, 64, lds, s>>>(t); break;
+        case 16: allow_lds(&predictive_kernel<16>, lds); predictive_kernel<16><<<total_wg, 64, lds, s>>>(t); break;
+        case 32: allow_lds(&predictive_kernel<32>, lds); predictive_kernel<32><<<total_wg, 64, lds, s>>>(t); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int B>
+static hipError_t launch_tfilter_b(const TFilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
+    if (wiener) {
+        allow_lds(&filter_temporal_kernel<B, true>, lds);
+        filter_temporal_kernel<B, true><<<total_wg, 64, lds, s>>>(t);
+    } else {
+        allow_lds(&filter_temporal_kernel<B, false>, lds);
+        filter_temporal_kernel<B, false><<<total_wg, 64, lds, s>>>(t);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
+                                  hipStream_t s) {
+    switch (block) {
+        case 1: return launch_tfilter_b<1>(t, total_wg, lds, wiener, s);
+        case 2: return launch_tfilter_b<2>(t, total_wg, lds, wiener, s);
+        case 4: return launch_tfilter_b<4>(t, total_wg, lds, wiener, s);
+        case 8: return launch_tfilter_b<8>(t, total_wg, lds, wiener, s);
+        case 12: return launch_tfilter_b<12>(t, total_wg, lds, wiener, s);
+        case 16: return launch_tfilter_b<16>(t, total_wg, lds, wiener, s);
+        case 32: return launch_tfilter_b<32>(t, total_wg, lds, wiener, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t upload_dct_tables(const float* host, size_t count) {

@@ -35,6 +35,10 @@ size_t match_run_lds_bytes(int block, int step, int range);
 size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);
 size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t);
+size_t predictive_lds_bytes(int block, int group, int ps_num, int radius);
+hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s);
+hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
+                                  hipStream_t s);
 
 thread_local std::string g_last_error;
 
@@ -177,6 +181,22 @@ struct nss_hip_bm3d {
     MatchRec* matches[3]{};
     int* counts[3]{};
     float* basic[3]{};
+    // Rolling temporal state (radius > 0): frames enter in clip order and
+    // stay on the device in rings (frame f at slot f % ring); each stage keeps
+    // frame-sized int64 sums for the 2r+1 frames its current centre touches.
+    struct Rolling {
+        int N = 0, r = 0, pushed = 0;
+        int src_n = 0, ref_n = 0, basic_n = 0, out_n = 0, acc_n = 0;
+        float* src[3]{};
+        float* ref[3]{};
+        float* basic[3]{};
+        float* out[3]{};
+        unsigned long long* num[2][3]{};
+        unsigned long long* den[2][3]{};
+        int next_center[2]{};
+        int finished[2]{};
+        int out_head = 0;
+    } roll;
 };
 
 namespace {
@@ -256,17 +276,40 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
 }
 
 void free_buffers(nss_hip_bm3d* h) {
+    auto& R = h->roll;
     for (int i = 0; i < 3; ++i) {
         for (void* p : {static_cast<void*>(h->num[i]), static_cast<void*>(h->den[i]), static_cast<void*>(h->part[i]),
                         static_cast<void*>(h->matches[i]), static_cast<void*>(h->counts[i]),
-                        static_cast<void*>(h->basic[i])})
+                        static_cast<void*>(h->basic[i]), static_cast<void*>(R.src[i]), static_cast<void*>(R.ref[i]),
+                        static_cast<void*>(R.basic[i]), static_cast<void*>(R.out[i]),
+                        static_cast<void*>(R.num[0][i]), static_cast<void*>(R.den[0][i]),
+                        static_cast<void*>(R.num[1][i]), static_cast<void*>(R.den[1][i])})
             if (p) (void)hipFree(p);
         h->num[i] = h->den[i] = nullptr;
         h->part[i] = nullptr;
         h->matches[i] = nullptr;
         h->counts[i] = nullptr;
         h->basic[i] = nullptr;
+        R.src[i] = R.ref[i] = R.basic[i] = R.out[i] = nullptr;
+        R.num[0][i] = R.den[0][i] = R.num[1][i] = R.den[1][i] = nullptr;
     }
+}
+
+// Event-traced launch on stream s (kind: 0 clear, 1 match, 2 filter, 3 finish).
+template <typename F>
+hipError_t traced(nss_hip_bm3d* h, int kind, hipStream_t s, F&& launch) {
+    hipEvent_t a = nullptr, z = nullptr;
+    if (h->tracer.on) {
+        a = h->tracer.take();
+        z = h->tracer.take();
+        if (a) (void)hipEventRecord(a, s);
+    }
+    const hipError_t err = launch();
+    if (a && z) {
+        (void)hipEventRecord(z, s);
+        h->tracer.spans[kind].emplace_back(a, z);
+    }
+    return err;
 }
 
 // One stage: match on `match`, filter `src` (Wiener reference `wref` when the
@@ -281,20 +324,6 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
                                               h->ph[i], hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return hip_fail(e, "plane copy");
     }
-    auto traced = [&](int kind, auto&& launch) -> hipError_t {
-        hipEvent_t a = nullptr, z = nullptr;
-        if (h->tracer.on) {
-            a = h->tracer.take();
-            z = h->tracer.take();
-            if (a) (void)hipEventRecord(a, s);
-        }
-        const hipError_t err = launch();
-        if (a && z) {
-            (void)hipEventRecord(z, s);
-            h->tracer.spans[kind].emplace_back(a, z);
-        }
-        return err;
-    };
     for (int bi = 0; bi < st.nbuckets; ++bi) {
         const Bucket& bk = st.bucket[bi];
         const int b = bk.block;
@@ -303,8 +332,35 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
         FilterTable ft{};
         FinishTable fn{};
         int zwg = 0, mwg = 0, fwg = 0, nwg = 0;
+        // CBM3D (chroma=1): the groups are matched once, on plane 0 of the
+        // match planes, and every filtered plane uses them
+        // (src/host/filter_bm3d.cpp:451-486).
+        int mplanes[3], nmp = 0;
+        if (h->p.chroma)
+            mplanes[nmp++] = 0;
+        else
+            for (int k = 0; k < bk.nplanes; ++k) mplanes[nmp++] = bk.planes[k];
+        for (int k = 0; k < nmp; ++k) {
+            const int i = mplanes[k];
+            const StagePlane& sp = st.plane[i];
+            const Grid g = make_grid(h->pw[i], h->ph[i], b, sp.step);
+            MatchJob& m = mt.job[mt.n++];
+            m.plane = match[i];
+            m.width = h->pw[i];
+            m.height = h->ph[i];
+            m.stride = match_stride[i];
+            m.grid = g;
+            m.range = sp.range;
+            m.group = sp.group;
+            m.t0 = 0;
+            m.wg_begin = mwg;
+            m.out = h->matches[i];
+            m.counts = h->counts[i];
+            mwg += bk.match_kind == 2 ? g.ny * ((g.nx + kRunRefs - 1) / kRunRefs) : g.nx * g.ny;
+        }
         for (int k = 0; k < bk.nplanes; ++k) {
             const int i = bk.planes[k];
+            const int mi = h->p.chroma ? 0 : i;
             const StagePlane& sp = st.plane[i];
             const int w = h->pw[i], hh = h->ph[i];
             const Grid g = make_grid(w, hh, b, sp.step);
@@ -319,20 +375,6 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
                     zwg += static_cast<int>((npx + 1023) / 1024);
                 }
             }
-            MatchJob& m = mt.job[mt.n++];
-            m.plane = match[i];
-            m.width = w;
-            m.height = hh;
-            m.stride = match_stride[i];
-            m.grid = g;
-            m.range = sp.range;
-            m.group = sp.group;
-            m.t0 = 0;
-            m.wg_begin = mwg;
-            m.out = h->matches[i];
-            m.counts = h->counts[i];
-            mwg += bk.match_kind == 2 ? g.ny * ((g.nx + kRunRefs - 1) / kRunRefs) : nref;
-
             FilterJob& f = ft.job[ft.n++];
             f.src = src[i];
             f.wref = st.wiener ? wref[i] : nullptr;
@@ -346,8 +388,8 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
             f.group = sp.group;
             f.sigma = sp.sigma;
             f.wg_begin = fwg;
-            f.matches = h->matches[i];
-            f.counts = h->counts[i];
+            f.matches = h->matches[mi];
+            f.counts = h->counts[mi];
             f.num = h->num[i];
             f.den = h->den[i];
             f.tiles = bk.geom[k];
@@ -376,15 +418,218 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
         }
         hipError_t e = hipSuccess;
         if (!bk.tiles) {
-            e = traced(0, [&] { return launch_zero(zt, zwg, s); });
+            e = traced(h, 0, s, [&] { return launch_zero(zt, zwg, s); });
             if (e != hipSuccess) return hip_fail(e, "clear launch");
         }
-        e = traced(1, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, s); });
+        e = traced(h, 1, s, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, s); });
         if (e != hipSuccess) return hip_fail(e, "match launch");
-        e = traced(2, [&] { return launch_filter(b, ft, fwg, bk.filter_lds, st.wiener, bk.tiles, bk.fast_group, s); });
+        e = traced(h, 2, s,
+                   [&] { return launch_filter(b, ft, fwg, bk.filter_lds, st.wiener, bk.tiles, bk.fast_group, s); });
         if (e != hipSuccess) return hip_fail(e, "filter launch");
-        e = traced(3, [&] { return launch_finish(b, fn, nwg, bk.tiles, s); });
+        e = traced(h, 3, s, [&] { return launch_finish(b, fn, nwg, bk.tiles, s); });
         if (e != hipSuccess) return hip_fail(e, "finish launch");
+    }
+    return NSS_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Rolling temporal processing (src/host/filter_bm3d.cpp:730-851): the centre
+// frames of a stage are processed in ascending order; centre c adds into the
+// sums of frames c-r .. c+r (clamped to the clip), and frame n is finished
+// once every centre that reaches it (up to min(n + r, N - 1)) is done.
+// ---------------------------------------------------------------------------
+
+FrameRing ring_of(float* base, int ring, int w, int hgt) {
+    return FrameRing{base, static_cast<long long>(w) * hgt, ring};
+}
+
+// Match frames of stage s: the user's ref clip when given, the basic
+// estimate in the Wiener stage of final=1, else the source.
+FrameRing match_ring(nss_hip_bm3d* h, int s, int i) {
+    auto& R = h->roll;
+    if (h->p.final_stage && s == 1) return ring_of(R.basic[i], R.basic_n, h->pw[i], h->ph[i]);
+    if (h->p.has_ref) return ring_of(R.ref[i], R.ref_n, h->pw[i], h->ph[i]);
+    return ring_of(R.src[i], R.src_n, h->pw[i], h->ph[i]);
+}
+
+FrameRing wref_ring(nss_hip_bm3d* h, int s, int i) {
+    auto& R = h->roll;
+    if (h->p.final_stage && s == 1) return ring_of(R.basic[i], R.basic_n, h->pw[i], h->ph[i]);
+    if (h->p.has_ref) return ring_of(R.ref[i], R.ref_n, h->pw[i], h->ph[i]);
+    return FrameRing{nullptr, 0, 1};
+}
+
+int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
+    auto& R = h->roll;
+    const Stage& stg = h->stages[s];
+    const Slots slots{c, R.N, R.r};
+    for (int bi = 0; bi < stg.nbuckets; ++bi) {
+        const Bucket& bk = stg.bucket[bi];
+        const int b = bk.block;
+        MatchTable mt{};
+        PredTable pt{};
+        TFilterTable ft{};
+        int mwg = 0, pwg = 0, fwg = 0;
+        size_t plds = 0, flds = 0;
+        int mplanes[3], nmp = 0;
+        if (h->p.chroma)
+            mplanes[nmp++] = 0;
+        else
+            for (int k = 0; k < bk.nplanes; ++k) mplanes[nmp++] = bk.planes[k];
+        bool predictive = false;
+        for (int k = 0; k < nmp; ++k) {
+            const int i = mplanes[k];
+            const StagePlane& sp = stg.plane[i];
+            const Grid g = make_grid(h->pw[i], h->ph[i], b, sp.step);
+            const FrameRing mr = match_ring(h, s, i);
+            MatchJob& m = mt.job[mt.n++];
+            m.plane = mr.frame(c);
+            m.width = h->pw[i];
+            m.height = h->ph[i];
+            m.stride = h->pw[i];
+            m.grid = g;
+            m.range = sp.range;
+            m.group = sp.group;
+            m.t0 = R.r;
+            m.wg_begin = mwg;
+            m.out = h->matches[i];
+            m.counts = h->counts[i];
+            mwg += bk.match_kind == 2 ? g.ny * ((g.nx + kRunRefs - 1) / kRunRefs) : g.nx * g.ny;
+            // group 1 keeps the reference alone (src/cpu/bm/matcher.hpp:323).
+            if (sp.group > 1) {
+                PredJob& p = pt.job[pt.n++];
+                p.match = mr;
+                p.width = h->pw[i];
+                p.height = h->ph[i];
+                p.stride = h->pw[i];
+                p.nref = g.nx * g.ny;
+                p.group = sp.group;
+                p.ps_num = sp.ps_num;
+                p.ps_range = sp.ps_range;
+                p.slots = slots;
+                p.wg_begin = pwg;
+                p.recs = h->matches[i];
+                p.counts = h->counts[i];
+                pwg += p.nref;
+                plds = std::max(plds, predictive_lds_bytes(b, sp.group, sp.ps_num, R.r));
+                predictive = true;
+            }
+        }
+        for (int k = 0; k < bk.nplanes; ++k) {
+            const int i = bk.planes[k];
+            const int mi = h->p.chroma ? 0 : i;
+            const StagePlane& sp = stg.plane[i];
+            const Grid g = make_grid(h->pw[i], h->ph[i], b, sp.step);
+            TFilterJob& f = ft.job[ft.n++];
+            f.src = ring_of(R.src[i], R.src_n, h->pw[i], h->ph[i]);
+            f.wref = wref_ring(h, s, i);
+            f.width = h->pw[i];
+            f.height = h->ph[i];
+            f.stride = h->pw[i];
+            f.wref_stride = h->pw[i];
+            f.nref = g.nx * g.ny;
+            f.group = sp.group;
+            f.sigma = sp.sigma;
+            f.slots = slots;
+            f.wg_begin = fwg;
+            f.matches = h->matches[mi];
+            f.counts = h->counts[mi];
+            f.num = R.num[s][i];
+            f.den = R.den[s][i];
+            f.acc_ring = R.acc_n;
+            fwg += f.nref;
+            flds = std::max(flds, filter_atomic_lds_bytes(b, sp.group, stg.wiener));
+        }
+        hipError_t e = traced(h, 1, st, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, st); });
+        if (e != hipSuccess) return hip_fail(e, "match launch");
+        if (predictive) {
+            e = traced(h, 1, st, [&] { return launch_predictive(b, pt, pwg, plds, st); });
+            if (e != hipSuccess) return hip_fail(e, "predictive match launch");
+        }
+        e = traced(h, 2, st, [&] { return launch_filter_temporal(b, ft, fwg, flds, stg.wiener, st); });
+        if (e != hipSuccess) return hip_fail(e, "filter launch");
+    }
+    return NSS_HIP_OK;
+}
+
+// Frame n of stage s: num/den (or the source where nothing landed) into the
+// basic ring (stage 0 of final=1) or the output ring; its sums are cleared
+// for the frame that reuses the accumulator slot.
+int temporal_finish(nss_hip_bm3d* h, int s, int n, hipStream_t st) {
+    auto& R = h->roll;
+    const Stage& stg = h->stages[s];
+    const bool to_out = s == h->nstages - 1;
+    for (int i = 0; i < h->nplanes; ++i) {
+        if (stg.plane[i].sigma != 0.f) continue;
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        float* dst = to_out ? R.out[i] + static_cast<size_t>(n % R.out_n) * npx
+                            : R.basic[i] + static_cast<size_t>(n % R.basic_n) * npx;
+        const hipError_t e = hipMemcpyAsync(dst, R.src[i] + static_cast<size_t>(n % R.src_n) * npx, npx * 4,
+                                            hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return hip_fail(e, "plane copy");
+    }
+    for (int bi = 0; bi < stg.nbuckets; ++bi) {
+        const Bucket& bk = stg.bucket[bi];
+        FinishTable fn{};
+        ZeroTable zt{};
+        int nwg = 0, zwg = 0;
+        for (int k = 0; k < bk.nplanes; ++k) {
+            const int i = bk.planes[k];
+            const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+            const size_t slot = static_cast<size_t>(n % R.acc_n) * npx;
+            float* dst = to_out ? R.out[i] + static_cast<size_t>(n % R.out_n) * npx
+                                : R.basic[i] + static_cast<size_t>(n % R.basic_n) * npx;
+            FinishJob& f = fn.job[fn.n++];
+            f.num = R.num[s][i] + slot;
+            f.den = R.den[s][i] + slot;
+            f.src = R.src[i] + static_cast<size_t>(n % R.src_n) * npx;
+            f.dst = dst;
+            f.width = h->pw[i];
+            f.height = h->ph[i];
+            f.sstride = h->pw[i];
+            f.dstride = h->pw[i];
+            f.tiles_x = (h->pw[i] + 15) / 16;
+            f.wg_begin = nwg;
+            nwg += f.tiles_x * ((h->ph[i] + 15) / 16);
+            for (unsigned long long* p : {R.num[s][i] + slot, R.den[s][i] + slot}) {
+                ZeroJob& z = zt.job[zt.n++];
+                z.p = p;
+                z.count = static_cast<long long>(npx);
+                z.wg_begin = zwg;
+                zwg += static_cast<int>((npx + 1023) / 1024);
+            }
+        }
+        hipError_t e = traced(h, 3, st, [&] { return launch_finish(bk.block, fn, nwg, false, st); });
+        if (e != hipSuccess) return hip_fail(e, "finish launch");
+        e = traced(h, 0, st, [&] { return launch_zero(zt, zwg, st); });
+        if (e != hipSuccess) return hip_fail(e, "clear launch");
+    }
+    return NSS_HIP_OK;
+}
+
+// Runs every centre that the frames pushed so far allow, in order, and
+// finishes the frames they complete.
+int temporal_advance(nss_hip_bm3d* h, hipStream_t st) {
+    auto& R = h->roll;
+    const int last = h->nstages - 1;
+    for (bool progress = true; progress;) {
+        progress = false;
+        for (int s = 0; s <= last; ++s) {
+            const int c = R.next_center[s];
+            if (c >= R.N) continue;
+            const int need = std::min(c + R.r, R.N - 1);  // newest frame centre c reads
+            const bool ready = s == 0 ? R.pushed > need : R.finished[0] > need;
+            if (!ready) continue;
+            int rc = temporal_center(h, s, c, st);
+            if (rc != NSS_HIP_OK) return rc;
+            R.next_center[s] = c + 1;
+            while (R.finished[s] < R.N && R.next_center[s] > std::min(R.finished[s] + R.r, R.N - 1)) {
+                rc = temporal_finish(h, s, R.finished[s], st);
+                if (rc != NSS_HIP_OK) return rc;
+                ++R.finished[s];
+            }
+            progress = true;
+        }
     }
     return NSS_HIP_OK;
 }
@@ -416,9 +661,14 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
     if (info->sample_type != 1 || info->bits_per_sample != 32 || info->width <= 0 || info->height <= 0 ||
         info->num_planes < 1 || info->num_planes > 3)
         return fail(NSS_HIP_INVALID_INPUT, "nss_hip.BM3D: constant Gray/YUV/RGB 32-bit float required");
-    if (params->radius != 0)
-        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: radius > 0 is not available in this build");
-    if (params->chroma) return fail(NSS_HIP_UNSUPPORTED, "nss_hip: chroma=1 is not available in this build");
+    if (params->radius != 0 && !params->temporal_mode)
+        return fail(NSS_HIP_UNSUPPORTED,
+                    "nss_hip: temporal_mode=legacy (the num/den slice output for VAggregate) is not available in "
+                    "this build; use temporal_mode=rolling");
+    if (params->radius != 0 && (info->width > 8191 || info->height > 8191))
+        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: radius > 0 supports planes up to 8191 x 8191");
+    if (params->radius != 0 && info->num_frames < 1)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip: radius > 0 needs the clip length");
     DeviceGuard guard(device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     int rc = ensure_tables(device);
@@ -464,6 +714,7 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
             return rc;
         }
     }
+    const bool rolling = p.radius > 0;
     for (int i = 0; i < h->nplanes; ++i) {
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
         size_t recs = 0, nref_max = 0, part = 0;
@@ -471,11 +722,17 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
         for (int s = 0; s < h->nstages; ++s) {
             const Stage& st = h->stages[s];
             const StagePlane& sp = st.plane[i];
-            if (sp.sigma == 0.f) continue;
-            const Grid g = make_grid(h->pw[i], h->ph[i], sp.block, sp.step);
-            const size_t nref = static_cast<size_t>(g.nx) * g.ny;
-            recs = std::max(recs, nref * sp.group);
-            nref_max = std::max(nref_max, nref);
+            bool any = false;
+            for (int q = 0; q < h->nplanes; ++q) any |= st.plane[q].sigma != 0.f;
+            // With chroma, plane 0 holds the shared groups whenever any plane is filtered.
+            const bool matched = p.chroma ? (i == 0 && any) : sp.sigma != 0.f;
+            if (matched) {
+                const Grid g = make_grid(h->pw[i], h->ph[i], sp.block, sp.step);
+                const size_t nref = static_cast<size_t>(g.nx) * g.ny;
+                recs = std::max(recs, nref * sp.group);
+                nref_max = std::max(nref_max, nref);
+            }
+            if (sp.sigma == 0.f || rolling) continue;
             for (int bi = 0; bi < st.nbuckets; ++bi)
                 for (int k = 0; k < st.bucket[bi].nplanes; ++k) {
                     if (st.bucket[bi].planes[k] != i) continue;
@@ -500,7 +757,34 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
             h->part_floats[i] = part;
             e = hipMalloc(&h->part[i], 2 * part * sizeof(float));
         }
-        if (e == hipSuccess && p.final_stage) e = hipMalloc(&h->basic[i], npx * 4);
+        if (e == hipSuccess && p.final_stage && !rolling) e = hipMalloc(&h->basic[i], npx * 4);
+        if (e == hipSuccess && rolling) {
+            // Ring depths: the oldest frame a pending centre or finish reads
+            // is 4r (final=1) or 2r (one stage) behind the newest pushed; the
+            // Wiener stage reads basic frames up to 3r behind the newest
+            // finished one; 4r + 1 finished frames can wait for pop at the
+            // clip's end.
+            auto& R = h->roll;
+            const int r = p.radius;
+            R.N = info->num_frames;
+            R.r = r;
+            R.src_n = p.final_stage ? 4 * r + 2 : 2 * r + 2;
+            R.ref_n = (p.has_ref && !p.final_stage) ? 2 * r + 2 : 0;
+            R.basic_n = p.final_stage ? 3 * r + 2 : 0;
+            R.out_n = 4 * r + 2;
+            R.acc_n = 2 * r + 1;
+            e = hipMalloc(&R.src[i], npx * 4 * R.src_n);
+            if (e == hipSuccess && R.ref_n) e = hipMalloc(&R.ref[i], npx * 4 * R.ref_n);
+            if (e == hipSuccess && R.basic_n) e = hipMalloc(&R.basic[i], npx * 4 * R.basic_n);
+            if (e == hipSuccess) e = hipMalloc(&R.out[i], npx * 4 * R.out_n);
+            for (int s = 0; s < h->nstages && e == hipSuccess; ++s) {
+                if (h->stages[s].plane[i].sigma == 0.f) continue;
+                e = hipMalloc(&R.num[s][i], npx * 8 * R.acc_n);
+                if (e == hipSuccess) e = hipMalloc(&R.den[s][i], npx * 8 * R.acc_n);
+                if (e == hipSuccess) e = hipMemset(R.num[s][i], 0, npx * 8 * R.acc_n);
+                if (e == hipSuccess) e = hipMemset(R.den[s][i], 0, npx * 8 * R.acc_n);
+            }
+        }
         if (e != hipSuccess) {
             free_buffers(h);
             delete h;
@@ -508,6 +792,67 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
         }
     }
     *out = h;
+    return NSS_HIP_OK;
+}
+
+int nss_hip_bm3d_push(nss_hip_bm3d* h, const float* const* src, const int* src_stride, const float* const* ref,
+                      const int* ref_stride, void* stream) {
+    if (!h || !src || !src_stride) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    auto& R = h->roll;
+    if (h->p.radius == 0) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: push/pop serve radius > 0; use process");
+    if (R.pushed >= R.N) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: every frame of the clip was already pushed");
+    const int last = h->nstages - 1;
+    const int pending = R.finished[last] - R.out_head;
+    const int may_finish = R.pushed + 1 == R.N ? R.N - R.finished[last] : 2;
+    if (pending + may_finish > R.out_n)
+        return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: pop the finished frames before pushing more");
+    const bool need_ref = h->p.has_ref && !h->p.final_stage;
+    for (int i = 0; i < h->nplanes; ++i) {
+        if (!src[i] || src_stride[i] < h->pw[i])
+            return fail(NSS_HIP_INVALID_INPUT, "nss_hip: missing plane or stride below width");
+        if (need_ref && (!ref || !ref_stride || !ref[i] || ref_stride[i] < h->pw[i]))
+            return fail(NSS_HIP_INVALID_INPUT, "nss_hip: ref planes required");
+    }
+    DeviceGuard guard(h->device);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int f = R.pushed;
+    for (int i = 0; i < h->nplanes; ++i) {
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        hipError_t e = hipMemcpy2DAsync(R.src[i] + static_cast<size_t>(f % R.src_n) * npx,
+                                        static_cast<size_t>(h->pw[i]) * 4, src[i], static_cast<size_t>(src_stride[i]) * 4,
+                                        static_cast<size_t>(h->pw[i]) * 4, h->ph[i], hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && need_ref)
+            e = hipMemcpy2DAsync(R.ref[i] + static_cast<size_t>(f % R.ref_n) * npx, static_cast<size_t>(h->pw[i]) * 4,
+                                 ref[i], static_cast<size_t>(ref_stride[i]) * 4, static_cast<size_t>(h->pw[i]) * 4,
+                                 h->ph[i], hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "frame upload");
+    }
+    R.pushed = f + 1;
+    return temporal_advance(h, s);
+}
+
+int nss_hip_bm3d_pop(nss_hip_bm3d* h, float* const* dst, const int* dst_stride, int* frame, void* stream) {
+    if (!h || !dst || !dst_stride) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    auto& R = h->roll;
+    if (h->p.radius == 0) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: push/pop serve radius > 0; use process");
+    if (R.out_head >= R.finished[h->nstages - 1]) return NSS_HIP_BYPASS;
+    DeviceGuard guard(h->device);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int n = R.out_head;
+    for (int i = 0; i < h->nplanes; ++i) {
+        if (!dst[i] || dst_stride[i] < h->pw[i])
+            return fail(NSS_HIP_INVALID_INPUT, "nss_hip: missing plane or stride below width");
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        const hipError_t e =
+            hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, R.out[i] + static_cast<size_t>(n % R.out_n) * npx,
+                             static_cast<size_t>(h->pw[i]) * 4, static_cast<size_t>(h->pw[i]) * 4, h->ph[i],
+                             hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "frame download");
+    }
+    if (frame) *frame = n;
+    R.out_head = n + 1;
     return NSS_HIP_OK;
 }
 
@@ -519,6 +864,7 @@ int nss_hip_bm3d_process(nss_hip_bm3d* h, const float* const* src, const int* sr
     (void)frame;
     (void)frame_count;
     if (!h || !src || !src_stride || !dst || !dst_stride) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    if (h->p.radius > 0) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: radius > 0 filters through push/pop");
     for (int i = 0; i < h->nplanes; ++i)
         if (!src[i] || !dst[i] || src_stride[i] < h->pw[i] || dst_stride[i] < h->pw[i])
             return fail(NSS_HIP_INVALID_INPUT, "nss_hip: missing plane or stride below width");

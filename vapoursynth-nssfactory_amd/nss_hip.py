@@ -90,7 +90,8 @@ _lib = None
 
 EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "nss_hip_bm3d_output_info",
            "nss_hip_bm3d_create", "nss_hip_bm3d_slots", "nss_hip_bm3d_process", "nss_hip_bm3d_destroy",
-           "nss_hip_spatial_match", "nss_hip_bm3d_set_tracing", "nss_hip_bm3d_kernel_times")
+           "nss_hip_spatial_match", "nss_hip_bm3d_set_tracing", "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push",
+           "nss_hip_bm3d_pop")
 KERNEL_KINDS = ("clear", "match", "filter", "finish")
 
 
@@ -115,6 +116,8 @@ def lib():
         L.nss_hip_spatial_match.argtypes = [vp, ci, ci, ci, ci, ci, ci, ci, vp, vp, vp]
         L.nss_hip_bm3d_set_tracing.argtypes = [vp, ci]
         L.nss_hip_bm3d_kernel_times.argtypes = [vp, P(ci), P(ctypes.c_double), ci]
+        L.nss_hip_bm3d_push.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), vp]
+        L.nss_hip_bm3d_pop.argtypes = [vp, P(vp), P(ci), P(ci), vp]
         for name in EXPORTS[1:]:
             getattr(L, name).restype = ci
         _lib = L
@@ -239,6 +242,58 @@ class BM3D:
         _check(lib().nss_hip_bm3d_kernel_times(self._h, n, ms, int(reset)))
         return {k: (n[i], ms[i]) for i, k in enumerate(KERNEL_KINDS)}
 
+    def _check_planes(self, planes, what="plane"):
+        import torch
+        n = len(self.plane_dims)
+        if len(planes) != n:
+            raise Error(f"{NAMESPACE}.BM3D: expected {n} {what}s, got {len(planes)}")
+        for i, p in enumerate(planes):
+            if p.dtype != torch.float32 or not p.is_cuda or p.dim() != 2 or p.stride(1) != 1:
+                raise Error(f"{NAMESPACE}.BM3D: {what} {i} must be a row-contiguous 2-D float32 CUDA tensor")
+            if tuple(p.shape) != self.plane_dims[i]:
+                raise Error(f"{NAMESPACE}.BM3D: {what} {i} has shape {tuple(p.shape)}, expected {self.plane_dims[i]}")
+
+    def push(self, planes, ref=None, stream=None):
+        """radius > 0: the clip's next frame (its planes as device tensors)."""
+        self._check_planes(planes)
+        if ref is not None:
+            self._check_planes(ref, "ref plane")
+        n = len(planes)
+        src = (ctypes.c_void_p * n)(*[p.data_ptr() for p in planes])
+        sst = (ctypes.c_int * n)(*[p.stride(0) for p in planes])
+        rp = (ctypes.c_void_p * n)(*[p.data_ptr() for p in ref]) if ref is not None else None
+        rst = (ctypes.c_int * n)(*[p.stride(0) for p in ref]) if ref is not None else None
+        _check(lib().nss_hip_bm3d_push(self._h, src, sst, rp, rst, _stream_handle(stream)))
+
+    def pop(self, out=None, stream=None):
+        """radius > 0: (frame index, planes) of the next finished frame, or None."""
+        import torch
+        n = len(self.plane_dims)
+        if out is None:
+            out = [torch.empty(d, dtype=torch.float32, device=torch.device("cuda", self.device))
+                   for d in self.plane_dims]
+        dp = (ctypes.c_void_p * n)(*[p.data_ptr() for p in out])
+        dst = (ctypes.c_int * n)(*[p.stride(0) for p in out])
+        frame = ctypes.c_int(-1)
+        rc = lib().nss_hip_bm3d_pop(self._h, dp, dst, ctypes.byref(frame), _stream_handle(stream))
+        if rc == BYPASS:
+            return None
+        _check(rc)
+        return frame.value, out
+
+    def filter_clip(self, frames, refs=None, stream=None):
+        """radius > 0: every frame of a clip (list of per-frame plane lists)
+        through push/pop; returns the finished frames in order."""
+        out = []
+        for i, f in enumerate(frames):
+            self.push(f, None if refs is None else refs[i], stream=stream)
+            while True:
+                r = self.pop(stream=stream)
+                if r is None:
+                    break
+                out.append(r[1])
+        return out
+
     def close(self):
         if getattr(self, "_h", None):
             lib().nss_hip_bm3d_destroy(self._h)
@@ -253,14 +308,9 @@ class BM3D:
     def __call__(self, planes, ref=None, out=None, frame=0, frame_count=1, stream=None):
         import torch
         n = len(self.plane_dims)
-        slots = lib().nss_hip_bm3d_slots(self._h)
-        if len(planes) != n * slots:
-            raise Error(f"{NAMESPACE}.BM3D: expected {n * slots} planes, got {len(planes)}")
-        for i, p in enumerate(planes):
-            if p.dtype != torch.float32 or not p.is_cuda or p.dim() != 2 or p.stride(1) != 1:
-                raise Error(f"{NAMESPACE}.BM3D: plane {i} must be a row-contiguous 2-D float32 CUDA tensor")
-            if tuple(p.shape) != self.plane_dims[i % n]:
-                raise Error(f"{NAMESPACE}.BM3D: plane {i} has shape {tuple(p.shape)}, expected {self.plane_dims[i % n]}")
+        self._check_planes(planes)
+        if ref is not None:
+            self._check_planes(ref, "ref plane")
         if out is None:
             out = [torch.empty(self.plane_dims[i], dtype=torch.float32, device=planes[i].device) for i in range(n)]
         VP = ctypes.c_void_p * len(planes)
