@@ -75,6 +75,39 @@ __device__ inline unsigned long long group_min_u64(unsigned long long v) {
     return v;
 }
 
+// min(v, v of the lane DPP control `ctrl` selects); lanes whose source is
+// outside the row (or rows outside row_mask) keep v.
+template <int CTRL, int ROW_MASK>
+__device__ inline unsigned long long dpp_min_u64(unsigned long long v) {
+    const int lo = static_cast<int>(static_cast<uint32_t>(v));
+    const int hi = static_cast<int>(static_cast<uint32_t>(v >> 32));
+    const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+    const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+    const unsigned long long o = (static_cast<unsigned long long>(static_cast<uint32_t>(ohi)) << 32) |
+                                 static_cast<uint32_t>(olo);
+    return o < v ? o : v;
+}
+
+// Minimum over each half-wave (lanes 0-31 and 32-63), returned to every lane
+// of that half. Must be reached by all 64 lanes. DPP row_shr 1, 2, 4, 8 leaves
+// each 16-lane row's minimum in its last lane; row_bcast15 folds row 0 into
+// row 1 and row 2 into row 3; lanes 31 and 63 then hold the two minima.
+__device__ inline unsigned long long half_min_u64(unsigned long long v) {
+    v = dpp_min_u64<0x111, 0xf>(v);
+    v = dpp_min_u64<0x112, 0xf>(v);
+    v = dpp_min_u64<0x114, 0xf>(v);
+    v = dpp_min_u64<0x118, 0xf>(v);
+    v = dpp_min_u64<0x142, 0xa>(v);
+    const int lo = static_cast<int>(static_cast<uint32_t>(v));
+    const int hi = static_cast<int>(static_cast<uint32_t>(v >> 32));
+    const uint32_t lo0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(lo, 31));
+    const uint32_t hi0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(hi, 31));
+    const uint32_t lo1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(lo, 63));
+    const uint32_t hi1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(hi, 63));
+    const bool upper = threadIdx.x & 32;
+    return (static_cast<unsigned long long>(upper ? hi1 : hi0) << 32) | (upper ? lo1 : lo0);
+}
+
 __device__ inline int wave_sum_i32(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -123,16 +156,21 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int top = max(cy - r, 0), bottom = min(cy + r, jb.height - B);
     const int rx0 = max(ref_x(jb.grid, i0, jb.width, B) - r, 0);
     const int rx1 = min(ref_x(jb.grid, i0 + nrun - 1, jb.width, B) + r, jb.width - B) + B - 1;
-    // 3 more rows and columns than any window needs: a lane's 4 x 4 tile of
-    // candidates may overhang the window (those candidates are discarded).
-    const int RW = rx1 - rx0 + 1 + 3, RH = bottom - top + B + 3;
+    // Lane tiles of 4 x 4 candidates start at window column 4*tx - (ty & 3):
+    // staggering the tile rows spreads the lanes' LDS addresses over all
+    // banks (aligned tiles would touch a quarter of them). The region keeps
+    // 3 columns before the windows and 6 after, 3 rows below (candidates of
+    // an overhanging tile are discarded), and a row pitch = 2 mod 4.
+    const int need = rx1 - rx0 + 1 + 9;
+    const int RW = need + (6 - need % 4) % 4;
+    const int RH = bottom - top + B + 3;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int yy = wave; yy < RH; yy += 4) {
         const int gy = top + yy;
         const float* src = jb.plane + static_cast<size_t>(min(gy, jb.height - 1)) * jb.stride;
         for (int xx = lane; xx < RW; xx += 64) {
-            const int gx = rx0 + xx;
-            lds[yy * RW + xx] = (gy < jb.height && gx < jb.width) ? src[gx] : 0.f;
+            const int gx = rx0 - 3 + xx;
+            lds[yy * RW + xx] = (gy < jb.height && gx >= 0 && gx < jb.width) ? src[gx] : 0.f;
         }
     }
     const int wmax = 2 * r + 1;
@@ -146,13 +184,13 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int ncand = ww * wh;
     uint32_t* dk = dist + half * wmax * wmax;
     if (half < nrun) {
-        const int ox = left - rx0;
-        const float* rp = lds + (cy - top) * RW + (cx - rx0);
-        const int tw = (ww + 3) >> 2, th = (wh + 3) >> 2;
+        const int ox = left - rx0 + 3;
+        const float* rp = lds + (cy - top) * RW + (cx - rx0 + 3);
+        const int tw = (ww + 6) >> 2, th = (wh + 3) >> 2;
         const int self = (cy - top) * ww + (cx - left);
         for (int t = hl; t < tw * th; t += 32) {
             const int ty = t / tw, tx = t - ty * tw;
-            const int dy0 = ty * 4, dx0 = tx * 4;
+            const int dy0 = ty * 4, dx0 = tx * 4 - (ty & 3);
             float acc[4][4];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -185,36 +223,42 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int dy = dy0 + i, dx = dx0 + j;
-                    if (dy < wh && dx < ww) {
+                    if (dy < wh && dx >= 0 && dx < ww) {
                         const int c = dy * ww + dx;
                         dk[c] = c == self ? kTaken : dist_key(acc[i][j]);
                     }
                 }
         }
-        wave_sync();
-        const int K = min(jb.group - 1, ncand - 1);
-        const int ref_id = row * jb.grid.nx + i0 + half;
-        MatchRec* out = jb.out + static_cast<size_t>(ref_id) * jb.group;
-        if (hl == 0) {
-            out[0] = MatchRec{cx, cy, jb.t0, 0u};
-            jb.counts[ref_id] = 1 + K;
-        }
-        for (int k = 0; k < K; ++k) {
-            unsigned long long best = ~0ull;
+    }
+    wave_sync();
+    // Selection, wave-uniform: both half-waves run max(K) rounds (a half with
+    // fewer candidates, or no reference, idles) so the DPP reduction always
+    // sees all 64 lanes.
+    const int K = half < nrun ? min(jb.group - 1, ncand - 1) : 0;
+    const int kmax = max(__builtin_amdgcn_readlane(K, 0), __builtin_amdgcn_readlane(K, 32));
+    const int ref_id = row * jb.grid.nx + i0 + half;
+    MatchRec* out = jb.out + static_cast<size_t>(ref_id) * jb.group;
+    if (half < nrun && hl == 0) {
+        out[0] = MatchRec{cx, cy, jb.t0, 0u};
+        jb.counts[ref_id] = 1 + K;
+    }
+    for (int k = 0; k < kmax; ++k) {
+        unsigned long long best = ~0ull;
+        if (k < K) {
             for (int c = hl; c < ncand; c += 32) {
                 const uint32_t d = dk[c];
                 const unsigned long long key = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
                 best = (d != kTaken && key < best) ? key : best;
             }
-            best = group_min_u64<32>(best);
-            const int c = static_cast<int>(static_cast<uint32_t>(best));
-            if (hl == (c & 31)) {
-                dk[c] = kTaken;
-                const int dy = c / ww, dx = c - dy * ww;
-                out[1 + k] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(best >> 32)};
-            }
-            wave_sync();
         }
+        best = half_min_u64(best);
+        const int c = static_cast<int>(static_cast<uint32_t>(best));
+        if (k < K && hl == (c & 31)) {
+            dk[c] = kTaken;
+            const int dy = c / ww, dx = c - dy * ww;
+            out[1 + k] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(best >> 32)};
+        }
+        wave_sync();
     }
 }
 
@@ -1132,7 +1176,8 @@ size_t match_wave_lds_bytes(int block, int range, bool stage) {
 
 size_t match_run_lds_bytes(int block, int step, int range) {
     const size_t w = 2 * static_cast<size_t>(range) + 1;
-    const size_t rw = static_cast<size_t>(kRunRefs - 1) * step + 2 * range + block + 3;
+    // windows' span + 9 padding columns, rounded up to a pitch of 2 mod 4
+    const size_t rw = static_cast<size_t>(kRunRefs - 1) * step + 2 * range + block + 9 + 3;
     const size_t rh = 2 * static_cast<size_t>(range) + block + 3;
     return (((rw * rh) + 3) & ~static_cast<size_t>(3)) * 4 + kRunRefs * w * w * 4;
 }
