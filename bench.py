@@ -1,10 +1,12 @@
 """Benchmark: two-stage nss_hip.BM3D on 1920x1080 YUV420PS frames, sigma 10.
 
 One step = one frame through both stages (basic + Wiener), all three planes,
-plain FP32, one frame per launch sequence, input already resident in HBM.
-Prints one JSON line. Per-kernel times come from the library's own event
-tracing on the launch stream; the CPU baseline is the oracle run
-frame-parallel on the host cores over a bounded sample.
+one frame per launch sequence, input already resident in HBM. Each transform
+form is timed (plain FP32, bf16x3 and fp16 MFMA); the headline value is the
+--form one (default bf16x3), the others are listed under "forms". Prints one
+JSON line. Per-kernel times come from the library's own event tracing on the
+launch stream; the CPU baseline is the oracle run frame-parallel on the host
+cores over a bounded sample, and each form's PSNR against it is reported.
 """
 import argparse
 import json
@@ -70,6 +72,62 @@ def cpu_baseline(noisy, params, threads):
     return threads * W * H / dt / 1e6, outs[0], dt
 
 
+FORMS = {"fp32": 0, "bf16x3": 1, "fp16": 2}
+FORM_DTYPE = {"fp32": "f32", "bf16x3": "bf16x3 (f32 accumulate)", "fp16": "f16 (f32 accumulate)"}
+PEAK_MFMA_TFLOPS = 2500.0  # dense bf16 / fp16
+
+
+def measure(nss_hip, dev, form, steps, warmup, dims):
+    """Times `steps` frames of one transform form; per-kernel averages from
+    the library's event tracing on the launch stream."""
+    import numpy as np
+    import torch
+    inst = nss_hip.BM3D(FMT, W, H, sigma=SIGMA, final=1, transform=form)
+    out = [torch.empty_like(p) for p in dev]
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        inst(dev, out=out)
+    torch.cuda.synchronize()
+    inst.set_tracing(True)
+    inst.kernel_times(reset=True)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        inst(dev, out=out)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = inst.kernel_times(reset=True)
+    inst.set_tracing(False)
+    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
+    params = inst.params
+    inst.close()
+    sw_b, tr_b, _ = stage_work(params, dims, True)
+    sw_f, tr_f, _ = stage_work(params, dims, False)
+    kernels = {}
+    for kind, (n, ms) in kt.items():
+        if n:
+            kernels[kind] = {"launches": n, "avg_us": ms / n * 1e3}
+    # Per launch of a kind: one stage of the frame (basic, then Wiener, alternating).
+    filter_peak = PEAK_FP32_TFLOPS if form == 0 else PEAK_MFMA_TFLOPS
+    work = {"match": ((sw_b + sw_f) / 2, "TFLOP/s", PEAK_FP32_TFLOPS),
+            "filter": ((tr_b + tr_f) / 2, "TFLOP/s", filter_peak),
+            "finish": (W * H * 1.5 * 16 / 1, "GB/s", PEAK_HBM_GBS)}
+    for kind, (amount, unit, peak) in work.items():
+        if kind in kernels:
+            avg_s = kernels[kind]["avg_us"] * 1e-6
+            ach = amount / avg_s / (1e12 if unit == "TFLOP/s" else 1e9)
+            kernels[kind].update({"achieved": ach, "unit": unit, "peak": peak, "frac": ach / peak,
+                                  "algorithmic_per_launch": amount})
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    return {"value": round(W * H * steps / wall / 1e6, 3), "ms_per_step": round(wall / steps * 1e3, 4),
+            "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+            "kernels": kernels, "dominant": dom, "params": params, "out": [o.cpu().numpy() for o in out]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +135,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--form", choices=list(FORMS), default="bf16x3",
+                    help="transform form the headline value is quoted for (all three are timed)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy that reads and writes MIB MiB: the known-size kernel "
                          "that scales rocprof's FETCH_SIZE / WRITE_SIZE readings")
@@ -102,77 +162,43 @@ def main():
     print(f"# generating {FMT} {W}x{H} sigma {SIGMA}", file=sys.stderr, flush=True)
     clean, noisy = synth.noisy_frame(FMT, W, H, SIGMA, seed=0)
     dev = [torch.from_numpy(p).cuda() for p in noisy]
-    inst = nss_hip.BM3D(FMT, W, H, sigma=SIGMA, final=1)
-    out = [torch.empty_like(p) for p in dev]
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        inst(dev, out=out)
-    torch.cuda.synchronize()
-    inst.set_tracing(True)
-    inst.kernel_times(reset=True)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        inst(dev, out=out)
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kt = inst.kernel_times(reset=True)
-    inst.set_tracing(False)
-    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
-    ms_per_step = wall / args.steps * 1e3
-    value = W * H * args.steps / wall / 1e6
-    got = [o.cpu().numpy() for o in out]
-    print(f"# {ms_per_step:.3f} ms/frame, kernels {kt}", file=sys.stderr, flush=True)
-
     dims = [tuple(p.shape) for p in noisy]
-    sw_b, tr_b, ag_b = stage_work(inst.params, dims, True)
-    sw_f, tr_f, ag_f = stage_work(inst.params, dims, False)
-    kernels = {}
-    for kind, (n, ms) in kt.items():
-        if n:
-            kernels[kind] = {"launches": n, "avg_us": ms / n * 1e3}
-    # Per launch of a kind: one stage of the frame (basic, then Wiener, alternating).
-    work = {"match": ((sw_b + sw_f) / 2, "TFLOP/s", PEAK_FP32_TFLOPS),
-            "filter": ((tr_b + tr_f) / 2, "TFLOP/s", PEAK_FP32_TFLOPS),
-            "finish": (W * H * 1.5 * 16 / 1, "GB/s", PEAK_HBM_GBS)}
-    for kind, (amount, unit, peak) in work.items():
-        if kind in kernels:
-            avg_s = kernels[kind]["avg_us"] * 1e-6
-            ach = amount / avg_s / (1e12 if unit == "TFLOP/s" else 1e9)
-            kernels[kind].update({"achieved": ach, "unit": unit, "peak": peak, "frac": ach / peak,
-                                  "algorithmic_per_launch": amount})
-    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
-    dk = kernels[dom]
-    roofline = {"kernel": dom, "bound": "hbm" if dk.get("unit") == "GB/s" else "mfma",
+    forms = {}
+    for name, code in FORMS.items():
+        forms[name] = measure(nss_hip, dev, code, args.steps, args.warmup, dims)
+        print(f"# {name}: {forms[name]['ms_per_step']:.4f} ms/frame, kernels "
+              f"{ {k: round(v['avg_us'], 2) for k, v in forms[name]['kernels'].items()} }", file=sys.stderr, flush=True)
+    head = forms[args.form]
+    dk = head["kernels"][head["dominant"]]
+    roofline = {"kernel": head["dominant"], "bound": "hbm" if dk.get("unit") == "GB/s" else "mfma",
                 "achieved": dk.get("achieved"), "peak": dk.get("peak"), "unit": dk.get("unit"),
                 "frac": dk.get("frac"), "traffic": None}
 
-    res = {"metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+    res = {"metric": METRIC, "value": head["value"], "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": FORM_DTYPE[args.form], "data": "synthetic",
            "config": {"workload": "two-stage BM3D (final=1, sigma 10, B8 s8 G8 r9) on one 1920x1080 YUV420PS "
-                                  "frame per step, plain FP32, one frame per launch sequence",
-                      "format": FMT, "width": W, "height": H, "sigma": SIGMA, "final": 1, "form": "plain-fp32"},
-           "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
-           "roofline": roofline, "kernels": kernels,
+                                  "frame per step, one frame per launch sequence",
+                      "format": FMT, "width": W, "height": H, "sigma": SIGMA, "final": 1, "form": args.form},
+           "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "roofline": roofline, "kernels": head["kernels"],
+           "forms": {k: {f: v[f] for f in ("value", "ms_per_step", "p50_ms", "p99_ms")} |
+                     {"filter_us": v["kernels"]["filter"]["avg_us"], "filter_frac": v["kernels"]["filter"]["frac"],
+                      "filter_peak": v["kernels"]["filter"]["peak"]} for k, v in forms.items()},
            "psnr_vs_clean": [round(float(10 * np.log10(1 / np.mean((g.astype(np.float64) - c) ** 2))), 3)
-                             for g, c in zip(got, clean)]}
+                             for g, c in zip(head["out"], clean)]}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"# cpu baseline: oracle on {threads} threads, one frame each", file=sys.stderr, flush=True)
         import oracle
         oracle.lib()
-        cpu_value, want, dt = cpu_baseline(noisy, inst.params, threads)
-        import oracle as o
-        res["psnr_vs_cpu_min_plane"] = round(min(o.psnr(g, w_) for g, w_ in zip(got, want)), 3)
+        cpu_value, want, dt = cpu_baseline(noisy, head["params"], threads)
+        res["psnr_vs_cpu_min_plane"] = round(min(oracle.psnr(g, w_) for g, w_ in zip(head["out"], want)), 3)
+        for k, v in forms.items():
+            res["forms"][k]["psnr_vs_cpu_min_plane"] = round(min(oracle.psnr(g, w_)
+                                                                 for g, w_ in zip(v["out"], want)), 3)
         res["cpu_baseline"] = {"value": round(cpu_value, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
                                "sample": f"{threads} copies of the same 1920x1080 YUV420PS frame, one per thread, "
                                          f"two stages ({dt:.1f} s wall)"}
-    inst.close()
     print(json.dumps(res), flush=True)
 
 

@@ -99,7 +99,17 @@ typedef struct nss_hip_bm3d_params {
     int memory_limit_mb; /* 0 = not given */
     int device_id;       /* backend tail device_id:int:opt (default 0) */
     int num_streams;     /* backend tail num_streams:int:opt (default 1) */
+    int transform;       /* nss_hip extension transform:int:opt, no counterpart
+                            in the reference: NSS_HIP_TRANSFORM_* (default FP32) */
 } nss_hip_bm3d_params;
+
+/* Arithmetic of the group transforms. FP32 keeps the reference's FP32
+ * arithmetic; BF16X3 and FP16 run the 2-D DCT and the Haar as 16x16x16 MFMA
+ * products with FP32 accumulation (block 8 with groups of 8 or 16; other
+ * shapes stay FP32). Matching, shrinkage and aggregation stay FP32. */
+#define NSS_HIP_TRANSFORM_FP32 0
+#define NSS_HIP_TRANSFORM_BF16X3 1
+#define NSS_HIP_TRANSFORM_FP16 2
 
 typedef struct nss_hip_bm3d nss_hip_bm3d;
 

@@ -1,0 +1,123 @@
+"""MFMA transform forms (transform=1 bf16x3, transform=2 fp16) of the block-8
+group filter, through the C ABI.
+
+Gates:
+- >= 60 dB per plane against the FP32 oracle (the same contract as the FP32
+  form, tests/data/cuda_tolerances_v1.json);
+- max and RMS error against the FP64 oracle, printed beside the FP32 form's;
+  bf16x3 within 1e-4 RMS (80 dB) and fp16 within 1e-3 RMS (60 dB) of it;
+- run-to-run bitwise identity;
+- shapes the MFMA kernel does not cover (block != 8, group 32) give exactly
+  the FP32 form's output.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MIN_PSNR = 60.0
+FORMS = {1: "bf16x3", 2: "fp16"}
+RMS_BOUND = {0: 1e-4, 1: 1e-4, 2: 1e-3}
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _run(gpu, fmt, planes, ref=None, **kw):
+    import torch
+    h, w = planes[0].shape
+    inst = gpu.BM3D(fmt, w, h, ref=ref is not None, **kw)
+    out = inst([_dev(p) for p in planes], ref=None if ref is None else [_dev(p) for p in ref])
+    torch.cuda.synchronize()
+    res = [o.cpu().numpy() for o in out]
+    inst.close()
+    return res
+
+
+def _oracle(oracle, gpu, fmt, planes, ref=None, fp64=False, **kw):
+    h, w = planes[0].shape
+    kw = {k: v for k, v in kw.items() if k != "transform"}
+    return oracle.bm3d_frame(planes, gpu.parse(fmt, w, h, ref=ref is not None, **kw), ref=ref, fp64=fp64)
+
+
+def _errors(got, want):
+    d = np.concatenate([(g.astype(np.float64) - w_).ravel() for g, w_ in zip(got, want)])
+    return float(np.max(np.abs(d))), float(np.sqrt(np.mean(d * d)))
+
+
+def _check_form(gpu, oracle, fmt, planes, label, ref=None, **kw):
+    want32 = _oracle(oracle, gpu, fmt, planes, ref=ref, **kw)
+    want64 = _oracle(oracle, gpu, fmt, planes, ref=ref, fp64=True, **kw)
+    plain = _run(gpu, fmt, planes, ref=ref, transform=0, **kw)
+    mx0, rms0 = _errors(plain, want64)
+    print(f"{label} fp32 form vs fp64 oracle: max {mx0:.3e} rms {rms0:.3e}", flush=True)
+    for form, name in FORMS.items():
+        got = _run(gpu, fmt, planes, ref=ref, transform=form, **kw)
+        mx, rms = _errors(got, want64)
+        ps = [oracle.psnr(g, w_) for g, w_ in zip(got, want32)]
+        print(f"{label} {name} vs fp64 oracle: max {mx:.3e} rms {rms:.3e}; "
+              f"PSNR vs fp32 oracle {' '.join(f'{p:.2f}' for p in ps)} dB", flush=True)
+        for i, p in enumerate(ps):
+            assert p >= MIN_PSNR, (label, name, i, p)
+        assert rms <= RMS_BOUND[form], (label, name, rms)
+
+
+def test_mfma_single_stage_group16(gpu, oracle):
+    """256x256 gray, sigma 5, block 8, step 3, group 16, range 16, basic only."""
+    import synth
+    _, noisy = synth.noisy_frame("GRAYS", 256, 256, 5, seed=0)
+    _check_form(gpu, oracle, "GRAYS", noisy, "gray256 g16",
+                sigma=5, block_size=8, block_step=3, group_size=16, bm_range=16)
+
+
+def test_mfma_two_stage_yuv420(gpu, oracle):
+    """Two stages at the default shape (block 8, group 8, step 8, range 9)."""
+    import synth
+    _, noisy = synth.noisy_frame("YUV420PS", 192, 128, 10, seed=1)
+    _check_form(gpu, oracle, "YUV420PS", noisy, "yuv420 two-stage", sigma=10, final=1)
+
+
+def test_mfma_wiener_with_ref_group16(gpu, oracle):
+    import synth
+    clean, noisy = synth.noisy_frame("GRAYS", 96, 80, 20, seed=4)
+    _check_form(gpu, oracle, "GRAYS", noisy, "wiener ref g16", ref=clean,
+                sigma=20, group_size=16, block_step=4, bm_range=7)
+
+
+def test_mfma_large_values_fp16_scaling(gpu, oracle):
+    """Pixels in [0, 1000]: the fp16 form's per-group power-of-two scaling
+    keeps the products in range (unscaled, 64-term sums would overflow fp16)."""
+    import synth
+    clean, noisy = synth.noisy_frame("GRAYS", 80, 64, 10, seed=9)
+    big = [p * 1000.0 for p in noisy]
+    kw = dict(sigma=10 * 1000, final=1)
+    want = _oracle(oracle, gpu, "GRAYS", big, **kw)
+    got = _run(gpu, "GRAYS", big, transform=2, **kw)
+    p = oracle.psnr(got[0], want[0], peak=1000.0)
+    print(f"fp16 form on [0, 1000] pixels: PSNR {p:.2f} dB", flush=True)
+    assert np.isfinite(got[0]).all()
+    assert p >= MIN_PSNR
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_mfma_run_to_run_identity(gpu, form):
+    import synth
+    _, noisy = synth.noisy_frame("YUV420PS", 128, 96, 10, seed=3)
+    a = _run(gpu, "YUV420PS", noisy, sigma=10, final=1, transform=form)
+    b = _run(gpu, "YUV420PS", noisy, sigma=10, final=1, transform=form)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("kw", [dict(block_size=4, block_step=2, group_size=16),
+                                dict(group_size=32, block_step=4),
+                                dict(block_size=16, group_size=8, block_step=8)])
+def test_uncovered_shapes_run_fp32(gpu, kw):
+    import synth
+    _, noisy = synth.noisy_frame("GRAYS", 72, 64, 10, seed=5)
+    plain = _run(gpu, "GRAYS", noisy, sigma=10, **kw)
+    for form in FORMS:
+        got = _run(gpu, "GRAYS", noisy, sigma=10, transform=form, **kw)
+        assert np.array_equal(got[0], plain[0]), (kw, form)

@@ -79,7 +79,8 @@ def _same(expected, actual):
 def test_signature_is_reference_plus_backend_tail():
     import nss_hip
     g = _golden()
-    assert nss_hip.BM3D_SIGNATURE == g["signature"] + "device_id:int:opt;num_streams:int:opt;"
+    assert nss_hip.BM3D_SIGNATURE == (g["signature"] + "device_id:int:opt;num_streams:int:opt;"
+                                      "transform:int:opt;")
 
 
 def test_replay_900_cases():
@@ -121,6 +122,8 @@ def test_adaptive_cache_limit_below_chunks_rejected():
     ("device_id", -1, "nss_hip.BM3D: device_id must be >= 0"),
     ("num_streams", 0, "nss_hip.BM3D: num_streams must be in [1, 16]"),
     ("num_streams", 17, "nss_hip.BM3D: num_streams must be in [1, 16]"),
+    ("transform", 3, "nss_hip.BM3D: transform must be 0 (fp32), 1 (bf16x3) or 2 (fp16)"),
+    ("transform", -1, "nss_hip.BM3D: transform must be 0 (fp32), 1 (bf16x3) or 2 (fp16)"),
 ])
 def test_backend_tail(key, value, msg):
     import nss_hip

@@ -402,6 +402,10 @@ int nss_hip_bm3d_parse(const nss_hip_video_info* clip_info, const nss_hip_video_
         if (!err && (streams < 1 || streams > kMaxStreams))
             throw Invalid(ns + ".BM3D: num_streams must be in [1, " + std::to_string(kMaxStreams) + "]");
         p.num_streams = err ? 1 : static_cast<int>(streams);
+        const std::int64_t form = in.get_int("transform", 0, &err);
+        if (!err && (form < NSS_HIP_TRANSFORM_FP32 || form > NSS_HIP_TRANSFORM_FP16))
+            throw Invalid(ns + ".BM3D: transform must be 0 (fp32), 1 (bf16x3) or 2 (fp16)");
+        p.transform = err ? NSS_HIP_TRANSFORM_FP32 : static_cast<int>(form);
         *out = p;
         nsship::g_last_error.clear();
         return NSS_HIP_OK;

@@ -868,6 +868,416 @@ __global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Block-8 group filter on MFMA (groups of 8 or 16), in bf16x3 or fp16 with
+// FP32 accumulation. One wave filters a unit of 16 patches (two groups of 8
+// or one of 16):
+//   2-D DCT   four patches at a time as a 16 x 16 tile Q = [[P0, P1], [P2, P3]]
+//             (pixel rows 8 pr + y, columns 8 pc + x): Z = bd(D) (Q bd(D^T))
+//             with bd(M) = diag(M, M), two 16x16x16 products per tile; the
+//             first product reads Q from memory in the A layout, its C-layout
+//             output is the B operand of the second.
+//   Haar      coefficient-major through LDS, then per 16 coefficients one
+//             16x16x16 product with the exact +-1 butterfly matrix (block
+//             diagonal for two groups of 8), scaled in FP32 to the
+//             orthonormal transform.
+//   shrink    hard threshold or Wiener in FP32 on the product's output.
+//   inverse   the same steps reversed: Haar^T product, LDS back to the tile
+//             layout, bd(D^T) (Z bd(D)).
+// bf16x3 splits each operand v = hi + lo (both bf16, round to nearest even)
+// and sums lo*hi + hi*lo + hi*hi; fp16 scales each group's pixels by a power
+// of two to max |x| in [0.5, 1) (exact, undone in FP32) before rounding.
+// Matching, weights and the fixed-point aggregation are those of the FP32
+// kernels. MFMA 16x16x16 operand layouts (lane l, j = 0..3):
+//   A[i][k]: i = l % 16, k = 4 (l / 16) + j;  B[k][n] and C[k][n]: n = l % 16,
+//   k = 4 (l / 16) + j. So A(M) holds the entries of B(M^T).
+// ---------------------------------------------------------------------------
+typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+inline constexpr int kFormBf16x3 = 1;
+inline constexpr int kFormF16 = 2;
+
+template <int FORM>
+struct Operand;
+template <>
+struct Operand<kFormBf16x3> {
+    s16x4 hi, lo;
+};
+template <>
+struct Operand<kFormF16> {
+    h16x4 v;
+};
+
+__device__ inline short bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<short>(u >> 16);
+}
+
+__device__ inline float bf16_value(short b) {
+    return __uint_as_float(static_cast<uint32_t>(static_cast<uint16_t>(b)) << 16);
+}
+
+template <int FORM>
+__device__ inline Operand<FORM> operand(f32x4 v) {
+    Operand<FORM> o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if constexpr (FORM == kFormBf16x3) {
+            o.hi[j] = bf16_bits(v[j]);
+            o.lo[j] = bf16_bits(v[j] - bf16_value(o.hi[j]));
+        } else {
+            o.v[j] = static_cast<_Float16>(v[j]);
+        }
+    }
+    return o;
+}
+
+// acc + a b (bf16x3 drops lo * lo only).
+template <int FORM>
+__device__ inline f32x4 mma(const Operand<FORM>& a, const Operand<FORM>& b, f32x4 acc) {
+    if constexpr (FORM == kFormBf16x3) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.lo, b.hi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.lo, acc, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.hi, acc, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a.v, b.v, acc, 0, 0, 0);
+    }
+}
+
+// a b for an A operand that is exact in the operand type (a.lo = 0).
+template <int FORM>
+__device__ inline f32x4 mma_exact_a(const Operand<FORM>& a, const Operand<FORM>& b) {
+    const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (FORM == kFormBf16x3) {
+        const f32x4 t = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.lo, z, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.hi, t, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a.v, b.v, z, 0, 0, 0);
+    }
+}
+
+// Lane's A-operand entries of bd(D) (transposed = false) or bd(D^T).
+__device__ inline f32x4 dct_bd_entries(bool transposed, int lane) {
+    const float* d = c_dct + dct_offset<8>();
+    const int i = lane & 15;
+    f32x4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int k = 4 * (lane >> 4) + j;
+        v[j] = (i >> 3) != (k >> 3) ? 0.f : transposed ? d[(k & 7) * 8 + (i & 7)] : d[(i & 7) * 8 + (k & 7)];
+    }
+    return v;
+}
+
+// Entry (j, g) of the unnormalised in-place Haar butterfly matrix: row 0 sums
+// all samples; row j > 0 with lowest set bit h is the sum over [j - h, j)
+// minus the sum over [j, j + h) (the level-h difference haar_forward writes
+// to position j).
+__device__ inline float haar_entry(int j, int g) {
+    if (j == 0) return 1.f;
+    const int h = j & -j;
+    return (g >= j - h && g < j) ? 1.f : (g >= j && g < j + h) ? -1.f : 0.f;
+}
+
+// Lane's A-operand entries of the unit's Haar matrix (block diagonal for two
+// groups of 8) or of its transpose.
+template <int G>
+__device__ inline f32x4 haar_unit_entries(bool transposed, int lane) {
+    const int i = lane & 15;
+    f32x4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int k = 4 * (lane >> 4) + j;
+        const int r = transposed ? k : i, c = transposed ? i : k;
+        v[j] = (G == 8 && (r >> 3) != (c >> 3)) ? 0.f : haar_entry(r & (G - 1), c & (G - 1));
+    }
+    return v;
+}
+
+// Coefficient-major transpose buffer of one wave: 64 coefficients x 16
+// patches (dwords), patch groups of 4 permuted per coefficient and a pad
+// after 32 coefficients, so that the b32 scatters and b128 reads below hit
+// distinct banks.
+inline constexpr int kMfmaBuf = 63 * 24 + 16 + 16;
+
+__device__ inline int mfma_slot(int coef, int patch) {
+    return coef * 24 + (coef >> 5) * 16 + (patch ^ (4 * (((coef >> 3) ^ (coef >> 5)) & 3)));
+}
+
+// Forward 3-D transform of a unit. x[q] = lane's pixels of tile q in the A
+// layout, already scaled. Out: raw Haar products out[t][j] = coefficient
+// 16 t + (lane & 15) of Haar row 4 (lane >> 4) + j (not yet normalised).
+template <int FORM>
+__device__ inline void forward_unit(const f32x4 (&x)[4], const Operand<FORM>& dct, const Operand<FORM>& haar,
+                                    float* T, int lane, f32x4 (&out)[4]) {
+    const f32x4 z0 = {0.f, 0.f, 0.f, 0.f};
+    const int h = (lane >> 4) & 1, kx = lane & 7, pc = (lane >> 3) & 1, pr = lane >> 5;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const f32x4 y = mma<FORM>(operand<FORM>(x[q]), dct, z0);  // Q bd(D^T)
+        const f32x4 z = mma<FORM>(dct, operand<FORM>(y), z0);     // bd(D) Q bd(D^T)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) T[mfma_slot(8 * (4 * h + j) + kx, 4 * q + 2 * pr + pc)] = z[j];
+    }
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const f32x4 c = *reinterpret_cast<const f32x4*>(T + mfma_slot(16 * t + (lane & 15), 4 * (lane >> 4)));
+        out[t] = mma_exact_a<FORM>(haar, operand<FORM>(c));
+    }
+    wave_sync();
+}
+
+// Inverse of a unit: c[t] = Haar-domain coefficients (layout of forward_unit's
+// output, in the operand scale and before the butterfly factors are undone).
+// Out: xo[q][j] = pixel (y = 4 ((lane >> 4) & 1) + j, x = lane & 7) of patch
+// 4 q + 2 (lane >> 5) + ((lane >> 3) & 1).
+template <int FORM>
+__device__ inline void inverse_unit(const f32x4 (&c)[4], const Operand<FORM>& dctt, const Operand<FORM>& haart,
+                                    float* T, int lane, f32x4 (&xo)[4]) {
+    const f32x4 z0 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const f32x4 u = mma_exact_a<FORM>(haart, operand<FORM>(c[t]));
+        *reinterpret_cast<f32x4*>(T + mfma_slot(16 * t + (lane & 15), 4 * (lane >> 4))) = u;
+    }
+    wave_sync();
+    const int ky = lane & 7, pr = (lane >> 3) & 1, pc = lane >> 5, kx0 = 4 * ((lane >> 4) & 1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        f32x4 z;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) z[j] = T[mfma_slot(8 * ky + kx0 + j, 4 * q + 2 * pr + pc)];
+        const f32x4 v = mma<FORM>(operand<FORM>(z), dctt, z0);  // Z bd(D)
+        xo[q] = mma<FORM>(dctt, operand<FORM>(v), z0);          // bd(D^T) Z bd(D)
+    }
+    wave_sync();
+}
+
+// Lane's pixels of a unit in the A layout (zero for patches beyond a group's
+// count or groups beyond the tile): tile q, patch 4 q + 2 ((lane >> 3) & 1) +
+// (lane >> 5), row lane & 7, columns 4 ((lane >> 4) & 1) + j.
+template <int G>
+__device__ inline void load_unit(const float* plane, int stride, const MatchRec* recs, const int* cnt, int ngroups,
+                                 int unit, int lane, f32x4 (&x)[4]) {
+    const int pr = (lane >> 3) & 1, pc = lane >> 5, y = lane & 7, x0 = 4 * ((lane >> 4) & 1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int p = 4 * q + 2 * pr + pc;
+        const int g = G == 8 ? 2 * unit + (p >> 3) : unit, idx = p & (G - 1);
+        if (g < ngroups && idx < cnt[g]) {
+            const MatchRec r = recs[g * G + idx];
+            const float* src = plane + static_cast<size_t>(r.y + y) * stride + r.x + x0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[q][j] = src[j];
+        } else {
+            x[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+}
+
+__device__ inline float wave_max_f32(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Power-of-two scale taking max |x| of a group to [0.5, 1); 1 for a zero or
+// non-finite maximum.
+__device__ inline float group_scale(float m) {
+    if (!(m > 0.f) || !(m <= 3.4e38f)) return 1.f;
+    return ldexpf(1.f, -__builtin_amdgcn_frexp_expf(m));
+}
+
+// Per-group scales of a unit's pixels (fp16 form); s[0] covers tiles 0-1 and
+// s[1] tiles 2-3 (the two groups of 8), equal for a group of 16.
+template <int G, int FORM>
+__device__ inline void unit_scales(f32x4 (&x)[4], float (&s)[2]) {
+    if constexpr (FORM != kFormF16) {
+        s[0] = s[1] = 1.f;
+        return;
+    } else {
+        float m[2] = {0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) m[q >> 1] = fmaxf(m[q >> 1], fabsf(x[q][j]));
+        if constexpr (G == 16) m[0] = m[1] = fmaxf(m[0], m[1]);
+        s[0] = group_scale(wave_max_f32(m[0]));
+        s[1] = group_scale(wave_max_f32(m[1]));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] *= s[q >> 1];
+    }
+}
+
+template <int G, bool WIENER, int FORM>
+__global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) {
+    extern __shared__ __align__(16) unsigned long long acc[];
+    constexpr int B = 8;
+    static_assert(G == 8 || G == 16, "units of 16 patches");
+    const int lb = xcd_block();
+    const FilterJob& jb = tab.job[find_job(tab, lb)];
+    const TileGeom& tg = jb.tiles;
+    const int tile = lb - jb.wg_begin;
+    const int tx = tile % tg.ntx, ty = tile / tg.ntx;
+    const int aw = tg.aw, ah = tg.ah;
+    const int cw = aw - B + 1, ch = ah - B + 1;
+    const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
+    const int ay0 = tg.ay0(ty, jb.height, B, jb.grid.step, jb.range);
+    const int i_begin = tx * tg.tw, j_begin = ty * tg.th;
+    const int ni = min(tg.tw, jb.grid.nx - i_begin), nj = min(tg.th, jb.grid.ny - j_begin);
+    const int ngroups = ni * nj;
+    const int nunits = G == 8 ? (ngroups + 1) / 2 : ngroups;
+    unsigned long long* num = acc;
+    unsigned long long* corner = acc + aw * ah;
+    // (records, counts and the transpose buffers start 16-byte aligned)
+    MatchRec* recs = reinterpret_cast<MatchRec*>(acc + ((aw * ah + cw * ch + 1) & ~1));
+    int* cnt = reinterpret_cast<int*>(recs + tg.tw * tg.th * G);
+    float* scratch = reinterpret_cast<float*>(cnt + ((tg.tw * tg.th + 3) & ~3));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* T = scratch + wave * kMfmaBuf;
+    for (int e = threadIdx.x; e < aw * ah + cw * ch; e += 256) acc[e] = 0ull;
+    for (int e = threadIdx.x; e < ngroups * G; e += 256) {
+        const int g = e / G, j = e - g * G;
+        const int gj = g / ni, gi = g - gj * ni;
+        recs[e] = jb.matches[static_cast<size_t>((j_begin + gj) * jb.grid.nx + i_begin + gi) * G + j];
+    }
+    for (int g = threadIdx.x; g < ngroups; g += 256) {
+        const int gj = g / ni, gi = g - gj * ni;
+        cnt[g] = jb.counts[(j_begin + gj) * jb.grid.nx + i_begin + gi];
+    }
+    __syncthreads();
+
+    const Operand<FORM> dct = operand<FORM>(dct_bd_entries(false, lane));
+    const Operand<FORM> dctt = operand<FORM>(dct_bd_entries(true, lane));
+    const Operand<FORM> haar = operand<FORM>(haar_unit_entries<G>(false, lane));
+    const Operand<FORM> haart = operand<FORM>(haar_unit_entries<G>(true, lane));
+    const float thr = 2.7f * jb.sigma;  // include/nss/params.hpp:46 kBmHardLambda
+    const float sig2 = jb.sigma * jb.sigma;
+    // Haar rows of this lane in the coefficient layout: 4 (lane >> 4) + j; for
+    // two groups of 8 the group is lane >> 5.
+    const int hgroup = G == 8 ? lane >> 5 : 0;
+    float hs[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) hs[j] = haar_scale(G, (4 * (lane >> 4) + j) & (G - 1));
+
+    f32x4 x[4], xr[4];
+    int u = wave;
+    if (u < nunits) {
+        load_unit<G>(jb.src, jb.stride, recs, cnt, ngroups, u, lane, x);
+        if constexpr (WIENER) load_unit<G>(jb.wref, jb.wref_stride, recs, cnt, ngroups, u, lane, xr);
+    }
+    for (; u < nunits; u += 4) {
+        float s[2], sr[2];
+        unit_scales<G, FORM>(x, s);
+        f32x4 c[4], r[4];
+        forward_unit<FORM>(x, dct, haar, T, lane, c);
+        if constexpr (WIENER) {
+            unit_scales<G, FORM>(xr, sr);
+            forward_unit<FORM>(xr, dct, haar, T, lane, r);
+        }
+        const int un = u + 4;
+        if (un < nunits) {
+            load_unit<G>(jb.src, jb.stride, recs, cnt, ngroups, un, lane, x);
+            if constexpr (WIENER) load_unit<G>(jb.wref, jb.wref_stride, recs, cnt, ngroups, un, lane, xr);
+        }
+        // Shrink in FP32 on the orthonormal coefficients of the unscaled data.
+        const float inv_s = 1.f / s[hgroup];
+        int kept = 0;
+        float w2 = 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool dc = t == 0 && (lane & 15) == 0 && ((4 * (lane >> 4) + j) & (G - 1)) == 0;
+                float v = c[t][j] * hs[j] * inv_s;
+                if constexpr (WIENER) {
+                    const float rv = r[t][j] * hs[j] * (1.f / sr[hgroup]);
+                    float w = 1.f;
+                    if (!dc) {
+                        const float r2 = rv * rv;
+                        const float den = r2 + sig2;
+                        w = den > 0.f ? r2 / den : 0.f;
+                    }
+                    v *= w;
+                    w2 = fmaf(w, w, w2);
+                } else {
+                    if (!dc && fabsf(v) < thr)
+                        v = 0.f;
+                    else
+                        ++kept;
+                }
+                // back to the operand scale, butterfly factor applied again
+                // (the inverse orthonormal Haar is H^T diag(factors))
+                c[t][j] = v * hs[j] * s[hgroup];
+            }
+        float wg[2];
+        if constexpr (WIENER) {
+            float sum = w2;
+#pragma unroll
+            for (int off = 16; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+            if constexpr (G == 16) sum += __shfl_xor(sum, 32, 64);
+            wg[0] = 1.f / fmaxf(__shfl(sum, 0, 64), 1e-12f);
+            wg[1] = 1.f / fmaxf(__shfl(sum, 32, 64), 1e-12f);
+        } else {
+            int sum = kept;
+#pragma unroll
+            for (int off = 16; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+            if constexpr (G == 16) sum += __shfl_xor(sum, 32, 64);
+            wg[0] = 1.f / static_cast<float>(max(__shfl(sum, 0, 64), 1));
+            wg[1] = 1.f / static_cast<float>(max(__shfl(sum, 32, 64), 1));
+        }
+        f32x4 xo[4];
+        inverse_unit<FORM>(c, dctt, haart, T, lane, xo);
+        const int pr = lane >> 5, pc = (lane >> 3) & 1, px = lane & 7, y0 = 4 * ((lane >> 4) & 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int p = 4 * q + 2 * pr + pc;
+            const int g = G == 8 ? 2 * u + (p >> 3) : u, idx = p & (G - 1);
+            const int gsel = G == 8 ? (q >> 1) : 0;
+            if (g < ngroups && idx < cnt[g]) {
+                const MatchRec rec = recs[g * G + idx];
+                const int lx = rec.x - ax0, ly = rec.y - ay0;
+                const float w = wg[gsel];
+                const float wq = w / s[gsel];
+                unsigned long long* cell = num + (ly + y0) * aw + lx + px;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) lds_add(cell + j * aw, to_fixed(wq * xo[q][j]));
+                if (px == 0 && y0 == 0) lds_add(corner + ly * cw + lx, to_fixed(w));
+            }
+        }
+    }
+    __syncthreads();
+
+    for (int e = threadIdx.x; e < ch * aw; e += 256) {
+        const int yy = e / aw, xx = e - (e / aw) * aw;
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cx = xx - i;
+            if (cx >= 0 && cx < cw) sum += corner[yy * cw + cx];
+        }
+        scratch[e] = static_cast<float>(static_cast<double>(static_cast<long long>(sum)) * (1.0 / 4294967296.0));
+    }
+    __syncthreads();
+    float* pn = jb.part_num + static_cast<size_t>(tile) * aw * ah;
+    float* pd = jb.part_den + static_cast<size_t>(tile) * aw * ah;
+    for (int e = threadIdx.x; e < aw * ah; e += 256) {
+        const int yy = e / aw, xx = e - (e / aw) * aw;
+        float d = 0.f;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cy = yy - i;
+            if (cy >= 0 && cy < ch) d += scratch[cy * aw + xx];
+        }
+        pn[e] = static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
+        pd[e] = d;
+    }
+}
+
 // Atomic path: one wave per group, frame-sized fixed-point sums.
 template <int B, bool WIENER>
 __global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
@@ -1196,6 +1606,16 @@ size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t) {
     return acc + recs + (tbuf > boxsum ? tbuf : boxsum);
 }
 
+size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t) {
+    const size_t aw = t.aw, ah = t.ah;
+    const size_t acc = ((aw * ah + (aw - 7) * (ah - 7) + 1) & ~static_cast<size_t>(1)) * 8;
+    const size_t recs = static_cast<size_t>(t.tw) * t.th * group * sizeof(MatchRec) +
+                        ((static_cast<size_t>(t.tw) * t.th + 3) & ~static_cast<size_t>(3)) * 4;
+    const size_t tbuf = 4 * static_cast<size_t>(kMfmaBuf) * 4;
+    const size_t boxsum = (ah - 7) * aw * 4;
+    return acc + recs + (tbuf > boxsum ? tbuf : boxsum);
+}
+
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t) {
     const size_t aw = t.aw, ah = t.ah;
     const size_t acc = aw * ah * 8 + (aw - block + 1) * (ah - block + 1) * 8;
@@ -1277,9 +1697,29 @@ static hipError_t launch_filter_b8(const FilterTable& t, int total_wg, size_t ld
     return hipGetLastError();
 }
 
-// fast_group: 8, 16 or 32 selects the block-8 register kernel (tile path).
+template <int G, int FORM>
+static hipError_t launch_filter_mfma(const FilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
+    if (wiener) {
+        allow_lds(&filter_tile_mfma_kernel<G, true, FORM>, lds);
+        filter_tile_mfma_kernel<G, true, FORM><<<total_wg, 256, lds, s>>>(t);
+    } else {
+        allow_lds(&filter_tile_mfma_kernel<G, false, FORM>, lds);
+        filter_tile_mfma_kernel<G, false, FORM><<<total_wg, 256, lds, s>>>(t);
+    }
+    return hipGetLastError();
+}
+
+// fast_group: 8, 16 or 32 selects the block-8 register kernel (tile path);
+// form 1 (bf16x3) or 2 (fp16) with fast_group 8 or 16 the MFMA kernel.
 hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
-                         int fast_group, hipStream_t s) {
+                         int fast_group, int form, hipStream_t s) {
+    if (tiles && block == 8 && form) {
+        if (fast_group == 8) return form == 1 ? launch_filter_mfma<8, kFormBf16x3>(t, total_wg, lds, wiener, s)
+                                              : launch_filter_mfma<8, kFormF16>(t, total_wg, lds, wiener, s);
+        if (fast_group == 16) return form == 1 ? launch_filter_mfma<16, kFormBf16x3>(t, total_wg, lds, wiener, s)
+                                               : launch_filter_mfma<16, kFormF16>(t, total_wg, lds, wiener, s);
+        return hipErrorInvalidValue;
+    }
     if (tiles && block == 8 && fast_group) {
         switch (fast_group) {
             case 8: return launch_filter_b8<8>(t, total_wg, lds, wiener, s);

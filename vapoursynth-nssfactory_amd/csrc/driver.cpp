@@ -26,7 +26,7 @@ namespace nsship {
 
 hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s);
 hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
-                         int fast_group, hipStream_t s);
+                         int fast_group, int form, hipStream_t s);
 hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s);
 hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s);
 hipError_t upload_dct_tables(const float* host, size_t count);
@@ -35,6 +35,7 @@ size_t match_run_lds_bytes(int block, int step, int range);
 size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);
 size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t);
+size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t);
 size_t predictive_lds_bytes(int block, int group, int ps_num, int radius);
 hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s);
 hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
@@ -125,6 +126,7 @@ struct Bucket {
     TileGeom geom[3]{};  // per plane of the bucket (tile path)
     size_t filter_lds = 0;
     int fast_group = 0;  // block 8 with one group size in {8, 16, 32}: the register kernel
+    int form = 0;        // 1 bf16x3, 2 fp16: the MFMA kernel (block 8, groups of 8 or 16)
 };
 
 struct Stage {
@@ -242,6 +244,7 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
                 if (st.plane[bk.planes[k]].group != fast) fast = 0;
             if (fast != 8 && fast != 16 && fast != 32) fast = 0;
         }
+        const int form = (fast == 8 || fast == 16) ? h->p.transform : 0;
         static const int shapes[][2] = {{8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}};
         for (const auto& sh : shapes) {
             size_t lds = 0;
@@ -250,13 +253,15 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
                 const int i = bk.planes[k];
                 const StagePlane& sp = st.plane[i];
                 g[k] = make_tiles(make_grid(h->pw[i], h->ph[i], b, sp.step), sh[0], sh[1], b, sp.range);
-                lds = std::max(lds, fast ? filter_tile_b8_lds_bytes(sp.group, g[k])
-                                         : filter_tile_lds_bytes(b, sp.group, st.wiener, g[k]));
+                lds = std::max(lds, form   ? filter_tile_mfma_lds_bytes(sp.group, g[k])
+                                    : fast ? filter_tile_b8_lds_bytes(sp.group, g[k])
+                                           : filter_tile_lds_bytes(b, sp.group, st.wiener, g[k]));
             }
             const size_t budget = (sh[0] == 1 && sh[1] == 1) ? kLdsLimit : kTileLdsBudget;
             if (lds <= budget) {
                 bk.tiles = true;
                 bk.fast_group = fast;
+                bk.form = form;
                 bk.filter_lds = lds;
                 for (int k = 0; k < bk.nplanes; ++k) bk.geom[k] = g[k];
                 break;
@@ -424,7 +429,8 @@ int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const i
         e = traced(h, 1, s, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, s); });
         if (e != hipSuccess) return hip_fail(e, "match launch");
         e = traced(h, 2, s,
-                   [&] { return launch_filter(b, ft, fwg, bk.filter_lds, st.wiener, bk.tiles, bk.fast_group, s); });
+                   [&] { return launch_filter(b, ft, fwg, bk.filter_lds, st.wiener, bk.tiles, bk.fast_group,
+                                                bk.form, s); });
         if (e != hipSuccess) return hip_fail(e, "filter launch");
         e = traced(h, 3, s, [&] { return launch_finish(b, fn, nwg, bk.tiles, s); });
         if (e != hipSuccess) return hip_fail(e, "finish launch");

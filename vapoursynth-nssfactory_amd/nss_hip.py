@@ -22,7 +22,8 @@ LIB_PATH = os.path.join(HERE, "libnss_hip.so")
 NAMESPACE = "nss_hip"
 
 # The reference signature (src/frontend/bm3d_args.hpp:8-12) plus the device
-# backend tail (src/cuda/runtime/context.hpp:17).
+# backend tail (src/cuda/runtime/context.hpp:17) and this library's transform
+# argument (no counterpart in the reference: the MFMA forms).
 BM3D_SIGNATURE = (
     "clip:vnode;ref:vnode:opt;sigma:float[]:opt;block_size:int[]:opt;group_size:int[]:opt;"
     "block_step:int[]:opt;bm_range:int[]:opt;radius:int:opt;ps_num:int[]:opt;ps_range:int[]:opt;chroma:int:opt;"
@@ -30,7 +31,11 @@ BM3D_SIGNATURE = (
     "temporal_mode:data:opt;rolling_chunk:int:opt;rolling_cache_chunks:int:opt;rolling_cache_limit:int:opt;"
     "memory_limit_mb:int:opt;"
     "device_id:int:opt;num_streams:int:opt;"
+    "transform:int:opt;"
 )
+
+# Group-transform arithmetic (the transform argument).
+TRANSFORM_FP32, TRANSFORM_BF16X3, TRANSFORM_FP16 = 0, 1, 2
 
 OK, BYPASS, INVALID_INPUT, NUMERICAL_FAILURE, UNSUPPORTED, RESOURCE_FAILURE = range(6)
 
@@ -76,7 +81,7 @@ class Params(ctypes.Structure):
                 ("has_ref", ctypes.c_int), ("temporal_mode", ctypes.c_int), ("rolling_chunk", ctypes.c_int),
                 ("rolling_cache_chunks", ctypes.c_int), ("rolling_cache_limit", ctypes.c_int),
                 ("rolling_cache_adaptive", ctypes.c_int), ("memory_limit_mb", ctypes.c_int),
-                ("device_id", ctypes.c_int), ("num_streams", ctypes.c_int)]
+                ("device_id", ctypes.c_int), ("num_streams", ctypes.c_int), ("transform", ctypes.c_int)]
 
     def as_dict(self):
         out = {}
