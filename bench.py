@@ -128,6 +128,104 @@ def measure(nss_hip, dev, form, steps, warmup, dims):
             "kernels": kernels, "dominant": dom, "params": params, "out": [o.cpu().numpy() for o in out]}
 
 
+def temporal_workload(args):
+    """60 frames of the 1080p base shifted by a cumulative random translation
+    (step U{-4..4} px), fresh noise sigma 5, radius 1, two stages, rolling
+    state on the device. One step = the whole clip through push/pop."""
+    import numpy as np
+    import torch
+
+    import nss_hip
+    import synth
+    frames = 60
+    print(f"# generating {frames} frames {FMT} {W}x{H}", file=sys.stderr, flush=True)
+    _, seq = synth.shifted_sequence(FMT, W, H, frames, 5, seed=0, max_step=4)
+    dev = [[torch.from_numpy(p).cuda() for p in f] for f in seq]
+    out = {}
+    # The temporal group filter is FP32 only (the transform argument selects
+    # the spatial tile kernels), so one form is timed.
+    for name, code in (("fp32", 0),):
+        def one_pass():
+            inst = nss_hip.BM3D(FMT, W, H, frames, sigma=5, radius=1, final=1, transform=code)
+            n = len(inst.filter_clip(dev))
+            inst.close()
+            assert n == frames
+        for _ in range(args.warmup):
+            one_pass()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            one_pass()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[name] = {"value": round(W * H * frames * args.steps / wall / 1e6, 3),
+                     "ms_per_frame": round(wall / (frames * args.steps) * 1e3, 4)}
+        print(f"# temporal {name}: {out[name]}", file=sys.stderr, flush=True)
+    head = out["fp32"]
+    print(json.dumps({"metric": METRIC, "value": head["value"], "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": round(head["ms_per_frame"] * frames, 3),
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": FORM_DTYPE["fp32"], "data": "synthetic",
+                      "config": {"workload": "temporal two-stage BM3D, radius 1, sigma 5, 60-frame 1920x1080 "
+                                             "YUV420PS shifted sequence per step (instance created per pass)",
+                                 "form": "fp32"}, "forms": out}), flush=True)
+
+
+def precision_workload(args):
+    """3840x2160 GRAYS, sigma 20, two stages: each form's throughput, its max
+    and RMS error against the FP64 oracle and PSNR against the FP32 oracle."""
+    import numpy as np
+    import torch
+
+    import nss_hip
+    import oracle
+    import synth
+    w, h = 3840, 2160
+    clean, noisy = synth.noisy_frame("GRAYS", w, h, 20, seed=0)
+    dev = [torch.from_numpy(p).cuda() for p in noisy]
+    out = {}
+    outs = {}
+    for name, code in FORMS.items():
+        inst = nss_hip.BM3D("GRAYS", w, h, sigma=20, final=1, transform=code)
+        o = [torch.empty_like(p) for p in dev]
+        for _ in range(args.warmup):
+            inst(dev, out=o)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            inst(dev, out=o)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        params = inst.params
+        inst.close()
+        outs[name] = o[0].cpu().numpy()
+        out[name] = {"value": round(w * h * args.steps / wall / 1e6, 3), "ms_per_frame": round(wall / args.steps * 1e3, 4)}
+        print(f"# 4k {name}: {out[name]}", file=sys.stderr, flush=True)
+    print("# oracle fp32 and fp64 (single-threaded each)", file=sys.stderr, flush=True)
+    results = {}
+
+    def run(fp64):
+        results[fp64] = oracle.bm3d_frame(noisy, params, fp64=fp64)[0]
+    ts = [threading.Thread(target=run, args=(f,)) for f in (False, True)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for name in FORMS:
+        d = outs[name].astype(np.float64) - results[True]
+        out[name].update({"max_err_vs_fp64": float(np.max(np.abs(d))), "rms_err_vs_fp64": float(np.sqrt(np.mean(d * d))),
+                          "psnr_vs_fp32_oracle": round(oracle.psnr(outs[name], results[False]), 3)})
+    d = results[False].astype(np.float64) - results[True]
+    head = out[args.form]
+    print(json.dumps({"metric": METRIC, "value": head["value"], "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": head["ms_per_frame"], "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": FORM_DTYPE[args.form], "data": "synthetic",
+                      "config": {"workload": "two-stage BM3D, sigma 20, one 3840x2160 GRAYS frame per step",
+                                 "form": args.form},
+                      "forms": out, "oracle_fp32_vs_fp64": {"max_err": float(np.max(np.abs(d))),
+                                                            "rms_err": float(np.sqrt(np.mean(d * d)))}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +238,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy that reads and writes MIB MiB: the known-size kernel "
                          "that scales rocprof's FETCH_SIZE / WRITE_SIZE readings")
+    ap.add_argument("--workload", choices=["two-stage-1080p", "temporal-1080p", "precision-4k"],
+                    default="two-stage-1080p", help="two-stage-1080p is the headline; the others print their "
+                                                    "own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; each frame is independent, so N GPUs would each run this "
@@ -152,6 +253,10 @@ def main():
     import nss_hip
     import synth
 
+    if args.workload == "temporal-1080p":
+        return temporal_workload(args)
+    if args.workload == "precision-4k":
+        return precision_workload(args)
     if args.copy_calibration:
         n = args.copy_calibration * 1024 * 1024 // 4
         a = torch.ones(n, device="cuda")

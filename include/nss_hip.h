@@ -192,6 +192,51 @@ int nss_hip_bm3d_destroy(nss_hip_bm3d* handle);
 int nss_hip_spatial_match(const float* plane, int width, int height, int stride, int block, int step, int bm_range,
                           int group, int* out_rec, int* out_count, void* stream);
 
+/* ---- Serving loop ------------------------------------------------------
+ * No counterpart in the reference: its filters answer one VapourSynth frame
+ * request at a time (src/host/filter_bm3d.cpp bm3dGetFrame). This loop is
+ * for callers that feed frames of several clips as they arrive: a scheduler
+ * thread admits queued frames into device frame slots under an HBM budget,
+ * packs the radius-0 frames admitted so far into one launch sequence per
+ * stage round (planes of several clips, sizes and sigmas in the same
+ * kernels) while earlier batches still run, and replays a captured hipGraph
+ * for a batch whose composition was seen before. Radius > 0 clips keep their
+ * rolling state on the device and run on a HIP stream of their own. */
+typedef struct nss_hip_server nss_hip_server;
+
+typedef struct nss_hip_server_config {
+    int device_id;
+    long long hbm_budget_bytes; /* frame slots + rolling state; 0 = a quarter of the device */
+    int max_batch_frames;       /* radius-0 frames per launch sequence, 1..16 */
+    int max_inflight;           /* batches on the device at once, 1..4 */
+    int use_graphs;             /* capture / replay hipGraphs per batch composition */
+} nss_hip_server_config;
+
+typedef struct nss_hip_server_stats {
+    long long submitted, completed, batches, batched_frames, graph_replays, graph_captures;
+    long long bytes_in_use, peak_bytes, admissions_deferred;
+} nss_hip_server_stats;
+
+int nss_hip_server_create(const nss_hip_server_config* config, nss_hip_server** out);
+/* A clip stream with resolved parameters (nss_hip_bm3d_parse) and format; a
+ * radius > 0 stream needs info->num_frames and takes its frames in order.
+ * RESOURCE_FAILURE when its rolling state does not fit the budget. */
+int nss_hip_server_open_stream(nss_hip_server* server, const nss_hip_bm3d_params* params,
+                               const nss_hip_video_info* info, int* stream_id);
+/* Queues one frame (device planes, read until the request completes; dst
+ * written when it completes). sigma_user > 0 overrides the stream's sigma
+ * for this frame on every filtered plane and both stages (radius-0 streams);
+ * 0 keeps it. */
+int nss_hip_server_submit(nss_hip_server* server, int stream_id, const float* const* src, const int* src_stride,
+                          float* const* dst, const int* dst_stride, double sigma_user, long long* request_id);
+/* NSS_HIP_OK when the request's output is complete on the device (latency_ms:
+ * submit to completion seen by the scheduler), NSS_HIP_BYPASS while pending. */
+int nss_hip_server_query(nss_hip_server* server, long long request_id, double* latency_ms);
+/* Blocks until every submitted request has completed. */
+int nss_hip_server_drain(nss_hip_server* server);
+int nss_hip_server_get_stats(nss_hip_server* server, nss_hip_server_stats* out);
+int nss_hip_server_destroy(nss_hip_server* server);
+
 #ifdef __cplusplus
 }
 #endif

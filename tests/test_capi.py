@@ -43,3 +43,20 @@ def test_null_arguments_are_errors():
     assert lib.nss_hip_bm3d_create(None, None, 0, None) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_bm3d_destroy(None) == nss_hip.OK
     assert lib.nss_hip_bm3d_set_tracing(None, 1) == nss_hip.INVALID_INPUT
+
+
+def test_server_config_validation_without_device():
+    """Configuration errors are reported before any device call."""
+    import nss_hip
+    lib = nss_hip.lib()
+    h = ctypes.c_void_p()
+    for cfg, msg in (((0, 0, 0, 1, 1), "max_batch_frames must be in [1, 16]"),
+                     ((0, 0, 17, 1, 1), "max_batch_frames must be in [1, 16]"),
+                     ((0, 0, 4, 0, 1), "max_inflight must be in [1, 4]"),
+                     ((0, -1, 4, 2, 1), "hbm_budget_bytes must be >= 0")):
+        c = nss_hip.ServerConfig(*cfg)
+        assert lib.nss_hip_server_create(ctypes.byref(c), ctypes.byref(h)) == nss_hip.INVALID_INPUT
+        assert msg in nss_hip.last_error()
+    assert lib.nss_hip_server_create(None, ctypes.byref(h)) == nss_hip.INVALID_INPUT
+    assert lib.nss_hip_server_destroy(None) == nss_hip.OK
+    assert lib.nss_hip_server_drain(None) == nss_hip.INVALID_INPUT

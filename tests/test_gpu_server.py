@@ -1,0 +1,153 @@
+"""Serving loop (nss_hip_server_*): frames of several clip streams queued at
+once are packed into shared launches; each result must be bitwise the one
+the stream's own filter instance gives for that frame alone (same kernels,
+same per-frame arithmetic), including per-request sigma, radius-1 streams,
+graph replays and admission under a tight HBM budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _alone(gpu, fmt, w, h, planes, sigma, **kw):
+    import torch
+    inst = gpu.BM3D(fmt, w, h, sigma=sigma, **kw)
+    out = inst([_dev(p) for p in planes])
+    torch.cuda.synchronize()
+    res = [o.cpu().numpy() for o in out]
+    inst.close()
+    return res
+
+
+STREAMS = [("YUV420PS", 96, 64, dict(final=1)), ("YUV420PS", 128, 96, dict(final=1)),
+           ("RGBS", 64, 48, dict(final=1)), ("GRAYS", 80, 56, dict(final=0, group_size=16, block_step=4))]
+
+
+def _frames(fmt, w, h, n, seed):
+    import synth
+    return [synth.noisy_frame(fmt, w, h, 10, seed=seed + k)[1] for k in range(n)]
+
+
+@pytest.mark.parametrize("use_graphs", [False, True])
+def test_mix_equals_alone(gpu, use_graphs):
+    import torch
+    srv = gpu.Server(max_batch_frames=6, max_inflight=2, use_graphs=use_graphs)
+    sids = [srv.open_stream(fmt, w, h, sigma=10, **kw) for fmt, w, h, kw in STREAMS]
+    rng = np.random.default_rng(5)
+    reqs = []
+    for rep in range(3):
+        for s, (fmt, w, h, kw) in enumerate(STREAMS):
+            for planes in _frames(fmt, w, h, 2, seed=10 * s + rep):
+                sig = float(rng.choice([3, 5, 10, 20]))
+                src = [_dev(p) for p in planes]
+                out = [torch.empty_like(p) for p in src]
+                torch.cuda.synchronize()
+                reqs.append((srv.submit(sids[s], src, out, sigma=sig), s, planes, sig, src, out))
+    srv.drain()
+    st = srv.stats()
+    print(f"server stats {st}", flush=True)
+    assert st["completed"] == len(reqs) and st["batches"] < len(reqs)
+    for rid, s, planes, sig, src, out in reqs:
+        assert srv.query(rid) is not None
+        fmt, w, h, kw = STREAMS[s]
+        want = _alone(gpu, fmt, w, h, planes, sig, **kw)
+        for g, w_ in zip(out, want):
+            assert np.array_equal(g.cpu().numpy(), w_), (s, sig)
+    srv.close()
+
+
+def test_graph_replays_match_eager(gpu):
+    """The same composition replayed several times, with eager work and
+    synchronisations between the replays; every replay equals the eager
+    (no-graph) result."""
+    import torch
+    fmt, w, h, kw = STREAMS[0]
+    planes = _frames(fmt, w, h, 2, seed=3)
+    src = [[_dev(p) for p in f] for f in planes]
+    outs = {}
+    for use_graphs in (False, True):
+        srv = gpu.Server(max_batch_frames=2, max_inflight=1, use_graphs=use_graphs)
+        sid = srv.open_stream(fmt, w, h, sigma=10, **kw)
+        got = []
+        for rep in range(4):
+            out = [[torch.empty_like(p) for p in f] for f in src]
+            rids = [srv.submit(sid, src[k], out[k], sigma=5.0) for k in range(2)]
+            srv.drain()
+            assert all(srv.query(r) is not None for r in rids)
+            got.append([[p.cpu().numpy() for p in f] for f in out])
+            x = torch.randn(1 << 20, device="cuda")
+            (x * 2).sum().item()
+            torch.cuda.synchronize()
+        outs[use_graphs] = got
+        st = srv.stats()
+        print(f"use_graphs={use_graphs} stats {st}", flush=True)
+        if use_graphs:
+            assert st["graph_replays"] >= 1
+        srv.close()
+    for rep in range(4):
+        for fe, fg in zip(outs[False][rep], outs[True][rep]):
+            for a, b in zip(fe, fg):
+                assert np.array_equal(a, b), rep
+
+
+def test_temporal_stream_in_mix(gpu):
+    import torch
+    import synth
+    _, clip = synth.shifted_sequence("YUV420PS", 64, 48, 5, 10, seed=2, max_step=2)
+    inst = gpu.BM3D("YUV420PS", 64, 48, 5, sigma=10, radius=1, final=1)
+    want = [[p.cpu().numpy() for p in f] for f in inst.filter_clip([[_dev(p) for p in f] for f in clip])]
+    inst.close()
+    srv = gpu.Server(max_batch_frames=4)
+    tid = srv.open_stream("YUV420PS", 64, 48, 5, sigma=10, radius=1, final=1)
+    sid = srv.open_stream("GRAYS", 72, 56, sigma=10)
+    reqs, side = [], []
+    for n, f in enumerate(clip):
+        src = [_dev(p) for p in f]
+        out = [torch.empty_like(p) for p in src]
+        g = _dev(synth.noisy_frame("GRAYS", 72, 56, 10, seed=n)[1][0])
+        go = torch.empty_like(g)
+        torch.cuda.synchronize()
+        reqs.append((srv.submit(tid, src, out), src, out))
+        side.append((srv.submit(sid, [g], [go]), g, go))
+    srv.drain()
+    for n, (rid, src, out) in enumerate(reqs):
+        assert srv.query(rid) is not None
+        for g, w_ in zip(out, want[n]):
+            assert np.array_equal(g.cpu().numpy(), w_), n
+    srv.close()
+
+
+def test_budget_admission(gpu):
+    """A budget that holds one frame slot per stream: frames wait for slots
+    (deferred admissions), all complete, the budget is never exceeded."""
+    import torch
+    probe = gpu.Server()
+    a = probe.open_stream("YUV420PS", 128, 96, sigma=10, final=1)
+    b = probe.open_stream("GRAYS", 96, 64, sigma=10)
+    base = probe.stats()["bytes_in_use"]
+    probe.close()
+    srv = gpu.Server(hbm_budget_bytes=base, max_batch_frames=8, max_inflight=2)
+    a = srv.open_stream("YUV420PS", 128, 96, sigma=10, final=1)
+    b = srv.open_stream("GRAYS", 96, 64, sigma=10)
+    reqs = []
+    for k in range(6):
+        for sid, fmt, w, h in ((a, "YUV420PS", 128, 96), (b, "GRAYS", 96, 64)):
+            planes = _frames(fmt, w, h, 1, seed=k)[0]
+            src = [_dev(p) for p in planes]
+            out = [torch.empty_like(p) for p in src]
+            torch.cuda.synchronize()
+            reqs.append((srv.submit(sid, src, out), fmt, w, h, planes, out))
+    srv.drain()
+    st = srv.stats()
+    print(f"budget stats {st}", flush=True)
+    assert st["completed"] == len(reqs) and st["peak_bytes"] <= base
+    for rid, fmt, w, h, planes, out in reqs:
+        want = _alone(gpu, fmt, w, h, planes, 10, **({"final": 1} if fmt == "YUV420PS" else {}))
+        for g, w_ in zip(out, want):
+            assert np.array_equal(g.cpu().numpy(), w_)
+    srv.close()

@@ -10,7 +10,7 @@
 
 namespace nsship {
 
-inline constexpr int kMaxJobs = 8;
+inline constexpr int kMaxJobs = 16;
 inline constexpr int kMaxGroup = 64;   // include/nss/params.hpp:39 kBmMaxGroup
 inline constexpr int kMaxRadius = 16;  // include/nss/params.hpp:44 kBmMaxRadius
 inline constexpr int kMaxSlots = 2 * kMaxRadius + 1;
@@ -56,6 +56,8 @@ struct TileGeom {
     int tw, th;        // references per tile
     int ntx, nty;      // tiles
     int aw, ah;        // accumulation rectangle
+    int pitch;         // row pitch of a tile's float partials: a multiple of 4,
+                       // >= aw + 6; column x - ax0 + (ax0 & 3) holds pixel x
     __host__ __device__ int ax0(int tx, int width, int block, int step, int range) const {
         const int x = tx * tw * step;
         return (x < width - block ? x : width - block) - range;

@@ -522,6 +522,43 @@ __device__ inline void lds_add(unsigned long long* p, unsigned long long v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Tile epilogue: den = B x B box sum of the corner weights (horizontal pass
+// into scratch, then vertical), and num, as the tile's float partials (rows of
+// tg.pitch floats, pixel x at column x - ax0 + (ax0 & 3), so that x = 0 mod 4
+// is 16-byte aligned in every tile). Called by all 256 threads after the
+// accumulation's barrier.
+template <int B>
+__device__ inline void tile_partials(const FilterJob& jb, int tile, int ax0, const unsigned long long* num,
+                                     const unsigned long long* corner, float* scratch) {
+    const int aw = jb.tiles.aw, ah = jb.tiles.ah, pitch = jb.tiles.pitch;
+    const int cw = aw - B + 1, ch = ah - B + 1;
+    for (int e = threadIdx.x; e < ch * aw; e += 256) {
+        const int y = e / aw, x = e - (e / aw) * aw;
+        unsigned long long s = 0;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cx = x - i;
+            if (cx >= 0 && cx < cw) s += corner[y * cw + cx];
+        }
+        scratch[e] = static_cast<float>(static_cast<double>(static_cast<long long>(s)) * (1.0 / 4294967296.0));
+    }
+    __syncthreads();
+    const size_t base = static_cast<size_t>(tile) * pitch * ah + (ax0 & 3);
+    float* pn = jb.part_num + base;
+    float* pd = jb.part_den + base;
+    for (int e = threadIdx.x; e < aw * ah; e += 256) {
+        const int y = e / aw, x = e - (e / aw) * aw;
+        float d = 0.f;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cy = y - i;
+            if (cy >= 0 && cy < ch) d += scratch[cy * aw + x];
+        }
+        pn[y * pitch + x] = static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
+        pd[y * pitch + x] = d;
+    }
+}
+
 // Tile path: sums of a tile of groups in LDS, then float partials.
 // LDS: num [ah][aw] int64 | den corners [ah-B+1][aw-B+1] int64 | per-wave
 // cubes (reused after filtering for the den box sum).
@@ -570,33 +607,7 @@ __global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
         wave_sync();
     }
     __syncthreads();
-
-    // den at each pixel: the B x B box sum of the corner weights ending there.
-    // Horizontal pass over the corner rows into scratch, then vertical.
-    for (int e = threadIdx.x; e < ch * aw; e += 256) {
-        const int y = e / aw, x = e - (e / aw) * aw;
-        unsigned long long s = 0;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int cx = x - i;
-            if (cx >= 0 && cx < cw) s += corner[y * cw + cx];
-        }
-        scratch[e] = static_cast<float>(static_cast<double>(static_cast<long long>(s)) * (1.0 / 4294967296.0));
-    }
-    __syncthreads();
-    float* pn = jb.part_num + static_cast<size_t>(tile) * aw * ah;
-    float* pd = jb.part_den + static_cast<size_t>(tile) * aw * ah;
-    for (int e = threadIdx.x; e < aw * ah; e += 256) {
-        const int y = e / aw, x = e - (e / aw) * aw;
-        float d = 0.f;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int cy = y - i;
-            if (cy >= 0 && cy < ch) d += scratch[cy * aw + x];
-        }
-        pn[e] = static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
-        pd[e] = d;
-    }
+    tile_partials<B>(jb, tile, ax0, num, corner, scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -841,31 +852,7 @@ __global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
         }
     }
     __syncthreads();
-
-    for (int e = threadIdx.x; e < ch * aw; e += 256) {
-        const int yy = e / aw, xx = e - (e / aw) * aw;
-        unsigned long long s = 0;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int cx = xx - i;
-            if (cx >= 0 && cx < cw) s += corner[yy * cw + cx];
-        }
-        scratch[e] = static_cast<float>(static_cast<double>(static_cast<long long>(s)) * (1.0 / 4294967296.0));
-    }
-    __syncthreads();
-    float* pn = jb.part_num + static_cast<size_t>(tile) * aw * ah;
-    float* pd = jb.part_den + static_cast<size_t>(tile) * aw * ah;
-    for (int e = threadIdx.x; e < aw * ah; e += 256) {
-        const int yy = e / aw, xx = e - (e / aw) * aw;
-        float d = 0.f;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int cy = yy - i;
-            if (cy >= 0 && cy < ch) d += scratch[cy * aw + xx];
-        }
-        pn[e] = static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
-        pd[e] = d;
-    }
+    tile_partials<B>(jb, tile, ax0, num, corner, scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -1251,31 +1238,7 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
         }
     }
     __syncthreads();
-
-    for (int e = threadIdx.x; e < ch * aw; e += 256) {
-        const int yy = e / aw, xx = e - (e / aw) * aw;
-        unsigned long long sum = 0;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int cx = xx - i;
-            if (cx >= 0 && cx < cw) sum += corner[yy * cw + cx];
-        }
-        scratch[e] = static_cast<float>(static_cast<double>(static_cast<long long>(sum)) * (1.0 / 4294967296.0));
-    }
-    __syncthreads();
-    float* pn = jb.part_num + static_cast<size_t>(tile) * aw * ah;
-    float* pd = jb.part_den + static_cast<size_t>(tile) * aw * ah;
-    for (int e = threadIdx.x; e < aw * ah; e += 256) {
-        const int yy = e / aw, xx = e - (e / aw) * aw;
-        float d = 0.f;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int cy = yy - i;
-            if (cy >= 0 && cy < ch) d += scratch[cy * aw + xx];
-        }
-        pn[e] = static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
-        pd[e] = d;
-    }
+    tile_partials<B>(jb, tile, ax0, num, corner, scratch);
 }
 
 // Atomic path: one wave per group, frame-sized fixed-point sums.
@@ -1493,33 +1456,49 @@ __global__ __launch_bounds__(64) void filter_temporal_kernel(TFilterTable tab) {
 // Finish (src/cpu/bm/agg.cpp:17-42 AggFinish): dst = den > 1e-12 ? num/den : src.
 // ---------------------------------------------------------------------------
 
-// Tile path: per pixel, the partials of every tile whose rectangle covers it.
+// Tile path: 64 x 16 pixels per block, four consecutive pixels of a row per
+// thread; per pixel, the partials of every tile whose rectangle covers it
+// (tiles in descending (ty, tx) order), read as one aligned 16-byte vector
+// per tile for the four pixels.
 __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
     const FinishJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
     const int t = static_cast<int>(blockIdx.x) - jb.wg_begin;
-    const int x = (t % jb.tiles_x) * 16 + (threadIdx.x & 15);
+    const int x0 = (t % jb.tiles_x) * 64 + (threadIdx.x & 15) * 4;
     const int y = (t / jb.tiles_x) * 16 + (threadIdx.x >> 4);
-    if (x >= jb.width || y >= jb.height) return;
+    if (x0 >= jb.width || y >= jb.height) return;
     const TileGeom& tg = jb.tiles;
     const int B = jb.block, s = jb.step, r = jb.range;
-    int txh = min(tg.ntx - 1, (x + r) / (tg.tw * s));
-    while (txh + 1 < tg.ntx && tg.ax0(txh + 1, jb.width, B, s, r) <= x) ++txh;
+    const int xr = min(x0 + 3, jb.width - 1);
+    int txh = min(tg.ntx - 1, (xr + r) / (tg.tw * s));
+    while (txh + 1 < tg.ntx && tg.ax0(txh + 1, jb.width, B, s, r) <= xr) ++txh;
     int tyh = min(tg.nty - 1, (y + r) / (tg.th * s));
     while (tyh + 1 < tg.nty && tg.ay0(tyh + 1, jb.height, B, s, r) <= y) ++tyh;
-    float n = 0.f, d = 0.f;
-    const size_t area = static_cast<size_t>(tg.aw) * tg.ah;
+    f32x4 n = {0.f, 0.f, 0.f, 0.f}, d = {0.f, 0.f, 0.f, 0.f};
+    const size_t area = static_cast<size_t>(tg.pitch) * tg.ah;
     for (int ty = tyh; ty >= 0; --ty) {
         const int oy = y - tg.ay0(ty, jb.height, B, s, r);
         if (oy >= tg.ah) break;
         for (int tx = txh; tx >= 0; --tx) {
-            const int ox = x - tg.ax0(tx, jb.width, B, s, r);
-            if (ox >= tg.aw) break;
-            const size_t o = (static_cast<size_t>(ty) * tg.ntx + tx) * area + static_cast<size_t>(oy) * tg.aw + ox;
-            n += jb.part_num[o];
-            d += jb.part_den[o];
+            const int ax0 = tg.ax0(tx, jb.width, B, s, r);
+            if (ax0 + tg.aw <= x0) break;
+            // column of x0: a multiple of 4 in [0, pitch - 4]
+            const size_t o = (static_cast<size_t>(ty) * tg.ntx + tx) * area + static_cast<size_t>(oy) * tg.pitch +
+                             (x0 - ax0 + (ax0 & 3));
+            const f32x4 pn = *reinterpret_cast<const f32x4*>(jb.part_num + o);
+            const f32x4 pd = *reinterpret_cast<const f32x4*>(jb.part_den + o);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bool in = static_cast<unsigned>(x0 + i - ax0) < static_cast<unsigned>(tg.aw);
+                n[i] += in ? pn[i] : 0.f;
+                d[i] += in ? pd[i] : 0.f;
+            }
         }
     }
-    jb.dst[static_cast<size_t>(y) * jb.dstride + x] = d > 1e-12f ? n / d : jb.src[static_cast<size_t>(y) * jb.sstride + x];
+    float* dst = jb.dst + static_cast<size_t>(y) * jb.dstride;
+    const float* src = jb.src + static_cast<size_t>(y) * jb.sstride;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (x0 + i < jb.width) dst[x0 + i] = d[i] > 1e-12f ? n[i] / d[i] : src[x0 + i];
 }
 
 // Atomic path: den kept at each patch's top-left corner, box-summed here.

@@ -19,27 +19,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/nss_hip.h"
-#include "bm3d_jobs.hpp"
+#include "instance.hpp"
 
 namespace nsship {
-
-hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s);
-hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
-                         int fast_group, int form, hipStream_t s);
-hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s);
-hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s);
-hipError_t upload_dct_tables(const float* host, size_t count);
-size_t match_wave_lds_bytes(int block, int range, bool stage);
-size_t match_run_lds_bytes(int block, int step, int range);
-size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
-size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);
-size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t);
-size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t);
-size_t predictive_lds_bytes(int block, int group, int ps_num, int radius);
-hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s);
-hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
-                                  hipStream_t s);
 
 thread_local std::string g_last_error;
 
@@ -51,25 +33,6 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what) {
     return fail(NSS_HIP_RESOURCE_FAILURE, std::string("nss_hip: ") + what + ": " + hipGetErrorString(e));
 }
-
-// Restores the caller's current device on scope exit.
-struct DeviceGuard {
-    int prev = -1;
-    hipError_t err = hipSuccess;
-    explicit DeviceGuard(int device) {
-        err = hipGetDevice(&prev);
-        if (err == hipSuccess && prev != device) err = hipSetDevice(device);
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-constexpr size_t kLdsLimit = 160 * 1024;  // LDS per workgroup on gfx950
-// Tile aggregation aims at two workgroups per CU; a one-reference tile may
-// take the whole LDS.
-constexpr size_t kTileLdsBudget = 76 * 1024;
-constexpr size_t kRunLdsBudget = 64 * 1024;
 
 // Per device, once: the DCT matrices in constant memory.
 std::mutex g_init_mutex;
@@ -106,35 +69,10 @@ TileGeom make_tiles(const Grid& g, int tw, int th, int block, int range) {
     t.ntx = (g.nx + tw - 1) / tw;
     t.nty = (g.ny + th - 1) / th;
     t.aw = (tw - 1) * g.step + 2 * range + block;
+    t.pitch = (t.aw + 6 + 3) & ~3;
     t.ah = (th - 1) * g.step + 2 * range + block;
     return t;
 }
-
-struct StagePlane {
-    float sigma;
-    int block, group, step, range, ps_num, ps_range;
-};
-
-// Launch plan of one block-size bucket of a stage.
-struct Bucket {
-    int block = 0;
-    int planes[3]{};
-    int nplanes = 0;
-    int match_kind = 0;  // 2 run, 1 wave + LDS window, 0 wave + global window
-    size_t match_lds = 0;
-    bool tiles = false;  // tile aggregation, else frame-sized atomics
-    TileGeom geom[3]{};  // per plane of the bucket (tile path)
-    size_t filter_lds = 0;
-    int fast_group = 0;  // block 8 with one group size in {8, 16, 32}: the register kernel
-    int form = 0;        // 1 bf16x3, 2 fp16: the MFMA kernel (block 8, groups of 8 or 16)
-};
-
-struct Stage {
-    StagePlane plane[3];
-    bool wiener;
-    Bucket bucket[7];
-    int nbuckets;
-};
 
 bool allowed_block(int b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 12 || b == 16 || b == 32; }
 bool allowed_group(int g) { return g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64; }
@@ -142,64 +80,6 @@ bool allowed_group(int g) { return g == 1 || g == 2 || g == 4 || g == 8 || g == 
 }  // namespace nsship
 
 using namespace nsship;
-
-// Event pairs around launches while tracing is on (one vector per kind).
-struct Tracer {
-    bool on = false;
-    std::vector<hipEvent_t> pool;
-    size_t used = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[NSS_HIP_KERNEL_KINDS];
-
-    hipEvent_t take() {
-        if (used == pool.size()) {
-            hipEvent_t e = nullptr;
-            if (hipEventCreate(&e) != hipSuccess) return nullptr;
-            pool.push_back(e);
-        }
-        return pool[used++];
-    }
-    void reset() {
-        used = 0;
-        for (auto& v : spans) v.clear();
-    }
-    ~Tracer() {
-        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
-    }
-};
-
-struct nss_hip_bm3d {
-    nss_hip_bm3d_params p{};
-    nss_hip_video_info vi{};
-    Tracer tracer;
-    int device = 0;
-    int nplanes = 0;
-    int pw[3]{}, ph[3]{};
-    Stage stages[2]{};
-    int nstages = 0;
-    unsigned long long* num[3]{};  // atomic path sums
-    unsigned long long* den[3]{};
-    float* part[3]{};              // tile path partials: num then den
-    size_t part_floats[3]{};       // per half (num or den)
-    MatchRec* matches[3]{};
-    int* counts[3]{};
-    float* basic[3]{};
-    // Rolling temporal state (radius > 0): frames enter in clip order and
-    // stay on the device in rings (frame f at slot f % ring); each stage keeps
-    // frame-sized int64 sums for the 2r+1 frames its current centre touches.
-    struct Rolling {
-        int N = 0, r = 0, pushed = 0;
-        int src_n = 0, ref_n = 0, basic_n = 0, out_n = 0, acc_n = 0;
-        float* src[3]{};
-        float* ref[3]{};
-        float* basic[3]{};
-        float* out[3]{};
-        unsigned long long* num[2][3]{};
-        unsigned long long* den[2][3]{};
-        int next_center[2]{};
-        int finished[2]{};
-        int out_head = 0;
-    } roll;
-};
 
 namespace {
 
@@ -281,162 +161,407 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
 }
 
 void free_buffers(nss_hip_bm3d* h) {
+    free_frame_buffers(h->buf);
     auto& R = h->roll;
     for (int i = 0; i < 3; ++i) {
-        for (void* p : {static_cast<void*>(h->num[i]), static_cast<void*>(h->den[i]), static_cast<void*>(h->part[i]),
-                        static_cast<void*>(h->matches[i]), static_cast<void*>(h->counts[i]),
-                        static_cast<void*>(h->basic[i]), static_cast<void*>(R.src[i]), static_cast<void*>(R.ref[i]),
-                        static_cast<void*>(R.basic[i]), static_cast<void*>(R.out[i]),
-                        static_cast<void*>(R.num[0][i]), static_cast<void*>(R.den[0][i]),
+        for (void* p : {static_cast<void*>(R.src[i]), static_cast<void*>(R.ref[i]), static_cast<void*>(R.basic[i]),
+                        static_cast<void*>(R.out[i]), static_cast<void*>(R.num[0][i]), static_cast<void*>(R.den[0][i]),
                         static_cast<void*>(R.num[1][i]), static_cast<void*>(R.den[1][i])})
             if (p) (void)hipFree(p);
-        h->num[i] = h->den[i] = nullptr;
-        h->part[i] = nullptr;
-        h->matches[i] = nullptr;
-        h->counts[i] = nullptr;
-        h->basic[i] = nullptr;
         R.src[i] = R.ref[i] = R.basic[i] = R.out[i] = nullptr;
         R.num[0][i] = R.den[0][i] = R.num[1][i] = R.den[1][i] = nullptr;
     }
 }
 
+// Per-plane buffer needs of an instance: match records and counts (the
+// largest stage), tile partial floats per half, frame-sized sums (atomic
+// path), basic estimate.
+struct PlaneNeeds {
+    size_t recs = 0, nref = 0, part = 0;
+    bool atomic = false, basic = false;
+};
+
+PlaneNeeds plane_needs(const nss_hip_bm3d* h, int i) {
+    PlaneNeeds n;
+    const nss_hip_bm3d_params& p = h->p;
+    const bool rolling = p.radius > 0;
+    for (int s = 0; s < h->nstages; ++s) {
+        const Stage& st = h->stages[s];
+        const StagePlane& sp = st.plane[i];
+        bool any = false;
+        for (int q = 0; q < h->nplanes; ++q) any |= st.plane[q].sigma != 0.f;
+        // With chroma, plane 0 holds the shared groups whenever any plane is filtered.
+        const bool matched = p.chroma ? (i == 0 && any) : sp.sigma != 0.f;
+        if (matched) {
+            const Grid g = make_grid(h->pw[i], h->ph[i], sp.block, sp.step);
+            const size_t nref = static_cast<size_t>(g.nx) * g.ny;
+            n.recs = std::max(n.recs, nref * sp.group);
+            n.nref = std::max(n.nref, nref);
+        }
+        if (sp.sigma == 0.f || rolling) continue;
+        for (int bi = 0; bi < st.nbuckets; ++bi)
+            for (int k = 0; k < st.bucket[bi].nplanes; ++k) {
+                if (st.bucket[bi].planes[k] != i) continue;
+                if (st.bucket[bi].tiles) {
+                    const TileGeom& tg = st.bucket[bi].geom[k];
+                    n.part = std::max(n.part, static_cast<size_t>(tg.ntx) * tg.nty * tg.pitch * tg.ah);
+                } else {
+                    n.atomic = true;
+                }
+            }
+    }
+    n.basic = p.final_stage && !rolling;
+    return n;
+}
+
+}  // namespace
+
+namespace nsship {
+
+size_t frame_buffer_bytes(const nss_hip_bm3d* h) {
+    size_t total = 0;
+    for (int i = 0; i < h->nplanes; ++i) {
+        const PlaneNeeds n = plane_needs(h, i);
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        total += n.recs * sizeof(MatchRec) + n.nref * sizeof(int) + 2 * n.part * sizeof(float) +
+                 (n.atomic ? 16 * npx : 0) + (n.basic ? 4 * npx : 0);
+    }
+    return total;
+}
+
+size_t instance_bytes(const nss_hip_bm3d* h) {
+    size_t total = frame_buffer_bytes(h);
+    const auto& R = h->roll;
+    for (int i = 0; i < h->nplanes; ++i) {
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        total += npx * 4 * static_cast<size_t>(R.src_n + R.ref_n + R.basic_n + R.out_n);
+        for (int s = 0; s < h->nstages; ++s)
+            if (R.num[s][i]) total += npx * 16 * static_cast<size_t>(R.acc_n);
+    }
+    return total;
+}
+
+void free_frame_buffers(FrameBuffers& fb) {
+    for (int i = 0; i < 3; ++i) {
+        for (void* p : {static_cast<void*>(fb.num[i]), static_cast<void*>(fb.den[i]), static_cast<void*>(fb.part[i]),
+                        static_cast<void*>(fb.matches[i]), static_cast<void*>(fb.counts[i]),
+                        static_cast<void*>(fb.basic[i])})
+            if (p) (void)hipFree(p);
+    }
+    fb = FrameBuffers{};
+}
+
+hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb) {
+    fb = FrameBuffers{};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < h->nplanes && e == hipSuccess; ++i) {
+        const PlaneNeeds n = plane_needs(h, i);
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        if (n.recs) {
+            e = hipMalloc(&fb.matches[i], n.recs * sizeof(MatchRec));
+            if (e == hipSuccess) e = hipMalloc(&fb.counts[i], n.nref * sizeof(int));
+        }
+        if (e == hipSuccess && n.atomic) {
+            e = hipMalloc(&fb.num[i], npx * 8);
+            if (e == hipSuccess) e = hipMalloc(&fb.den[i], npx * 8);
+        }
+        if (e == hipSuccess && n.part) e = hipMalloc(&fb.part[i], 2 * n.part * sizeof(float));
+        if (e == hipSuccess && n.basic) e = hipMalloc(&fb.basic[i], npx * 4);
+    }
+    if (e != hipSuccess) free_frame_buffers(fb);
+    return e;
+}
+
+}  // namespace nsship
+
+namespace {
+
 // Event-traced launch on stream s (kind: 0 clear, 1 match, 2 filter, 3 finish).
 template <typename F>
-hipError_t traced(nss_hip_bm3d* h, int kind, hipStream_t s, F&& launch) {
+hipError_t traced(Tracer* tr, int kind, hipStream_t s, F&& launch) {
     hipEvent_t a = nullptr, z = nullptr;
-    if (h->tracer.on) {
-        a = h->tracer.take();
-        z = h->tracer.take();
+    if (tr && tr->on) {
+        a = tr->take();
+        z = tr->take();
         if (a) (void)hipEventRecord(a, s);
     }
     const hipError_t err = launch();
     if (a && z) {
         (void)hipEventRecord(z, s);
-        h->tracer.spans[kind].emplace_back(a, z);
+        tr->spans[kind].emplace_back(a, z);
     }
     return err;
 }
 
-// One stage: match on `match`, filter `src` (Wiener reference `wref` when the
-// stage is Wiener), finish into `dst`.
-int run_stage(nss_hip_bm3d* h, const Stage& st, const float* const* src, const int* src_stride,
-              const float* const* match, const int* match_stride, const float* const* wref, const int* wref_stride,
-              float* const* dst, const int* dst_stride, hipStream_t s) {
-    for (int i = 0; i < h->nplanes; ++i) {
-        if (st.plane[i].sigma != 0.f) continue;
-        const hipError_t e = hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, src[i],
-                                              static_cast<size_t>(src_stride[i]) * 4, static_cast<size_t>(h->pw[i]) * 4,
-                                              h->ph[i], hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return hip_fail(e, "plane copy");
-    }
-    for (int bi = 0; bi < st.nbuckets; ++bi) {
-        const Bucket& bk = st.bucket[bi];
-        const int b = bk.block;
-        ZeroTable zt{};
-        MatchTable mt{};
-        FilterTable ft{};
-        FinishTable fn{};
-        int zwg = 0, mwg = 0, fwg = 0, nwg = 0;
-        // CBM3D (chroma=1): the groups are matched once, on plane 0 of the
-        // match planes, and every filtered plane uses them
-        // (src/host/filter_bm3d.cpp:451-486).
-        int mplanes[3], nmp = 0;
-        if (h->p.chroma)
-            mplanes[nmp++] = 0;
-        else
-            for (int k = 0; k < bk.nplanes; ++k) mplanes[nmp++] = bk.planes[k];
-        for (int k = 0; k < nmp; ++k) {
-            const int i = mplanes[k];
-            const StagePlane& sp = st.plane[i];
-            const Grid g = make_grid(h->pw[i], h->ph[i], b, sp.step);
-            MatchJob& m = mt.job[mt.n++];
-            m.plane = match[i];
-            m.width = h->pw[i];
-            m.height = h->ph[i];
-            m.stride = match_stride[i];
-            m.grid = g;
-            m.range = sp.range;
-            m.group = sp.group;
-            m.t0 = 0;
-            m.wg_begin = mwg;
-            m.out = h->matches[i];
-            m.counts = h->counts[i];
-            mwg += bk.match_kind == 2 ? g.ny * ((g.nx + kRunRefs - 1) / kRunRefs) : g.nx * g.ny;
-        }
-        for (int k = 0; k < bk.nplanes; ++k) {
-            const int i = bk.planes[k];
-            const int mi = h->p.chroma ? 0 : i;
-            const StagePlane& sp = st.plane[i];
-            const int w = h->pw[i], hh = h->ph[i];
-            const Grid g = make_grid(w, hh, b, sp.step);
-            const int nref = g.nx * g.ny;
-            if (!bk.tiles) {
-                const long long npx = static_cast<long long>(w) * hh;
-                for (unsigned long long* buf : {h->num[i], h->den[i]}) {
-                    ZeroJob& z = zt.job[zt.n++];
-                    z.p = buf;
-                    z.count = npx;
-                    z.wg_begin = zwg;
-                    zwg += static_cast<int>((npx + 1023) / 1024);
-                }
-            }
-            FilterJob& f = ft.job[ft.n++];
-            f.src = src[i];
-            f.wref = st.wiener ? wref[i] : nullptr;
-            f.width = w;
-            f.height = hh;
-            f.stride = src_stride[i];
-            f.wref_stride = st.wiener ? wref_stride[i] : 0;
-            f.grid = g;
-            f.range = sp.range;
-            f.nref = nref;
-            f.group = sp.group;
-            f.sigma = sp.sigma;
-            f.wg_begin = fwg;
-            f.matches = h->matches[mi];
-            f.counts = h->counts[mi];
-            f.num = h->num[i];
-            f.den = h->den[i];
-            f.tiles = bk.geom[k];
-            f.part_num = h->part[i];
-            f.part_den = h->part[i] ? h->part[i] + h->part_floats[i] : nullptr;
-            fwg += bk.tiles ? bk.geom[k].ntx * bk.geom[k].nty : nref;
+template <typename F>
+hipError_t traced(nss_hip_bm3d* h, int kind, hipStream_t s, F&& launch) {
+    return traced(&h->tracer, kind, s, static_cast<F&&>(launch));
+}
 
-            FinishJob& n = fn.job[fn.n++];
-            n.num = h->num[i];
-            n.den = h->den[i];
-            n.part_num = f.part_num;
-            n.part_den = f.part_den;
-            n.tiles = bk.geom[k];
-            n.block = b;
-            n.step = sp.step;
-            n.range = sp.range;
-            n.src = src[i];
-            n.dst = dst[i];
-            n.width = w;
-            n.height = hh;
-            n.sstride = src_stride[i];
-            n.dstride = dst_stride[i];
-            n.tiles_x = (w + 15) / 16;
-            n.wg_begin = nwg;
-            nwg += n.tiles_x * ((hh + 15) / 16);
+// Planes of one frame for one stage: match on `match`, filter `src` (Wiener
+// reference `wref` in a Wiener stage), finish into `dst`.
+struct StageIO {
+    const float* src[3]{};
+    int src_stride[3]{};
+    const float* match[3]{};
+    int match_stride[3]{};
+    const float* wref[3]{};
+    int wref_stride[3]{};
+    float* dst[3]{};
+    int dst_stride[3]{};
+};
+
+// Stage k's planes of a frame (src/host/filter_bm3d.cpp:1086-1110): final=1
+// runs basic into the instance's basic planes, then Wiener matched on and
+// referenced to them; a ref clip alone gives one Wiener stage on the ref.
+StageIO stage_io(const FrameRef& f, int k) {
+    const nss_hip_bm3d* h = f.h;
+    StageIO io;
+    for (int i = 0; i < h->nplanes; ++i) {
+        io.src[i] = f.src[i];
+        io.src_stride[i] = f.src_stride[i];
+        io.dst[i] = f.dst[i];
+        io.dst_stride[i] = f.dst_stride[i];
+        io.match[i] = f.src[i];
+        io.match_stride[i] = f.src_stride[i];
+        if (h->p.final_stage) {
+            if (k == 0) {
+                io.dst[i] = f.buf.basic[i];
+                io.dst_stride[i] = h->pw[i];
+            } else {
+                io.match[i] = io.wref[i] = f.buf.basic[i];
+                io.match_stride[i] = io.wref_stride[i] = h->pw[i];
+            }
+        } else if (h->stages[0].wiener) {
+            io.match[i] = io.wref[i] = f.ref[i];
+            io.match_stride[i] = io.wref_stride[i] = f.ref_stride[i];
         }
-        hipError_t e = hipSuccess;
-        if (!bk.tiles) {
-            e = traced(h, 0, s, [&] { return launch_zero(zt, zwg, s); });
-            if (e != hipSuccess) return hip_fail(e, "clear launch");
-        }
-        e = traced(h, 1, s, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, s); });
+    }
+    return io;
+}
+
+// Jobs of one bucket shape gathered over the frames of a batch, launched as
+// one clear / match / filter / finish sequence.
+struct LaunchGroup {
+    int block = 0, match_kind = 0, fast_group = 0, form = 0;
+    bool tiles = false, wiener = false;
+    size_t match_lds = 0, filter_lds = 0;
+    ZeroTable zt{};
+    MatchTable mt{};
+    FilterTable ft{};
+    FinishTable fn{};
+    int zwg = 0, mwg = 0, fwg = 0, nwg = 0;
+
+    bool fits(const Bucket& bk, bool w) const {
+        return block == bk.block && match_kind == bk.match_kind && fast_group == bk.fast_group && form == bk.form &&
+               tiles == bk.tiles && wiener == w;
+    }
+    bool empty() const { return ft.n == 0 && mt.n == 0; }
+};
+
+int flush_group(LaunchGroup& g, Tracer* tr, hipStream_t s) {
+    if (g.empty()) return NSS_HIP_OK;
+    const int b = g.block;
+    hipError_t e = hipSuccess;
+    if (!g.tiles && g.zt.n) {
+        e = traced(tr, 0, s, [&] { return launch_zero(g.zt, g.zwg, s); });
+        if (e != hipSuccess) return hip_fail(e, "clear launch");
+    }
+    if (g.mt.n) {
+        e = traced(tr, 1, s, [&] { return launch_match(b, g.mt, g.mwg, g.match_lds, g.match_kind, s); });
         if (e != hipSuccess) return hip_fail(e, "match launch");
-        e = traced(h, 2, s,
-                   [&] { return launch_filter(b, ft, fwg, bk.filter_lds, st.wiener, bk.tiles, bk.fast_group,
-                                                bk.form, s); });
+    }
+    if (g.ft.n) {
+        e = traced(tr, 2, s, [&] {
+            return launch_filter(b, g.ft, g.fwg, g.filter_lds, g.wiener, g.tiles, g.fast_group, g.form, s);
+        });
         if (e != hipSuccess) return hip_fail(e, "filter launch");
-        e = traced(h, 3, s, [&] { return launch_finish(b, fn, nwg, bk.tiles, s); });
+        e = traced(tr, 3, s, [&] { return launch_finish(b, g.fn, g.nwg, g.tiles, s); });
         if (e != hipSuccess) return hip_fail(e, "finish launch");
+    }
+    const LaunchGroup fresh{};
+    g.zt = fresh.zt;
+    g.mt = fresh.mt;
+    g.ft = fresh.ft;
+    g.fn = fresh.fn;
+    g.zwg = g.mwg = g.fwg = g.nwg = 0;
+    g.match_lds = g.filter_lds = 0;
+    return NSS_HIP_OK;
+}
+
+// Appends one frame's jobs of bucket bk (stage st, sigma per plane) to g.
+void add_bucket_jobs(LaunchGroup& g, const FrameRef& f, const Stage& st, const Bucket& bk, const StageIO& io,
+                     const float* sigma) {
+    nss_hip_bm3d* h = f.h;
+    const int b = bk.block;
+    g.match_lds = std::max(g.match_lds, bk.match_lds);
+    g.filter_lds = std::max(g.filter_lds, bk.filter_lds);
+    // CBM3D (chroma=1): the groups are matched once, on plane 0 of the match
+    // planes, and every filtered plane uses them (src/host/filter_bm3d.cpp:451-486).
+    int mplanes[3], nmp = 0;
+    if (h->p.chroma)
+        mplanes[nmp++] = 0;
+    else
+        for (int k = 0; k < bk.nplanes; ++k) mplanes[nmp++] = bk.planes[k];
+    for (int k = 0; k < nmp; ++k) {
+        const int i = mplanes[k];
+        const StagePlane& sp = st.plane[i];
+        const Grid gr = make_grid(h->pw[i], h->ph[i], b, sp.step);
+        MatchJob& m = g.mt.job[g.mt.n++];
+        m.plane = io.match[i];
+        m.width = h->pw[i];
+        m.height = h->ph[i];
+        m.stride = io.match_stride[i];
+        m.grid = gr;
+        m.range = sp.range;
+        m.group = sp.group;
+        m.t0 = 0;
+        m.wg_begin = g.mwg;
+        m.out = f.buf.matches[i];
+        m.counts = f.buf.counts[i];
+        g.mwg += bk.match_kind == 2 ? gr.ny * ((gr.nx + kRunRefs - 1) / kRunRefs) : gr.nx * gr.ny;
+    }
+    for (int k = 0; k < bk.nplanes; ++k) {
+        const int i = bk.planes[k];
+        const int mi = h->p.chroma ? 0 : i;
+        const StagePlane& sp = st.plane[i];
+        const int w = h->pw[i], hh = h->ph[i];
+        const Grid gr = make_grid(w, hh, b, sp.step);
+        const int nref = gr.nx * gr.ny;
+        if (!bk.tiles) {
+            const long long npx = static_cast<long long>(w) * hh;
+            for (unsigned long long* buf : {f.buf.num[i], f.buf.den[i]}) {
+                ZeroJob& z = g.zt.job[g.zt.n++];
+                z.p = buf;
+                z.count = npx;
+                z.wg_begin = g.zwg;
+                g.zwg += static_cast<int>((npx + 1023) / 1024);
+            }
+        }
+        FilterJob& fj = g.ft.job[g.ft.n++];
+        fj.src = io.src[i];
+        fj.wref = st.wiener ? io.wref[i] : nullptr;
+        fj.width = w;
+        fj.height = hh;
+        fj.stride = io.src_stride[i];
+        fj.wref_stride = st.wiener ? io.wref_stride[i] : 0;
+        fj.grid = gr;
+        fj.range = sp.range;
+        fj.nref = nref;
+        fj.group = sp.group;
+        fj.sigma = sigma[i];
+        fj.wg_begin = g.fwg;
+        fj.matches = f.buf.matches[mi];
+        fj.counts = f.buf.counts[mi];
+        fj.num = f.buf.num[i];
+        fj.den = f.buf.den[i];
+        fj.tiles = bk.geom[k];
+        fj.part_num = f.buf.part[i];
+        fj.part_den = f.buf.part[i] ? f.buf.part[i] + h->part_floats[i] : nullptr;
+        g.fwg += bk.tiles ? bk.geom[k].ntx * bk.geom[k].nty : nref;
+
+        FinishJob& n = g.fn.job[g.fn.n++];
+        n.num = f.buf.num[i];
+        n.den = f.buf.den[i];
+        n.part_num = fj.part_num;
+        n.part_den = fj.part_den;
+        n.tiles = bk.geom[k];
+        n.block = b;
+        n.step = sp.step;
+        n.range = sp.range;
+        n.src = io.src[i];
+        n.dst = io.dst[i];
+        n.width = w;
+        n.height = hh;
+        n.sstride = io.src_stride[i];
+        n.dstride = io.dst_stride[i];
+        // finish blocks: 64 x 16 pixels on the tile path, 16 x 16 otherwise
+        n.tiles_x = bk.tiles ? (w + 63) / 64 : (w + 15) / 16;
+        n.wg_begin = g.nwg;
+        g.nwg += n.tiles_x * ((hh + 15) / 16);
+    }
+}
+
+}  // namespace
+
+namespace nsship {
+
+// Both stages of a batch of radius-0 frames: per stage round, every frame's
+// planes with sigma 0 are copied and the jobs of equal bucket shape share one
+// launch sequence (at most kMaxJobs planes per table, else several).
+int run_batch(const FrameRef* frames, int nframes, Tracer* tr, hipStream_t s) {
+    int rounds = 0;
+    for (int q = 0; q < nframes; ++q) rounds = std::max(rounds, frames[q].h->nstages);
+    std::vector<LaunchGroup> groups;
+    for (int k = 0; k < rounds; ++k) {
+        groups.clear();
+        for (int q = 0; q < nframes; ++q) {
+            const FrameRef& f = frames[q];
+            nss_hip_bm3d* h = f.h;
+            if (k >= h->nstages) continue;
+            const Stage& st = h->stages[k];
+            const StageIO io = stage_io(f, k);
+            for (int i = 0; i < h->nplanes; ++i) {
+                if (st.plane[i].sigma != 0.f) continue;
+                const hipError_t e =
+                    hipMemcpy2DAsync(io.dst[i], static_cast<size_t>(io.dst_stride[i]) * 4, io.src[i],
+                                     static_cast<size_t>(io.src_stride[i]) * 4, static_cast<size_t>(h->pw[i]) * 4,
+                                     h->ph[i], hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) return hip_fail(e, "plane copy");
+            }
+            for (int bi = 0; bi < st.nbuckets; ++bi) {
+                const Bucket& bk = st.bucket[bi];
+                LaunchGroup* g = nullptr;
+                for (auto& cand : groups)
+                    if (cand.fits(bk, st.wiener)) g = &cand;
+                if (!g) {
+                    groups.emplace_back();
+                    g = &groups.back();
+                    g->block = bk.block;
+                    g->match_kind = bk.match_kind;
+                    g->fast_group = bk.fast_group;
+                    g->form = bk.form;
+                    g->tiles = bk.tiles;
+                    g->wiener = st.wiener;
+                }
+                const int nmp = h->p.chroma ? 1 : bk.nplanes;
+                if (g->mt.n + nmp > kMaxJobs || g->ft.n + bk.nplanes > kMaxJobs) {
+                    const int rc = flush_group(*g, tr, s);
+                    if (rc != NSS_HIP_OK) return rc;
+                }
+                add_bucket_jobs(*g, f, st, bk, io, f.sigma[k]);
+            }
+        }
+        for (auto& g : groups) {
+            const int rc = flush_group(g, tr, s);
+            if (rc != NSS_HIP_OK) return rc;
+        }
     }
     return NSS_HIP_OK;
 }
+
+// The instance's own buffers and sigma, for frame planes src / ref / dst.
+FrameRef frame_ref(nss_hip_bm3d* h, const float* const* src, const int* src_stride, const float* const* ref,
+                   const int* ref_stride, float* const* dst, const int* dst_stride) {
+    FrameRef f{};
+    f.h = h;
+    for (int i = 0; i < h->nplanes; ++i) {
+        f.src[i] = src[i];
+        f.src_stride[i] = src_stride[i];
+        f.ref[i] = ref ? ref[i] : nullptr;
+        f.ref_stride[i] = ref_stride ? ref_stride[i] : 0;
+        f.dst[i] = dst[i];
+        f.dst_stride[i] = dst_stride[i];
+    }
+    f.buf = h->buf;
+    for (int k = 0; k < h->nstages; ++k)
+        for (int i = 0; i < 3; ++i) f.sigma[k][i] = h->stages[k].plane[i].sigma;
+    return f;
+}
+
+}  // namespace nsship
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // Rolling temporal processing (src/host/filter_bm3d.cpp:730-851): the centre
@@ -498,8 +623,8 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             m.group = sp.group;
             m.t0 = R.r;
             m.wg_begin = mwg;
-            m.out = h->matches[i];
-            m.counts = h->counts[i];
+            m.out = h->buf.matches[i];
+            m.counts = h->buf.counts[i];
             mwg += bk.match_kind == 2 ? g.ny * ((g.nx + kRunRefs - 1) / kRunRefs) : g.nx * g.ny;
             // group 1 keeps the reference alone (src/cpu/bm/matcher.hpp:323).
             if (sp.group > 1) {
@@ -514,8 +639,8 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
                 p.ps_range = sp.ps_range;
                 p.slots = slots;
                 p.wg_begin = pwg;
-                p.recs = h->matches[i];
-                p.counts = h->counts[i];
+                p.recs = h->buf.matches[i];
+                p.counts = h->buf.counts[i];
                 pwg += p.nref;
                 plds = std::max(plds, predictive_lds_bytes(b, sp.group, sp.ps_num, R.r));
                 predictive = true;
@@ -538,8 +663,8 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             f.sigma = sp.sigma;
             f.slots = slots;
             f.wg_begin = fwg;
-            f.matches = h->matches[mi];
-            f.counts = h->counts[mi];
+            f.matches = h->buf.matches[mi];
+            f.counts = h->buf.counts[mi];
             f.num = R.num[s][i];
             f.den = R.den[s][i];
             f.acc_ring = R.acc_n;
@@ -721,49 +846,15 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
         }
     }
     const bool rolling = p.radius > 0;
+    for (int i = 0; i < h->nplanes; ++i) h->part_floats[i] = plane_needs(h, i).part;
+    hipError_t ea = alloc_frame_buffers(h, h->buf);
+    if (ea != hipSuccess) {
+        delete h;
+        return hip_fail(ea, "device allocation");
+    }
     for (int i = 0; i < h->nplanes; ++i) {
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        size_t recs = 0, nref_max = 0, part = 0;
-        bool atomic = false;
-        for (int s = 0; s < h->nstages; ++s) {
-            const Stage& st = h->stages[s];
-            const StagePlane& sp = st.plane[i];
-            bool any = false;
-            for (int q = 0; q < h->nplanes; ++q) any |= st.plane[q].sigma != 0.f;
-            // With chroma, plane 0 holds the shared groups whenever any plane is filtered.
-            const bool matched = p.chroma ? (i == 0 && any) : sp.sigma != 0.f;
-            if (matched) {
-                const Grid g = make_grid(h->pw[i], h->ph[i], sp.block, sp.step);
-                const size_t nref = static_cast<size_t>(g.nx) * g.ny;
-                recs = std::max(recs, nref * sp.group);
-                nref_max = std::max(nref_max, nref);
-            }
-            if (sp.sigma == 0.f || rolling) continue;
-            for (int bi = 0; bi < st.nbuckets; ++bi)
-                for (int k = 0; k < st.bucket[bi].nplanes; ++k) {
-                    if (st.bucket[bi].planes[k] != i) continue;
-                    if (st.bucket[bi].tiles) {
-                        const TileGeom& tg = st.bucket[bi].geom[k];
-                        part = std::max(part, static_cast<size_t>(tg.ntx) * tg.nty * tg.aw * tg.ah);
-                    } else {
-                        atomic = true;
-                    }
-                }
-        }
         hipError_t e = hipSuccess;
-        if (recs) {
-            e = hipMalloc(&h->matches[i], recs * sizeof(MatchRec));
-            if (e == hipSuccess) e = hipMalloc(&h->counts[i], nref_max * sizeof(int));
-        }
-        if (e == hipSuccess && atomic) {
-            e = hipMalloc(&h->num[i], npx * 8);
-            if (e == hipSuccess) e = hipMalloc(&h->den[i], npx * 8);
-        }
-        if (e == hipSuccess && part) {
-            h->part_floats[i] = part;
-            e = hipMalloc(&h->part[i], 2 * part * sizeof(float));
-        }
-        if (e == hipSuccess && p.final_stage && !rolling) e = hipMalloc(&h->basic[i], npx * 4);
         if (e == hipSuccess && rolling) {
             // Ring depths: the oldest frame a pending centre or finish reads
             // is 4r (final=1) or 2r (one stage) behind the newest pushed; the
@@ -882,18 +973,8 @@ int nss_hip_bm3d_process(nss_hip_bm3d* h, const float* const* src, const int* sr
     }
     DeviceGuard guard(h->device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->p.final_stage) {
-        float* basic[3] = {h->basic[0], h->basic[1], h->basic[2]};
-        int bstride[3] = {h->pw[0], h->pw[1], h->pw[2]};
-        int rc = run_stage(h, h->stages[0], src, src_stride, src, src_stride, nullptr, nullptr, basic, bstride, s);
-        if (rc != NSS_HIP_OK) return rc;
-        const float* cb[3] = {basic[0], basic[1], basic[2]};
-        return run_stage(h, h->stages[1], src, src_stride, cb, bstride, cb, bstride, dst, dst_stride, s);
-    }
-    if (h->stages[0].wiener)
-        return run_stage(h, h->stages[0], src, src_stride, ref, ref_stride, ref, ref_stride, dst, dst_stride, s);
-    return run_stage(h, h->stages[0], src, src_stride, src, src_stride, nullptr, nullptr, dst, dst_stride, s);
+    const FrameRef f = frame_ref(h, src, src_stride, ref, ref_stride, dst, dst_stride);
+    return run_batch(&f, 1, &h->tracer, static_cast<hipStream_t>(stream));
 }
 
 int nss_hip_bm3d_set_tracing(nss_hip_bm3d* h, int on) {

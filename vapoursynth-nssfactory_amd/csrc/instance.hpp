@@ -1,0 +1,182 @@
+// Host-side state shared by the filter instance (driver.cpp) and the serving
+// loop (server.cpp): launch plans, per-frame device buffers, the batched
+// stage runner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/nss_hip.h"
+#include "bm3d_jobs.hpp"
+
+namespace nsship {
+
+hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s);
+hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
+                         int fast_group, int form, hipStream_t s);
+hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s);
+hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s);
+hipError_t upload_dct_tables(const float* host, size_t count);
+size_t match_wave_lds_bytes(int block, int range, bool stage);
+size_t match_run_lds_bytes(int block, int step, int range);
+size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
+size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);
+size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t);
+size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t);
+size_t predictive_lds_bytes(int block, int group, int ps_num, int radius);
+hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s);
+hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
+                                  hipStream_t s);
+
+extern thread_local std::string g_last_error;
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceGuard(int device) {
+        err = hipGetDevice(&prev);
+        if (err == hipSuccess && prev != device) err = hipSetDevice(device);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+constexpr size_t kLdsLimit = 160 * 1024;  // LDS per workgroup on gfx950
+// Tile aggregation aims at two workgroups per CU; a one-reference tile may
+// take the whole LDS.
+constexpr size_t kTileLdsBudget = 76 * 1024;
+constexpr size_t kRunLdsBudget = 64 * 1024;
+
+int ensure_tables(int device);
+Grid make_grid(int width, int height, int block, int step);
+TileGeom make_tiles(const Grid& g, int tw, int th, int block, int range);
+bool allowed_block(int b);
+bool allowed_group(int g);
+
+struct StagePlane {
+    float sigma;
+    int block, group, step, range, ps_num, ps_range;
+};
+
+// Launch plan of one block-size bucket of a stage.
+struct Bucket {
+    int block = 0;
+    int planes[3]{};
+    int nplanes = 0;
+    int match_kind = 0;  // 2 run, 1 wave + LDS window, 0 wave + global window
+    size_t match_lds = 0;
+    bool tiles = false;  // tile aggregation, else frame-sized atomics
+    TileGeom geom[3]{};  // per plane of the bucket (tile path)
+    size_t filter_lds = 0;
+    int fast_group = 0;  // block 8 with one group size in {8, 16, 32}: the register kernel
+    int form = 0;        // 1 bf16x3, 2 fp16: the MFMA kernel (block 8, groups of 8 or 16)
+};
+
+struct Stage {
+    StagePlane plane[3];
+    bool wiener;
+    Bucket bucket[7];
+    int nbuckets;
+};
+
+// Event pairs around launches while tracing is on (one vector per kind).
+struct Tracer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[NSS_HIP_KERNEL_KINDS];
+
+    hipEvent_t take() {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void reset() {
+        used = 0;
+        for (auto& v : spans) v.clear();
+    }
+    ~Tracer() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// Device buffers one radius-0 frame needs while it is processed: match lists,
+// tile partials (or frame-sized sums on the atomic path) and the basic
+// estimate of a two-stage filter.
+struct FrameBuffers {
+    unsigned long long* num[3]{};  // atomic path sums
+    unsigned long long* den[3]{};
+    float* part[3]{};              // tile path partials: num then den
+    MatchRec* matches[3]{};
+    int* counts[3]{};
+    float* basic[3]{};
+};
+
+}  // namespace nsship
+
+struct nss_hip_bm3d {
+    nss_hip_bm3d_params p{};
+    nss_hip_video_info vi{};
+    nsship::Tracer tracer;
+    int device = 0;
+    int nplanes = 0;
+    int pw[3]{}, ph[3]{};
+    nsship::Stage stages[2]{};
+    int nstages = 0;
+    size_t part_floats[3]{};  // per half (num or den) of a plane's tile partials
+    nsship::FrameBuffers buf;  // the instance's own frame (process)
+    // Rolling temporal state (radius > 0): frames enter in clip order and
+    // stay on the device in rings (frame f at slot f % ring); each stage keeps
+    // frame-sized int64 sums for the 2r+1 frames its current centre touches.
+    struct Rolling {
+        int N = 0, r = 0, pushed = 0;
+        int src_n = 0, ref_n = 0, basic_n = 0, out_n = 0, acc_n = 0;
+        float* src[3]{};
+        float* ref[3]{};
+        float* basic[3]{};
+        float* out[3]{};
+        unsigned long long* num[2][3]{};
+        unsigned long long* den[2][3]{};
+        int next_center[2]{};
+        int finished[2]{};
+        int out_head = 0;
+    } roll;
+};
+
+namespace nsship {
+
+// One radius-0 frame of a batch: its instance (plan), planes, buffers and
+// effective sigma per stage and plane.
+struct FrameRef {
+    nss_hip_bm3d* h = nullptr;
+    const float* src[3]{};
+    int src_stride[3]{};
+    const float* ref[3]{};
+    int ref_stride[3]{};
+    float* dst[3]{};
+    int dst_stride[3]{};
+    FrameBuffers buf;
+    float sigma[2][3]{};
+};
+
+// Device bytes of one frame's buffers for instance h (radius 0), and of all
+// the instance holds (frame buffers plus rolling rings and sums).
+size_t frame_buffer_bytes(const nss_hip_bm3d* h);
+size_t instance_bytes(const nss_hip_bm3d* h);
+hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb);
+void free_frame_buffers(FrameBuffers& fb);
+FrameRef frame_ref(nss_hip_bm3d* h, const float* const* src, const int* src_stride, const float* const* ref,
+                   const int* ref_stride, float* const* dst, const int* dst_stride);
+int run_batch(const FrameRef* frames, int nframes, Tracer* tr, hipStream_t s);
+
+}  // namespace nsship

@@ -96,8 +96,20 @@ _lib = None
 EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "nss_hip_bm3d_output_info",
            "nss_hip_bm3d_create", "nss_hip_bm3d_slots", "nss_hip_bm3d_process", "nss_hip_bm3d_destroy",
            "nss_hip_spatial_match", "nss_hip_bm3d_set_tracing", "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push",
-           "nss_hip_bm3d_pop")
+           "nss_hip_bm3d_pop", "nss_hip_server_create", "nss_hip_server_open_stream", "nss_hip_server_submit",
+           "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy")
 KERNEL_KINDS = ("clear", "match", "filter", "finish")
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device_id", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_longlong),
+                ("max_batch_frames", ctypes.c_int), ("max_inflight", ctypes.c_int), ("use_graphs", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_longlong) for n in ("submitted", "completed", "batches", "batched_frames",
+                                                 "graph_replays", "graph_captures", "bytes_in_use", "peak_bytes",
+                                                 "admissions_deferred")]
 
 
 def lib():
@@ -123,6 +135,14 @@ def lib():
         L.nss_hip_bm3d_kernel_times.argtypes = [vp, P(ci), P(ctypes.c_double), ci]
         L.nss_hip_bm3d_push.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_bm3d_pop.argtypes = [vp, P(vp), P(ci), P(ci), vp]
+        L.nss_hip_server_create.argtypes = [P(ServerConfig), P(vp)]
+        L.nss_hip_server_open_stream.argtypes = [vp, P(Params), P(VideoInfo), P(ci)]
+        L.nss_hip_server_submit.argtypes = [vp, ci, P(vp), P(ci), P(vp), P(ci), ctypes.c_double,
+                                            P(ctypes.c_longlong)]
+        L.nss_hip_server_query.argtypes = [vp, ctypes.c_longlong, P(ctypes.c_double)]
+        L.nss_hip_server_drain.argtypes = [vp]
+        L.nss_hip_server_get_stats.argtypes = [vp, P(ServerStats)]
+        L.nss_hip_server_destroy.argtypes = [vp]
         for name in EXPORTS[1:]:
             getattr(L, name).restype = ci
         _lib = L
@@ -353,3 +373,77 @@ def spatial_match(plane, block=8, step=8, bm_range=9, group=8, stream=None):
                                        bm_range, group, ctypes.c_void_p(rec.data_ptr()),
                                        ctypes.c_void_p(cnt.data_ptr()), _stream_handle(stream)))
     return rec, cnt
+
+
+class Server:
+    """Continuous-batching serving loop (nss_hip_server_*): clip streams of
+    any format, frames queued as they arrive, packed into shared launches by
+    a native scheduler thread under an HBM budget."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_frames=8, max_inflight=2, use_graphs=True):
+        cfg = ServerConfig(device, int(hbm_budget_bytes), max_batch_frames, max_inflight, int(bool(use_graphs)))
+        h = ctypes.c_void_p()
+        _check(lib().nss_hip_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.device = device
+        self._dims = []
+
+    def open_stream(self, fmt, width, height, frames=1, **kwargs):
+        """A clip stream with nss.BM3D arguments; returns its id."""
+        kwargs.setdefault("device_id", self.device)
+        params = parse(fmt, width, height, frames, **kwargs)
+        info = video_info(fmt, width, height, frames)
+        sid = ctypes.c_int(-1)
+        _check(lib().nss_hip_server_open_stream(self._h, ctypes.byref(params), ctypes.byref(info), ctypes.byref(sid)))
+        _, sw, sh, n = FORMATS[fmt]
+        self._dims.append([(height >> (sh if i else 0), width >> (sw if i else 0)) for i in range(n)])
+        return sid.value
+
+    def submit(self, stream, planes, out, sigma=0.0):
+        """Queues one frame (device planes, ready before the call); `out`
+        planes are written when the request completes. Returns its id."""
+        import torch
+        dims = self._dims[stream]
+        for what, ps in (("plane", planes), ("output plane", out)):
+            if len(ps) != len(dims):
+                raise Error(f"{NAMESPACE}.Server: expected {len(dims)} {what}s")
+            for i, p in enumerate(ps):
+                if p.dtype != torch.float32 or not p.is_cuda or p.stride(1) != 1 or tuple(p.shape) != dims[i]:
+                    raise Error(f"{NAMESPACE}.Server: {what} {i} must be a row-contiguous float32 CUDA tensor "
+                                f"of shape {dims[i]}")
+        n = len(planes)
+        src = (ctypes.c_void_p * n)(*[p.data_ptr() for p in planes])
+        sst = (ctypes.c_int * n)(*[p.stride(0) for p in planes])
+        dst = (ctypes.c_void_p * n)(*[p.data_ptr() for p in out])
+        dstr = (ctypes.c_int * n)(*[p.stride(0) for p in out])
+        rid = ctypes.c_longlong(-1)
+        _check(lib().nss_hip_server_submit(self._h, stream, src, sst, dst, dstr, float(sigma), ctypes.byref(rid)))
+        return rid.value
+
+    def query(self, rid):
+        """Latency in ms once the request completed, else None."""
+        lat = ctypes.c_double(0)
+        rc = lib().nss_hip_server_query(self._h, rid, ctypes.byref(lat))
+        if rc == BYPASS:
+            return None
+        _check(rc)
+        return lat.value
+
+    def drain(self):
+        _check(lib().nss_hip_server_drain(self._h))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().nss_hip_server_get_stats(self._h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in st._fields_}
+
+    def close(self):
+        if self._h:
+            lib().nss_hip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
