@@ -73,6 +73,16 @@ def cpu_baseline(noisy, params, threads):
 
 
 FORMS = {"fp32": 0, "bf16x3": 1, "fp16": 2}
+# HBM bytes per launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch, averaged over the basic and Wiener launches)
+# scaled by what the counters read for a 256 MiB elementwise copy in the same
+# process (FETCH x2, WRITE x1); profiles/bm3d_1080p_yuv420_two_stage_kernels.md.
+TRAFFIC_KIB = {("match", "fp32"): (6608, 6280), ("match", "bf16x3"): (6608, 6280), ("match", "fp16"): (6608, 6280),
+               ("filter", "fp32"): ((9645 + 16061) / 2, 52598.5),
+               ("filter", "bf16x3"): ((9707 + 16152) / 2, 66938), ("filter", "fp16"): ((9690 + 16127) / 2, 66938),
+               ("finish", "fp32"): (30857, 12723), ("finish", "bf16x3"): (30857, 12723),
+               ("finish", "fp16"): (30857, 12723)}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 FORM_DTYPE = {"fp32": "f32", "bf16x3": "bf16x3 (f32 accumulate)", "fp16": "f16 (f32 accumulate)"}
 PEAK_MFMA_TFLOPS = 2500.0  # dense bf16 / fp16
 
@@ -226,6 +236,124 @@ def precision_workload(args):
                                                             "rms_err": float(np.sqrt(np.mean(d * d)))}}), flush=True)
 
 
+def serving_workload(args):
+    """Three clip streams with Poisson arrivals at 24 fps each (720x480
+    YUV420PS radius 1 sigma 5; 1920x1080 YUV420PS and 3840x2160 RGBS radius 0
+    with per-request sigma in {3, 5, 10, 20}), two stages, for --seconds of
+    arrivals: the continuous-batching server against the plain form (each
+    frame alone through its stream's instance, in arrival order)."""
+    import numpy as np
+    import torch
+
+    import nss_hip
+    import synth
+    rng = np.random.default_rng(11)
+    fps, seconds = 24.0, args.seconds
+    specs = [("YUV420PS", 720, 480, 1), ("YUV420PS", 1920, 1080, 0), ("RGBS", 3840, 2160, 0)]
+    arrivals = []
+    for s in range(len(specs)):
+        t = 0.0
+        while True:
+            t += rng.exponential(1.0 / fps)
+            if t >= seconds:
+                break
+            arrivals.append((t, s))
+    arrivals.sort()
+    counts = [sum(1 for _, s in arrivals if s == k) for k in range(len(specs))]
+    sigmas = [float(rng.choice([3, 5, 10, 20])) for _ in arrivals]
+    print(f"# serving: {len(arrivals)} arrivals {counts} over {seconds} s", file=sys.stderr, flush=True)
+    inputs = []
+    for k, (fmt, w, h, r) in enumerate(specs):
+        if r:
+            _, seq = synth.shifted_sequence(fmt, w, h, counts[k], 5, seed=k, max_step=4)
+        else:
+            seq = [synth.noisy_frame(fmt, w, h, 10, seed=k * 10 + j)[1] for j in range(3)]
+        inputs.append([[torch.from_numpy(p).cuda() for p in f] for f in seq])
+    outs = [[[torch.empty_like(p) for p in inputs[k][0]] for _ in range(8)] for k in range(len(specs))]
+    torch.cuda.synchronize()
+    pix = sum(specs[s][1] * specs[s][2] for _, s in arrivals)
+
+    def frame_of(k, j):
+        return inputs[k][j] if specs[k][3] else inputs[k][j % len(inputs[k])]
+
+    # --- server ---
+    srv = nss_hip.Server(max_batch_frames=8, max_inflight=2, use_graphs=True)
+    sids = [srv.open_stream(fmt, w, h, counts[k] if r else 1, sigma=5, radius=r, final=1)
+            for k, (fmt, w, h, r) in enumerate(specs)]
+    seen = [0] * len(specs)
+    pending = [[None] * 8 for _ in specs]
+    rids = []
+    t0 = time.perf_counter()
+    for n, (t, k) in enumerate(arrivals):
+        while time.perf_counter() - t0 < t:
+            time.sleep(50e-6)
+        j = seen[k]
+        seen[k] += 1
+        slot = j % 8
+        if pending[k][slot] is not None:
+            while srv.query(pending[k][slot]) is None:
+                time.sleep(50e-6)
+        rid = srv.submit(sids[k], frame_of(k, j), outs[k][slot], sigma=0.0 if specs[k][3] else sigmas[n])
+        pending[k][slot] = rid
+        rids.append((rid, k))
+    srv.drain()
+    wall = time.perf_counter() - t0
+    lat = np.array([srv.query(r) for r, _ in rids])
+    st = srv.stats()
+    srv.close()
+    server = {"frames_per_s": round(len(rids) / wall, 2), "mpix_per_s": round(pix / wall / 1e6, 2),
+              "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+              "per_stream_p99_ms": [round(float(np.percentile(lat[[k == s for _, k in rids]], 99)), 3)
+                                    for s in range(len(specs))],
+              "wall_s": round(wall, 3), "batches": st["batches"],
+              "frames_per_batch": round(st["batched_frames"] / max(st["batches"], 1), 2),
+              "graph_replays": st["graph_replays"], "graph_captures": st["graph_captures"],
+              "peak_bytes": st["peak_bytes"]}
+    print(f"# server {server}", file=sys.stderr, flush=True)
+
+    # --- plain: each frame alone, in arrival order, as it arrives ---
+    insts = []
+    for k, (fmt, w, h, r) in enumerate(specs):
+        insts.append(nss_hip.BM3D(fmt, w, h, counts[k] if r else 1, sigma=5, radius=r, final=1))
+    per_sigma = {(k, sg): nss_hip.BM3D(fmt, w, h, sigma=sg, final=1)
+                 for k, (fmt, w, h, r) in enumerate(specs) if not r for sg in (3.0, 5.0, 10.0, 20.0)}
+    lat_p = []
+    seen = [0] * len(specs)
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    for n, (t, k) in enumerate(arrivals):
+        while time.perf_counter() - t0 < t:
+            time.sleep(50e-6)
+        j = seen[k]
+        seen[k] += 1
+        fmt, w, h, r = specs[k]
+        if r:
+            insts[k].push(frame_of(k, j))
+            while insts[k].pop(out=outs[k][0]) is not None:
+                pass
+        else:
+            per_sigma[(k, sigmas[n])](frame_of(k, j), out=outs[k][0])
+        stream.synchronize()
+        lat_p.append((time.perf_counter() - t0 - t) * 1e3)
+    wall_p = time.perf_counter() - t0
+    for i in insts + list(per_sigma.values()):
+        i.close()
+    lat_p = np.array(lat_p)
+    plain = {"frames_per_s": round(len(arrivals) / wall_p, 2), "mpix_per_s": round(pix / wall_p / 1e6, 2),
+             "p50_ms": round(float(np.percentile(lat_p, 50)), 3), "p99_ms": round(float(np.percentile(lat_p, 99)), 3),
+             "wall_s": round(wall_p, 3)}
+    print(f"# plain {plain}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": METRIC, "value": server["mpix_per_s"], "unit": "Mpix/s", "n_gpus": 1,
+                      "steps": len(arrivals), "warmup": 0, "ms_per_step": round(wall / len(arrivals) * 1e3, 4),
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                      "data": "synthetic",
+                      "config": {"workload": "serving: 720x480 YUV420PS r1, 1920x1080 YUV420PS, 3840x2160 RGBS, "
+                                             f"Poisson 24 fps per stream for {seconds} s, two stages, per-request "
+                                             "sigma in {3,5,10,20} on the radius-0 streams"},
+                      "p50_ms": server["p50_ms"], "p99_ms": server["p99_ms"], "server": server, "plain": plain}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,12 +361,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--form", choices=list(FORMS), default="bf16x3",
+    ap.add_argument("--form", choices=list(FORMS), default="fp32",
                     help="transform form the headline value is quoted for (all three are timed)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy that reads and writes MIB MiB: the known-size kernel "
                          "that scales rocprof's FETCH_SIZE / WRITE_SIZE readings")
-    ap.add_argument("--workload", choices=["two-stage-1080p", "temporal-1080p", "precision-4k"],
+    ap.add_argument("--seconds", type=float, default=5.0, help="serving: seconds of arrivals")
+    ap.add_argument("--workload", choices=["two-stage-1080p", "temporal-1080p", "precision-4k", "serving"],
                     default="two-stage-1080p", help="two-stage-1080p is the headline; the others print their "
                                                     "own line")
     args = ap.parse_args()
@@ -257,6 +386,8 @@ def main():
         return temporal_workload(args)
     if args.workload == "precision-4k":
         return precision_workload(args)
+    if args.workload == "serving":
+        return serving_workload(args)
     if args.copy_calibration:
         n = args.copy_calibration * 1024 * 1024 // 4
         a = torch.ones(n, device="cuda")
@@ -275,9 +406,11 @@ def main():
               f"{ {k: round(v['avg_us'], 2) for k, v in forms[name]['kernels'].items()} }", file=sys.stderr, flush=True)
     head = forms[args.form]
     dk = head["kernels"][head["dominant"]]
+    tk = TRAFFIC_KIB.get((head["dominant"], args.form))
     roofline = {"kernel": head["dominant"], "bound": "hbm" if dk.get("unit") == "GB/s" else "mfma",
                 "achieved": dk.get("achieved"), "peak": dk.get("peak"), "unit": dk.get("unit"),
-                "frac": dk.get("frac"), "traffic": None}
+                "frac": dk.get("frac"),
+                "traffic": None if tk is None else round((tk[0] * FETCH_SCALE + tk[1] * WRITE_SCALE) * 1024)}
 
     res = {"metric": METRIC, "value": head["value"], "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",

@@ -176,6 +176,13 @@ int nss_hip_bm3d_pop(nss_hip_bm3d* handle, float* const* dst, const int* dst_str
  * reports, per kernel kind (0 clear, 1 match, 2 filter + aggregate,
  * 3 finish), the launch count and total milliseconds since the last reset;
  * reset != 0 starts a new window. */
+/* Tile-shape autotuner (no counterpart in the reference): times every
+ * filter tile shape that fits the LDS budget, one bucket at a time, on a
+ * synthetic frame of the instance's format (process on `stream`, repeats
+ * frames per shape) and keeps the fastest plan; best_ms = its time per frame.
+ * Radius > 0 and ref-only instances have no tile plan (OK, nothing done). */
+int nss_hip_bm3d_autotune(nss_hip_bm3d* handle, int repeats, void* stream, double* best_ms);
+
 #define NSS_HIP_KERNEL_KINDS 4
 int nss_hip_bm3d_set_tracing(nss_hip_bm3d* handle, int on);
 int nss_hip_bm3d_kernel_times(nss_hip_bm3d* handle, int* launches, double* total_ms, int reset);
@@ -210,6 +217,7 @@ typedef struct nss_hip_server_config {
     int max_batch_frames;       /* radius-0 frames per launch sequence, 1..16 */
     int max_inflight;           /* batches on the device at once, 1..4 */
     int use_graphs;             /* capture / replay hipGraphs per batch composition */
+    int autotune;               /* run nss_hip_bm3d_autotune when a radius-0 stream opens */
 } nss_hip_server_config;
 
 typedef struct nss_hip_server_stats {

@@ -50,10 +50,10 @@ def test_server_config_validation_without_device():
     import nss_hip
     lib = nss_hip.lib()
     h = ctypes.c_void_p()
-    for cfg, msg in (((0, 0, 0, 1, 1), "max_batch_frames must be in [1, 16]"),
-                     ((0, 0, 17, 1, 1), "max_batch_frames must be in [1, 16]"),
-                     ((0, 0, 4, 0, 1), "max_inflight must be in [1, 4]"),
-                     ((0, -1, 4, 2, 1), "hbm_budget_bytes must be >= 0")):
+    for cfg, msg in (((0, 0, 0, 1, 1, 0), "max_batch_frames must be in [1, 16]"),
+                     ((0, 0, 17, 1, 1, 0), "max_batch_frames must be in [1, 16]"),
+                     ((0, 0, 4, 0, 1, 0), "max_inflight must be in [1, 4]"),
+                     ((0, -1, 4, 2, 1, 0), "hbm_budget_bytes must be >= 0")):
         c = nss_hip.ServerConfig(*cfg)
         assert lib.nss_hip_server_create(ctypes.byref(c), ctypes.byref(h)) == nss_hip.INVALID_INPUT
         assert msg in nss_hip.last_error()

@@ -36,18 +36,21 @@ def _frames(fmt, w, h, n, seed):
 @pytest.mark.parametrize("use_graphs", [False, True])
 def test_mix_equals_alone(gpu, use_graphs):
     import torch
-    srv = gpu.Server(max_batch_frames=6, max_inflight=2, use_graphs=use_graphs)
+    # One batch in flight at a time, requests submitted back to back: the
+    # ones arriving while a batch runs are packed into the next.
+    srv = gpu.Server(max_batch_frames=6, max_inflight=1, use_graphs=use_graphs)
     sids = [srv.open_stream(fmt, w, h, sigma=10, **kw) for fmt, w, h, kw in STREAMS]
     rng = np.random.default_rng(5)
-    reqs = []
+    work = []
     for rep in range(3):
         for s, (fmt, w, h, kw) in enumerate(STREAMS):
             for planes in _frames(fmt, w, h, 2, seed=10 * s + rep):
                 sig = float(rng.choice([3, 5, 10, 20]))
                 src = [_dev(p) for p in planes]
                 out = [torch.empty_like(p) for p in src]
-                torch.cuda.synchronize()
-                reqs.append((srv.submit(sids[s], src, out, sigma=sig), s, planes, sig, src, out))
+                work.append((s, planes, sig, src, out))
+    torch.cuda.synchronize()
+    reqs = [(srv.submit(sids[s], src, out, sigma=sig), s, planes, sig, src, out) for s, planes, sig, src, out in work]
     srv.drain()
     st = srv.stats()
     print(f"server stats {st}", flush=True)
@@ -71,11 +74,16 @@ def test_graph_replays_match_eager(gpu):
     src = [[_dev(p) for p in f] for f in planes]
     outs = {}
     for use_graphs in (False, True):
-        srv = gpu.Server(max_batch_frames=2, max_inflight=1, use_graphs=use_graphs)
+        # one frame per batch: the same two compositions every repetition
+        srv = gpu.Server(max_batch_frames=1, max_inflight=1, use_graphs=use_graphs)
         sid = srv.open_stream(fmt, w, h, sigma=10, **kw)
         got = []
+        out = [[torch.empty_like(p) for p in f] for f in src]
         for rep in range(4):
-            out = [[torch.empty_like(p) for p in f] for f in src]
+            for f in out:
+                for p in f:
+                    p.fill_(-1.0)
+            torch.cuda.synchronize()
             rids = [srv.submit(sid, src[k], out[k], sigma=5.0) for k in range(2)]
             srv.drain()
             assert all(srv.query(r) is not None for r in rids)
@@ -87,7 +95,7 @@ def test_graph_replays_match_eager(gpu):
         st = srv.stats()
         print(f"use_graphs={use_graphs} stats {st}", flush=True)
         if use_graphs:
-            assert st["graph_replays"] >= 1
+            assert st["graph_captures"] == 2 and st["graph_replays"] == 6
         srv.close()
     for rep in range(4):
         for fe, fg in zip(outs[False][rep], outs[True][rep]):
@@ -151,3 +159,23 @@ def test_budget_admission(gpu):
         for g, w_ in zip(out, want):
             assert np.array_equal(g.cpu().numpy(), w_)
     srv.close()
+
+
+def test_autotune_keeps_parity(gpu, oracle):
+    """The autotuned plan (fastest tile shape per bucket) still meets the
+    oracle gate; radius-0 two-stage YUV420 and a one-stage group-16 gray."""
+    import torch
+    import synth
+    for fmt, w, h, kw in (("YUV420PS", 160, 120, dict(sigma=10, final=1)),
+                          ("GRAYS", 128, 96, dict(sigma=5, group_size=16, block_step=3, bm_range=16))):
+        _, noisy = synth.noisy_frame(fmt, w, h, 10, seed=2)
+        inst = gpu.BM3D(fmt, w, h, **kw)
+        ms = inst.autotune(repeats=2)
+        out = inst([_dev(p) for p in noisy])
+        torch.cuda.synchronize()
+        got = [o.cpu().numpy() for o in out]
+        inst.close()
+        want = oracle.bm3d_frame(noisy, gpu.parse(fmt, w, h, **kw))
+        ps = [oracle.psnr(g, w_) for g, w_ in zip(got, want)]
+        print(f"autotune {fmt}: {ms:.3f} ms/frame, PSNR {ps}", flush=True)
+        assert ms > 0 and min(ps) >= 60.0

@@ -897,27 +897,27 @@ struct Operand<kFormF16> {
     h16x4 v;
 };
 
-__device__ inline short bf16_bits(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<short>(u >> 16);
-}
+typedef __bf16 b16x4 __attribute__((ext_vector_type(4)));
 
-__device__ inline float bf16_value(short b) {
-    return __uint_as_float(static_cast<uint32_t>(static_cast<uint16_t>(b)) << 16);
-}
-
+// Round to nearest even (v_cvt_pk_bf16_f32 / v_cvt_pk_f16_f32, two values per
+// instruction); bf16x3 keeps the rounding residual as the lo part.
 template <int FORM>
 __device__ inline Operand<FORM> operand(f32x4 v) {
     Operand<FORM> o;
+    if constexpr (FORM == kFormBf16x3) {
+        b16x4 hi, lo;
+        f32x4 r;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if constexpr (FORM == kFormBf16x3) {
-            o.hi[j] = bf16_bits(v[j]);
-            o.lo[j] = bf16_bits(v[j] - bf16_value(o.hi[j]));
-        } else {
-            o.v[j] = static_cast<_Float16>(v[j]);
-        }
+        for (int j = 0; j < 4; ++j) hi[j] = static_cast<__bf16>(v[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = v[j] - static_cast<float>(hi[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) lo[j] = static_cast<__bf16>(r[j]);
+        o.hi = __builtin_bit_cast(s16x4, hi);
+        o.lo = __builtin_bit_cast(s16x4, lo);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o.v[j] = static_cast<_Float16>(v[j]);
     }
     return o;
 }
@@ -1461,8 +1461,11 @@ __global__ __launch_bounds__(64) void filter_temporal_kernel(TFilterTable tab) {
 // (tiles in descending (ty, tx) order), read as one aligned 16-byte vector
 // per tile for the four pixels.
 __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
-    const FinishJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
-    const int t = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    // XCD-contiguous blocks: neighbouring blocks read the same tile partial
+    // lines through one L2.
+    const int lb = xcd_block();
+    const FinishJob& jb = tab.job[find_job(tab, lb)];
+    const int t = lb - jb.wg_begin;
     const int x0 = (t % jb.tiles_x) * 64 + (threadIdx.x & 15) * 4;
     const int y = (t / jb.tiles_x) * 16 + (threadIdx.x >> 4);
     if (x0 >= jb.width || y >= jb.height) return;

@@ -90,6 +90,32 @@ int check_stage_plane(const StagePlane& s, int w, int h) {
     return NSS_HIP_OK;
 }
 
+// Tile shapes (references per tile, x by y) in the order the plan tries them;
+// the autotuner times every one that fits.
+constexpr int kTileShapes[][2] = {{8, 4}, {4, 4}, {4, 2}, {8, 2}, {16, 2}, {2, 2}, {1, 1}};
+
+// Sets bucket bk to tiles of tw x th references if every plane's tile fits the
+// LDS budget (the whole LDS for a one-reference tile); false otherwise.
+bool apply_tile_shape(const nss_hip_bm3d* h, const Stage& st, Bucket& bk, int tw, int th) {
+    const int b = bk.block;
+    size_t lds = 0;
+    TileGeom g[3]{};
+    for (int k = 0; k < bk.nplanes; ++k) {
+        const int i = bk.planes[k];
+        const StagePlane& sp = st.plane[i];
+        g[k] = make_tiles(make_grid(h->pw[i], h->ph[i], b, sp.step), tw, th, b, sp.range);
+        lds = std::max(lds, bk.form         ? filter_tile_mfma_lds_bytes(sp.group, g[k])
+                            : bk.fast_group ? filter_tile_b8_lds_bytes(sp.group, g[k])
+                                            : filter_tile_lds_bytes(b, sp.group, st.wiener, g[k]));
+    }
+    const size_t budget = (tw == 1 && th == 1) ? kLdsLimit : kTileLdsBudget;
+    if (lds > budget) return false;
+    bk.tiles = true;
+    bk.filter_lds = lds;
+    for (int k = 0; k < bk.nplanes; ++k) bk.geom[k] = g[k];
+    return true;
+}
+
 // Chooses the kernels of every (stage, block size) bucket.
 int plan_stage(nss_hip_bm3d* h, Stage& st) {
     st.nbuckets = 0;
@@ -125,29 +151,12 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
             if (fast != 8 && fast != 16 && fast != 32) fast = 0;
         }
         const int form = (fast == 8 || fast == 16) ? h->p.transform : 0;
-        static const int shapes[][2] = {{8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}};
-        for (const auto& sh : shapes) {
-            size_t lds = 0;
-            TileGeom g[3]{};
-            for (int k = 0; k < bk.nplanes; ++k) {
-                const int i = bk.planes[k];
-                const StagePlane& sp = st.plane[i];
-                g[k] = make_tiles(make_grid(h->pw[i], h->ph[i], b, sp.step), sh[0], sh[1], b, sp.range);
-                lds = std::max(lds, form   ? filter_tile_mfma_lds_bytes(sp.group, g[k])
-                                    : fast ? filter_tile_b8_lds_bytes(sp.group, g[k])
-                                           : filter_tile_lds_bytes(b, sp.group, st.wiener, g[k]));
-            }
-            const size_t budget = (sh[0] == 1 && sh[1] == 1) ? kLdsLimit : kTileLdsBudget;
-            if (lds <= budget) {
-                bk.tiles = true;
-                bk.fast_group = fast;
-                bk.form = form;
-                bk.filter_lds = lds;
-                for (int k = 0; k < bk.nplanes; ++k) bk.geom[k] = g[k];
-                break;
-            }
-        }
+        bk.fast_group = fast;
+        bk.form = form;
+        for (const auto& sh : kTileShapes)
+            if (apply_tile_shape(h, st, bk, sh[0], sh[1])) break;
         if (!bk.tiles) {
+            bk.fast_group = bk.form = 0;
             for (int k = 0; k < bk.nplanes; ++k)
                 bk.filter_lds =
                     std::max(bk.filter_lds, filter_atomic_lds_bytes(b, st.plane[bk.planes[k]].group, st.wiener));
@@ -223,7 +232,7 @@ size_t frame_buffer_bytes(const nss_hip_bm3d* h) {
     for (int i = 0; i < h->nplanes; ++i) {
         const PlaneNeeds n = plane_needs(h, i);
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        total += n.recs * sizeof(MatchRec) + n.nref * sizeof(int) + 2 * n.part * sizeof(float) +
+        total += n.recs * sizeof(MatchRec) + n.nref * sizeof(int) + 2 * h->part_floats[i] * sizeof(float) +
                  (n.atomic ? 16 * npx : 0) + (n.basic ? 4 * npx : 0);
     }
     return total;
@@ -265,7 +274,7 @@ hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb) {
             e = hipMalloc(&fb.num[i], npx * 8);
             if (e == hipSuccess) e = hipMalloc(&fb.den[i], npx * 8);
         }
-        if (e == hipSuccess && n.part) e = hipMalloc(&fb.part[i], 2 * n.part * sizeof(float));
+        if (e == hipSuccess && h->part_floats[i]) e = hipMalloc(&fb.part[i], 2 * h->part_floats[i] * sizeof(float));
         if (e == hipSuccess && n.basic) e = hipMalloc(&fb.basic[i], npx * 4);
     }
     if (e != hipSuccess) free_frame_buffers(fb);
@@ -975,6 +984,112 @@ int nss_hip_bm3d_process(nss_hip_bm3d* h, const float* const* src, const int* sr
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     const FrameRef f = frame_ref(h, src, src_stride, ref, ref_stride, dst, dst_stride);
     return run_batch(&f, 1, &h->tracer, static_cast<hipStream_t>(stream));
+}
+
+int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* best_ms) {
+    if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null handle");
+    if (repeats < 1 || repeats > 100) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: repeats must be in [1, 100]");
+    if (h->p.radius > 0 || (h->p.has_ref && !h->p.final_stage)) return NSS_HIP_OK;  // nothing to tune
+    DeviceGuard guard(h->device);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // Partials large enough for every candidate shape.
+    size_t need[3]{};
+    for (int k = 0; k < h->nstages; ++k)
+        for (int bi = 0; bi < h->stages[k].nbuckets; ++bi) {
+            Bucket probe = h->stages[k].bucket[bi];
+            if (!probe.tiles) continue;
+            for (const auto& sh : kTileShapes)
+                if (apply_tile_shape(h, h->stages[k], probe, sh[0], sh[1]))
+                    for (int q = 0; q < probe.nplanes; ++q) {
+                        const TileGeom& tg = probe.geom[q];
+                        need[probe.planes[q]] = std::max(need[probe.planes[q]],
+                                                         static_cast<size_t>(tg.ntx) * tg.nty * tg.pitch * tg.ah);
+                    }
+        }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "synchronize");
+    for (int i = 0; i < h->nplanes; ++i) {
+        if (need[i] <= h->part_floats[i]) continue;
+        if (h->buf.part[i]) (void)hipFree(h->buf.part[i]);
+        h->buf.part[i] = nullptr;
+        h->part_floats[i] = 0;
+        e = hipMalloc(&h->buf.part[i], 2 * need[i] * sizeof(float));
+        if (e != hipSuccess) return hip_fail(e, "device allocation");
+        h->part_floats[i] = need[i];
+    }
+    // A synthetic frame: smooth pattern plus pseudo-random noise (sigma 10/255).
+    float* src[3]{};
+    float* dst[3]{};
+    int stride[3]{};
+    std::vector<float> host;
+    unsigned lcg = 12345u;
+    int rc = NSS_HIP_OK;
+    for (int i = 0; i < h->nplanes && rc == NSS_HIP_OK; ++i) {
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        host.resize(npx);
+        for (size_t o = 0; o < npx; ++o) {
+            const int x = static_cast<int>(o % h->pw[i]), y = static_cast<int>(o / h->pw[i]);
+            lcg = lcg * 1664525u + 1013904223u;
+            const float noise = (static_cast<float>(lcg >> 8) / 16777216.f - 0.5f) * (10.f / 255.f) * 3.46f;
+            host[o] = 0.5f + 0.25f * std::sin(0.05f * x) * std::cos(0.07f * y) + noise;
+        }
+        stride[i] = h->pw[i];
+        e = hipMalloc(&src[i], npx * 4);
+        if (e == hipSuccess) e = hipMalloc(&dst[i], npx * 4);
+        if (e == hipSuccess) e = hipMemcpy(src[i], host.data(), npx * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "autotune buffers");
+    }
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (rc == NSS_HIP_OK && (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess))
+        rc = fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: event create");
+    const float* csrc[3] = {src[0], src[1], src[2]};
+    auto time_frame = [&](double* ms) -> int {
+        int r = nss_hip_bm3d_process(h, csrc, stride, nullptr, nullptr, dst, stride, 0, 1, s);
+        if (r != NSS_HIP_OK) return r;
+        (void)hipEventRecord(ev[0], s);
+        for (int k = 0; k < repeats && r == NSS_HIP_OK; ++k)
+            r = nss_hip_bm3d_process(h, csrc, stride, nullptr, nullptr, dst, stride, 0, 1, s);
+        (void)hipEventRecord(ev[1], s);
+        if (r != NSS_HIP_OK) return r;
+        const hipError_t x = hipEventSynchronize(ev[1]);
+        if (x != hipSuccess) return hip_fail(x, "autotune timing");
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, ev[0], ev[1]);
+        *ms = t / repeats;
+        return NSS_HIP_OK;
+    };
+    double best = 0.0;
+    if (rc == NSS_HIP_OK) rc = time_frame(&best);
+    // One bucket at a time, every shape that fits; the others stay as set.
+    for (int k = 0; k < h->nstages && rc == NSS_HIP_OK; ++k)
+        for (int bi = 0; bi < h->stages[k].nbuckets && rc == NSS_HIP_OK; ++bi) {
+            Bucket& bk = h->stages[k].bucket[bi];
+            if (!bk.tiles) continue;
+            Bucket chosen = bk;
+            for (const auto& sh : kTileShapes) {
+                Bucket trial = chosen;
+                if (!apply_tile_shape(h, h->stages[k], trial, sh[0], sh[1])) continue;
+                bk = trial;
+                double ms = 0.0;
+                rc = time_frame(&ms);
+                if (rc != NSS_HIP_OK) break;
+                if (ms < best) {
+                    best = ms;
+                    chosen = trial;
+                }
+            }
+            bk = chosen;
+        }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    (void)hipStreamSynchronize(s);
+    for (int i = 0; i < 3; ++i) {
+        if (src[i]) (void)hipFree(src[i]);
+        if (dst[i]) (void)hipFree(dst[i]);
+    }
+    if (rc == NSS_HIP_OK && best_ms) *best_ms = best;
+    return rc;
 }
 
 int nss_hip_bm3d_set_tracing(nss_hip_bm3d* h, int on) {

@@ -443,6 +443,17 @@ int nss_hip_server_open_stream(nss_hip_server* s, const nss_hip_bm3d_params* par
             return hip_fail(e, "stream create");
         }
     } else {
+        if (s->cfg.autotune) {
+            hipStream_t ts = nullptr;
+            hipError_t e = hipStreamCreateWithFlags(&ts, hipStreamNonBlocking);
+            rc = e == hipSuccess ? nss_hip_bm3d_autotune(h, 3, ts, nullptr) : hip_fail(e, "stream create");
+            if (ts) (void)hipStreamDestroy(ts);
+            if (rc != NSS_HIP_OK) {
+                nss_hip_bm3d_destroy(h);
+                return rc;
+            }
+            cs.slot_bytes = frame_buffer_bytes(h);
+        }
         Slot own;
         own.buf = h->buf;
         own.own = true;

@@ -97,13 +97,15 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_bm3d_create", "nss_hip_bm3d_slots", "nss_hip_bm3d_process", "nss_hip_bm3d_destroy",
            "nss_hip_spatial_match", "nss_hip_bm3d_set_tracing", "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push",
            "nss_hip_bm3d_pop", "nss_hip_server_create", "nss_hip_server_open_stream", "nss_hip_server_submit",
-           "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy")
+           "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy",
+           "nss_hip_bm3d_autotune")
 KERNEL_KINDS = ("clear", "match", "filter", "finish")
 
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device_id", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_longlong),
-                ("max_batch_frames", ctypes.c_int), ("max_inflight", ctypes.c_int), ("use_graphs", ctypes.c_int)]
+                ("max_batch_frames", ctypes.c_int), ("max_inflight", ctypes.c_int), ("use_graphs", ctypes.c_int),
+                ("autotune", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
@@ -135,6 +137,7 @@ def lib():
         L.nss_hip_bm3d_kernel_times.argtypes = [vp, P(ci), P(ctypes.c_double), ci]
         L.nss_hip_bm3d_push.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_bm3d_pop.argtypes = [vp, P(vp), P(ci), P(ci), vp]
+        L.nss_hip_bm3d_autotune.argtypes = [vp, ci, vp, P(ctypes.c_double)]
         L.nss_hip_server_create.argtypes = [P(ServerConfig), P(vp)]
         L.nss_hip_server_open_stream.argtypes = [vp, P(Params), P(VideoInfo), P(ci)]
         L.nss_hip_server_submit.argtypes = [vp, ci, P(vp), P(ci), P(vp), P(ci), ctypes.c_double,
@@ -256,6 +259,13 @@ class BM3D:
         self._h = h
         cf, sw, sh, n = FORMATS[fmt]
         self.plane_dims = [(height >> (sh if i else 0), width >> (sw if i else 0)) for i in range(n)]
+
+    def autotune(self, repeats=3, stream=None):
+        """Times every filter tile shape that fits (per bucket) on a synthetic
+        frame and keeps the fastest plan; returns its ms per frame."""
+        ms = ctypes.c_double(0)
+        _check(lib().nss_hip_bm3d_autotune(self._h, repeats, _stream_handle(stream), ctypes.byref(ms)))
+        return ms.value
 
     def set_tracing(self, on=True):
         _check(lib().nss_hip_bm3d_set_tracing(self._h, int(on)))
@@ -380,8 +390,10 @@ class Server:
     any format, frames queued as they arrive, packed into shared launches by
     a native scheduler thread under an HBM budget."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_frames=8, max_inflight=2, use_graphs=True):
-        cfg = ServerConfig(device, int(hbm_budget_bytes), max_batch_frames, max_inflight, int(bool(use_graphs)))
+    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_frames=8, max_inflight=2, use_graphs=True,
+                 autotune=False):
+        cfg = ServerConfig(device, int(hbm_budget_bytes), max_batch_frames, max_inflight, int(bool(use_graphs)),
+                           int(bool(autotune)))
         h = ctypes.c_void_p()
         _check(lib().nss_hip_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

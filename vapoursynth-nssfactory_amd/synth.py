@@ -117,7 +117,7 @@ def shifted_sequence(fmt: str, width: int, height: int, frames: int, sigma_user:
             h, w = p.shape
             yi = np.clip(np.arange(h) - sy, 0, h - 1)
             xi = np.clip(np.arange(w) - sx, 0, w - 1)
-            planes.append(p[yi][:, xi])
+            planes.append(np.ascontiguousarray(p[yi][:, xi]))
         clean_seq.append(planes)
         noisy_seq.append([add_noise(p, sigma_user, 50000 + f * 16 + i + seed * 7919) for i, p in enumerate(planes)])
     return clean_seq, noisy_seq
