@@ -319,6 +319,7 @@ def serving_workload(args):
                  for k, (fmt, w, h, r) in enumerate(specs) if not r for sg in (3.0, 5.0, 10.0, 20.0)}
     lat_p = []
     seen = [0] * len(specs)
+    arrived = [[] for _ in specs]  # radius > 0: arrival time of each frame
     stream = torch.cuda.current_stream()
     t0 = time.perf_counter()
     for n, (t, k) in enumerate(arrivals):
@@ -328,21 +329,53 @@ def serving_workload(args):
         seen[k] += 1
         fmt, w, h, r = specs[k]
         if r:
+            # a frame's latency runs from its own arrival to its pop
+            arrived[k].append(t)
             insts[k].push(frame_of(k, j))
-            while insts[k].pop(out=outs[k][0]) is not None:
-                pass
+            done = []
+            while (res := insts[k].pop(out=outs[k][0])) is not None:
+                done.append(res[0])
+            stream.synchronize()
+            now = time.perf_counter() - t0
+            lat_p += [(now - arrived[k][f]) * 1e3 for f in done]
         else:
             per_sigma[(k, sigmas[n])](frame_of(k, j), out=outs[k][0])
-        stream.synchronize()
-        lat_p.append((time.perf_counter() - t0 - t) * 1e3)
+            stream.synchronize()
+            lat_p.append((time.perf_counter() - t0 - t) * 1e3)
     wall_p = time.perf_counter() - t0
-    for i in insts + list(per_sigma.values()):
-        i.close()
     lat_p = np.array(lat_p)
     plain = {"frames_per_s": round(len(arrivals) / wall_p, 2), "mpix_per_s": round(pix / wall_p / 1e6, 2),
              "p50_ms": round(float(np.percentile(lat_p, 50)), 3), "p99_ms": round(float(np.percentile(lat_p, 99)), 3),
              "wall_s": round(wall_p, 3)}
     print(f"# plain {plain}", file=sys.stderr, flush=True)
+
+    # --- saturated: every radius-0 frame queued at once (throughput) ---
+    burst = [(k, j) for k in range(len(specs)) if not specs[k][3] for j in range(counts[k])]
+    pix_b = sum(specs[k][1] * specs[k][2] for k, _ in burst)
+    srv = nss_hip.Server(max_batch_frames=8, max_inflight=2, use_graphs=False)
+    sids = [srv.open_stream(fmt, w, h, sigma=5, final=1) if not r else None
+            for k, (fmt, w, h, r) in enumerate(specs)]
+    bouts = {k: [[torch.empty_like(p) for p in inputs[k][0]] for _ in range(counts[k])]
+             for k in range(len(specs)) if not specs[k][3]}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, j in burst:
+        srv.submit(sids[k], frame_of(k, j), bouts[k][j], sigma=sigmas[j % len(sigmas)])
+    srv.drain()
+    wall_b = time.perf_counter() - t0
+    stb = srv.stats()
+    srv.close()
+    t0 = time.perf_counter()
+    for k, j in burst:
+        per_sigma[(k, sigmas[j % len(sigmas)])](frame_of(k, j), out=outs[k][0])
+    stream.synchronize()
+    wall_bp = time.perf_counter() - t0
+    saturated = {"frames": len(burst), "server_mpix_per_s": round(pix_b / wall_b / 1e6, 1),
+                 "plain_mpix_per_s": round(pix_b / wall_bp / 1e6, 1),
+                 "server_frames_per_batch": round(stb["batched_frames"] / max(stb["batches"], 1), 2)}
+    print(f"# saturated {saturated}", file=sys.stderr, flush=True)
+    for i in insts + list(per_sigma.values()):
+        i.close()
     print(json.dumps({"metric": METRIC, "value": server["mpix_per_s"], "unit": "Mpix/s", "n_gpus": 1,
                       "steps": len(arrivals), "warmup": 0, "ms_per_step": round(wall / len(arrivals) * 1e3, 4),
                       "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -350,7 +383,8 @@ def serving_workload(args):
                       "config": {"workload": "serving: 720x480 YUV420PS r1, 1920x1080 YUV420PS, 3840x2160 RGBS, "
                                              f"Poisson 24 fps per stream for {seconds} s, two stages, per-request "
                                              "sigma in {3,5,10,20} on the radius-0 streams"},
-                      "p50_ms": server["p50_ms"], "p99_ms": server["p99_ms"], "server": server, "plain": plain}),
+                      "p50_ms": server["p50_ms"], "p99_ms": server["p99_ms"], "server": server, "plain": plain,
+                      "saturated": saturated}),
           flush=True)
 
 

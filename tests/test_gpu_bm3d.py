@@ -210,3 +210,29 @@ def test_run_to_run_identity(gpu):
     b = _run(gpu, "YUV420PS", noisy, sigma=10, final=1)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_final_equals_two_calls(gpu):
+    """final=1 is the basic filter (the *_basic values) followed by the
+    Wiener filter with the basic estimate as ref, bitwise
+    (src/frontend/bm3d_args.cpp:161-196)."""
+    import synth
+    _, noisy = synth.noisy_frame("YUV420PS", 128, 96, 10, seed=12)
+    # basic stage: sigma 12, group 8; Wiener: sigma 10, group 16; both step 4
+    fused = _run(gpu, "YUV420PS", noisy, final=1, sigma=10, block_step=4, group_size=16, sigma_basic=12,
+                 group_size_basic=8)
+    basic = _run(gpu, "YUV420PS", noisy, sigma=12, group_size=8, block_step=4)
+    wiener = _run(gpu, "YUV420PS", noisy, ref=basic, sigma=10, block_step=4, group_size=16)
+    for a, b in zip(fused, wiener):
+        assert np.array_equal(a, b)
+
+
+def test_memory_limit_does_not_change_output(gpu):
+    """tests/cuda/test_cuda_bm3d_live.py:352-369: memory_limit_mb never
+    changes the result."""
+    import synth
+    _, noisy = synth.noisy_frame("YUV420PS", 96, 64, 10, seed=13)
+    a = _run(gpu, "YUV420PS", noisy, sigma=10, final=1)
+    b = _run(gpu, "YUV420PS", noisy, sigma=10, final=1, memory_limit_mb=1)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

@@ -897,27 +897,33 @@ struct Operand<kFormF16> {
     h16x4 v;
 };
 
-typedef __bf16 b16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 b16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-// Round to nearest even (v_cvt_pk_bf16_f32 / v_cvt_pk_f16_f32, two values per
-// instruction); bf16x3 keeps the rounding residual as the lo part.
+// Two values rounded to nearest even into one dword (v_cvt_pk_bf16_f32 /
+// v_cvt_pk_f16_f32).
+__device__ inline uint32_t pack_bf16(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, b16x2));
+}
+__device__ inline uint32_t pack_f16(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, h16x2));
+}
+
+// bf16x3 keeps the rounding residual as the lo part; the hi values are read
+// back from the packed bits (low half << 16, high half masked).
 template <int FORM>
 __device__ inline Operand<FORM> operand(f32x4 v) {
     Operand<FORM> o;
     if constexpr (FORM == kFormBf16x3) {
-        b16x4 hi, lo;
-        f32x4 r;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) hi[j] = static_cast<__bf16>(v[j]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) r[j] = v[j] - static_cast<float>(hi[j]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) lo[j] = static_cast<__bf16>(r[j]);
-        o.hi = __builtin_bit_cast(s16x4, hi);
-        o.lo = __builtin_bit_cast(s16x4, lo);
+        const uint32_t h0 = pack_bf16(v[0], v[1]), h1 = pack_bf16(v[2], v[3]);
+        const uint32_t l0 = pack_bf16(v[0] - __uint_as_float(h0 << 16), v[1] - __uint_as_float(h0 & 0xffff0000u));
+        const uint32_t l1 = pack_bf16(v[2] - __uint_as_float(h1 << 16), v[3] - __uint_as_float(h1 & 0xffff0000u));
+        o.hi = __builtin_bit_cast(s16x4, u32x2{h0, h1});
+        o.lo = __builtin_bit_cast(s16x4, u32x2{l0, l1});
     } else {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) o.v[j] = static_cast<_Float16>(v[j]);
+        o.v = __builtin_bit_cast(h16x4, u32x2{pack_f16(v[0], v[1]), pack_f16(v[2], v[3])});
     }
     return o;
 }
@@ -985,14 +991,12 @@ __device__ inline f32x4 haar_unit_entries(bool transposed, int lane) {
 }
 
 // Coefficient-major transpose buffer of one wave: 64 coefficients x 16
-// patches (dwords), patch groups of 4 permuted per coefficient and a pad
-// after 32 coefficients, so that the b32 scatters and b128 reads below hit
-// distinct banks.
-inline constexpr int kMfmaBuf = 63 * 24 + 16 + 16;
+// patches (dwords) at a pitch of 20, patch quads permuted per coefficient
+// (XOR by 4 ((coef >> 4) & 3)), so that the b32 scatters / gathers and the
+// b128 accesses below are at most 2-way bank conflicts.
+inline constexpr int kMfmaBuf = 64 * 20;
 
-__device__ inline int mfma_slot(int coef, int patch) {
-    return coef * 24 + (coef >> 5) * 16 + (patch ^ (4 * (((coef >> 3) ^ (coef >> 5)) & 3)));
-}
+__device__ inline int mfma_slot(int coef, int patch) { return coef * 20 + (patch ^ (4 * ((coef >> 4) & 3))); }
 
 // Forward 3-D transform of a unit. x[q] = lane's pixels of tile q in the A
 // layout, already scaled. Out: raw Haar products out[t][j] = coefficient
@@ -1046,9 +1050,10 @@ __device__ inline void inverse_unit(const f32x4 (&c)[4], const Operand<FORM>& dc
 
 // Lane's pixels of a unit in the A layout (zero for patches beyond a group's
 // count or groups beyond the tile): tile q, patch 4 q + 2 ((lane >> 3) & 1) +
-// (lane >> 5), row lane & 7, columns 4 ((lane >> 4) & 1) + j.
+// (lane >> 5), row lane & 7, columns 4 ((lane >> 4) & 1) + j. Patch positions
+// as x | y << 16.
 template <int G>
-__device__ inline void load_unit(const float* plane, int stride, const MatchRec* recs, const int* cnt, int ngroups,
+__device__ inline void load_unit(const float* plane, int stride, const uint32_t* pos, const int* cnt, int ngroups,
                                  int unit, int lane, f32x4 (&x)[4]) {
     const int pr = (lane >> 3) & 1, pc = lane >> 5, y = lane & 7, x0 = 4 * ((lane >> 4) & 1);
 #pragma unroll
@@ -1056,8 +1061,8 @@ __device__ inline void load_unit(const float* plane, int stride, const MatchRec*
         const int p = 4 * q + 2 * pr + pc;
         const int g = G == 8 ? 2 * unit + (p >> 3) : unit, idx = p & (G - 1);
         if (g < ngroups && idx < cnt[g]) {
-            const MatchRec r = recs[g * G + idx];
-            const float* src = plane + static_cast<size_t>(r.y + y) * stride + r.x + x0;
+            const uint32_t xy = pos[g * G + idx];
+            const float* src = plane + static_cast<size_t>((xy >> 16) + y) * stride + (xy & 0xffffu) + x0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[q][j] = src[j];
         } else {
@@ -1120,9 +1125,10 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     const int nunits = G == 8 ? (ngroups + 1) / 2 : ngroups;
     unsigned long long* num = acc;
     unsigned long long* corner = acc + aw * ah;
-    // (records, counts and the transpose buffers start 16-byte aligned)
-    MatchRec* recs = reinterpret_cast<MatchRec*>(acc + ((aw * ah + cw * ch + 1) & ~1));
-    int* cnt = reinterpret_cast<int*>(recs + tg.tw * tg.th * G);
+    // Patch positions (x | y << 16) and counts of the tile's groups, then the
+    // transpose buffers, 16-byte aligned.
+    uint32_t* recs = reinterpret_cast<uint32_t*>(acc + ((aw * ah + cw * ch + 1) & ~1));
+    int* cnt = reinterpret_cast<int*>(recs + ((tg.tw * tg.th * G + 3) & ~3));
     float* scratch = reinterpret_cast<float*>(cnt + ((tg.tw * tg.th + 3) & ~3));
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* T = scratch + wave * kMfmaBuf;
@@ -1130,7 +1136,8 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     for (int e = threadIdx.x; e < ngroups * G; e += 256) {
         const int g = e / G, j = e - g * G;
         const int gj = g / ni, gi = g - gj * ni;
-        recs[e] = jb.matches[static_cast<size_t>((j_begin + gj) * jb.grid.nx + i_begin + gi) * G + j];
+        const MatchRec r = jb.matches[static_cast<size_t>((j_begin + gj) * jb.grid.nx + i_begin + gi) * G + j];
+        recs[e] = static_cast<uint32_t>(r.x) | (static_cast<uint32_t>(r.y) << 16);
     }
     for (int g = threadIdx.x; g < ngroups; g += 256) {
         const int gj = g / ni, gi = g - gj * ni;
@@ -1226,8 +1233,8 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
             const int g = G == 8 ? 2 * u + (p >> 3) : u, idx = p & (G - 1);
             const int gsel = G == 8 ? (q >> 1) : 0;
             if (g < ngroups && idx < cnt[g]) {
-                const MatchRec rec = recs[g * G + idx];
-                const int lx = rec.x - ax0, ly = rec.y - ay0;
+                const uint32_t xy = recs[g * G + idx];
+                const int lx = static_cast<int>(xy & 0xffffu) - ax0, ly = static_cast<int>(xy >> 16) - ay0;
                 const float w = wg[gsel];
                 const float wq = w / s[gsel];
                 unsigned long long* cell = num + (ly + y0) * aw + lx + px;
@@ -1591,7 +1598,7 @@ size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t) {
 size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t) {
     const size_t aw = t.aw, ah = t.ah;
     const size_t acc = ((aw * ah + (aw - 7) * (ah - 7) + 1) & ~static_cast<size_t>(1)) * 8;
-    const size_t recs = static_cast<size_t>(t.tw) * t.th * group * sizeof(MatchRec) +
+    const size_t recs = ((static_cast<size_t>(t.tw) * t.th * group + 3) & ~static_cast<size_t>(3)) * 4 +
                         ((static_cast<size_t>(t.tw) * t.th + 3) & ~static_cast<size_t>(3)) * 4;
     const size_t tbuf = 4 * static_cast<size_t>(kMfmaBuf) * 4;
     const size_t boxsum = (ah - 7) * aw * 4;

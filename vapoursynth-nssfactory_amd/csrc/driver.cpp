@@ -150,7 +150,10 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
                 if (st.plane[bk.planes[k]].group != fast) fast = 0;
             if (fast != 8 && fast != 16 && fast != 32) fast = 0;
         }
-        const int form = (fast == 8 || fast == 16) ? h->p.transform : 0;
+        // (the MFMA kernel keeps patch positions as 16-bit x and y)
+        bool small = true;
+        for (int k = 0; k < bk.nplanes; ++k) small &= h->pw[bk.planes[k]] <= 65535 && h->ph[bk.planes[k]] <= 65535;
+        const int form = ((fast == 8 || fast == 16) && small) ? h->p.transform : 0;
         bk.fast_group = fast;
         bk.form = form;
         for (const auto& sh : kTileShapes)

@@ -51,7 +51,7 @@ struct DeviceGuard {
 constexpr size_t kLdsLimit = 160 * 1024;  // LDS per workgroup on gfx950
 // Tile aggregation aims at two workgroups per CU; a one-reference tile may
 // take the whole LDS.
-constexpr size_t kTileLdsBudget = 76 * 1024;
+constexpr size_t kTileLdsBudget = 80 * 1024;
 constexpr size_t kRunLdsBudget = 64 * 1024;
 
 int ensure_tables(int device);

@@ -13,8 +13,8 @@
 //           launch sequence over the planes of every frame, sizes and sigmas
 //           mixed) on one of max_inflight HIP streams, so a batch starts while
 //           the previous ones still run; a batch whose composition (frames,
-//           planes, slots, sigmas) was launched before replays the hipGraph
-//           captured then.
+//           planes, slots, sigmas) is seen a second time is captured as a
+//           hipGraph, and replayed from then on.
 // Radius > 0 streams keep their rolling state in their own instance and run
 // push / pop on a HIP stream of their own, in frame order.
 #include <hip/hip_runtime.h>
@@ -30,6 +30,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "instance.hpp"
@@ -106,6 +107,7 @@ struct nss_hip_server {
     int next_hs = 0;
     std::vector<hipEvent_t> event_pool;
     std::unordered_map<std::string, GraphEntry> graphs;
+    std::unordered_set<std::string> seen_once;
     unsigned long long use_clock = 0;
     long long next_id = 1;
     nss_hip_server_stats st{};
@@ -240,10 +242,19 @@ struct nss_hip_server {
         return key;
     }
 
+    // A composition runs direct the first time it is seen and is captured the
+    // second time; later sightings replay the graph.
     int run_graph(const std::vector<FrameRef>& frames, hipStream_t s) {
         std::string key = graph_key(frames);
         auto it = graphs.find(key);
         if (it == graphs.end()) {
+            auto seen = seen_once.find(key);
+            if (seen == seen_once.end()) {
+                if (seen_once.size() >= 4 * kGraphCacheLimit) seen_once.clear();
+                seen_once.insert(key);
+                return run_batch(frames.data(), static_cast<int>(frames.size()), nullptr, s);
+            }
+            seen_once.erase(seen);
             if (graphs.size() >= kGraphCacheLimit) {
                 auto lru = graphs.begin();
                 for (auto g = graphs.begin(); g != graphs.end(); ++g)
