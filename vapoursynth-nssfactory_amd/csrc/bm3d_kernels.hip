@@ -242,23 +242,46 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
         out[0] = MatchRec{cx, cy, jb.t0, 0u};
         jb.counts[ref_id] = 1 + K;
     }
-    for (int k = 0; k < kmax; ++k) {
-        unsigned long long best = ~0ull;
-        if (k < K) {
-            for (int c = hl; c < ncand; c += 32) {
-                const uint32_t d = dk[c];
-                const unsigned long long key = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
-                best = (d != kTaken && key < best) ? key : best;
+    // Each lane keeps the four smallest of its keys (window indices c = hl
+    // mod 32) sorted in registers; a round's winner is the half-wave minimum
+    // of the lanes' heads, and only the winning lane advances (rebuilding its
+    // four from LDS, where given-away keys are marked taken, after its fourth).
+    unsigned long long best4[4];
+    auto rebuild = [&]() {
+        best4[0] = best4[1] = best4[2] = best4[3] = ~0ull;
+        for (int c = hl; c < ncand; c += 32) {
+            const uint32_t d = dk[c];
+            const unsigned long long x = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
+            if (d != kTaken && x < best4[3]) {
+                best4[3] = x;
+                if (best4[3] < best4[2]) { const unsigned long long t = best4[2]; best4[2] = best4[3]; best4[3] = t; }
+                if (best4[2] < best4[1]) { const unsigned long long t = best4[1]; best4[1] = best4[2]; best4[2] = t; }
+                if (best4[1] < best4[0]) { const unsigned long long t = best4[0]; best4[0] = best4[1]; best4[1] = t; }
             }
         }
-        best = half_min_u64(best);
+    };
+    if (K > 0) {
+        rebuild();
+    } else {
+        best4[0] = best4[1] = best4[2] = best4[3] = ~0ull;
+    }
+    int given = 0;
+    for (int k = 0; k < kmax; ++k) {
+        const unsigned long long best = half_min_u64(k < K ? best4[0] : ~0ull);
         const int c = static_cast<int>(static_cast<uint32_t>(best));
         if (k < K && hl == (c & 31)) {
             dk[c] = kTaken;
             const int dy = c / ww, dx = c - dy * ww;
             out[1 + k] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(best >> 32)};
+            best4[0] = best4[1];
+            best4[1] = best4[2];
+            best4[2] = best4[3];
+            best4[3] = ~0ull;
+            if (++given == 4) {
+                given = 0;
+                rebuild();
+            }
         }
-        wave_sync();
     }
 }
 
@@ -1459,6 +1482,100 @@ __global__ __launch_bounds__(64) void filter_temporal_kernel(TFilterTable tab) {
     }
 }
 
+// Block 8, G in {8, 16, 32}: four groups per workgroup, a wave per group with
+// the cube in registers (the spatial register kernel's transforms); patches
+// from their slot's frame, device-scope int64 atomics into the sums of that
+// frame (den at the patch corner).
+template <int G, bool WIENER>
+__global__ __launch_bounds__(256) void filter_temporal_b8_kernel(TFilterTable tab) {
+    extern __shared__ __align__(16) float tbuf[];
+    constexpr int M = G / 8;
+    const int lb = xcd_block();
+    const TFilterJob& jb = tab.job[find_job(tab, lb)];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = (lb - jb.wg_begin) * 4 + wave;
+    if (g >= jb.nref) return;  // whole wave; no workgroup barrier below
+    float* T = tbuf + wave * (G * kTs);
+    const int k = jb.counts[g];
+    const MatchRec* rec = jb.matches + static_cast<size_t>(g) * G;
+    const int gb = lane >> 3, y = lane & 7;
+    float v[M][8], vr[M][8];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int p = gb + 8 * m;
+        if (p < k) {
+            const MatchRec r = rec[p];
+            const int f = jb.slots.frame(r.t);
+            const size_t o = static_cast<size_t>(r.y + y) * jb.stride + r.x;
+            const float* sp = jb.src.frame(f) + o;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) v[m][x] = sp[x];
+            if constexpr (WIENER) {
+                const float* wp = jb.wref.frame(f) + static_cast<size_t>(r.y + y) * jb.wref_stride + r.x;
+#pragma unroll
+                for (int x = 0; x < 8; ++x) vr[m][x] = wp[x];
+            }
+        } else {
+#pragma unroll
+            for (int x = 0; x < 8; ++x) v[m][x] = vr[m][x] = 0.f;
+        }
+    }
+    float c[G], r[G];
+    forward_b8<G>(v, c, T, lane);
+    if constexpr (WIENER) forward_b8<G>(vr, r, T, lane);
+    const float thr = 2.7f * jb.sigma;  // include/nss/params.hpp:46 kBmHardLambda
+    const float sig2 = jb.sigma * jb.sigma;
+    int kept = 0;
+    float w2 = 0.f;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const bool dc = (lane == 0 && j == 0);
+        if constexpr (WIENER) {
+            float w = 1.f;
+            if (!dc) {
+                const float r2 = r[j] * r[j];
+                const float den = r2 + sig2;
+                w = den > 0.f ? r2 / den : 0.f;
+            }
+            c[j] *= w;
+            w2 = fmaf(w, w, w2);
+        } else {
+            if (!dc && fabsf(c[j]) < thr)
+                c[j] = 0.f;
+            else
+                ++kept;
+        }
+    }
+    haar_inverse_reg<G>(c);
+    float u[M][8];
+    group_to_columns<G>(c, u, T, lane);
+#pragma unroll
+    for (int m = 0; m < M; ++m) dct8<true>(u[m]);
+    transpose_rc<M>(u, T, lane);
+#pragma unroll
+    for (int m = 0; m < M; ++m) dct8<true>(u[m]);
+    float w;
+    if constexpr (WIENER) {
+        w = 1.f / fmaxf(wave_sum_f32(w2), 1e-12f);
+    } else {
+        w = 1.f / static_cast<float>(max(wave_sum_i32(kept), 1));
+    }
+    const unsigned long long wfix = to_fixed(w);
+    const size_t plane = static_cast<size_t>(jb.width) * jb.height;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int p = gb + 8 * m;
+        if (p < k) {
+            const MatchRec rr = rec[p];
+            const size_t slot = static_cast<size_t>(jb.slots.frame(rr.t) % jb.acc_ring) * plane;
+            unsigned long long* row = jb.num + slot + static_cast<size_t>(rr.y + y) * jb.width + rr.x;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) atomicAdd(row + x, to_fixed(w * u[m][x]));
+            if (y == 0) atomicAdd(jb.den + slot + static_cast<size_t>(rr.y) * jb.width + rr.x, wfix);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Finish (src/cpu/bm/agg.cpp:17-42 AggFinish): dst = den > 1e-12 ? num/den : src.
 // ---------------------------------------------------------------------------
@@ -1785,8 +1902,31 @@ static hipError_t launch_tfilter_b(const TFilterTable& t, int total_wg, size_t l
     return hipGetLastError();
 }
 
+size_t filter_temporal_b8_lds_bytes(int group) { return 4 * static_cast<size_t>(group) * kTs * 4; }
+
+template <int G>
+static hipError_t launch_tfilter_b8(const TFilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
+    if (wiener) {
+        allow_lds(&filter_temporal_b8_kernel<G, true>, lds);
+        filter_temporal_b8_kernel<G, true><<<total_wg, 256, lds, s>>>(t);
+    } else {
+        allow_lds(&filter_temporal_b8_kernel<G, false>, lds);
+        filter_temporal_b8_kernel<G, false><<<total_wg, 256, lds, s>>>(t);
+    }
+    return hipGetLastError();
+}
+
+// fast_group 8, 16 or 32 (block 8): four groups per workgroup, register cube.
 hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
-                                  hipStream_t s) {
+                                  int fast_group, hipStream_t s) {
+    if (block == 8 && fast_group) {
+        switch (fast_group) {
+            case 8: return launch_tfilter_b8<8>(t, total_wg, lds, wiener, s);
+            case 16: return launch_tfilter_b8<16>(t, total_wg, lds, wiener, s);
+            case 32: return launch_tfilter_b8<32>(t, total_wg, lds, wiener, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (block) {
         case 1: return launch_tfilter_b<1>(t, total_wg, lds, wiener, s);
         case 2: return launch_tfilter_b<2>(t, total_wg, lds, wiener, s);

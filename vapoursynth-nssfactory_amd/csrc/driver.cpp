@@ -658,6 +658,15 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
                 predictive = true;
             }
         }
+        // Block 8 with one group size in {8, 16, 32}: the register kernel,
+        // four groups per workgroup.
+        int fast = 0;
+        if (b == 8) {
+            fast = stg.plane[bk.planes[0]].group;
+            for (int k = 1; k < bk.nplanes; ++k)
+                if (stg.plane[bk.planes[k]].group != fast) fast = 0;
+            if (fast != 8 && fast != 16 && fast != 32) fast = 0;
+        }
         for (int k = 0; k < bk.nplanes; ++k) {
             const int i = bk.planes[k];
             const int mi = h->p.chroma ? 0 : i;
@@ -680,8 +689,9 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             f.num = R.num[s][i];
             f.den = R.den[s][i];
             f.acc_ring = R.acc_n;
-            fwg += f.nref;
-            flds = std::max(flds, filter_atomic_lds_bytes(b, sp.group, stg.wiener));
+            fwg += fast ? (f.nref + 3) / 4 : f.nref;
+            flds = std::max(flds, fast ? filter_temporal_b8_lds_bytes(sp.group)
+                                       : filter_atomic_lds_bytes(b, sp.group, stg.wiener));
         }
         hipError_t e = traced(h, 1, st, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, st); });
         if (e != hipSuccess) return hip_fail(e, "match launch");
@@ -689,7 +699,7 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             e = traced(h, 1, st, [&] { return launch_predictive(b, pt, pwg, plds, st); });
             if (e != hipSuccess) return hip_fail(e, "predictive match launch");
         }
-        e = traced(h, 2, st, [&] { return launch_filter_temporal(b, ft, fwg, flds, stg.wiener, st); });
+        e = traced(h, 2, st, [&] { return launch_filter_temporal(b, ft, fwg, flds, stg.wiener, fast, st); });
         if (e != hipSuccess) return hip_fail(e, "filter launch");
     }
     return NSS_HIP_OK;

@@ -29,7 +29,8 @@ size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t);
 size_t predictive_lds_bytes(int block, int group, int ps_num, int radius);
 hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s);
 hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
-                                  hipStream_t s);
+                                  int fast_group, hipStream_t s);
+size_t filter_temporal_b8_lds_bytes(int group);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
