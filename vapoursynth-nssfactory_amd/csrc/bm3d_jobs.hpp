@@ -182,6 +182,58 @@ struct TFilterTable {
     TFilterJob job[kMaxJobs];
 };
 
+// Temporal tile path (block 8, radius <= 2): every slot's contributions of a
+// tile of references accumulate in LDS over that slot's apron (range +
+// |t - r| * ps_range around the tile's references, the predictive search's
+// reach), then go out as float partials of the centre.
+inline constexpr int kMaxTileSlots = 5;
+
+struct TTileJob {
+    FrameRing src, wref;
+    int width, height, stride, wref_stride;
+    Grid grid;
+    int nslots;
+    TileGeom tiles[kMaxTileSlots];  // per slot; tw, th, ntx, nty shared
+    int range[kMaxTileSlots];
+    float* part;                    // this centre's partials
+    long long slot_off[kMaxTileSlots];  // floats: slot t's num at part + slot_off[t], den after it
+    int nref, group;
+    float sigma;
+    Slots slots;
+    int wg_begin;
+    const MatchRec* matches;
+    const int* counts;
+};
+
+struct TTileTable {
+    int n;
+    TTileJob job[3];
+};
+
+// Frame n's finish from the partials of the centres that reach it.
+struct TSource {
+    const float* num;
+    const float* den;
+    TileGeom tiles;
+    int range;
+};
+
+struct TFinishJob {
+    TSource src[kMaxTileSlots];
+    int nsrc;
+    int block, step;
+    const float* fallback;  // the source frame
+    float* dst;
+    int width, height;
+    int tiles_x;  // 64 x 16 output blocks per row
+    int wg_begin;
+};
+
+struct TFinishTable {
+    int n;
+    TFinishJob job[3];
+};
+
 struct ZeroJob {
     unsigned long long* p;
     long long count;

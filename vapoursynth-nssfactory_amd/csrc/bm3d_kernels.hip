@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "bm3d_jobs.hpp"
 
 namespace nsship {
@@ -547,29 +549,41 @@ __device__ inline void lds_add(unsigned long long* p, unsigned long long v) {
 
 // Tile epilogue: den = B x B box sum of the corner weights (horizontal pass
 // into scratch, then vertical), and num, as the tile's float partials (rows of
-// tg.pitch floats, pixel x at column x - ax0 + (ax0 & 3), so that x = 0 mod 4
-// is 16-byte aligned in every tile). Called by all 256 threads after the
-// accumulation's barrier.
+// `pitch` floats; pn / pd point at column ax0 & 3 of the tile's first row, so
+// that x = 0 mod 4 is 16-byte aligned in every tile). Called by every thread
+// of the block after the accumulation's barrier; scratch holds
+// (ah - B + 1) * aw floats.
 template <int B>
-__device__ inline void tile_partials(const FilterJob& jb, int tile, int ax0, const unsigned long long* num,
-                                     const unsigned long long* corner, float* scratch) {
-    const int aw = jb.tiles.aw, ah = jb.tiles.ah, pitch = jb.tiles.pitch;
+__device__ inline void partials_out(int aw, int ah, int pitch, const unsigned long long* num,
+                                    const unsigned long long* corner, float* scratch, float* pn, float* pd) {
     const int cw = aw - B + 1, ch = ah - B + 1;
-    for (int e = threadIdx.x; e < ch * aw; e += 256) {
-        const int y = e / aw, x = e - (e / aw) * aw;
-        unsigned long long s = 0;
+    const int nt = static_cast<int>(blockDim.x);
+    // Horizontal: a thread slides the B-wide window over 8 outputs of a row
+    // (exact int64 sums, so the order of the adds does not matter).
+    const int segs = (aw + 7) >> 3;
+    for (int e = threadIdx.x; e < ch * segs; e += nt) {
+        const int y = e / segs, x0 = (e - y * segs) * 8;
+        const unsigned long long* row = corner + y * cw;
+        unsigned long long w[B + 7];  // corners x0 - B + 1 .. x0 + 7
 #pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int cx = x - i;
-            if (cx >= 0 && cx < cw) s += corner[y * cw + cx];
+        for (int i = 0; i < B + 7; ++i) {
+            const int cx = x0 - B + 1 + i;
+            w[i] = (cx >= 0 && cx < cw) ? row[cx] : 0ull;
         }
-        scratch[e] = static_cast<float>(static_cast<double>(static_cast<long long>(s)) * (1.0 / 4294967296.0));
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int i = 0; i < B - 1; ++i) sum += w[i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            sum += w[B - 1 + j];
+            if (x0 + j < aw)
+                scratch[y * aw + x0 + j] =
+                    static_cast<float>(static_cast<double>(static_cast<long long>(sum)) * (1.0 / 4294967296.0));
+            sum -= w[j];
+        }
     }
     __syncthreads();
-    const size_t base = static_cast<size_t>(tile) * pitch * ah + (ax0 & 3);
-    float* pn = jb.part_num + base;
-    float* pd = jb.part_den + base;
-    for (int e = threadIdx.x; e < aw * ah; e += 256) {
+    for (int e = threadIdx.x; e < aw * ah; e += nt) {
         const int y = e / aw, x = e - (e / aw) * aw;
         float d = 0.f;
 #pragma unroll
@@ -580,6 +594,14 @@ __device__ inline void tile_partials(const FilterJob& jb, int tile, int ax0, con
         pn[y * pitch + x] = static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
         pd[y * pitch + x] = d;
     }
+}
+
+template <int B>
+__device__ inline void tile_partials(const FilterJob& jb, int tile, int ax0, const unsigned long long* num,
+                                     const unsigned long long* corner, float* scratch) {
+    const size_t base = static_cast<size_t>(tile) * jb.tiles.pitch * jb.tiles.ah + (ax0 & 3);
+    partials_out<B>(jb.tiles.aw, jb.tiles.ah, jb.tiles.pitch, num, corner, scratch, jb.part_num + base,
+                    jb.part_den + base);
 }
 
 // Tile path: sums of a tile of groups in LDS, then float partials.
@@ -1365,30 +1387,32 @@ __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
             }
             const int bw = xmax - xmin + 1;
             const int cells = bw * (ymax - ymin + 1);
+            auto ssd = [&](const float* base, int stride) {
+                float acc = 0.f;
+                for (int py = 0; py < B; ++py) {
+                    const float* row = base + py * stride;
+#pragma unroll
+                    for (int px = 0; px < B; ++px) {
+                        const float d = refp[py * B + px] - row[px];
+                        acc = fmaf(d, d, acc);
+                    }
+                }
+                return acc;
+            };
             int nb = 0;
             for (int base = 0; base < cells; base += kPredChunk) {
                 const int cnt = min(kPredChunk, cells - base);
                 for (int c = lane; c < cnt; c += 64) {
                     const int cell = base + c;
-                    const int y = ymin + cell / bw, x = xmin + cell % bw;
+                    const int dy = cell / bw, dx = cell - dy * bw;
+                    const int y = ymin + dy, x = xmin + dx;
                     bool member = false;
                     for (int i = 0; i < nc; ++i) {
                         const int ccy = centers[i] >> 13, ccx = centers[i] & 8191;
                         member |= abs(y - ccy) <= ps && x >= max(0, ccx - ps) && x <= min(maxx, ccx + ps);
                     }
                     uint32_t key = kTaken;
-                    if (member) {
-                        float acc = 0.f;
-                        for (int py = 0; py < B; ++py) {
-                            const float* row = pt + static_cast<size_t>(y + py) * jb.stride + x;
-#pragma unroll
-                            for (int px = 0; px < B; ++px) {
-                                const float d = refp[py * B + px] - row[px];
-                                acc = fmaf(d, d, acc);
-                            }
-                        }
-                        key = dist_key(acc);
-                    }
+                    if (member) key = dist_key(ssd(pt + static_cast<size_t>(y) * jb.stride + x, jb.stride));
                     cand[c] = key;
                 }
                 __syncthreads();
@@ -1574,6 +1598,208 @@ __global__ __launch_bounds__(256) void filter_temporal_b8_kernel(TFilterTable ta
             if (y == 0) atomicAdd(jb.den + slot + static_cast<size_t>(rr.y) * jb.width + rr.x, wfix);
         }
     }
+}
+
+// Temporal tile path, block 8, G in {8, 16, 32}: one workgroup (8 waves) per
+// tile of references; a wave per group with the register cube; every slot's
+// contributions go into that slot's LDS sums (apron = the slot's reach around
+// the tile), then out as the centre's float partials, slot by slot.
+// LDS: slot table | per slot num [ah][aw] and corners [ch][cw] (int64) |
+// match records | counts | per-wave transpose buffers (also the box sum).
+inline constexpr int kTTileWaves = 8;
+
+struct SlotLds {
+    int off;       // int64 offset of the slot's num (corners follow)
+    int ax0, ay0;  // apron origin
+    int aw, ah;
+};
+
+template <int G, bool WIENER>
+__global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable tab) {
+    extern __shared__ __align__(16) unsigned long long acc[];
+    constexpr int B = 8;
+    constexpr int M = G / 8;
+    const int lb = xcd_block();
+    const TTileJob& jb = tab.job[find_job(tab, lb)];
+    const TileGeom& tg = jb.tiles[0];
+    const int tile = lb - jb.wg_begin;
+    const int tx = tile % tg.ntx, ty = tile / tg.ntx;
+    const int i_begin = tx * tg.tw, j_begin = ty * tg.th;
+    const int ni = min(tg.tw, jb.grid.nx - i_begin), nj = min(tg.th, jb.grid.ny - j_begin);
+    const int ngroups = ni * nj;
+    const int ns = jb.nslots;
+    __shared__ SlotLds sl[kMaxTileSlots];
+    int total = 0;
+    for (int t = 0; t < ns; ++t) {
+        const TileGeom& g = jb.tiles[t];
+        total += g.aw * g.ah + (g.aw - B + 1) * (g.ah - B + 1);
+    }
+    if (threadIdx.x < ns) {
+        const int t = threadIdx.x;
+        int o = 0;
+        for (int q = 0; q < t; ++q) o += jb.tiles[q].aw * jb.tiles[q].ah + (jb.tiles[q].aw - B + 1) * (jb.tiles[q].ah - B + 1);
+        const TileGeom& g = jb.tiles[t];
+        sl[t] = SlotLds{o, g.ax0(tx, jb.width, B, jb.grid.step, jb.range[t]),
+                        g.ay0(ty, jb.height, B, jb.grid.step, jb.range[t]), g.aw, g.ah};
+    }
+    MatchRec* recs = reinterpret_cast<MatchRec*>(acc + ((total + 1) & ~1));
+    int* cnt = reinterpret_cast<int*>(recs + tg.tw * tg.th * G);
+    float* scratch = reinterpret_cast<float*>(cnt + ((tg.tw * tg.th + 3) & ~3));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* T = scratch + wave * (G * kTs);
+    for (int e = threadIdx.x; e < total; e += 512) acc[e] = 0ull;
+    for (int e = threadIdx.x; e < ngroups * G; e += 512) {
+        const int g = e / G, j = e - g * G;
+        const int gj = g / ni, gi = g - gj * ni;
+        recs[e] = jb.matches[static_cast<size_t>((j_begin + gj) * jb.grid.nx + i_begin + gi) * G + j];
+    }
+    for (int g = threadIdx.x; g < ngroups; g += 512) {
+        const int gj = g / ni, gi = g - gj * ni;
+        cnt[g] = jb.counts[(j_begin + gj) * jb.grid.nx + i_begin + gi];
+    }
+    __syncthreads();
+
+    const float thr = 2.7f * jb.sigma;  // include/nss/params.hpp:46 kBmHardLambda
+    const float sig2 = jb.sigma * jb.sigma;
+    const int gb = lane >> 3, y = lane & 7;
+    for (int g = wave; g < ngroups; g += kTTileWaves) {
+        const int k = cnt[g];
+        const MatchRec* rec = recs + g * G;
+        float v[M][8], vr[M][8];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int p = gb + 8 * m;
+            if (p < k) {
+                const MatchRec r = rec[p];
+                const int f = jb.slots.frame(r.t);
+                const float* sp = jb.src.frame(f) + static_cast<size_t>(r.y + y) * jb.stride + r.x;
+#pragma unroll
+                for (int x = 0; x < 8; ++x) v[m][x] = sp[x];
+                if constexpr (WIENER) {
+                    const float* wp = jb.wref.frame(f) + static_cast<size_t>(r.y + y) * jb.wref_stride + r.x;
+#pragma unroll
+                    for (int x = 0; x < 8; ++x) vr[m][x] = wp[x];
+                }
+            } else {
+#pragma unroll
+                for (int x = 0; x < 8; ++x) v[m][x] = vr[m][x] = 0.f;
+            }
+        }
+        float c[G], r[G];
+        forward_b8<G>(v, c, T, lane);
+        if constexpr (WIENER) forward_b8<G>(vr, r, T, lane);
+        int kept = 0;
+        float w2 = 0.f;
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const bool dc = (lane == 0 && j == 0);
+            if constexpr (WIENER) {
+                float w = 1.f;
+                if (!dc) {
+                    const float r2 = r[j] * r[j];
+                    const float den = r2 + sig2;
+                    w = den > 0.f ? r2 / den : 0.f;
+                }
+                c[j] *= w;
+                w2 = fmaf(w, w, w2);
+            } else {
+                if (!dc && fabsf(c[j]) < thr)
+                    c[j] = 0.f;
+                else
+                    ++kept;
+            }
+        }
+        haar_inverse_reg<G>(c);
+        float u[M][8];
+        group_to_columns<G>(c, u, T, lane);
+#pragma unroll
+        for (int m = 0; m < M; ++m) dct8<true>(u[m]);
+        transpose_rc<M>(u, T, lane);
+#pragma unroll
+        for (int m = 0; m < M; ++m) dct8<true>(u[m]);
+        float w;
+        if constexpr (WIENER) {
+            w = 1.f / fmaxf(wave_sum_f32(w2), 1e-12f);
+        } else {
+            w = 1.f / static_cast<float>(max(wave_sum_i32(kept), 1));
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int p = gb + 8 * m;
+            if (p < k) {
+                const MatchRec rr = rec[p];
+                const SlotLds q = sl[rr.t];
+                const int lx = rr.x - q.ax0, ly = rr.y - q.ay0;
+                // The predictive search keeps every slot-t match within the
+                // slot's reach; a patch outside the apron is dropped rather
+                // than written out of bounds.
+                if (lx >= 0 && ly >= 0 && lx <= q.aw - B && ly <= q.ah - B) {
+                    unsigned long long* row = acc + q.off + (ly + y) * q.aw + lx;
+#pragma unroll
+                    for (int x = 0; x < 8; ++x) lds_add(row + x, to_fixed(w * u[m][x]));
+                    if (y == 0) lds_add(acc + q.off + q.aw * q.ah + ly * (q.aw - B + 1) + lx, to_fixed(w));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = 0; t < ns; ++t) {
+        const SlotLds q = sl[t];
+        const TileGeom& g = jb.tiles[t];
+        const size_t area = static_cast<size_t>(g.pitch) * g.ah;
+        float* pn = jb.part + jb.slot_off[t] + static_cast<size_t>(tile) * area + (q.ax0 & 3);
+        float* pd = pn + static_cast<size_t>(g.ntx) * g.nty * area;
+        partials_out<B>(q.aw, q.ah, g.pitch, acc + q.off, acc + q.off + q.aw * q.ah, scratch, pn, pd);
+        __syncthreads();
+    }
+}
+
+// Temporal finish: frame n from the partials of every centre that reaches
+// it (sources in ascending centre order), 64 x 16 pixels per block, four per
+// thread, as finish_tiles_kernel.
+__global__ __launch_bounds__(256) void finish_temporal_tiles_kernel(TFinishTable tab) {
+    const int lb = xcd_block();
+    const TFinishJob& jb = tab.job[find_job(tab, lb)];
+    const int t = lb - jb.wg_begin;
+    const int x0 = (t % jb.tiles_x) * 64 + (threadIdx.x & 15) * 4;
+    const int y = (t / jb.tiles_x) * 16 + (threadIdx.x >> 4);
+    if (x0 >= jb.width || y >= jb.height) return;
+    const int B = jb.block, s = jb.step;
+    const int xr = min(x0 + 3, jb.width - 1);
+    f32x4 n = {0.f, 0.f, 0.f, 0.f}, d = {0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < jb.nsrc; ++q) {
+        const TSource& src = jb.src[q];
+        const TileGeom& tg = src.tiles;
+        const int r = src.range;
+        int txh = min(tg.ntx - 1, (xr + r) / (tg.tw * s));
+        while (txh + 1 < tg.ntx && tg.ax0(txh + 1, jb.width, B, s, r) <= xr) ++txh;
+        int tyh = min(tg.nty - 1, (y + r) / (tg.th * s));
+        while (tyh + 1 < tg.nty && tg.ay0(tyh + 1, jb.height, B, s, r) <= y) ++tyh;
+        const size_t area = static_cast<size_t>(tg.pitch) * tg.ah;
+        for (int ty = tyh; ty >= 0; --ty) {
+            const int oy = y - tg.ay0(ty, jb.height, B, s, r);
+            if (oy >= tg.ah) break;
+            for (int tx = txh; tx >= 0; --tx) {
+                const int ax0 = tg.ax0(tx, jb.width, B, s, r);
+                if (ax0 + tg.aw <= x0) break;
+                const size_t o = (static_cast<size_t>(ty) * tg.ntx + tx) * area + static_cast<size_t>(oy) * tg.pitch +
+                                 (x0 - ax0 + (ax0 & 3));
+                const f32x4 pn = *reinterpret_cast<const f32x4*>(src.num + o);
+                const f32x4 pd = *reinterpret_cast<const f32x4*>(src.den + o);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const bool in = static_cast<unsigned>(x0 + i - ax0) < static_cast<unsigned>(tg.aw);
+                    n[i] += in ? pn[i] : 0.f;
+                    d[i] += in ? pd[i] : 0.f;
+                }
+            }
+        }
+    }
+    float* dst = jb.dst + static_cast<size_t>(y) * jb.width;
+    const float* fb = jb.fallback + static_cast<size_t>(y) * jb.width;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (x0 + i < jb.width) dst[x0 + i] = d[i] > 1e-12f ? n[i] / d[i] : fb[x0 + i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1870,8 +2096,9 @@ hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s) {
 }
 
 size_t predictive_lds_bytes(int block, int group, int ps_num, int radius) {
-    const size_t kl = static_cast<size_t>(ps_num < group ? ps_num : group);
-    const size_t merged = static_cast<size_t>(group - 1) + 2 * static_cast<size_t>(radius) * kl;
+    // merged candidates: (G - 1) spatial + min(ps_num, G) per layer and direction
+    const int kl = ps_num < group ? ps_num : group;
+    const size_t merged = static_cast<size_t>(group - 1 + 2 * radius * kl);
     return ((static_cast<size_t>(block) * block + 3) & ~static_cast<size_t>(3)) * 4 + kPredChunk * 4 +
            2 * kMaxGroup * 8 + kMaxGroup * 4 + merged * 8;
 }
@@ -1903,6 +2130,47 @@ static hipError_t launch_tfilter_b(const TFilterTable& t, int total_wg, size_t l
 }
 
 size_t filter_temporal_b8_lds_bytes(int group) { return 4 * static_cast<size_t>(group) * kTs * 4; }
+
+// Dynamic LDS of filter_temporal_tile_kernel for per-slot geometries g[0..ns).
+size_t filter_temporal_tile_lds_bytes(int group, const TileGeom* g, int ns) {
+    size_t acc = 0, boxsum = 0;
+    for (int t = 0; t < ns; ++t) {
+        acc += static_cast<size_t>(g[t].aw) * g[t].ah + static_cast<size_t>(g[t].aw - 7) * (g[t].ah - 7);
+        boxsum = std::max(boxsum, static_cast<size_t>(g[t].ah - 7) * g[t].aw * 4);
+    }
+    acc = ((acc + 1) & ~static_cast<size_t>(1)) * 8;
+    const size_t recs = static_cast<size_t>(g[0].tw) * g[0].th * group * sizeof(MatchRec) +
+                        ((static_cast<size_t>(g[0].tw) * g[0].th + 3) & ~static_cast<size_t>(3)) * 4;
+    const size_t tbuf = static_cast<size_t>(kTTileWaves) * group * kTs * 4;
+    return acc + recs + std::max(tbuf, boxsum);
+}
+
+template <int G>
+static hipError_t launch_ttile(const TTileTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
+    if (wiener) {
+        allow_lds(&filter_temporal_tile_kernel<G, true>, lds);
+        filter_temporal_tile_kernel<G, true><<<total_wg, 512, lds, s>>>(t);
+    } else {
+        allow_lds(&filter_temporal_tile_kernel<G, false>, lds);
+        filter_temporal_tile_kernel<G, false><<<total_wg, 512, lds, s>>>(t);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int total_wg, size_t lds, bool wiener,
+                                        hipStream_t s) {
+    switch (group) {
+        case 8: return launch_ttile<8>(t, total_wg, lds, wiener, s);
+        case 16: return launch_ttile<16>(t, total_wg, lds, wiener, s);
+        case 32: return launch_ttile<32>(t, total_wg, lds, wiener, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_finish_temporal_tiles(const TFinishTable& t, int total_wg, hipStream_t s) {
+    finish_temporal_tiles_kernel<<<total_wg, 256, 0, s>>>(t);
+    return hipGetLastError();
+}
 
 template <int G>
 static hipError_t launch_tfilter_b8(const TFilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {

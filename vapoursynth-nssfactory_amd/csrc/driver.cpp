@@ -178,10 +178,12 @@ void free_buffers(nss_hip_bm3d* h) {
     for (int i = 0; i < 3; ++i) {
         for (void* p : {static_cast<void*>(R.src[i]), static_cast<void*>(R.ref[i]), static_cast<void*>(R.basic[i]),
                         static_cast<void*>(R.out[i]), static_cast<void*>(R.num[0][i]), static_cast<void*>(R.den[0][i]),
-                        static_cast<void*>(R.num[1][i]), static_cast<void*>(R.den[1][i])})
+                        static_cast<void*>(R.num[1][i]), static_cast<void*>(R.den[1][i]), static_cast<void*>(R.part[0][i]),
+                        static_cast<void*>(R.part[1][i])})
             if (p) (void)hipFree(p);
         R.src[i] = R.ref[i] = R.basic[i] = R.out[i] = nullptr;
         R.num[0][i] = R.den[0][i] = R.num[1][i] = R.den[1][i] = nullptr;
+        R.part[0][i] = R.part[1][i] = nullptr;
     }
 }
 
@@ -247,8 +249,10 @@ size_t instance_bytes(const nss_hip_bm3d* h) {
     for (int i = 0; i < h->nplanes; ++i) {
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
         total += npx * 4 * static_cast<size_t>(R.src_n + R.ref_n + R.basic_n + R.out_n);
-        for (int s = 0; s < h->nstages; ++s)
+        for (int s = 0; s < h->nstages; ++s) {
             if (R.num[s][i]) total += npx * 16 * static_cast<size_t>(R.acc_n);
+            if (R.part[s][i]) total += R.part_center[s][i] * (2 * static_cast<size_t>(R.r) + 1) * sizeof(float);
+        }
     }
     return total;
 }
@@ -602,6 +606,62 @@ FrameRing wref_ring(nss_hip_bm3d* h, int s, int i) {
     return FrameRing{nullptr, 0, 1};
 }
 
+// Slot geometries of the temporal tile path for plane i of stage s: slot t
+// reaches range + |t - r| * ps_range around its references (the predictive
+// search's seeds are spatial matches, each layer widens by ps_range).
+void slot_geoms(const nss_hip_bm3d* h, int s, int i, int tw, int th, TileGeom* g, int* range) {
+    const StagePlane& sp = h->stages[s].plane[i];
+    const int r = h->p.radius;
+    const Grid gr = make_grid(h->pw[i], h->ph[i], 8, sp.step);
+    for (int t = 0; t < 2 * r + 1; ++t) {
+        range[t] = sp.range + (t > r ? t - r : r - t) * sp.ps_range;
+        g[t] = make_tiles(gr, tw, th, 8, range[t]);
+    }
+}
+
+// Floats of one centre's partials of plane i (every slot, num and den), and
+// slot offsets.
+size_t centre_partials(const TileGeom* g, int ns, long long* slot_off) {
+    size_t total = 0;
+    for (int t = 0; t < ns; ++t) {
+        if (slot_off) slot_off[t] = static_cast<long long>(total);
+        total += 2 * static_cast<size_t>(g[t].ntx) * g[t].nty * g[t].pitch * g[t].ah;
+    }
+    return total;
+}
+
+// The temporal tile path for stage s when it has one bucket of block 8 with
+// one group size in {8, 16, 32}, radius <= 2 and a tile shape whose slot
+// sums fit the LDS (one workgroup per CU); else the atomic kernels.
+void plan_temporal_tiles(nss_hip_bm3d* h, int s) {
+    auto& R = h->roll;
+    const Stage& st = h->stages[s];
+    R.tiles[s] = false;
+    if (R.r < 1 || 2 * R.r + 1 > kMaxTileSlots || st.nbuckets != 1 || st.bucket[0].block != 8) return;
+    const Bucket& bk = st.bucket[0];
+    const int G = st.plane[bk.planes[0]].group;
+    for (int k = 1; k < bk.nplanes; ++k)
+        if (st.plane[bk.planes[k]].group != G) return;
+    if (G != 8 && G != 16 && G != 32) return;
+    static const int shapes[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+    for (const auto& sh : shapes) {
+        size_t lds = 0;
+        for (int k = 0; k < bk.nplanes; ++k) {
+            TileGeom g[kMaxTileSlots];
+            int range[kMaxTileSlots];
+            slot_geoms(h, s, bk.planes[k], sh[0], sh[1], g, range);
+            lds = std::max(lds, filter_temporal_tile_lds_bytes(G, g, 2 * R.r + 1));
+        }
+        if (lds + 1024 <= kLdsLimit) {
+            R.tiles[s] = true;
+            R.tw[s] = sh[0];
+            R.th[s] = sh[1];
+            R.tile_lds[s] = lds;
+            return;
+        }
+    }
+}
+
 int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
     auto& R = h->roll;
     const Stage& stg = h->stages[s];
@@ -699,7 +759,41 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             e = traced(h, 1, st, [&] { return launch_predictive(b, pt, pwg, plds, st); });
             if (e != hipSuccess) return hip_fail(e, "predictive match launch");
         }
-        e = traced(h, 2, st, [&] { return launch_filter_temporal(b, ft, fwg, flds, stg.wiener, fast, st); });
+        if (R.tiles[s]) {
+            TTileTable tt{};
+            int twg = 0;
+            const int ns = 2 * R.r + 1;
+            for (int k = 0; k < bk.nplanes; ++k) {
+                const int i = bk.planes[k];
+                const TFilterJob& f = ft.job[k];
+                TTileJob& j = tt.job[tt.n++];
+                j.src = f.src;
+                j.wref = f.wref;
+                j.width = f.width;
+                j.height = f.height;
+                j.stride = f.stride;
+                j.wref_stride = f.wref_stride;
+                j.grid = make_grid(h->pw[i], h->ph[i], 8, stg.plane[i].step);
+                j.nslots = ns;
+                slot_geoms(h, s, i, R.tw[s], R.th[s], j.tiles, j.range);
+                centre_partials(j.tiles, ns, j.slot_off);
+                j.part = R.part[s][i] + static_cast<size_t>(c % ns) * R.part_center[s][i];
+                j.nref = f.nref;
+                j.group = f.group;
+                j.sigma = f.sigma;
+                j.slots = f.slots;
+                j.wg_begin = twg;
+                j.matches = f.matches;
+                j.counts = f.counts;
+                twg += j.tiles[0].ntx * j.tiles[0].nty;
+            }
+            e = traced(h, 2, st, [&] {
+                return launch_filter_temporal_tiles(stg.plane[bk.planes[0]].group, tt, twg, R.tile_lds[s], stg.wiener,
+                                                    st);
+            });
+        } else {
+            e = traced(h, 2, st, [&] { return launch_filter_temporal(b, ft, fwg, flds, stg.wiener, fast, st); });
+        }
         if (e != hipSuccess) return hip_fail(e, "filter launch");
     }
     return NSS_HIP_OK;
@@ -720,6 +814,44 @@ int temporal_finish(nss_hip_bm3d* h, int s, int n, hipStream_t st) {
         const hipError_t e = hipMemcpyAsync(dst, R.src[i] + static_cast<size_t>(n % R.src_n) * npx, npx * 4,
                                             hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) return hip_fail(e, "plane copy");
+    }
+    if (R.tiles[s]) {
+        const Bucket& bk = stg.bucket[0];
+        const int ns = 2 * R.r + 1;
+        TFinishTable fn{};
+        int nwg = 0;
+        for (int k = 0; k < bk.nplanes; ++k) {
+            const int i = bk.planes[k];
+            const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+            TFinishJob& f = fn.job[fn.n++];
+            TileGeom g[kMaxTileSlots];
+            int range[kMaxTileSlots];
+            long long off[kMaxTileSlots];
+            slot_geoms(h, s, i, R.tw[s], R.th[s], g, range);
+            centre_partials(g, ns, off);
+            f.nsrc = 0;
+            for (int c = std::max(0, n - R.r); c <= std::min(R.N - 1, n + R.r); ++c) {
+                const int t = n - c + R.r;
+                const float* base = R.part[s][i] + static_cast<size_t>(c % ns) * R.part_center[s][i] + off[t];
+                TSource& src = f.src[f.nsrc++];
+                src.num = base;
+                src.den = base + static_cast<size_t>(g[t].ntx) * g[t].nty * g[t].pitch * g[t].ah;
+                src.tiles = g[t];
+                src.range = range[t];
+            }
+            f.block = 8;
+            f.step = stg.plane[i].step;
+            f.fallback = R.src[i] + static_cast<size_t>(n % R.src_n) * npx;
+            f.dst = to_out ? R.out[i] + static_cast<size_t>(n % R.out_n) * npx
+                           : R.basic[i] + static_cast<size_t>(n % R.basic_n) * npx;
+            f.width = h->pw[i];
+            f.height = h->ph[i];
+            f.tiles_x = (h->pw[i] + 63) / 64;
+            f.wg_begin = nwg;
+            nwg += f.tiles_x * ((h->ph[i] + 15) / 16);
+        }
+        const hipError_t e = traced(h, 3, st, [&] { return launch_finish_temporal_tiles(fn, nwg, st); });
+        return e == hipSuccess ? NSS_HIP_OK : hip_fail(e, "finish launch");
     }
     for (int bi = 0; bi < stg.nbuckets; ++bi) {
         const Bucket& bk = stg.bucket[bi];
@@ -896,8 +1028,18 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
             if (e == hipSuccess && R.ref_n) e = hipMalloc(&R.ref[i], npx * 4 * R.ref_n);
             if (e == hipSuccess && R.basic_n) e = hipMalloc(&R.basic[i], npx * 4 * R.basic_n);
             if (e == hipSuccess) e = hipMalloc(&R.out[i], npx * 4 * R.out_n);
+            if (i == 0)
+                for (int s = 0; s < h->nstages; ++s) plan_temporal_tiles(h, s);
             for (int s = 0; s < h->nstages && e == hipSuccess; ++s) {
                 if (h->stages[s].plane[i].sigma == 0.f) continue;
+                if (R.tiles[s]) {
+                    TileGeom g[kMaxTileSlots];
+                    int range[kMaxTileSlots];
+                    slot_geoms(h, s, i, R.tw[s], R.th[s], g, range);
+                    R.part_center[s][i] = centre_partials(g, 2 * r + 1, nullptr);
+                    e = hipMalloc(&R.part[s][i], R.part_center[s][i] * (2 * r + 1) * sizeof(float));
+                    continue;
+                }
                 e = hipMalloc(&R.num[s][i], npx * 8 * R.acc_n);
                 if (e == hipSuccess) e = hipMalloc(&R.den[s][i], npx * 8 * R.acc_n);
                 if (e == hipSuccess) e = hipMemset(R.num[s][i], 0, npx * 8 * R.acc_n);

@@ -31,6 +31,10 @@ hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t
 hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
                                   int fast_group, hipStream_t s);
 size_t filter_temporal_b8_lds_bytes(int group);
+size_t filter_temporal_tile_lds_bytes(int group, const TileGeom* g, int ns);
+hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int total_wg, size_t lds, bool wiener,
+                                        hipStream_t s);
+hipError_t launch_finish_temporal_tiles(const TFinishTable& t, int total_wg, hipStream_t s);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
@@ -151,6 +155,14 @@ struct nss_hip_bm3d {
         int next_center[2]{};
         int finished[2]{};
         int out_head = 0;
+        // Tile path of a stage (block 8 register kernel, radius <= 2): the
+        // tile partials of the last 2r+1 centres per plane, centre c at
+        // part + (c % (2r+1)) * part_center.
+        bool tiles[2]{};
+        int tw[2]{}, th[2]{};
+        float* part[2][3]{};
+        size_t part_center[2][3]{};
+        size_t tile_lds[2]{};
     } roll;
 };
 
