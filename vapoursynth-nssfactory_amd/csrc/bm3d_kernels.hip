@@ -67,16 +67,6 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int WIDTH>
-__device__ inline unsigned long long group_min_u64(unsigned long long v) {
-#pragma unroll
-    for (int off = WIDTH / 2; off >= 1; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, WIDTH);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
 // min(v, v of the lane DPP control `ctrl` selects); lanes whose source is
 // outside the row (or rows outside row_mask) keep v.
 template <int CTRL, int ROW_MASK>
@@ -108,6 +98,25 @@ __device__ inline unsigned long long half_min_u64(unsigned long long v) {
     const uint32_t hi1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(hi, 63));
     const bool upper = threadIdx.x & 32;
     return (static_cast<unsigned long long>(upper ? hi1 : hi0) << 32) | (upper ? lo1 : lo0);
+}
+
+// Minimum over the whole wave, returned to every lane (same DPP steps as
+// half_min_u64, then the smaller of lanes 31 and 63). All 64 lanes must call.
+__device__ inline unsigned long long wave_min_u64(unsigned long long v) {
+    v = dpp_min_u64<0x111, 0xf>(v);
+    v = dpp_min_u64<0x112, 0xf>(v);
+    v = dpp_min_u64<0x114, 0xf>(v);
+    v = dpp_min_u64<0x118, 0xf>(v);
+    v = dpp_min_u64<0x142, 0xa>(v);
+    const int lo = static_cast<int>(static_cast<uint32_t>(v));
+    const int hi = static_cast<int>(static_cast<uint32_t>(v >> 32));
+    const unsigned long long a =
+        (static_cast<unsigned long long>(static_cast<uint32_t>(__builtin_amdgcn_readlane(hi, 31))) << 32) |
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(lo, 31));
+    const unsigned long long b =
+        (static_cast<unsigned long long>(static_cast<uint32_t>(__builtin_amdgcn_readlane(hi, 63))) << 32) |
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(lo, 63));
+    return a < b ? a : b;
 }
 
 __device__ inline int wave_sum_i32(int v) {
@@ -350,7 +359,7 @@ __global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
             const unsigned long long key = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
             best = (d != kTaken && key < best) ? key : best;
         }
-        best = group_min_u64<64>(best);
+        best = wave_min_u64(best);
         const int c = static_cast<int>(static_cast<uint32_t>(best));
         if (lane == (c & 63)) {
             dk[c] = kTaken;
@@ -1323,7 +1332,7 @@ __global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
 // Temporal: predictive search (src/cpu/bm/matcher.hpp:319-373) and the
 // group filter with per-slot sources and accumulators.
 // ---------------------------------------------------------------------------
-inline constexpr int kPredChunk = 2048;  // candidates of a layer evaluated per LDS pass
+inline constexpr int kPredChunk = 1024;  // candidates of a layer evaluated per LDS pass
 
 __device__ inline unsigned long long pack_pos(int t, int y, int x) {
     return (static_cast<unsigned long long>(t) << 26) | (static_cast<unsigned long long>(y) << 13) |
@@ -1331,11 +1340,11 @@ __device__ inline unsigned long long pack_pos(int t, int y, int x) {
 }
 
 // One wave per reference. The spatial matches (slot radius) seed the search:
-// per direction, layer dt scans the union of (2 ps_range + 1)^2 windows
-// around the previous layer's winners (exact membership over the union's
-// bounding box, raster order), keeps its min(ps_num, G) smallest (dist, y, x),
-// and those feed the global G-1 smallest (dist, t, y, x) with the spatial
-// matches. Layer 1's centres are the first ps_num spatial matches.
+// per direction, layer dt scores the union of (2 ps_range + 1)^2 windows
+// around the previous layer's winners (every union position once), keeps its
+// min(ps_num, G) smallest (dist, y, x), and those feed the global G-1
+// smallest (dist, t, y, x) with the spatial matches. Layer 1's centres are
+// the first ps_num spatial matches.
 // LDS: reference patch | layer chunk keys | best, next best | centres | merged.
 template <int B>
 __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
@@ -1351,7 +1360,7 @@ __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
     const int n = jb.counts[ref];
     const int cx = rec[0].x, cy = rec[0].y;
     float* refp = reinterpret_cast<float*>(smem);
-    uint32_t* cand = reinterpret_cast<uint32_t*>(refp + ((B * B + 3) & ~3));
+    unsigned long long* cand = reinterpret_cast<unsigned long long*>(refp + ((B * B + 3) & ~3));
     unsigned long long* best = reinterpret_cast<unsigned long long*>(cand + kPredChunk);
     unsigned long long* nbest = best + kMaxGroup;
     int* centers = reinterpret_cast<int*>(nbest + kMaxGroup);
@@ -1377,16 +1386,11 @@ __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
             const int t = t0 + sign * dt;
             if (t < vb || t >= ve) break;
             const float* pt = jb.match.frame(sl.frame(t));
-            int ymin = jb.height, ymax = -1, xmin = jb.width, xmax = -1;
-            for (int i = 0; i < nc; ++i) {
-                const int ccy = centers[i] >> 13, ccx = centers[i] & 8191;
-                ymin = min(ymin, max(0, ccy - ps));
-                ymax = max(ymax, min(maxy, ccy + ps));
-                xmin = min(xmin, max(0, ccx - ps));
-                xmax = max(xmax, min(maxx, ccx + ps));
-            }
-            const int bw = xmax - xmin + 1;
-            const int cells = bw * (ymax - ymin + 1);
+            // The union of the centres' (2 ps + 1)^2 windows (clamped to the
+            // plane), enumerated window by window: a cell belongs to the first
+            // window that contains it, so every union position is scored once.
+            const int W = 2 * ps + 1, WW = W * W;
+            const int cells = nc * WW;
             auto ssd = [&](const float* base, int stride) {
                 float acc = 0.f;
                 for (int py = 0; py < B; ++py) {
@@ -1404,40 +1408,50 @@ __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
                 const int cnt = min(kPredChunk, cells - base);
                 for (int c = lane; c < cnt; c += 64) {
                     const int cell = base + c;
-                    const int dy = cell / bw, dx = cell - dy * bw;
-                    const int y = ymin + dy, x = xmin + dx;
-                    bool member = false;
-                    for (int i = 0; i < nc; ++i) {
-                        const int ccy = centers[i] >> 13, ccx = centers[i] & 8191;
-                        member |= abs(y - ccy) <= ps && x >= max(0, ccx - ps) && x <= min(maxx, ccx + ps);
+                    const int w = cell / WW, o = cell - w * WW;
+                    const int dy = o / W;
+                    const int cw = centers[w];
+                    const int y = (cw >> 13) - ps + dy, x = (cw & 8191) - ps + (o - dy * W);
+                    bool member = y >= 0 && y <= maxy && x >= 0 && x <= maxx;
+                    for (int q = 0; q < w && member; ++q) {
+                        const int cq = centers[q];
+                        member = !(abs(y - (cq >> 13)) <= ps && abs(x - (cq & 8191)) <= ps);
                     }
-                    uint32_t key = kTaken;
-                    if (member) key = dist_key(ssd(pt + static_cast<size_t>(y) * jb.stride + x, jb.stride));
+                    unsigned long long key = ~0ull;
+                    if (member)
+                        key = (static_cast<unsigned long long>(dist_key(ssd(pt + static_cast<size_t>(y) * jb.stride + x,
+                                                                            jb.stride)))
+                               << 32) |
+                              pack_pos(0, y, x);
                     cand[c] = key;
                 }
                 __syncthreads();
-                // The chunk's candidates and the running best: the kl smallest.
+                // The chunk's candidates and the running best: the kl smallest
+                // keys (unique: one per position), each taken by its owner lane.
                 int nn = 0;
                 for (int k = 0; k < kl; ++k) {
-                    unsigned long long m = ~0ull;
-                    for (int c = lane; c < cnt; c += 64) {
-                        if (cand[c] == kTaken) continue;
-                        const int cell = base + c;
-                        const unsigned long long key = (static_cast<unsigned long long>(cand[c]) << 32) |
-                                                       pack_pos(0, ymin + cell / bw, xmin + cell % bw);
-                        m = key < m ? key : m;
-                    }
-                    for (int i = lane; i < nb; i += 64) m = best[i] < m ? best[i] : m;
-                    m = group_min_u64<64>(m);
-                    if (m == ~0ull) break;
-                    const int my = static_cast<int>((m >> 13) & 8191), mx = static_cast<int>(m & 8191);
-                    const int mc = (my - ymin) * bw + (mx - xmin) - base;
-                    if (mc >= 0 && mc < cnt && (mc & 63) == lane && cand[mc] != kTaken &&
-                        cand[mc] == static_cast<uint32_t>(m >> 32))
-                        cand[mc] = kTaken;
+                    unsigned long long m = ~0ull, mb = ~0ull;
+                    int mi = 0, bi = 0;
+                    for (int c = lane; c < cnt; c += 64)
+                        if (cand[c] < m) {
+                            m = cand[c];
+                            mi = c;
+                        }
                     for (int i = lane; i < nb; i += 64)
-                        if (best[i] == m) best[i] = ~0ull;
-                    if (lane == 0) nbest[nn] = m;
+                        if (best[i] < mb) {
+                            mb = best[i];
+                            bi = i;
+                        }
+                    const unsigned long long mine = m < mb ? m : mb;
+                    const unsigned long long win = wave_min_u64(mine);
+                    if (win == ~0ull) break;
+                    if (mine == win) {
+                        if (m < mb)
+                            cand[mi] = ~0ull;
+                        else
+                            best[bi] = ~0ull;
+                    }
+                    if (lane == 0) nbest[nn] = win;
                     ++nn;
                     __syncthreads();
                 }
@@ -1462,7 +1476,7 @@ __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
     for (int k = 0; k < K; ++k) {
         unsigned long long m = ~0ull;
         for (int i = lane; i < nm; i += 64) m = merged[i] < m ? merged[i] : m;
-        m = group_min_u64<64>(m);
+        m = wave_min_u64(m);
         for (int i = lane; i < nm; i += 64)
             if (merged[i] == m) merged[i] = ~0ull;
         if (lane == 0)
@@ -2099,7 +2113,7 @@ size_t predictive_lds_bytes(int block, int group, int ps_num, int radius) {
     // merged candidates: (G - 1) spatial + min(ps_num, G) per layer and direction
     const int kl = ps_num < group ? ps_num : group;
     const size_t merged = static_cast<size_t>(group - 1 + 2 * radius * kl);
-    return ((static_cast<size_t>(block) * block + 3) & ~static_cast<size_t>(3)) * 4 + kPredChunk * 4 +
+    return ((static_cast<size_t>(block) * block + 3) & ~static_cast<size_t>(3)) * 4 + kPredChunk * 8 +
            2 * kMaxGroup * 8 + kMaxGroup * 4 + merged * 8;
 }
 
