@@ -24,9 +24,9 @@
 //   filter_atomic_kernel<B, W> one wave per group, device-scope int64 atomics
 //                              into frame-sized sums (den at the corner).
 //   finish_corner_kernel<B>    box-sums the corner weights, divides.
-// Workgroups are dealt round-robin over the 8 XCDs; the main-path kernels
-// remap their block index so that each XCD works on a contiguous part of the
-// frame (its L2 then holds the overlapping windows and patches).
+// Workgroups are dealt round-robin over the 8 XCDs; every kernel remaps its
+// block index so that each XCD works on a contiguous part of the frame (its
+// L2 then holds the overlapping windows, patches and partial rows).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -301,8 +301,9 @@ template <int B, bool STAGE>
 __global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
-    const MatchJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
-    const int ref = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    const int lb = xcd_block();
+    const MatchJob& jb = tab.job[find_job(tab, lb)];
+    const int ref = lb - jb.wg_begin;
     if (ref >= jb.grid.nx * jb.grid.ny) return;
     const int cx = ref_x(jb.grid, ref % jb.grid.nx, jb.width, B);
     const int cy = ref_y(jb.grid, ref / jb.grid.nx, jb.height, B);
@@ -1309,8 +1310,9 @@ __global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
     constexpr int A = B * B;
     constexpr int PS = A + 1;
     const int lane = threadIdx.x;
-    const FilterJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
-    const int g = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    const int lb = xcd_block();
+    const FilterJob& jb = tab.job[find_job(tab, lb)];
+    const int g = lb - jb.wg_begin;
     if (g >= jb.nref) return;
     const int G = jb.group;
     const int k = jb.counts[g];
@@ -1350,8 +1352,10 @@ template <int B>
 __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
-    const PredJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
-    const int ref = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    // XCD-contiguous references: neighbours share their frames' rows in L2.
+    const int lb = xcd_block();
+    const PredJob& jb = tab.job[find_job(tab, lb)];
+    const int ref = lb - jb.wg_begin;
     if (ref >= jb.nref) return;
     const int G = jb.group;
     const Slots& sl = jb.slots;
@@ -1495,8 +1499,9 @@ __global__ __launch_bounds__(64) void filter_temporal_kernel(TFilterTable tab) {
     constexpr int A = B * B;
     constexpr int PS = A + 1;
     const int lane = threadIdx.x;
-    const TFilterJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
-    const int g = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    const int lb = xcd_block();
+    const TFilterJob& jb = tab.job[find_job(tab, lb)];
+    const int g = lb - jb.wg_begin;
     if (g >= jb.nref) return;
     const int G = jb.group;
     const int k = jb.counts[g];
@@ -1875,8 +1880,9 @@ __global__ __launch_bounds__(256) void finish_corner_kernel(FinishTable tab) {
     constexpr int TS = T + B - 1;
     __shared__ unsigned long long corner[TS * TS];
     __shared__ unsigned long long colsum[T * TS];
-    const FinishJob& jb = tab.job[find_job(tab, static_cast<int>(blockIdx.x))];
-    const int tile = static_cast<int>(blockIdx.x) - jb.wg_begin;
+    const int lb = xcd_block();
+    const FinishJob& jb = tab.job[find_job(tab, lb)];
+    const int tile = lb - jb.wg_begin;
     const int x0 = (tile % jb.tiles_x) * T, y0 = (tile / jb.tiles_x) * T;
     const int tid = threadIdx.x;
     for (int e = tid; e < TS * TS; e += 256) {
