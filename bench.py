@@ -77,11 +77,11 @@ FORMS = {"fp32": 0, "bf16x3": 1, "fp16": 2}
 # WRITE_SIZE (KiB per dispatch, averaged over the basic and Wiener launches)
 # scaled by what the counters read for a 256 MiB elementwise copy in the same
 # process (FETCH x2, WRITE x1); profiles/bm3d_1080p_yuv420_two_stage_kernels.md.
-TRAFFIC_KIB = {("match", "fp32"): (6608, 6280), ("match", "bf16x3"): (6608, 6280), ("match", "fp16"): (6608, 6280),
-               ("filter", "fp32"): ((9645 + 16061) / 2, 52598.5),
-               ("filter", "bf16x3"): ((9707 + 16152) / 2, 66938), ("filter", "fp16"): ((9690 + 16127) / 2, 66938),
-               ("finish", "fp32"): (30857, 12723), ("finish", "bf16x3"): (30857, 12723),
-               ("finish", "fp16"): (30857, 12723)}
+TRAFFIC_KIB = {("match", "fp32"): (6612, 6280), ("match", "bf16x3"): (6612, 6280), ("match", "fp16"): (6612, 6280),
+               ("filter", "fp32"): ((9652 + 16071) / 2, 52597.5),
+               ("filter", "bf16x3"): ((9703 + 16148) / 2, 52599), ("filter", "fp16"): ((9688 + 16137) / 2, 52601.5),
+               ("finish", "fp32"): (25972, 12233), ("finish", "bf16x3"): (25972, 12233),
+               ("finish", "fp16"): (25972, 12233)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 FORM_DTYPE = {"fp32": "f32", "bf16x3": "bf16x3 (f32 accumulate)", "fp16": "f16 (f32 accumulate)"}
 PEAK_MFMA_TFLOPS = 2500.0  # dense bf16 / fp16
