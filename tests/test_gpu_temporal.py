@@ -68,13 +68,16 @@ def test_temporal_rolling(gpu, oracle, radius, final):
 
 
 def test_temporal_shapes_and_ref(gpu, oracle):
-    """Other block / group / predictive settings, and a ref clip (Wiener)."""
+    """Other block / group / predictive settings (staged and unstaged
+    predictive windows), and a ref clip (Wiener)."""
     import synth
     clip = _moving_clip("GRAYS", 56, 44, 5, 20, seed=3)
     clean = [[synth.clean_image(56, 44, seed=3)] for _ in range(5)]
     for kw in (dict(block_size=4, block_step=2, group_size=16, ps_num=3, ps_range=2),
                dict(block_size=8, block_step=4, group_size=32, bm_range=6, ps_range=5),
-               dict(block_size=16, block_step=8, group_size=4, bm_range=4)):
+               dict(block_size=16, block_step=8, group_size=4, bm_range=4),
+               # windows too large to stage in LDS: scored from the frame in chunks
+               dict(block_size=8, block_step=4, group_size=8, bm_range=4, ps_range=20)):
         kw = dict(kw, sigma=20, radius=1)
         _check(oracle, _gpu_clip(gpu, "GRAYS", clip, **kw), _oracle_clip(oracle, gpu, "GRAYS", clip, **kw), str(kw))
         got = _gpu_clip(gpu, "GRAYS", clip, ref_clip=clean, **kw)

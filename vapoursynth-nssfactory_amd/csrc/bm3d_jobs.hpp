@@ -156,8 +156,18 @@ struct PredJob {
     int* counts;
 };
 
+// Predictive search LDS limits: candidate keys per pass, and the staged
+// windows' floats (each window's (2 ps + 1 + B - 1)^2 pixels at a row pitch
+// that covers the last strip of three candidates).
+inline constexpr int kPredChunk = 1024;
+inline constexpr int kPredRegionCap = 8192;
+constexpr int pred_region_pitch(int w, int block) { return (3 * ((w + 2) / 3) + block - 1) | 1; }
+
 struct PredTable {
     int n;
+    int chunk;   // candidate keys held in LDS per pass (a multiple of 64)
+    int region;  // floats of LDS for the staged windows (0: windows read from the frame)
+    int region_offset;  // bytes from the start of LDS to the staged windows
     PredJob job[kMaxJobs];
 };
 

@@ -6,7 +6,7 @@
 //                              grid row; the union of their search windows is
 //                              staged in LDS once; each half-wave owns one
 //                              reference (patch in registers) and each lane a
-//                              4 x 4 tile of candidates; then the G-1 smallest
+//                              2 x 7 tile of candidates; then the G-1 smallest
 //                              keys (dist, y, x) by rounds of a half-wave min.
 //   filter_tile_kernel<B, W>   one workgroup per tile of references; each wave
 //                              filters whole groups (LDS cube: 2-D DCT, Haar
@@ -167,20 +167,20 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int top = max(cy - r, 0), bottom = min(cy + r, jb.height - B);
     const int rx0 = max(ref_x(jb.grid, i0, jb.width, B) - r, 0);
     const int rx1 = min(ref_x(jb.grid, i0 + nrun - 1, jb.width, B) + r, jb.width - B) + B - 1;
-    // Lane tiles of 4 x 4 candidates start at window column 4*tx - (ty & 3):
-    // staggering the tile rows spreads the lanes' LDS addresses over all
-    // banks (aligned tiles would touch a quarter of them). The region keeps
-    // 3 columns before the windows and 6 after, 3 rows below (candidates of
-    // an overhanging tile are discarded), and a row pitch = 2 mod 4.
-    const int need = rx1 - rx0 + 1 + 9;
+    // Lane tiles of 2 x 7 candidates (2 rows, 7 columns): a 19 x 19 window
+    // is 10 x 3 tiles, 448 lane slots for 361 candidates. Tile columns 7 tx
+    // spread the lanes over the banks; the region keeps 6 columns after the
+    // windows and 1 row below (candidates of an overhanging tile are
+    // discarded), with a row pitch = 2 mod 4.
+    const int need = rx1 - rx0 + 1 + 6;
     const int RW = need + (6 - need % 4) % 4;
-    const int RH = bottom - top + B + 3;
+    const int RH = bottom - top + B + 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int yy = wave; yy < RH; yy += 4) {
         const int gy = top + yy;
         const float* src = jb.plane + static_cast<size_t>(min(gy, jb.height - 1)) * jb.stride;
         for (int xx = lane; xx < RW; xx += 64) {
-            const int gx = rx0 - 3 + xx;
+            const int gx = rx0 + xx;
             lds[yy * RW + xx] = (gy < jb.height && gx >= 0 && gx < jb.width) ? src[gx] : 0.f;
         }
     }
@@ -195,33 +195,33 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int ncand = ww * wh;
     uint32_t* dk = dist + half * wmax * wmax;
     if (half < nrun) {
-        const int ox = left - rx0 + 3;
-        const float* rp = lds + (cy - top) * RW + (cx - rx0 + 3);
-        const int tw = (ww + 6) >> 2, th = (wh + 3) >> 2;
+        const int ox = left - rx0;
+        const float* rp = lds + (cy - top) * RW + (cx - rx0);
+        const int tw = (ww + 6) / 7, th = (wh + 1) >> 1;
         const int self = (cy - top) * ww + (cx - left);
         for (int t = hl; t < tw * th; t += 32) {
             const int ty = t / tw, tx = t - ty * tw;
-            const int dy0 = ty * 4, dx0 = tx * 4 - (ty & 3);
-            float acc[4][4];
+            const int dy0 = ty * 2, dx0 = tx * 7;
+            float acc[2][7];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+                for (int j = 0; j < 7; ++j) acc[i][j] = 0.f;
             const float* base = lds + dy0 * RW + ox + dx0;
-            // Block row py of the reference against block row py of the 4 x 4
-            // candidates: region rows dy0 + py + i, columns dx0 .. dx0 + B + 2.
+            // Block row py of the reference against block row py of the 2 x 7
+            // candidates: region rows dy0 + py + i, columns dx0 .. dx0 + B + 5.
             for (int py = 0; py < B; ++py) {
                 float rrow[B];
 #pragma unroll
                 for (int px = 0; px < B; ++px) rrow[px] = rp[py * RW + px];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < 2; ++i) {
                     const float* rowp = base + (py + i) * RW;
-                    float v[B + 3];
+                    float v[B + 6];
 #pragma unroll
-                    for (int c = 0; c < B + 3; ++c) v[c] = rowp[c];
+                    for (int c = 0; c < B + 6; ++c) v[c] = rowp[c];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
+                    for (int j = 0; j < 7; ++j)
 #pragma unroll
                         for (int px = 0; px < B; ++px) {
                             const float d = rrow[px] - v[j + px];
@@ -230,11 +230,11 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
                 }
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < 7; ++j) {
                     const int dy = dy0 + i, dx = dx0 + j;
-                    if (dy < wh && dx >= 0 && dx < ww) {
+                    if (dy < wh && dx < ww) {
                         const int c = dy * ww + dx;
                         dk[c] = c == self ? kTaken : dist_key(acc[i][j]);
                     }
@@ -1334,8 +1334,6 @@ __global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
 // Temporal: predictive search (src/cpu/bm/matcher.hpp:319-373) and the
 // group filter with per-slot sources and accumulators.
 // ---------------------------------------------------------------------------
-inline constexpr int kPredChunk = 1024;  // candidates of a layer evaluated per LDS pass
-
 __device__ inline unsigned long long pack_pos(int t, int y, int x) {
     return (static_cast<unsigned long long>(t) << 26) | (static_cast<unsigned long long>(y) << 13) |
            static_cast<unsigned long long>(x);
@@ -1346,10 +1344,14 @@ __device__ inline unsigned long long pack_pos(int t, int y, int x) {
 // around the previous layer's winners (every union position once), keeps its
 // min(ps_num, G) smallest (dist, y, x), and those feed the global G-1
 // smallest (dist, t, y, x) with the spatial matches. Layer 1's centres are
-// the first ps_num spatial matches.
-// LDS: reference patch | layer chunk keys | best, next best | centres | merged.
+// the first ps_num spatial matches. When the layer's windows fit the staging
+// buffer, their pixels are copied to LDS once and each lane scores three
+// horizontally adjacent candidates from one register row per block row
+// (same sums, same order as scoring from the frame).
+// LDS: reference patch | layer chunk keys | best, next best | centres | merged
+// | staged windows.
 template <int B>
-__global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void predictive_kernel(PredTable tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
     // XCD-contiguous references: neighbours share their frames' rows in L2.
@@ -1365,10 +1367,11 @@ __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
     const int cx = rec[0].x, cy = rec[0].y;
     float* refp = reinterpret_cast<float*>(smem);
     unsigned long long* cand = reinterpret_cast<unsigned long long*>(refp + ((B * B + 3) & ~3));
-    unsigned long long* best = reinterpret_cast<unsigned long long*>(cand + kPredChunk);
+    unsigned long long* best = reinterpret_cast<unsigned long long*>(cand + tab.chunk);
     unsigned long long* nbest = best + kMaxGroup;
     int* centers = reinterpret_cast<int*>(nbest + kMaxGroup);
     unsigned long long* merged = reinterpret_cast<unsigned long long*>(centers + kMaxGroup);
+    float* region = reinterpret_cast<float*>(smem + tab.region_offset);
     {
         const float* p0 = jb.match.frame(sl.frame(t0));
         for (int e = lane; e < B * B; e += 64) refp[e] = p0[static_cast<size_t>(cy + e / B) * jb.stride + cx + e % B];
@@ -1407,22 +1410,85 @@ __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
                 }
                 return acc;
             };
+            auto member_of = [&](int w, int y, int x) {
+                bool m = y >= 0 && y <= maxy && x >= 0 && x <= maxx;
+                for (int q = 0; q < w && m; ++q) {
+                    const int cq = centers[q];
+                    m = !(abs(y - (cq >> 13)) <= ps && abs(x - (cq & 8191)) <= ps);
+                }
+                return m;
+            };
+            const int R = W + B - 1, P = pred_region_pitch(W, B), SW = (W + 2) / 3;
+            const bool staged = cells <= tab.chunk && nc * R * P <= tab.region;
+            if (staged) {
+                // Window w's pixels (rows y_w - ps .., columns x_w - ps ..) at
+                // region[w R P + j P + i]; columns from R on and pixels off the
+                // plane hold 0 (read only for discarded candidates).
+                // Lanes cover 64 / P rows of P columns per step (P <= 64), or
+                // one row in steps of 64 columns.
+                const int rows_per = P <= 64 ? 64 / P : 1;
+                const int lr = P <= 64 ? lane / P : 0, li = P <= 64 ? lane - lr * P : lane;
+                for (int w = 0; w < nc; ++w) {
+                    const int cw = centers[w];
+                    const int y0 = (cw >> 13) - ps, x0 = (cw & 8191) - ps;
+                    float* dst = region + w * (R * P);
+                    if (lr < rows_per)
+                        for (int j = lr; j < R; j += rows_per)
+                            for (int i = li; i < P; i += (P <= 64 ? P : 64)) {
+                                const int y = y0 + j, x = x0 + i;
+                                dst[j * P + i] = (i < R && y >= 0 && y < jb.height && x >= 0 && x < jb.width)
+                                                     ? pt[static_cast<size_t>(y) * jb.stride + x]
+                                                     : 0.f;
+                            }
+                }
+                __syncthreads();
+                const int strips = nc * W * SW;
+                for (int sidx = lane; sidx < strips; sidx += 64) {
+                    const int w = sidx / (W * SW), o = sidx - w * (W * SW);
+                    const int dy = o / SW, dx0 = 3 * (o - dy * SW);
+                    const float* rb = region + w * (R * P) + dy * P + dx0;
+                    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 1
+                    for (int py = 0; py < B; ++py) {
+                        float v[B + 2];
+#pragma unroll
+                        for (int q = 0; q < B + 2; ++q) v[q] = rb[py * P + q];
+#pragma unroll
+                        for (int px = 0; px < B; ++px) {
+                            const float rv = refp[py * B + px];
+                            const float d0 = rv - v[px], d1 = rv - v[px + 1], d2 = rv - v[px + 2];
+                            a0 = fmaf(d0, d0, a0);
+                            a1 = fmaf(d1, d1, a1);
+                            a2 = fmaf(d2, d2, a2);
+                        }
+                    }
+                    const int cw = centers[w];
+                    const int y = (cw >> 13) - ps + dy;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const int dx = dx0 + q;
+                        if (dx < W) {
+                            const int x = (cw & 8191) - ps + dx;
+                            const float a = q == 0 ? a0 : q == 1 ? a1 : a2;
+                            cand[w * WW + dy * W + dx] =
+                                member_of(w, y, x)
+                                    ? (static_cast<unsigned long long>(dist_key(a)) << 32) | pack_pos(0, y, x)
+                                    : ~0ull;
+                        }
+                    }
+                }
+            }
             int nb = 0;
-            for (int base = 0; base < cells; base += kPredChunk) {
-                const int cnt = min(kPredChunk, cells - base);
-                for (int c = lane; c < cnt; c += 64) {
+            for (int base = 0; base < cells; base += tab.chunk) {
+                const int cnt = min(tab.chunk, cells - base);
+                for (int c = lane; c < cnt && !staged; c += 64) {
                     const int cell = base + c;
                     const int w = cell / WW, o = cell - w * WW;
                     const int dy = o / W;
                     const int cw = centers[w];
                     const int y = (cw >> 13) - ps + dy, x = (cw & 8191) - ps + (o - dy * W);
-                    bool member = y >= 0 && y <= maxy && x >= 0 && x <= maxx;
-                    for (int q = 0; q < w && member; ++q) {
-                        const int cq = centers[q];
-                        member = !(abs(y - (cq >> 13)) <= ps && abs(x - (cq & 8191)) <= ps);
-                    }
                     unsigned long long key = ~0ull;
-                    if (member)
+                    if (member_of(w, y, x))
                         key = (static_cast<unsigned long long>(dist_key(ssd(pt + static_cast<size_t>(y) * jb.stride + x,
                                                                             jb.stride)))
                                << 32) |
@@ -1476,17 +1542,16 @@ __global__ __launch_bounds__(64) void predictive_kernel(PredTable tab) {
             if (nc == 0) break;
         }
     }
+    // The G-1 smallest merged keys in ascending order: keys are unique (one
+    // per (t, y, x)), so each key's rank is the count of smaller keys.
     const int K = min(G - 1, nm);
-    for (int k = 0; k < K; ++k) {
-        unsigned long long m = ~0ull;
-        for (int i = lane; i < nm; i += 64) m = merged[i] < m ? merged[i] : m;
-        m = wave_min_u64(m);
-        for (int i = lane; i < nm; i += 64)
-            if (merged[i] == m) merged[i] = ~0ull;
-        if (lane == 0)
-            rec[1 + k] = MatchRec{static_cast<int>(m & 8191), static_cast<int>((m >> 13) & 8191),
-                                  static_cast<int>((m >> 26) & 63), static_cast<uint32_t>(m >> 32)};
-        __syncthreads();
+    for (int i = lane; i < nm; i += 64) {
+        const unsigned long long m = merged[i];
+        int rank = 0;
+        for (int j = 0; j < nm; ++j) rank += merged[j] < m ? 1 : 0;
+        if (rank < K)
+            rec[1 + rank] = MatchRec{static_cast<int>(m & 8191), static_cast<int>((m >> 13) & 8191),
+                                     static_cast<int>((m >> 26) & 63), static_cast<uint32_t>(m >> 32)};
     }
     if (lane == 0) jb.counts[ref] = 1 + K;
 }
@@ -1938,9 +2003,10 @@ size_t match_wave_lds_bytes(int block, int range, bool stage) {
 
 size_t match_run_lds_bytes(int block, int step, int range) {
     const size_t w = 2 * static_cast<size_t>(range) + 1;
-    // windows' span + 9 padding columns, rounded up to a pitch of 2 mod 4
-    const size_t rw = static_cast<size_t>(kRunRefs - 1) * step + 2 * range + block + 9 + 3;
-    const size_t rh = 2 * static_cast<size_t>(range) + block + 3;
+    // windows' span + 6 padding columns, rounded up to a pitch of 2 mod 4;
+    // one row below the windows
+    const size_t rw = static_cast<size_t>(kRunRefs - 1) * step + 2 * range + block + 6 + 3;
+    const size_t rh = 2 * static_cast<size_t>(range) + block + 1;
     return (((rw * rh) + 3) & ~static_cast<size_t>(3)) * 4 + kRunRefs * w * w * 4;
 }
 
@@ -2115,12 +2181,26 @@ hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t predictive_lds_bytes(int block, int group, int ps_num, int radius) {
-    // merged candidates: (G - 1) spatial + min(ps_num, G) per layer and direction
-    const int kl = ps_num < group ? ps_num : group;
-    const size_t merged = static_cast<size_t>(group - 1 + 2 * radius * kl);
-    return ((static_cast<size_t>(block) * block + 3) & ~static_cast<size_t>(3)) * 4 + kPredChunk * 8 +
-           2 * kMaxGroup * 8 + kMaxGroup * 4 + merged * 8;
+size_t predictive_table_lds(int block, PredTable& t, int radius) {
+    // Candidate keys: one pass for a layer's union when it fits kPredChunk.
+    // Merged candidates: (G - 1) spatial + min(ps_num, G) per layer and direction.
+    int chunk = 64, region = 0;
+    size_t merged = 0;
+    for (int j = 0; j < t.n; ++j) {
+        const PredJob& p = t.job[j];
+        const int kl = std::min(p.ps_num, p.group);
+        const int w = 2 * p.ps_range + 1, cells = kl * w * w;
+        chunk = std::max(chunk, std::min(kPredChunk, (cells + 63) / 64 * 64));
+        merged = std::max(merged, static_cast<size_t>(p.group - 1 + 2 * radius * kl));
+        const int need = kl * (w + block - 1) * pred_region_pitch(w, block);
+        if (cells <= kPredChunk && need <= kPredRegionCap) region = std::max(region, need);
+    }
+    const size_t head = ((static_cast<size_t>(block) * block + 3) & ~static_cast<size_t>(3)) * 4 + chunk * 8 +
+                        2 * kMaxGroup * 8 + kMaxGroup * 4 + merged * 8;
+    t.chunk = chunk;
+    t.region = region;
+    t.region_offset = static_cast<int>((head + 15) & ~static_cast<size_t>(15));
+    return t.region_offset + static_cast<size_t>(region) * 4;
 }
 
 hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s) {

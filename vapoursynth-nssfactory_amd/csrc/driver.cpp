@@ -714,7 +714,6 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
                 p.recs = h->buf.matches[i];
                 p.counts = h->buf.counts[i];
                 pwg += p.nref;
-                plds = std::max(plds, predictive_lds_bytes(b, sp.group, sp.ps_num, R.r));
                 predictive = true;
             }
         }
@@ -756,6 +755,7 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
         hipError_t e = traced(h, 1, st, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, st); });
         if (e != hipSuccess) return hip_fail(e, "match launch");
         if (predictive) {
+            plds = predictive_table_lds(b, pt, R.r);
             e = traced(h, 1, st, [&] { return launch_predictive(b, pt, pwg, plds, st); });
             if (e != hipSuccess) return hip_fail(e, "predictive match launch");
         }

@@ -26,7 +26,9 @@ size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);
 size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t);
 size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t);
-size_t predictive_lds_bytes(int block, int group, int ps_num, int radius);
+// Sets the table's LDS layout (chunk, staged-window capacity) from its jobs;
+// returns the bytes per workgroup.
+size_t predictive_table_lds(int block, PredTable& t, int radius);
 hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s);
 hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
                                   int fast_group, hipStream_t s);
