@@ -677,19 +677,47 @@ __global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
 inline constexpr int kTs = 72;  // patch stride in the transpose buffer
 inline constexpr int kTr = 9;   // row stride
 
+// 8-point orthonormal DCT-II (forward) or its transpose, DCT-III (inverse),
+// by the even/odd split of the definition X_k = a_k sum_n x_n cos(pi (2n+1) k / 16):
+// X_{2m} is the 4-point transform of x_n + x_{7-n} (split once more into
+// sums and differences), X_{2m+1} the 4 x 4 cosine matrix times
+// x_n - x_{7-n}; the inverse runs the same steps transposed. 36 operations
+// instead of 64 multiply-adds; the constants are cos(j pi / 16) / 2 and
+// 1 / sqrt(8) rounded from double.
 template <bool INVERSE>
 __device__ inline void dct8(float v[8]) {
-    const float* m = c_dct + dct_offset<8>();
-    float o[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        float acc = 0.f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) acc = fmaf(INVERSE ? m[i * 8 + k] : m[k * 8 + i], v[i], acc);
-        o[k] = acc;
+    constexpr float k0 = 0.353553391f;  // 1/sqrt(8) = cos(4 pi/16) / 2
+    constexpr float c1 = 0.490392640f, c2 = 0.461939766f, c3 = 0.415734806f;
+    constexpr float c5 = 0.277785117f, c6 = 0.191341716f, c7 = 0.0975451610f;
+    if (!INVERSE) {
+        const float a0 = v[0] + v[7], a1 = v[1] + v[6], a2 = v[2] + v[5], a3 = v[3] + v[4];
+        const float b0 = v[0] - v[7], b1 = v[1] - v[6], b2 = v[2] - v[5], b3 = v[3] - v[4];
+        const float e0 = a0 + a3, e1 = a1 + a2, d0 = a0 - a3, d1 = a1 - a2;
+        v[0] = k0 * (e0 + e1);
+        v[4] = k0 * (e0 - e1);
+        v[2] = fmaf(c2, d0, c6 * d1);
+        v[6] = fmaf(c6, d0, -c2 * d1);
+        v[1] = fmaf(c1, b0, fmaf(c3, b1, fmaf(c5, b2, c7 * b3)));
+        v[3] = fmaf(c3, b0, fmaf(-c7, b1, fmaf(-c1, b2, -c5 * b3)));
+        v[5] = fmaf(c5, b0, fmaf(-c1, b1, fmaf(c7, b2, c3 * b3)));
+        v[7] = fmaf(c7, b0, fmaf(-c5, b1, fmaf(c3, b2, -c1 * b3)));
+    } else {
+        const float s = k0 * (v[0] + v[4]), t = k0 * (v[0] - v[4]);
+        const float p = fmaf(c2, v[2], c6 * v[6]), q = fmaf(c6, v[2], -c2 * v[6]);
+        const float e0 = s + p, e3 = s - p, e1 = t + q, e2 = t - q;
+        const float o0 = fmaf(c1, v[1], fmaf(c3, v[3], fmaf(c5, v[5], c7 * v[7])));
+        const float o1 = fmaf(c3, v[1], fmaf(-c7, v[3], fmaf(-c1, v[5], -c5 * v[7])));
+        const float o2 = fmaf(c5, v[1], fmaf(-c1, v[3], fmaf(c7, v[5], c3 * v[7])));
+        const float o3 = fmaf(c7, v[1], fmaf(-c5, v[3], fmaf(c3, v[5], -c1 * v[7])));
+        v[0] = e0 + o0;
+        v[7] = e0 - o0;
+        v[1] = e1 + o1;
+        v[6] = e1 - o1;
+        v[2] = e2 + o2;
+        v[5] = e2 - o2;
+        v[3] = e3 + o3;
+        v[4] = e3 - o3;
     }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = o[i];
 }
 
 // rows (or columns) layout -> the transposed layout within each patch.
