@@ -1712,12 +1712,17 @@ __global__ __launch_bounds__(256) void filter_temporal_b8_kernel(TFilterTable ta
     }
 }
 
-// Temporal tile path, block 8, G in {8, 16, 32}: one workgroup (8 waves) per
-// tile of references; a wave per group with the register cube; every slot's
-// contributions go into that slot's LDS sums (apron = the slot's reach around
-// the tile), then out as the centre's float partials, slot by slot.
-// LDS: slot table | per slot num [ah][aw] and corners [ch][cw] (int64) |
-// match records | counts | per-wave transpose buffers (also the box sum).
+// Temporal tile path, block 8, G in {8, 16, 32}: 8 waves per workgroup, a
+// wave per group with the register cube; every slot's contributions go into
+// that slot's LDS sums (apron = the slot's reach around the tile), then out as
+// the centre's float partials, slot by slot. The LDS (well over half a CU's)
+// leaves one workgroup per CU, so the kernel is persistent: workgroup L (in
+// XCD-contiguous order) filters tiles [L T / n, (L + 1) T / n) of the T tiles,
+// and while one tile's partials are written it loads the next tile's records
+// and each wave's first patches; within a tile, a wave loads its next
+// group's patches while the current one is filtered.
+// LDS: match records | counts | per slot num [ah][aw] and corners [ch][cw]
+// (int64) | per-wave transpose buffers (also the box sum).
 inline constexpr int kTTileWaves = 8;
 
 struct SlotLds {
@@ -1726,143 +1731,216 @@ struct SlotLds {
     int aw, ah;
 };
 
+struct TileAt {
+    int job, tile, tx, ty, i_begin, j_begin, ni, ngroups;
+};
+
+__device__ inline TileAt ttile_at(const TTileTable& tab, int lt) {
+    TileAt a;
+    a.job = find_job(tab, lt);
+    const TTileJob& jb = tab.job[a.job];
+    const TileGeom& tg = jb.tiles[0];
+    a.tile = lt - jb.wg_begin;
+    a.tx = a.tile % tg.ntx;
+    a.ty = a.tile / tg.ntx;
+    a.i_begin = a.tx * tg.tw;
+    a.j_begin = a.ty * tg.th;
+    a.ni = min(tg.tw, jb.grid.nx - a.i_begin);
+    a.ngroups = a.ni * min(tg.th, jb.grid.ny - a.j_begin);
+    return a;
+}
+
+// This lane's rows (patch gb + 8 m, row y) of a group with k patches, from
+// their slots' frames (zero beyond k); the Wiener reference rows too.
 template <int G, bool WIENER>
-__global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable tab) {
-    extern __shared__ __align__(16) unsigned long long acc[];
+__device__ inline void ttile_gather(const TTileJob& jb, const MatchRec* rec, int k, int gb, int y,
+                                    float (&v)[G / 8][8], float (&vr)[G / 8][8]) {
+#pragma unroll
+    for (int m = 0; m < G / 8; ++m) {
+        const int p = gb + 8 * m;
+        if (p < k) {
+            const MatchRec r = rec[p];
+            const int f = jb.slots.frame(r.t);
+            const float* sp = jb.src.frame(f) + static_cast<size_t>(r.y + y) * jb.stride + r.x;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) v[m][x] = sp[x];
+            if constexpr (WIENER) {
+                const float* wp = jb.wref.frame(f) + static_cast<size_t>(r.y + y) * jb.wref_stride + r.x;
+#pragma unroll
+                for (int x = 0; x < 8; ++x) vr[m][x] = wp[x];
+            }
+        } else {
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                v[m][x] = 0.f;
+                if constexpr (WIENER) vr[m][x] = 0.f;
+            }
+        }
+    }
+}
+
+template <int G, bool WIENER>
+__global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable tab, int total_tiles) {
+    extern __shared__ __align__(16) unsigned long long smem64[];
     constexpr int B = 8;
     constexpr int M = G / 8;
-    const int lb = xcd_block();
-    const TTileJob& jb = tab.job[find_job(tab, lb)];
-    const TileGeom& tg = jb.tiles[0];
-    const int tile = lb - jb.wg_begin;
-    const int tx = tile % tg.ntx, ty = tile / tg.ntx;
-    const int i_begin = tx * tg.tw, j_begin = ty * tg.th;
-    const int ni = min(tg.tw, jb.grid.nx - i_begin), nj = min(tg.th, jb.grid.ny - j_begin);
-    const int ngroups = ni * nj;
-    const int ns = jb.nslots;
+    const int L = xcd_block(), nb = static_cast<int>(gridDim.x);
+    const int lo = static_cast<int>(static_cast<long long>(L) * total_tiles / nb);
+    const int hi = static_cast<int>(static_cast<long long>(L + 1) * total_tiles / nb);
+    if (lo >= hi) return;
+    // Groups per tile (one tile shape per launch; tw * th * G <= 512, one
+    // record per thread).
+    const int cap = tab.job[0].tiles[0].tw * tab.job[0].tiles[0].th;
+    MatchRec* recs = reinterpret_cast<MatchRec*>(smem64);
+    int* cnt = reinterpret_cast<int*>(recs + cap * G);
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(cnt + ((cap + 3) & ~3));
     __shared__ SlotLds sl[kMaxTileSlots];
-    int total = 0;
-    for (int t = 0; t < ns; ++t) {
-        const TileGeom& g = jb.tiles[t];
-        total += g.aw * g.ah + (g.aw - B + 1) * (g.ah - B + 1);
-    }
-    if (threadIdx.x < ns) {
-        const int t = threadIdx.x;
-        int o = 0;
-        for (int q = 0; q < t; ++q) o += jb.tiles[q].aw * jb.tiles[q].ah + (jb.tiles[q].aw - B + 1) * (jb.tiles[q].ah - B + 1);
-        const TileGeom& g = jb.tiles[t];
-        sl[t] = SlotLds{o, g.ax0(tx, jb.width, B, jb.grid.step, jb.range[t]),
-                        g.ay0(ty, jb.height, B, jb.grid.step, jb.range[t]), g.aw, g.ah};
-    }
-    MatchRec* recs = reinterpret_cast<MatchRec*>(acc + ((total + 1) & ~1));
-    int* cnt = reinterpret_cast<int*>(recs + tg.tw * tg.th * G);
-    float* scratch = reinterpret_cast<float*>(cnt + ((tg.tw * tg.th + 3) & ~3));
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float* T = scratch + wave * (G * kTs);
-    for (int e = threadIdx.x; e < total; e += 512) acc[e] = 0ull;
-    for (int e = threadIdx.x; e < ngroups * G; e += 512) {
-        const int g = e / G, j = e - g * G;
-        const int gj = g / ni, gi = g - gj * ni;
-        recs[e] = jb.matches[static_cast<size_t>((j_begin + gj) * jb.grid.nx + i_begin + gi) * G + j];
-    }
-    for (int g = threadIdx.x; g < ngroups; g += 512) {
-        const int gj = g / ni, gi = g - gj * ni;
-        cnt[g] = jb.counts[(j_begin + gj) * jb.grid.nx + i_begin + gi];
-    }
-    __syncthreads();
-
-    const float thr = 2.7f * jb.sigma;  // include/nss/params.hpp:46 kBmHardLambda
-    const float sig2 = jb.sigma * jb.sigma;
     const int gb = lane >> 3, y = lane & 7;
-    for (int g = wave; g < ngroups; g += kTTileWaves) {
-        const int k = cnt[g];
-        const MatchRec* rec = recs + g * G;
-        float v[M][8], vr[M][8];
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const int p = gb + 8 * m;
-            if (p < k) {
-                const MatchRec r = rec[p];
-                const int f = jb.slots.frame(r.t);
-                const float* sp = jb.src.frame(f) + static_cast<size_t>(r.y + y) * jb.stride + r.x;
-#pragma unroll
-                for (int x = 0; x < 8; ++x) v[m][x] = sp[x];
-                if constexpr (WIENER) {
-                    const float* wp = jb.wref.frame(f) + static_cast<size_t>(r.y + y) * jb.wref_stride + r.x;
-#pragma unroll
-                    for (int x = 0; x < 8; ++x) vr[m][x] = wp[x];
-                }
-            } else {
-#pragma unroll
-                for (int x = 0; x < 8; ++x) v[m][x] = vr[m][x] = 0.f;
-            }
+
+    // Thread e's record (e < ngroups G) and count (e < ngroups) of tile a,
+    // as four record words and the count.
+    struct Fetched {
+        int x, y, t;
+        uint32_t dist;
+        int count;
+    };
+    auto fetch = [&](const TileAt& a) {
+        const int e = threadIdx.x;
+        const TTileJob& jb = tab.job[a.job];
+        Fetched f{0, 0, 0, 0u, 0};
+        if (e < a.ngroups * G) {
+            const int g = e / G, j = e - g * G;
+            const int gj = g / a.ni, gi = g - gj * a.ni;
+            const MatchRec r = jb.matches[static_cast<size_t>((a.j_begin + gj) * jb.grid.nx + a.i_begin + gi) * G + j];
+            f.x = r.x;
+            f.y = r.y;
+            f.t = r.t;
+            f.dist = r.dist;
         }
-        float c[G], r[G];
-        forward_b8<G>(v, c, T, lane);
-        if constexpr (WIENER) forward_b8<G>(vr, r, T, lane);
-        int kept = 0;
-        float w2 = 0.f;
-#pragma unroll
-        for (int j = 0; j < G; ++j) {
-            const bool dc = (lane == 0 && j == 0);
-            if constexpr (WIENER) {
-                float w = 1.f;
-                if (!dc) {
-                    const float r2 = r[j] * r[j];
-                    const float den = r2 + sig2;
-                    w = den > 0.f ? r2 / den : 0.f;
-                }
-                c[j] *= w;
-                w2 = fmaf(w, w, w2);
-            } else {
-                if (!dc && fabsf(c[j]) < thr)
-                    c[j] = 0.f;
-                else
-                    ++kept;
-            }
+        if (e < a.ngroups) {
+            const int gj = e / a.ni, gi = e - gj * a.ni;
+            f.count = jb.counts[(a.j_begin + gj) * jb.grid.nx + a.i_begin + gi];
         }
-        haar_inverse_reg<G>(c);
-        float u[M][8];
-        group_to_columns<G>(c, u, T, lane);
-#pragma unroll
-        for (int m = 0; m < M; ++m) dct8<true>(u[m]);
-        transpose_rc<M>(u, T, lane);
-#pragma unroll
-        for (int m = 0; m < M; ++m) dct8<true>(u[m]);
-        float w;
-        if constexpr (WIENER) {
-            w = 1.f / fmaxf(wave_sum_f32(w2), 1e-12f);
-        } else {
-            w = 1.f / static_cast<float>(max(wave_sum_i32(kept), 1));
-        }
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const int p = gb + 8 * m;
-            if (p < k) {
-                const MatchRec rr = rec[p];
-                const SlotLds q = sl[rr.t];
-                const int lx = rr.x - q.ax0, ly = rr.y - q.ay0;
-                // The predictive search keeps every slot-t match within the
-                // slot's reach; a patch outside the apron is dropped rather
-                // than written out of bounds.
-                if (lx >= 0 && ly >= 0 && lx <= q.aw - B && ly <= q.ah - B) {
-                    unsigned long long* row = acc + q.off + (ly + y) * q.aw + lx;
-#pragma unroll
-                    for (int x = 0; x < 8; ++x) lds_add(row + x, to_fixed(w * u[m][x]));
-                    if (y == 0) lds_add(acc + q.off + q.aw * q.ah + ly * (q.aw - B + 1) + lx, to_fixed(w));
-                }
-            }
-        }
-    }
+        return f;
+    };
+    auto publish = [&](const TileAt& a, const Fetched& f) {
+        const int e = threadIdx.x;
+        if (e < a.ngroups * G) recs[e] = MatchRec{f.x, f.y, f.t, f.dist};
+        if (e < a.ngroups) cnt[e] = f.count;
+    };
+    auto gather = [&](const TileAt& a, int g, float (&v)[M][8], float (&vr)[M][8]) {
+        ttile_gather<G, WIENER>(tab.job[a.job], recs + g * G, cnt[g], gb, y, v, vr);
+    };
+
+    TileAt cur = ttile_at(tab, lo);
+    publish(cur, fetch(cur));
     __syncthreads();
-    for (int t = 0; t < ns; ++t) {
-        const SlotLds q = sl[t];
-        const TileGeom& g = jb.tiles[t];
-        const size_t area = static_cast<size_t>(g.pitch) * g.ah;
-        float* pn = jb.part + jb.slot_off[t] + static_cast<size_t>(tile) * area + (q.ax0 & 3);
-        float* pd = pn + static_cast<size_t>(g.ntx) * g.nty * area;
-        partials_out<B>(q.aw, q.ah, g.pitch, acc + q.off, acc + q.off + q.aw * q.ah, scratch, pn, pd);
+    float v[M][8], vr[M][8];
+    if (wave < cur.ngroups) gather(cur, wave, v, vr);
+    for (int lt = lo; lt < hi; ++lt) {
+        const TTileJob& jb = tab.job[cur.job];
+        const int ns = jb.nslots;
+        int total = 0;
+        for (int t = 0; t < ns; ++t) {
+            const TileGeom& g = jb.tiles[t];
+            total += g.aw * g.ah + (g.aw - B + 1) * (g.ah - B + 1);
+        }
+        if (threadIdx.x < ns) {
+            const int t = threadIdx.x;
+            int o = 0;
+            for (int q = 0; q < t; ++q)
+                o += jb.tiles[q].aw * jb.tiles[q].ah + (jb.tiles[q].aw - B + 1) * (jb.tiles[q].ah - B + 1);
+            const TileGeom& g = jb.tiles[t];
+            sl[t] = SlotLds{o, g.ax0(cur.tx, jb.width, B, jb.grid.step, jb.range[t]),
+                            g.ay0(cur.ty, jb.height, B, jb.grid.step, jb.range[t]), g.aw, g.ah};
+        }
+        float* scratch = reinterpret_cast<float*>(acc + ((total + 1) & ~1));
+        float* T = scratch + wave * (G * kTs);
+        for (int e = threadIdx.x; e < total; e += 512) acc[e] = 0ull;
         __syncthreads();
+
+        const float thr = 2.7f * jb.sigma;  // include/nss/params.hpp:46 kBmHardLambda
+        const float sig2 = jb.sigma * jb.sigma;
+        for (int g = wave; g < cur.ngroups; g += kTTileWaves) {
+            const int k = cnt[g];
+            const MatchRec* rec = recs + g * G;
+            float c[G], r[G];
+            forward_b8<G>(v, c, T, lane);
+            if constexpr (WIENER) forward_b8<G>(vr, r, T, lane);
+            if (g + kTTileWaves < cur.ngroups) gather(cur, g + kTTileWaves, v, vr);
+            int kept = 0;
+            float w2 = 0.f;
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const bool dc = (lane == 0 && j == 0);
+                if constexpr (WIENER) {
+                    float w = 1.f;
+                    if (!dc) {
+                        const float r2 = r[j] * r[j];
+                        const float den = r2 + sig2;
+                        w = den > 0.f ? r2 / den : 0.f;
+                    }
+                    c[j] *= w;
+                    w2 = fmaf(w, w, w2);
+                } else {
+                    if (!dc && fabsf(c[j]) < thr)
+                        c[j] = 0.f;
+                    else
+                        ++kept;
+                }
+            }
+            haar_inverse_reg<G>(c);
+            float u[M][8];
+            group_to_columns<G>(c, u, T, lane);
+#pragma unroll
+            for (int m = 0; m < M; ++m) dct8<true>(u[m]);
+            transpose_rc<M>(u, T, lane);
+#pragma unroll
+            for (int m = 0; m < M; ++m) dct8<true>(u[m]);
+            float w;
+            if constexpr (WIENER) {
+                w = 1.f / fmaxf(wave_sum_f32(w2), 1e-12f);
+            } else {
+                w = 1.f / static_cast<float>(max(wave_sum_i32(kept), 1));
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const int p = gb + 8 * m;
+                if (p < k) {
+                    const MatchRec rr = rec[p];
+                    const SlotLds q = sl[rr.t];
+                    const int lx = rr.x - q.ax0, ly = rr.y - q.ay0;
+                    // The predictive search keeps every slot-t match within the
+                    // slot's reach; a patch outside the apron is dropped rather
+                    // than written out of bounds.
+                    if (lx >= 0 && ly >= 0 && lx <= q.aw - B && ly <= q.ah - B) {
+                        unsigned long long* row = acc + q.off + (ly + y) * q.aw + lx;
+#pragma unroll
+                        for (int x = 0; x < 8; ++x) lds_add(row + x, to_fixed(w * u[m][x]));
+                        if (y == 0) lds_add(acc + q.off + q.aw * q.ah + ly * (q.aw - B + 1) + lx, to_fixed(w));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // The next tile's records: loaded now, stored once the first slot's
+        // partials are out (the records area is idle until then).
+        const bool more = lt + 1 < hi;
+        const TileAt nxt = more ? ttile_at(tab, lt + 1) : cur;
+        const Fetched nf = more ? fetch(nxt) : Fetched{0, 0, 0, 0u, 0};
+        for (int t = 0; t < ns; ++t) {
+            const SlotLds q = sl[t];
+            const TileGeom& g = jb.tiles[t];
+            const size_t area = static_cast<size_t>(g.pitch) * g.ah;
+            float* pn = jb.part + jb.slot_off[t] + static_cast<size_t>(cur.tile) * area + (q.ax0 & 3);
+            float* pd = pn + static_cast<size_t>(g.ntx) * g.nty * area;
+            partials_out<B>(q.aw, q.ah, g.pitch, acc + q.off, acc + q.off + q.aw * q.ah, scratch, pn, pd);
+            if (t == 0 && more) publish(nxt, nf);
+            __syncthreads();
+            if (t == 0 && more && wave < nxt.ngroups) gather(nxt, wave, v, vr);
+        }
+        cur = nxt;
     }
 }
 
@@ -2270,19 +2348,29 @@ size_t filter_temporal_tile_lds_bytes(int group, const TileGeom* g, int ns) {
     const size_t recs = static_cast<size_t>(g[0].tw) * g[0].th * group * sizeof(MatchRec) +
                         ((static_cast<size_t>(g[0].tw) * g[0].th + 3) & ~static_cast<size_t>(3)) * 4;
     const size_t tbuf = static_cast<size_t>(kTTileWaves) * group * kTs * 4;
-    return acc + recs + std::max(tbuf, boxsum);
+    return recs + acc + std::max(tbuf, boxsum);
+}
+
+// Persistent grid: as many workgroups as the device holds at once (at most
+// one per tile).
+template <typename Kernel>
+static hipError_t launch_persistent_ttile(Kernel k, const TTileTable& t, int tiles, size_t lds, hipStream_t s) {
+    allow_lds(k, lds);
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 512, lds);
+    if (e != hipSuccess) return e;
+    const int grid = std::max(1, std::min(tiles, std::max(cus, 1) * std::max(per_cu, 1)));
+    k<<<grid, 512, lds, s>>>(t, tiles);
+    return hipGetLastError();
 }
 
 template <int G>
-static hipError_t launch_ttile(const TTileTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
-    if (wiener) {
-        allow_lds(&filter_temporal_tile_kernel<G, true>, lds);
-        filter_temporal_tile_kernel<G, true><<<total_wg, 512, lds, s>>>(t);
-    } else {
-        allow_lds(&filter_temporal_tile_kernel<G, false>, lds);
-        filter_temporal_tile_kernel<G, false><<<total_wg, 512, lds, s>>>(t);
-    }
-    return hipGetLastError();
+static hipError_t launch_ttile(const TTileTable& t, int tiles, size_t lds, bool wiener, hipStream_t s) {
+    if (t.job[0].tiles[0].tw * t.job[0].tiles[0].th * G > 512) return hipErrorInvalidValue;
+    return wiener ? launch_persistent_ttile(&filter_temporal_tile_kernel<G, true>, t, tiles, lds, s)
+                  : launch_persistent_ttile(&filter_temporal_tile_kernel<G, false>, t, tiles, lds, s);
 }
 
 hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int total_wg, size_t lds, bool wiener,

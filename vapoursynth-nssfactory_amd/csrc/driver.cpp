@@ -645,6 +645,7 @@ void plan_temporal_tiles(nss_hip_bm3d* h, int s) {
     if (G != 8 && G != 16 && G != 32) return;
     static const int shapes[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
     for (const auto& sh : shapes) {
+        if (sh[0] * sh[1] * G > 512) continue;  // one match record per thread of the tile kernel
         size_t lds = 0;
         for (int k = 0; k < bk.nplanes; ++k) {
             TileGeom g[kMaxTileSlots];
