@@ -402,8 +402,13 @@ def serving_workload(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    # Defaults per workload (None): the headline times 200 frames after 500
+    # untimed ones (the GPU's clocks take a few hundred frames of this load to
+    # settle; the first frames of a process run several percent slower); the
+    # temporal workload counts passes over its 60-frame clip (2 after 1), the
+    # 4K precision workload frames (20 after 50).
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--form", choices=list(FORMS), default="fp32",
@@ -420,6 +425,13 @@ def main():
         print("bench.py: this build measures one GPU; each frame is independent, so N GPUs would each run this "
               "workload on their own frames (see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    defaults = {"two-stage-1080p": (200, 500), "temporal-1080p": (2, 1), "precision-4k": (20, 50),
+                "serving": (0, 0)}[args.workload]
+    if args.steps is None:
+        args.steps = defaults[0]
+    if args.warmup is None:
+        args.warmup = defaults[1]
 
     import numpy as np
     import torch

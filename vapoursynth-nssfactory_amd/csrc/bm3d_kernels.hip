@@ -143,6 +143,13 @@ __device__ inline uint32_t dist_key(float d) {
     return (k & kNonFinite) == kNonFinite ? kNonFinite : k;
 }
 
+// floor(c / d) for 0 <= c < 2^20 and 1 <= d <= 256, from inv = 1.f / d:
+// (c + 1/2) / d lies at least 1/(2d) away from an integer, far more than the
+// float rounding of the product.
+__device__ inline int div_small(int c, float inv) {
+    return static_cast<int>((static_cast<float>(c) + 0.5f) * inv);
+}
+
 __device__ inline int ref_x(const Grid& g, int i, int width, int block) { return min(i * g.step, width - block); }
 __device__ inline int ref_y(const Grid& g, int j, int height, int block) { return min(j * g.step, height - block); }
 
@@ -247,6 +254,7 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     // sees all 64 lanes.
     const int K = half < nrun ? min(jb.group - 1, ncand - 1) : 0;
     const int kmax = max(__builtin_amdgcn_readlane(K, 0), __builtin_amdgcn_readlane(K, 32));
+    const float inv_ww = 1.f / static_cast<float>(ww);
     const int ref_id = row * jb.grid.nx + i0 + half;
     MatchRec* out = jb.out + static_cast<size_t>(ref_id) * jb.group;
     if (half < nrun && hl == 0) {
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
         const int c = static_cast<int>(static_cast<uint32_t>(best));
         if (k < K && hl == (c & 31)) {
             dk[c] = kTaken;
-            const int dy = c / ww, dx = c - dy * ww;
+            const int dy = div_small(c, inv_ww), dx = c - dy * ww;
             out[1 + k] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(best >> 32)};
             best4[0] = best4[1];
             best4[1] = best4[2];

@@ -156,8 +156,14 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
         const int form = ((fast == 8 || fast == 16) && small) ? h->p.transform : 0;
         bk.fast_group = fast;
         bk.form = form;
+        // The FP32 register filter is latency bound at two workgroups per CU,
+        // so it starts from smaller tiles (more workgroups per CU): on 1080p
+        // YUV420 (B8, G8) 8 x 2 references filter fastest for the hard
+        // threshold and 4 x 4 for Wiener. The autotuner refines per input.
+        bool placed = false;
+        if (fast && !form) placed = apply_tile_shape(h, st, bk, st.wiener ? 4 : 8, st.wiener ? 4 : 2);
         for (const auto& sh : kTileShapes)
-            if (apply_tile_shape(h, st, bk, sh[0], sh[1])) break;
+            if (placed || (placed = apply_tile_shape(h, st, bk, sh[0], sh[1]))) break;
         if (!bk.tiles) {
             bk.fast_group = bk.form = 0;
             for (int k = 0; k < bk.nplanes; ++k)
