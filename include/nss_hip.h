@@ -155,7 +155,7 @@ int nss_hip_bm3d_process(nss_hip_bm3d* handle, const float* const* src, const in
                          const float* const* ref, const int* ref_stride, float* const* dst, const int* dst_stride,
                          int frame, int frame_count, void* stream);
 
-/* Temporal instances (radius > 0, temporal_mode rolling): the rolling
+/* Temporal instances (radius > 0; rolling, or legacy below): the rolling
  * computation of rollingGetFrame (src/host/filter_bm3d.cpp:730-851,
  * :985-1072) with all state on the device. Push the clip's frames in order
  * (device pointers, copied into the instance's device ring; ref planes when a
@@ -168,6 +168,31 @@ int nss_hip_bm3d_process(nss_hip_bm3d* handle, const float* const* src, const in
 int nss_hip_bm3d_push(nss_hip_bm3d* handle, const float* const* src, const int* src_stride, const float* const* ref,
                       const int* ref_stride, void* stream);
 int nss_hip_bm3d_pop(nss_hip_bm3d* handle, float* const* dst, const int* dst_stride, int* frame, void* stream);
+
+/* Legacy temporal output (radius > 0, temporal_mode "legacy"): push as
+ * above; pop returns, per centre frame c in order, its contribution frame
+ * (src/host/filter_bm3d.cpp:322-334): per plane 2r+1 slice pairs of plane
+ * height, num of slot t at rows 2t*h, den at (2t+1)*h, holding the sums of
+ * c's groups' patches taken from frame c-r+t (zeros where that frame is
+ * outside the clip; planes with sigma 0 carry the source in the centre slice
+ * with weight 1, src/frontend/temporal.hpp:23-36). The VapourSynth glue
+ * stamps these frame properties (src/frontend/contribution.hpp:12-27). */
+#define NSS_HIP_FAT_VERSION 2        /* _NSSFatVersion, include/nss/contracts.hpp:7 */
+#define NSS_HIP_FAT_LAYOUT 1         /* _NSSFatLayout: num/den slices, contracts.hpp:8 */
+#define NSS_HIP_MODEL_BM3D 1         /* _NSSModel, contracts.hpp:9 */
+#define NSS_HIP_MODEL_VERSION_BM3D 3 /* _NSSModelVersion, contracts.hpp:10-17 */
+#define NSS_HIP_NOISE_PROFILE_BM3D 1 /* _NSSNoiseProfile, contracts.hpp:19-20 */
+
+/* nss.VAggregate's reduction of one plane of output frame n
+ * (vaggregate_target, src/cpu/bm/agg.cpp:159-201, with the frame logic of
+ * src/host/filter_vaggregate.cpp:29-100): nums[k] / dens[k] are device
+ * pointers to the num and den rows of contribution k (the slice n-c+r of
+ * contribution frame c, for c = max(0, n-r) .. min(N-1, n+r) ascending),
+ * strides[k] in floats; dst = sum(num) / sum(den) in that order, or src where
+ * sum(den) <= 1e-12. 1 <= count <= 33. Asynchronous on stream. */
+int nss_hip_vaggregate(float* dst, int dst_stride, const float* const* nums, const float* const* dens,
+                       const int* strides, int count, const float* src, int src_stride, int width, int height,
+                       void* stream);
 
 /* Kernel tracing (the role of the reference's NVTX ranges group.match /
  * group.filter / group.aggregate, src/cuda/runtime/nvtx.hpp:15-32): while on,

@@ -266,6 +266,90 @@ def bm3d_clip(clip, params, ref_clip=None, fp64=False):
     return _temporal_stage(clip, params, False, None, clip, fp64)
 
 
+def _legacy_stage(clip, params, wref_clip, match_clip, fp64):
+    """Legacy temporal output of the last stage: per centre frame c and plane,
+    the 2r+1 num/den slice pairs stacked vertically (slot t's num at rows
+    2 t h, den at (2 t + 1) h; src/host/filter_bm3d.cpp:322-334), zeros for
+    slots whose frame lies outside the clip; planes with sigma 0 get the
+    identity contribution (src/frontend/temporal.hpp:23-36)."""
+    n_frames = len(clip)
+    r = params.radius
+    ns = 2 * r + 1
+    chroma = bool(getattr(params, "chroma", False))
+    out = [[None] * len(clip[0]) for _ in clip]
+    for p in range(len(clip[0])):
+        a = _stage_args(params, p, False)
+        frames = [f[p] for f in clip]
+        h, w = frames[0].shape
+        mplane = 0 if chroma else p
+        for c in range(n_frames):
+            fat = np.zeros((2 * ns * h, w), np.float32)
+            if a["sigma"] == 0:
+                fat[2 * r * h:(2 * r + 1) * h] = frames[c]
+                fat[(2 * r + 1) * h:(2 * r + 2) * h] = 1.0
+                out[c][p] = fat
+                continue
+            slot_frames = [c + min(max(t - r, -c), n_frames - 1 - c) for t in range(ns)]
+            srcs = [frames[f] for f in slot_frames]
+            wr = [wref_clip[f][p] for f in slot_frames] if wref_clip is not None else None
+            mt = [match_clip[f][mplane] for f in slot_frames]
+            _, num, den = bm3d_plane_temporal(srcs, r, a["sigma"], a["block"], a["group"], a["step"], a["bm_range"],
+                                              a["ps_num"], a["ps_range"], r, wrefs=wr, matches=mt, center=c,
+                                              frame_count=n_frames, fp64=fp64, want_slices=True)
+            for t in range(ns):
+                if 0 <= c + t - r < n_frames:
+                    fat[2 * t * h:(2 * t + 1) * h] = num[t]
+                    fat[(2 * t + 1) * h:(2 * t + 2) * h] = den[t]
+            out[c][p] = fat
+    return out
+
+
+def bm3d_clip_legacy(clip, params, ref_clip=None, fp64=False):
+    """nss.BM3D with radius > 0 and temporal_mode "legacy" over a clip: the
+    contribution frames of the centres in order. With final=1 the basic
+    stage runs rolling and feeds the Wiener stage, which emits the slices
+    (src/frontend/bm3d_args.cpp:186-221)."""
+    if params.radius < 1:
+        raise ValueError("legacy temporal output needs radius > 0")
+    if _is_final(params):
+        basic = _temporal_stage(clip, params, True, None, clip, fp64)
+        return _legacy_stage(clip, params, basic, basic, fp64)
+    if ref_clip is not None:
+        return _legacy_stage(clip, params, ref_clip, ref_clip, fp64)
+    return _legacy_stage(clip, params, None, clip, fp64)
+
+
+def vaggregate(fat_clip, src_clip, radius, planes=None):
+    """nss.VAggregate (src/host/filter_vaggregate.cpp:29-100 with
+    vaggregate_target, src/cpu/bm/agg.cpp:159-201): frame n of plane p is the
+    float32 sum over c = max(0, n - r) .. min(N - 1, n + r), ascending, of
+    slice n - c + r's num, over the same sum of den, or the source where that
+    den sum is <= 1e-12; planes not selected are the source."""
+    n_frames = len(src_clip)
+    rows = 2 * (2 * radius + 1)
+    out = []
+    for n in range(n_frames):
+        frame = []
+        for p, s in enumerate(src_clip[n]):
+            s = np.asarray(s, np.float32)
+            if planes is not None and p not in planes:
+                frame.append(s.copy())
+                continue
+            h = s.shape[0]
+            num = np.zeros_like(s)
+            den = np.zeros_like(s)
+            for c in range(max(0, n - radius), min(n_frames - 1, n + radius) + 1):
+                f = np.asarray(fat_clip[c][p], np.float32)
+                assert f.shape == (h * rows, s.shape[1])
+                sl = n - c + radius
+                num = num + f[2 * sl * h:(2 * sl + 1) * h]
+                den = den + f[(2 * sl + 1) * h:(2 * sl + 2) * h]
+            ok = den > np.float32(1e-12)
+            frame.append(np.where(ok, num / np.where(ok, den, np.float32(1)), s).astype(np.float32))
+        out.append(frame)
+    return out
+
+
 @dataclass
 class Params:
     """Mirror of Bm3dParams with the reference defaults (effective sigma)."""

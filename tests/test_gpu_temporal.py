@@ -157,3 +157,98 @@ def test_chroma_temporal(gpu, oracle):
     kw = dict(sigma=10, chroma=1, radius=1, final=1)
     _check(oracle, _gpu_clip(gpu, "YUV444PS", clip, **kw), _oracle_clip(oracle, gpu, "YUV444PS", clip, **kw),
            "chroma temporal")
+
+
+def _gpu_legacy(gpu, fmt, clip, **kw):
+    h, w = clip[0][0].shape
+    inst = gpu.BM3D(fmt, w, h, len(clip), temporal_mode="legacy", **kw)
+    assert inst.legacy
+    out = inst.filter_clip([[_dev(p) for p in f] for f in clip])
+    inst.close()
+    assert len(out) == len(clip)
+    return out
+
+
+@pytest.mark.parametrize("final", [0, 1])
+@pytest.mark.parametrize("shape", ["tiles", "atomic"])
+def test_legacy_slices(gpu, oracle, final, shape):
+    """temporal_mode="legacy": per centre the 2r+1 num/den slice pairs
+    (src/host/filter_bm3d.cpp:322-334) against the oracle's, >= 60 dB per
+    slice relative to its peak; reduced with the device VAggregate they give
+    the rolling output (>= 60 dB against the oracle, within float rounding of
+    the device's own rolling frames). "tiles" is the block-8 tile path,
+    "atomic" a block-4 shape on frame-sized sums."""
+    import torch
+    fmt, w, h, frames = "YUV420PS", 64, 48, 5
+    clip = _moving_clip(fmt, w, h, frames, 10, seed=21 + final)
+    kw = dict(sigma=[10, 10, 0], radius=1, final=final)
+    if shape == "atomic":
+        kw.update(block_size=4, block_step=2, group_size=8, bm_range=4)
+    fat = _gpu_legacy(gpu, fmt, clip, **kw)
+    p = gpu.parse(fmt, w, h, frames, temporal_mode="legacy", **kw)
+    want = oracle.bm3d_clip_legacy(clip, p)
+    for c in range(frames):
+        for i in range(3):
+            g = fat[c][i].cpu().numpy()
+            assert g.shape == want[c][i].shape
+            for s in range(6):
+                ph = g.shape[0] // 6
+                a, b = g[s * ph:(s + 1) * ph], want[c][i][s * ph:(s + 1) * ph]
+                peak = max(float(np.abs(b).max()), 1e-6)
+                if not b.any():
+                    assert not a.any(), (c, i, s)
+                    continue
+                assert oracle.psnr(a / peak, b / peak) >= MIN_PSNR, (shape, final, c, i, s)
+    src = [[_dev(q) for q in f] for f in clip]
+    red = gpu.VAggregate(fat, src, 1)
+    rolling = _gpu_clip(gpu, fmt, clip, **kw)
+    want_rolling = _oracle_clip(oracle, gpu, fmt, clip, **kw)
+    got = [_host(f) for f in red]
+    _check(oracle, got, want_rolling, f"legacy+VAggregate {shape} final={final}")
+    for gf, rf in zip(got, rolling):
+        for g, r_ in zip(gf, rf):
+            np.testing.assert_allclose(g, r_, rtol=0, atol=2e-5)
+    # the plane with sigma 0 comes back unchanged
+    for n in range(frames):
+        assert np.array_equal(got[n][2], clip[n][2])
+    torch.cuda.synchronize()
+
+
+def test_vaggregate_kernel_is_exact(gpu, oracle):
+    """The device reduction against the oracle's vaggregate_target on random
+    slices with dens at, below and around the 1e-12 cut, three frames at
+    radius 1 and a selected plane subset: bit for bit (float32 sums in
+    ascending centre order, IEEE division)."""
+    rng = np.random.default_rng(5)
+    h, w, r, frames = 12, 37, 1, 4
+    src, fat = [], []
+    for _ in range(frames):
+        src.append([rng.uniform(-1, 1, (h, w)).astype(np.float32) for _ in range(2)])
+        f = []
+        for _ in range(2):
+            a = rng.uniform(-2, 2, (6 * h, w)).astype(np.float32)
+            for s in range(3):
+                d = a[(2 * s + 1) * h:(2 * s + 2) * h]
+                d[:] = np.abs(d)
+                d[rng.uniform(size=d.shape) < 0.2] = 0
+                d[rng.uniform(size=d.shape) < 0.1] = 1e-13
+                d[rng.uniform(size=d.shape) < 0.1] = 1
+            f.append(a)
+        fat.append(f)
+    want = oracle.vaggregate(fat, src, r, planes=[1])
+    got = gpu.VAggregate([[_dev(q) for q in f] for f in fat], [[_dev(q) for q in f] for f in src], r, planes=[1])
+    for n in range(frames):
+        for i in range(2):
+            assert np.array_equal(_host(got[n])[i], want[n][i]), (n, i)
+
+
+def test_vaggregate_argument_errors(gpu):
+    """src/frontend/vaggregate_args.cpp:13-45 messages."""
+    a = [[_dev(np.zeros((6 * 4, 5)))]]
+    s = [[_dev(np.zeros((4, 5)))]]
+    with pytest.raises(gpu.Error, match="radius must be in"):
+        gpu.VAggregate(a, s, 17)
+    with pytest.raises(gpu.Error, match="height=src.height"):
+        gpu.VAggregate(a, s, 2)
+    with pytest.raises(gpu.Error, match="unique valid plane"):
+        gpu.VAggregate(a, s, 1, planes=[0, 0])

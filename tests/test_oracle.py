@@ -245,3 +245,47 @@ def test_static_clip_temporal_denoises_more(oracle):
     ps = np.mean([oracle.psnr(f[0], clean) for f in s])
     pt = np.mean([oracle.psnr(f[0], clean) for f in t])
     assert pt > ps + 0.5, (ps, pt)
+
+
+@pytest.mark.parametrize("final", [0, 1])
+def test_legacy_slices_reduce_to_rolling(oracle, final):
+    """The reference adds rolling contributions in ascending centre order "as
+    legacy + VAggregate" (src/host/filter_bm3d.cpp:588, :726): reducing the
+    legacy slices with VAggregate's float32 sums gives the rolling frames bit
+    for bit, including a plane with sigma 0 (identity slice) and the clip's
+    first and last frames (slices outside the clip are zero and unread)."""
+    import nss_hip
+    import synth
+    _, clip = synth.shifted_sequence("YUV420PS", 40, 32, 4, 10, seed=11, max_step=2)
+    kw = dict(sigma=[10, 10, 0], radius=1, final=final, block_size=4, block_step=2, group_size=8, bm_range=3)
+    rolling = oracle.bm3d_clip(clip, nss_hip.parse("YUV420PS", 40, 32, 4, **kw))
+    pl = nss_hip.parse("YUV420PS", 40, 32, 4, temporal_mode="legacy", **kw)
+    fat = oracle.bm3d_clip_legacy(clip, pl)
+    assert fat[0][0].shape == (32 * 6, 40) and fat[0][1].shape == (16 * 6, 20)
+    # slot 0 of the first centre and slot 2 of the last lie outside the clip
+    assert not fat[0][0][:2 * 32].any() and not fat[3][0][4 * 32:].any()
+    got = oracle.vaggregate(fat, clip, 1)
+    for n in range(4):
+        for p in range(3):
+            assert np.array_equal(got[n][p], rolling[n][p]), (final, n, p)
+
+
+def test_vaggregate_oracle_known_answers(oracle):
+    """vaggregate_target (src/cpu/bm/agg.cpp:159-201) by hand on a 1 x 3 plane,
+    radius 1, three frames: sums over the contributing centres, the source
+    where the den sum is <= 1e-12, and unselected planes copied."""
+    h, w, r = 1, 3, 1
+    src = [[np.array([[5., 6., 7.]], np.float32)] for _ in range(3)]
+    fat = []
+    for c in range(3):
+        f = np.zeros((2 * 3 * h, w), np.float32)
+        for t in range(3):
+            f[2 * t] = [c + 1, 2 * (c + 1), 0]          # num of slot t
+            f[2 * t + 1] = [1, 2, 1e-13]                 # den of slot t
+        fat.append([f])
+    out = oracle.vaggregate(fat, src, r)
+    # frame 0: centres 0, 1 -> num (1 + 2, 2 + 4), den (2, 4); third pixel falls back
+    np.testing.assert_array_equal(out[0][0], [[1.5, 1.5, 7.]])
+    # frame 1: centres 0, 1, 2 -> num (6, 12), den (3, 6)
+    np.testing.assert_array_equal(out[1][0], [[2., 2., 7.]])
+    assert np.array_equal(oracle.vaggregate(fat, src, r, planes=[])[2][0], src[2][0])

@@ -104,8 +104,11 @@ struct FinishJob {
     int block, step, range;
     const float* src;   // fallback where no patch landed
     float* dst;
+    // Slice output (legacy temporal): when set, dst receives the summed num
+    // and dst_den the summed den (no division, no fallback).
+    float* dst_den;
     int width, height, sstride, dstride;
-    int tiles_x;        // 16 x 16 output tiles per row
+    int tiles_x;        // output blocks per row (64 x 16 tile path, 16 x 16 atomic path)
     int wg_begin;
 };
 
@@ -237,6 +240,19 @@ struct TFinishJob {
     int width, height;
     int tiles_x;  // 64 x 16 output blocks per row
     int wg_begin;
+};
+
+// Reduction of legacy temporal slices (nss.VAggregate): contribution k's
+// num and den rows of one plane, summed in k order.
+inline constexpr int kVAggMax = 2 * kMaxRadius + 1;
+struct VAggJob {
+    const float* num[kVAggMax];
+    const float* den[kVAggMax];
+    int stride[kVAggMax];
+    int count;
+    const float* src;
+    float* dst;
+    int width, height, sstride, dstride;
 };
 
 struct TFinishTable {

@@ -2046,6 +2046,16 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
         }
     }
     float* dst = jb.dst + static_cast<size_t>(y) * jb.dstride;
+    if (jb.dst_den) {
+        float* dd = jb.dst_den + static_cast<size_t>(y) * jb.dstride;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (x0 + i < jb.width) {
+                dst[x0 + i] = n[i];
+                dd[x0 + i] = d[i];
+            }
+        return;
+    }
     const float* src = jb.src + static_cast<size_t>(y) * jb.sstride;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -2088,10 +2098,40 @@ __global__ __launch_bounds__(256) void finish_corner_kernel(FinishTable tab) {
         for (int i = 0; i < B; ++i) d += colsum[oy * TS + ox + i];
         const long long n = static_cast<long long>(jb.num[static_cast<size_t>(y) * jb.width + x]);
         const long long dd = static_cast<long long>(d);
+        if (jb.dst_den) {
+            jb.dst[static_cast<size_t>(y) * jb.dstride + x] =
+                static_cast<float>(static_cast<double>(n) * (1.0 / 4294967296.0));
+            jb.dst_den[static_cast<size_t>(y) * jb.dstride + x] =
+                static_cast<float>(static_cast<double>(dd) * (1.0 / 4294967296.0));
+            return;
+        }
         jb.dst[static_cast<size_t>(y) * jb.dstride + x] =
             dd > 0 ? static_cast<float>(static_cast<double>(n) / static_cast<double>(dd))
                    : jb.src[static_cast<size_t>(y) * jb.sstride + x];
     }
+}
+
+// nss.VAggregate's reduction of one plane (src/cpu/bm/agg.cpp:159-201
+// vaggregate_target): sum(num) / sum(den) over the contributions in their
+// order, the source where sum(den) <= 1e-12. 64 x 4 pixels per block.
+__global__ __launch_bounds__(256) void vaggregate_kernel(VAggJob jb) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= jb.width || y >= jb.height) return;
+    float n = 0.f, d = 0.f;
+    for (int k = 0; k < jb.count; ++k) {
+        const size_t o = static_cast<size_t>(y) * jb.stride[k] + x;
+        n += jb.num[k][o];
+        d += jb.den[k][o];
+    }
+    jb.dst[static_cast<size_t>(y) * jb.dstride + x] =
+        d > 1e-12f ? n / d : jb.src[static_cast<size_t>(y) * jb.sstride + x];
+}
+
+hipError_t launch_vaggregate(const VAggJob& jb, hipStream_t s) {
+    const dim3 grid((jb.width + 63) / 64, (jb.height + 3) / 4);
+    vaggregate_kernel<<<grid, 256, 0, s>>>(jb);
+    return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void zero_kernel(ZeroTable tab) {

@@ -254,7 +254,7 @@ size_t instance_bytes(const nss_hip_bm3d* h) {
     const auto& R = h->roll;
     for (int i = 0; i < h->nplanes; ++i) {
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        total += npx * 4 * static_cast<size_t>(R.src_n + R.ref_n + R.basic_n + R.out_n);
+        total += npx * 4 * static_cast<size_t>(R.src_n + R.ref_n + R.basic_n + R.out_n * R.out_rows);
         for (int s = 0; s < h->nstages; ++s) {
             if (R.num[s][i]) total += npx * 16 * static_cast<size_t>(R.acc_n);
             if (R.part[s][i]) total += R.part_center[s][i] * (2 * static_cast<size_t>(R.r) + 1) * sizeof(float);
@@ -901,6 +901,108 @@ int temporal_finish(nss_hip_bm3d* h, int s, int n, hipStream_t st) {
 
 // Runs every centre that the frames pushed so far allow, in order, and
 // finishes the frames they complete.
+// Legacy temporal output of centre c (src/host/filter_bm3d.cpp:322-334):
+// for each slot t, the num and den sums of c's groups' patches taken from
+// slot t's frame, as float planes at rows 2t and 2t + 1 (in plane heights) of
+// output frame c; slots outside the clip, and clamped repeats of an edge
+// frame, hold zeros. Planes not filtered get the identity contribution of
+// temporal_identity (src/frontend/temporal.hpp:23-36): the source in the
+// centre slice with weight 1.
+int legacy_emit(nss_hip_bm3d* h, int c, hipStream_t st) {
+    auto& R = h->roll;
+    const int s = h->nstages - 1;
+    const Stage& stg = h->stages[s];
+    const int ns = 2 * R.r + 1;
+    const Slots slots{c, R.N, R.r};
+    const int vb = slots.valid_begin(), ve = slots.valid_end();
+    auto fat = [&](int i) {
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        return R.out[i] + static_cast<size_t>(c % R.out_n) * npx * R.out_rows;
+    };
+    for (int i = 0; i < h->nplanes; ++i) {
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        float* f = fat(i);
+        const bool filtered = stg.plane[i].sigma != 0.f;
+        // zeros in every slice the kernels below do not write
+        for (int t = 0; t < ns; ++t) {
+            if (filtered && (R.tiles[s] || (t >= vb && t < ve))) continue;
+            hipError_t e = hipMemsetAsync(f + static_cast<size_t>(2 * t) * npx, 0, 2 * npx * 4, st);
+            if (e != hipSuccess) return hip_fail(e, "slice clear");
+        }
+        if (!filtered) {
+            hipError_t e = hipMemcpyAsync(f + static_cast<size_t>(2 * R.r) * npx,
+                                          R.src[i] + static_cast<size_t>(c % R.src_n) * npx, npx * 4,
+                                          hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess) e = hipMemsetD32Async(f + static_cast<size_t>(2 * R.r + 1) * npx, 0x3f800000u, npx, st);
+            if (e != hipSuccess) return hip_fail(e, "identity slice");
+        }
+    }
+    for (int bi = 0; bi < stg.nbuckets; ++bi) {
+        const Bucket& bk = stg.bucket[bi];
+        FinishTable fn{};
+        int nwg = 0;
+        auto flush = [&]() -> int {
+            if (!fn.n) return NSS_HIP_OK;
+            const hipError_t e = traced(h, 3, st, [&] { return launch_finish(bk.block, fn, nwg, R.tiles[s], st); });
+            fn = FinishTable{};
+            nwg = 0;
+            return e == hipSuccess ? NSS_HIP_OK : hip_fail(e, "slice launch");
+        };
+        for (int k = 0; k < bk.nplanes; ++k) {
+            const int i = bk.planes[k];
+            const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+            TileGeom g[kMaxTileSlots];
+            int range[kMaxTileSlots];
+            long long off[kMaxTileSlots];
+            if (R.tiles[s]) {
+                slot_geoms(h, s, i, R.tw[s], R.th[s], g, range);
+                centre_partials(g, ns, off);
+            }
+            for (int t = 0; t < ns; ++t) {
+                if (!R.tiles[s] && (t < vb || t >= ve)) continue;
+                if (fn.n == kMaxJobs) {
+                    const int rc = flush();
+                    if (rc != NSS_HIP_OK) return rc;
+                }
+                FinishJob& f = fn.job[fn.n++];
+                if (R.tiles[s]) {
+                    f.part_num = R.part[s][i] + static_cast<size_t>(c % ns) * R.part_center[s][i] + off[t];
+                    f.part_den = f.part_num + static_cast<size_t>(g[t].ntx) * g[t].nty * g[t].pitch * g[t].ah;
+                    f.tiles = g[t];
+                    f.range = range[t];
+                    f.tiles_x = (h->pw[i] + 63) / 64;
+                } else {
+                    const size_t slot = static_cast<size_t>(slots.frame(t) % R.acc_n) * npx;
+                    f.num = R.num[s][i] + slot;
+                    f.den = R.den[s][i] + slot;
+                    f.tiles_x = (h->pw[i] + 15) / 16;
+                }
+                f.block = bk.block;
+                f.step = stg.plane[i].step;
+                f.dst = fat(i) + static_cast<size_t>(2 * t) * npx;
+                f.dst_den = f.dst + npx;
+                f.width = h->pw[i];
+                f.height = h->ph[i];
+                f.sstride = f.dstride = h->pw[i];
+                f.wg_begin = nwg;
+                nwg += f.tiles_x * ((h->ph[i] + 15) / 16);
+            }
+        }
+        const int rc = flush();
+        if (rc != NSS_HIP_OK) return rc;
+        if (!R.tiles[s])
+            for (int k = 0; k < bk.nplanes; ++k) {
+                // the next centre starts from empty sums
+                const int i = bk.planes[k];
+                const size_t bytes = static_cast<size_t>(h->pw[i]) * h->ph[i] * 8 * R.acc_n;
+                hipError_t e = hipMemsetAsync(R.num[s][i], 0, bytes, st);
+                if (e == hipSuccess) e = hipMemsetAsync(R.den[s][i], 0, bytes, st);
+                if (e != hipSuccess) return hip_fail(e, "sum clear");
+            }
+    }
+    return NSS_HIP_OK;
+}
+
 int temporal_advance(nss_hip_bm3d* h, hipStream_t st) {
     auto& R = h->roll;
     const int last = h->nstages - 1;
@@ -915,6 +1017,13 @@ int temporal_advance(nss_hip_bm3d* h, hipStream_t st) {
             int rc = temporal_center(h, s, c, st);
             if (rc != NSS_HIP_OK) return rc;
             R.next_center[s] = c + 1;
+            if (R.legacy && s == last) {
+                rc = legacy_emit(h, c, st);
+                if (rc != NSS_HIP_OK) return rc;
+                R.finished[s] = c + 1;
+                progress = true;
+                continue;
+            }
             while (R.finished[s] < R.N && R.next_center[s] > std::min(R.finished[s] + R.r, R.N - 1)) {
                 rc = temporal_finish(h, s, R.finished[s], st);
                 if (rc != NSS_HIP_OK) return rc;
@@ -953,10 +1062,6 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
     if (info->sample_type != 1 || info->bits_per_sample != 32 || info->width <= 0 || info->height <= 0 ||
         info->num_planes < 1 || info->num_planes > 3)
         return fail(NSS_HIP_INVALID_INPUT, "nss_hip.BM3D: constant Gray/YUV/RGB 32-bit float required");
-    if (params->radius != 0 && !params->temporal_mode)
-        return fail(NSS_HIP_UNSUPPORTED,
-                    "nss_hip: temporal_mode=legacy (the num/den slice output for VAggregate) is not available in "
-                    "this build; use temporal_mode=rolling");
     if (params->radius != 0 && (info->width > 8191 || info->height > 8191))
         return fail(NSS_HIP_UNSUPPORTED, "nss_hip: radius > 0 supports planes up to 8191 x 8191");
     if (params->radius != 0 && info->num_frames < 1)
@@ -1031,10 +1136,12 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
             R.basic_n = p.final_stage ? 3 * r + 2 : 0;
             R.out_n = 4 * r + 2;
             R.acc_n = 2 * r + 1;
+            R.legacy = !p.temporal_mode;
+            R.out_rows = R.legacy ? 2 * (2 * r + 1) : 1;
             e = hipMalloc(&R.src[i], npx * 4 * R.src_n);
             if (e == hipSuccess && R.ref_n) e = hipMalloc(&R.ref[i], npx * 4 * R.ref_n);
             if (e == hipSuccess && R.basic_n) e = hipMalloc(&R.basic[i], npx * 4 * R.basic_n);
-            if (e == hipSuccess) e = hipMalloc(&R.out[i], npx * 4 * R.out_n);
+            if (e == hipSuccess) e = hipMalloc(&R.out[i], npx * 4 * R.out_n * R.out_rows);
             if (i == 0)
                 for (int s = 0; s < h->nstages; ++s) plan_temporal_tiles(h, s);
             for (int s = 0; s < h->nstages && e == hipSuccess; ++s) {
@@ -1112,11 +1219,11 @@ int nss_hip_bm3d_pop(nss_hip_bm3d* h, float* const* dst, const int* dst_stride, 
     for (int i = 0; i < h->nplanes; ++i) {
         if (!dst[i] || dst_stride[i] < h->pw[i])
             return fail(NSS_HIP_INVALID_INPUT, "nss_hip: missing plane or stride below width");
-        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        const size_t fpx = static_cast<size_t>(h->pw[i]) * h->ph[i] * R.out_rows;
         const hipError_t e =
-            hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, R.out[i] + static_cast<size_t>(n % R.out_n) * npx,
-                             static_cast<size_t>(h->pw[i]) * 4, static_cast<size_t>(h->pw[i]) * 4, h->ph[i],
-                             hipMemcpyDeviceToDevice, s);
+            hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, R.out[i] + static_cast<size_t>(n % R.out_n) * fpx,
+                             static_cast<size_t>(h->pw[i]) * 4, static_cast<size_t>(h->pw[i]) * 4,
+                             static_cast<size_t>(h->ph[i]) * R.out_rows, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return hip_fail(e, "frame download");
     }
     if (frame) *frame = n;
@@ -1125,6 +1232,33 @@ int nss_hip_bm3d_pop(nss_hip_bm3d* h, float* const* dst, const int* dst_stride, 
 }
 
 int nss_hip_bm3d_slots(const nss_hip_bm3d* handle) { return handle ? 2 * handle->p.radius + 1 : 0; }
+
+int nss_hip_vaggregate(float* dst, int dst_stride, const float* const* nums, const float* const* dens,
+                       const int* strides, int count, const float* src, int src_stride, int width, int height,
+                       void* stream) {
+    if (!dst || !nums || !dens || !strides || !src) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    if (count < 1 || count > kVAggMax)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.VAggregate: 1 to 33 contributions per frame");
+    if (width < 1 || height < 1 || dst_stride < width || src_stride < width)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.VAggregate: invalid plane dimensions or strides");
+    VAggJob jb{};
+    for (int k = 0; k < count; ++k) {
+        if (!nums[k] || !dens[k] || strides[k] < width)
+            return fail(NSS_HIP_INVALID_INPUT, "nss_hip.VAggregate: missing contribution or stride below width");
+        jb.num[k] = nums[k];
+        jb.den[k] = dens[k];
+        jb.stride[k] = strides[k];
+    }
+    jb.count = count;
+    jb.src = src;
+    jb.dst = dst;
+    jb.width = width;
+    jb.height = height;
+    jb.sstride = src_stride;
+    jb.dstride = dst_stride;
+    const hipError_t e = launch_vaggregate(jb, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? NSS_HIP_OK : hip_fail(e, "vaggregate launch");
+}
 
 int nss_hip_bm3d_process(nss_hip_bm3d* h, const float* const* src, const int* src_stride, const float* const* ref,
                          const int* ref_stride, float* const* dst, const int* dst_stride, int frame, int frame_count,

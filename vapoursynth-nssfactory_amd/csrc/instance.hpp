@@ -37,6 +37,7 @@ size_t filter_temporal_tile_lds_bytes(int group, const TileGeom* g, int ns);
 hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int total_wg, size_t lds, bool wiener,
                                         hipStream_t s);
 hipError_t launch_finish_temporal_tiles(const TFinishTable& t, int total_wg, hipStream_t s);
+hipError_t launch_vaggregate(const VAggJob& jb, hipStream_t s);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
@@ -157,6 +158,11 @@ struct nss_hip_bm3d {
         int next_center[2]{};
         int finished[2]{};
         int out_head = 0;
+        // Legacy temporal output: the last stage emits, per centre, its 2r+1
+        // num/den slice pairs (out_rows = 2 (2r + 1) plane heights per output
+        // frame) instead of finished frames (out_rows = 1).
+        bool legacy = false;
+        int out_rows = 1;
         // Tile path of a stage (block 8 register kernel, radius <= 2): the
         // tile partials of the last 2r+1 centres per plane, centre c at
         // part + (c % (2r+1)) * part_center.

@@ -433,6 +433,8 @@ int nss_hip_server_open_stream(nss_hip_server* s, const nss_hip_bm3d_params* par
     if (params->device_id != s->cfg.device_id)
         return fail(NSS_HIP_INVALID_INPUT, "nss_hip: stream device_id differs from the server's");
     if (params->has_ref) return fail(NSS_HIP_UNSUPPORTED, "nss_hip: server streams take no ref clip");
+    if (params->radius > 0 && !params->temporal_mode)
+        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: server streams return finished frames; use temporal_mode=rolling");
     DeviceGuard guard(s->cfg.device_id);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     std::lock_guard<std::mutex> lock(s->mu);

@@ -98,7 +98,11 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_spatial_match", "nss_hip_bm3d_set_tracing", "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push",
            "nss_hip_bm3d_pop", "nss_hip_server_create", "nss_hip_server_open_stream", "nss_hip_server_submit",
            "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy",
-           "nss_hip_bm3d_autotune")
+           "nss_hip_bm3d_autotune", "nss_hip_vaggregate")
+# Contribution identity of legacy temporal frames (frame properties the
+# VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
+# src/frontend/contribution.hpp:12-27).
+FAT_VERSION, FAT_LAYOUT, MODEL_BM3D, MODEL_VERSION_BM3D, NOISE_PROFILE_BM3D = 2, 1, 1, 3, 1
 KERNEL_KINDS = ("clear", "match", "filter", "finish")
 
 
@@ -138,6 +142,7 @@ def lib():
         L.nss_hip_bm3d_push.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_bm3d_pop.argtypes = [vp, P(vp), P(ci), P(ci), vp]
         L.nss_hip_bm3d_autotune.argtypes = [vp, ci, vp, P(ctypes.c_double)]
+        L.nss_hip_vaggregate.argtypes = [vp, ci, P(vp), P(vp), P(ci), ci, vp, ci, ci, ci, vp]
         L.nss_hip_server_create.argtypes = [P(ServerConfig), P(vp)]
         L.nss_hip_server_open_stream.argtypes = [vp, P(Params), P(VideoInfo), P(ci)]
         L.nss_hip_server_submit.argtypes = [vp, ci, P(vp), P(ci), P(vp), P(ci), ctypes.c_double,
@@ -259,6 +264,11 @@ class BM3D:
         self._h = h
         cf, sw, sh, n = FORMATS[fmt]
         self.plane_dims = [(height >> (sh if i else 0), width >> (sw if i else 0)) for i in range(n)]
+        # radius > 0 with temporal_mode "legacy": pop returns contribution
+        # frames of 2 (2r + 1) plane heights (num/den slice pairs).
+        self.legacy = self.params.radius > 0 and self.params.temporal_mode == 0
+        rows = 2 * (2 * self.params.radius + 1) if self.legacy else 1
+        self.out_dims = [(hh * rows, ww) for hh, ww in self.plane_dims]
 
     def autotune(self, repeats=3, stream=None):
         """Times every filter tile shape that fits (per bucket) on a synthetic
@@ -306,7 +316,7 @@ class BM3D:
         n = len(self.plane_dims)
         if out is None:
             out = [torch.empty(d, dtype=torch.float32, device=torch.device("cuda", self.device))
-                   for d in self.plane_dims]
+                   for d in self.out_dims]
         dp = (ctypes.c_void_p * n)(*[p.data_ptr() for p in out])
         dst = (ctypes.c_int * n)(*[p.stride(0) for p in out])
         frame = ctypes.c_int(-1)
@@ -318,7 +328,8 @@ class BM3D:
 
     def filter_clip(self, frames, refs=None, stream=None):
         """radius > 0: every frame of a clip (list of per-frame plane lists)
-        through push/pop; returns the finished frames in order."""
+        through push/pop; returns the finished frames in order (legacy
+        temporal mode: the contribution frames of the centres in order)."""
         out = []
         for i, f in enumerate(frames):
             self.push(f, None if refs is None else refs[i], stream=stream)
@@ -383,6 +394,67 @@ def spatial_match(plane, block=8, step=8, bm_range=9, group=8, stream=None):
                                        bm_range, group, ctypes.c_void_p(rec.data_ptr()),
                                        ctypes.c_void_p(cnt.data_ptr()), _stream_handle(stream)))
     return rec, cnt
+
+
+def VAggregate(clip, src, radius, planes=None, stream=None):
+    """nss.VAggregate on the device (frame logic src/host/filter_vaggregate.cpp:29-100,
+    argument checks src/frontend/vaggregate_args.cpp:13-45): ``clip`` holds the
+    legacy contribution frames (per frame, per plane a row-contiguous 2-D
+    float32 CUDA tensor of 2 (2 radius + 1) plane heights), ``src`` the
+    source frames. Output frame n is sum(num) / sum(den) over the
+    contribution frames c = max(0, n - radius) .. min(N - 1, n + radius),
+    slice n - c + radius, added in ascending c, or the source where nothing
+    contributed; planes not in ``planes`` are copied from src."""
+    import torch
+    ns = f"{NAMESPACE}.VAggregate"
+    if not 0 <= radius <= 16:
+        raise Error(f"{ns}: radius must be in [0, 16]")
+    nframes = len(src)
+    nplanes = len(src[0]) if nframes else 0
+    if planes is None:
+        sel = [True] * nplanes
+    else:
+        if len(set(planes)) != len(planes) or any(not 0 <= q < nplanes for q in planes):
+            raise Error(f"{ns}: planes must contain unique valid plane indices")
+        sel = [q in planes for q in range(nplanes)]
+    rows = 2 * (2 * radius + 1)
+    shape_msg = f"{ns}: clip must match src format, width, and frame count, with height=src.height*(2*radius+1)*2"
+    if len(clip) != nframes:
+        raise Error(shape_msg)
+    for n in range(nframes):
+        if len(clip[n]) != nplanes:
+            raise Error(shape_msg)
+        for q in range(nplanes):
+            a, b = clip[n][q], src[n][q]
+            if a.shape != (b.shape[0] * rows, b.shape[1]) or a.dtype != torch.float32 or b.dtype != torch.float32:
+                raise Error(shape_msg)
+            if a.stride(1) != 1 or b.stride(1) != 1:
+                raise Error(f"{ns}: planes must be row-contiguous")
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    out = []
+    for n in range(nframes):
+        first, last = max(0, n - radius), min(nframes - 1, n + radius)
+        frame = []
+        for q in range(nplanes):
+            s = src[n][q]
+            if not sel[q]:
+                frame.append(s.clone())
+                continue
+            h, w = s.shape
+            dst = torch.empty_like(s)
+            nums, dens, strides = [], [], []
+            for c in range(first, last + 1):
+                sl, f = n - c + radius, clip[c][q]
+                nums.append(f.data_ptr() + 2 * sl * h * f.stride(0) * 4)
+                dens.append(f.data_ptr() + (2 * sl + 1) * h * f.stride(0) * 4)
+                strides.append(f.stride(0))
+            k = len(nums)
+            _check(lib().nss_hip_vaggregate(dst.data_ptr(), dst.stride(0), (vp * k)(*nums), (vp * k)(*dens),
+                                            (ci * k)(*strides), k, s.data_ptr(), s.stride(0), w, h,
+                                            _stream_handle(stream)))
+            frame.append(dst)
+        out.append(frame)
+    return out
 
 
 class Server:
