@@ -77,11 +77,11 @@ FORMS = {"fp32": 0, "bf16x3": 1, "fp16": 2}
 # WRITE_SIZE (KiB per dispatch, averaged over the basic and Wiener launches)
 # scaled by what the counters read for a 256 MiB elementwise copy in the same
 # process (FETCH x2, WRITE x1); profiles/bm3d_1080p_yuv420_two_stage_kernels.md.
-TRAFFIC_KIB = {("match", "fp32"): (6612, 6280), ("match", "bf16x3"): (6612, 6280), ("match", "fp16"): (6612, 6280),
-               ("filter", "fp32"): ((9652 + 16071) / 2, 52597.5),
-               ("filter", "bf16x3"): ((9703 + 16148) / 2, 52599), ("filter", "fp16"): ((9688 + 16137) / 2, 52601.5),
-               ("finish", "fp32"): (25972, 12233), ("finish", "bf16x3"): (25972, 12233),
-               ("finish", "fp16"): (25972, 12233)}
+TRAFFIC_KIB = {("match", "fp32"): (6562, 6280), ("match", "bf16x3"): (6562, 6280), ("match", "fp16"): (6562, 6280),
+               ("filter", "fp32"): ((9649 + 16064) / 2, (71536 + 66940) / 2),
+               ("filter", "bf16x3"): ((9696 + 16149) / 2, 52599), ("filter", "fp16"): ((9695 + 16138) / 2, 52601),
+               ("finish", "fp32"): (28788, 12374), ("finish", "bf16x3"): (28788, 12374),
+               ("finish", "fp16"): (28788, 12374)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 FORM_DTYPE = {"fp32": "f32", "bf16x3": "bf16x3 (f32 accumulate)", "fp16": "f16 (f32 accumulate)"}
 PEAK_MFMA_TFLOPS = 2500.0  # dense bf16 / fp16
