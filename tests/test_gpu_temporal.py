@@ -252,3 +252,45 @@ def test_vaggregate_argument_errors(gpu):
         gpu.VAggregate(a, s, 2)
     with pytest.raises(gpu.Error, match="unique valid plane"):
         gpu.VAggregate(a, s, 1, planes=[0, 0])
+
+
+@pytest.mark.parametrize("kw", [dict(radius=2), dict(radius=3), dict(radius=1, chroma=1),
+                                dict(radius=1, block_size=16, block_step=8, group_size=4, bm_range=4)])
+def test_legacy_more_shapes(gpu, oracle, kw):
+    """Legacy slices reduced on the device against the oracle's rolling
+    frames (>= 60 dB per plane): radius 2 (tile path), radius 3 (beyond the
+    tile path's slot count: frame-sized sums), CBM3D, and block 16."""
+    fmt, w, h, frames = ("YUV444PS" if kw.get("chroma") else "YUV420PS"), 48, 40, 6
+    clip = _moving_clip(fmt, w, h, frames, 10, seed=31)
+    kw = dict(kw, sigma=10)
+    fat = _gpu_legacy(gpu, fmt, clip, **kw)
+    red = [_host(f) for f in gpu.VAggregate(fat, [[_dev(q) for q in f] for f in clip], kw["radius"])]
+    _check(oracle, red, _oracle_clip(oracle, gpu, fmt, clip, **kw), f"legacy {kw}")
+
+
+def test_legacy_with_ref_and_identity(gpu, oracle):
+    """Legacy output of a Wiener stage with a ref clip against the oracle's
+    slices, and bitwise identical on a second run."""
+    import synth
+    fmt, w, h, frames = "GRAYS", 40, 32, 4
+    clip = _moving_clip(fmt, w, h, frames, 20, seed=41)
+    ref = [[synth.clean_image(w, h, seed=41)] for _ in range(frames)]
+    kw = dict(sigma=20, radius=1, temporal_mode="legacy")
+    outs = []
+    for _ in range(2):
+        inst = gpu.BM3D(fmt, w, h, frames, ref=True, **kw)
+        fat = inst.filter_clip([[_dev(p) for p in f] for f in clip], [[_dev(p) for p in f] for f in ref])
+        inst.close()
+        outs.append([_host(f) for f in fat])
+    for a, b in zip(*outs):
+        assert np.array_equal(a[0], b[0])
+    want = oracle.bm3d_clip_legacy(clip, gpu.parse(fmt, w, h, frames, ref=True, **kw), ref_clip=ref)
+    for c in range(frames):
+        g, o = outs[0][c][0], want[c][0]
+        for s in range(6):
+            a, b = g[s * h:(s + 1) * h], o[s * h:(s + 1) * h]
+            if not b.any():
+                assert not a.any()
+                continue
+            peak = float(np.abs(b).max())
+            assert oracle.psnr(a / peak, b / peak) >= MIN_PSNR, (c, s)
