@@ -37,9 +37,13 @@ def window_candidates(width, height, block, step, rng):
     return int(np.sum(wy[:, None] * wx[None, :] - 1)), nx * ny
 
 
-def stage_work(params, dims, basic):
+def stage_work(params, dims, basic, directions=None):
     """Algorithmic work of one stage over the frame's planes (one launch of
-    each kernel): SSD flops, transform flops, aggregation + finish bytes."""
+    each kernel): SSD flops, transform flops, aggregation + finish bytes.
+    Transform directions: 2 (hard threshold) in the basic stage, 3 (Wiener)
+    in the final one, unless given."""
+    if directions is None:
+        directions = 2 if basic else 3
     ssd = tr = agg = 0
     for i, (h, w) in enumerate(dims):
         b = params.block_size_basic[i] if basic else params.block_size[i]
@@ -48,7 +52,7 @@ def stage_work(params, dims, basic):
         cands, nref = window_candidates(w, h, b, s, params.bm_range[i])
         ssd += 3 * b * b * cands
         per_dir = g * 4 * b ** 3 + 2 * g * g * b * b
-        tr += nref * per_dir * (2 if basic else 3)
+        tr += nref * per_dir * directions
         agg += nref * g * b * b * 20 + w * h * 16  # k <= G; G at these sizes (window >> G)
     return ssd, tr, agg
 
@@ -247,6 +251,79 @@ def precision_workload(args):
                                                             "rms_err": float(np.sqrt(np.mean(d * d)))}}), flush=True)
 
 
+def single_stage_256_workload(args):
+    """One 256x256 GRAYS frame per step (the config-0 generator), sigma 5, one
+    hard-threshold stage with block 8, step 3, group 16, range 16: GPU frame
+    time and per-kernel rooflines, the CPU port on the host's cores, and PSNR
+    against it."""
+    import numpy as np
+    import torch
+
+    import nss_hip
+    import oracle
+    import synth
+    w, h = 256, 256
+    kw = dict(sigma=5, block_size=8, block_step=3, group_size=16, bm_range=16)
+    clean, noisy = synth.noisy_frame("GRAYS", w, h, 5, seed=0)
+    dev = [torch.from_numpy(p).cuda() for p in noisy]
+    inst = nss_hip.BM3D("GRAYS", w, h, **kw)
+    out = [torch.empty_like(p) for p in dev]
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        inst(dev, out=out)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        inst(dev, out=out)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
+    inst.set_tracing(True)
+    inst.kernel_times(reset=True)
+    for _ in range(max(args.steps // 2, 1)):
+        inst(dev, out=out)
+    torch.cuda.synchronize()
+    kt = inst.kernel_times(reset=True)
+    params = inst.params
+    inst.close()
+    got = out[0].cpu().numpy()
+    ssd, tr, agg = stage_work(params, [(h, w)], False, directions=2)
+    work = {"match": (ssd, "TFLOP/s", PEAK_FP32_TFLOPS), "filter": (tr, "TFLOP/s", PEAK_FP32_TFLOPS),
+            "finish": (w * h * 16, "GB/s", PEAK_HBM_GBS)}
+    kernels = {}
+    for kind, (n, ms) in kt.items():
+        if n and kind in work:
+            amount, unit, peak = work[kind]
+            avg_s = ms / n * 1e-3
+            ach = amount / avg_s / (1e12 if unit == "TFLOP/s" else 1e9)
+            kernels[kind] = {"launches": n, "avg_us": avg_s * 1e6, "achieved": ach, "unit": unit, "peak": peak,
+                             "frac": ach / peak, "algorithmic_per_launch": amount}
+    threads = min(16, os.cpu_count() or 1) if not args.cpu_threads else args.cpu_threads
+    print(f"# gpu {wall / args.steps * 1e3:.4f} ms/frame; cpu port on {threads} threads", file=sys.stderr, flush=True)
+    cpu_value, want, dt = cpu_baseline(noisy, params, threads)
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    res = {"metric": "256x256 one-stage BM3D Mpix/s (block 8, step 3, group 16, range 16)",
+           "value": round(w * h * args.steps / wall / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "one-stage BM3D, sigma 5, block 8, step 3, group 16, range 16, one 256x256 GRAYS "
+                                  "frame per step"},
+           "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+           "roofline": {"kernel": dom, "bound": "hbm" if kernels[dom]["unit"] == "GB/s" else "mfma",
+                        "achieved": kernels[dom]["achieved"], "peak": kernels[dom]["peak"],
+                        "unit": kernels[dom]["unit"], "frac": kernels[dom]["frac"], "traffic": None},
+           "kernels": kernels,
+           "psnr_vs_cpu": round(oracle.psnr(got, want[0]), 3),
+           "cpu_baseline": {"value": round(cpu_value, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                            "sample": f"{threads} copies of the same 256x256 GRAYS frame, one per thread "
+                                      f"({dt:.1f} s wall)"}}
+    print(json.dumps(res), flush=True)
+
+
 def serving_workload(args):
     """Three clip streams with Poisson arrivals at 24 fps each (720x480
     YUV420PS radius 1 sigma 5; 1920x1080 YUV420PS and 3840x2160 RGBS radius 0
@@ -417,7 +494,8 @@ def main():
                     help="first run one elementwise copy that reads and writes MIB MiB: the known-size kernel "
                          "that scales rocprof's FETCH_SIZE / WRITE_SIZE readings")
     ap.add_argument("--seconds", type=float, default=5.0, help="serving: seconds of arrivals")
-    ap.add_argument("--workload", choices=["two-stage-1080p", "temporal-1080p", "precision-4k", "serving"],
+    ap.add_argument("--workload", choices=["two-stage-1080p", "temporal-1080p", "precision-4k", "serving",
+                                           "single-stage-256"],
                     default="two-stage-1080p", help="two-stage-1080p is the headline; the others print their "
                                                     "own line")
     args = ap.parse_args()
@@ -427,7 +505,7 @@ def main():
         sys.exit(2)
 
     defaults = {"two-stage-1080p": (200, 500), "temporal-1080p": (2, 1), "precision-4k": (20, 50),
-                "serving": (0, 0)}[args.workload]
+                "serving": (0, 0), "single-stage-256": (500, 500)}[args.workload]
     if args.steps is None:
         args.steps = defaults[0]
     if args.warmup is None:
@@ -445,6 +523,8 @@ def main():
         return precision_workload(args)
     if args.workload == "serving":
         return serving_workload(args)
+    if args.workload == "single-stage-256":
+        return single_stage_256_workload(args)
     if args.copy_calibration:
         n = args.copy_calibration * 1024 * 1024 // 4
         a = torch.ones(n, device="cuda")
