@@ -100,6 +100,37 @@ __device__ inline unsigned long long half_min_u64(unsigned long long v) {
     return (static_cast<unsigned long long>(upper ? hi1 : hi0) << 32) | (upper ? lo1 : lo0);
 }
 
+// min(v, v of the lane DPP control `ctrl` selects); lanes whose source is
+// outside the row (or rows outside row_mask) see the identity ~0, so the
+// move folds into the min as one DPP instruction.
+template <int CTRL, int ROW_MASK>
+__device__ inline uint32_t dpp_min_u32(uint32_t v) {
+    const int o = __builtin_amdgcn_update_dpp(-1, static_cast<int>(v), CTRL, ROW_MASK, 0xf, false);
+    return min(v, static_cast<uint32_t>(o));
+}
+
+// Half-wave minimum of 32-bit values (the DPP steps of half_min_u64).
+__device__ inline uint32_t half_min_u32(uint32_t v) {
+    v = dpp_min_u32<0x111, 0xf>(v);
+    v = dpp_min_u32<0x112, 0xf>(v);
+    v = dpp_min_u32<0x114, 0xf>(v);
+    v = dpp_min_u32<0x118, 0xf>(v);
+    v = dpp_min_u32<0x142, 0xa>(v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 31));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+    return (threadIdx.x & 32) ? hi : lo;
+}
+
+// Half-wave minimum of 64-bit keys (dist << 32 | index) in two 32-bit
+// passes: the smallest dist, then the smallest index among the lanes holding
+// it. Equal to half_min_u64's result; all 64 lanes must call.
+__device__ inline unsigned long long half_min_key(unsigned long long v) {
+    const uint32_t d = static_cast<uint32_t>(v >> 32);
+    const uint32_t md = half_min_u32(d);
+    const uint32_t mc = half_min_u32(d == md ? static_cast<uint32_t>(v) : 0xffffffffu);
+    return (static_cast<unsigned long long>(md) << 32) | mc;
+}
+
 // Minimum over the whole wave, returned to every lane (same DPP steps as
 // half_min_u64, then the smaller of lanes 31 and 63). All 64 lanes must call.
 __device__ inline unsigned long long wave_min_u64(unsigned long long v) {
@@ -284,14 +315,20 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     } else {
         best4[0] = best4[1] = best4[2] = best4[3] = ~0ull;
     }
+    // Round k's winner is kept by lane k of the half (k < 32) and the
+    // records are written together after the rounds.
     int given = 0;
+    unsigned long long won = ~0ull;
     for (int k = 0; k < kmax; ++k) {
-        const unsigned long long best = half_min_u64(k < K ? best4[0] : ~0ull);
+        const unsigned long long best = half_min_key(k < K ? best4[0] : ~0ull);
         const int c = static_cast<int>(static_cast<uint32_t>(best));
-        if (k < K && hl == (c & 31)) {
-            dk[c] = kTaken;
+        if (hl == k) won = best;
+        if (k < K && k >= 32 && hl == 0) {
             const int dy = div_small(c, inv_ww), dx = c - dy * ww;
             out[1 + k] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(best >> 32)};
+        }
+        if (k < K && hl == (c & 31)) {
+            dk[c] = kTaken;
             best4[0] = best4[1];
             best4[1] = best4[2];
             best4[2] = best4[3];
@@ -301,6 +338,11 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
                 rebuild();
             }
         }
+    }
+    if (hl < K && hl < 32) {
+        const int c = static_cast<int>(static_cast<uint32_t>(won));
+        const int dy = div_small(c, inv_ww), dx = c - dy * ww;
+        out[1 + hl] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(won >> 32)};
     }
 }
 
