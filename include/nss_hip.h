@@ -224,6 +224,15 @@ int nss_hip_bm3d_destroy(nss_hip_bm3d* handle);
 int nss_hip_spatial_match(const float* plane, int width, int height, int stride, int block, int step, int bm_range,
                           int group, int* out_rec, int* out_count, void* stream);
 
+/* Matcher choice for this process (instances created afterwards, and
+ * nss_hip_spatial_match): 0 automatic, 1 the FP32 matchers only, 2 the MFMA
+ * matcher wherever it applies (block 8, group <= 16). Every choice returns
+ * the same records; 1 and 2 exist to compare them. The reference picks its
+ * tensor-core matcher by window side (src/cuda/common/match.cu:1609-1615
+ * spatial_match_tensor) and exposes the forced form to its tests
+ * (:1690 spatial_match_tensor_only). */
+int nss_hip_set_match_path(int path);
+
 /* ---- Serving loop ------------------------------------------------------
  * No counterpart in the reference: its filters answer one VapourSynth frame
  * request at a time (src/host/filter_bm3d.cpp bm3dGetFrame). This loop is

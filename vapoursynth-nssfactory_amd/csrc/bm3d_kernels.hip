@@ -31,6 +31,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "bm3d_jobs.hpp"
 
@@ -420,6 +421,493 @@ __global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
         __syncthreads();
     }
     if (lane == 0) jb.counts[ref] = 1 + K;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA matcher (B = 8, G <= 16): the role of the reference's tensor-core
+// matcher (src/cuda/common/match.cu:1165-1620, spatial_match_tensor), built
+// for CDNA4. A workgroup of four waves takes a block of 4 x 4 references.
+// The union of their windows is staged in LDS as fp16 values
+// h = fp16(v - mu). Dot products of reference and candidate patches run on
+// v_mfma_f32_16x16x32_f16 (16 references x 16 candidates of one row per
+// tile), giving d' = |h_r|^2 + |h_c|^2 - 2 h_r.h_c for every pair; wave w
+// takes the candidate rows w mod 4. Two passes over the tiles:
+//   A. per reference, the four smallest d' of each lane's candidate column
+//      (16 lanes hold disjoint columns): the K-th smallest of those 64
+//      distinct candidates bounds the K-th exact key from above;
+//   B. every candidate whose d' does not rule it out against that bound is
+//      listed (at most kMmList per reference).
+// The listed candidates are then scored in the FP32 order of the other
+// matchers (rows, then columns, one fmaf per pixel) and ranked by
+// (dist, y, x): the records are those of match_run_kernel bit for bit. A
+// reference whose list overflows, or a block holding a value fp16 cannot
+// carry (non-finite, or |v - mu| >= 32768), scores its whole window exactly.
+//
+// Error bound (all quantities per pair, u = 2^-24):
+//   |v - mu - h| <= e per element, e measured while staging (the fp16
+//     rounding, values below 2^-14 flushed to 0, plus the FP32 rounding of
+//     v - mu); over a patch ||.||_2 <= 8 e_max, so by the triangle inequality
+//     |sqrt(d*) - sqrt(d'')| <= E = 16 e_max (d* the real SSD, d'' the exact
+//     SSD of the fp16 patches);
+//   |d' - d''| <= 200 u (|h_r|^2 + |h_c|^2) (squares exact, 64-term FP32 sums
+//     and the MFMA's FP32 accumulation), taken as kMmC1 = 512 u;
+//   the FP32 key differs from d* by at most 66 u d* (taken as 2e-4).
+// ---------------------------------------------------------------------------
+
+typedef _Float16 mm_h16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 mm_h16x2 __attribute__((ext_vector_type(2)));
+typedef float mm_f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t mm_u32x4 __attribute__((ext_vector_type(4)));
+
+inline constexpr int kMmList = 128;
+inline constexpr float kMmC1 = 3.0517578125e-05f;    // 512 * 2^-24
+inline constexpr float kMmTiny = 6.103515625e-05f;   // 2^-14, fp16's smallest normal
+inline constexpr float kMmLimit = 32768.f;
+inline constexpr float kMmInf = __builtin_inff();
+
+// fp16 bits of a (0 below 2^-14); err = max(err, bound of |a - h| plus the
+// rounding of a = v - mu); bad when a cannot be carried.
+__device__ inline uint32_t mm_half(float a, float& err, bool& bad) {
+    const float m = fabsf(a);
+    bad |= !(m < kMmLimit);
+    const _Float16 h = m < kMmTiny ? static_cast<_Float16>(0.f) : static_cast<_Float16>(a);
+    err = fmaxf(err, fabsf(static_cast<float>(h) - a) + 1.2e-7f * m);
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+// Eight fp16 values at region row `row`, columns col .. col + 7 (two per
+// word; odd columns realigned with a funnel shift).
+__device__ inline mm_h16x8 mm_frag(const uint32_t* reg, int pitch, int row, int col) {
+    const uint32_t* p = reg + row * pitch + (col >> 1);
+    const uint32_t w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
+    const uint32_t sh = static_cast<uint32_t>(col & 1) * 16u;
+    const mm_u32x4 o = {__builtin_amdgcn_alignbit(w1, w0, sh), __builtin_amdgcn_alignbit(w2, w1, sh),
+                        __builtin_amdgcn_alignbit(w3, w2, sh), __builtin_amdgcn_alignbit(w4, w3, sh)};
+    return __builtin_bit_cast(mm_h16x8, o);
+}
+
+__device__ inline float mm_sq(mm_h16x8 f) {
+    float s = __builtin_amdgcn_fdot2(mm_h16x2{f[0], f[1]}, mm_h16x2{f[0], f[1]}, 0.f, false);
+    s = __builtin_amdgcn_fdot2(mm_h16x2{f[2], f[3]}, mm_h16x2{f[2], f[3]}, s, false);
+    s = __builtin_amdgcn_fdot2(mm_h16x2{f[4], f[5]}, mm_h16x2{f[4], f[5]}, s, false);
+    return __builtin_amdgcn_fdot2(mm_h16x2{f[6], f[7]}, mm_h16x2{f[6], f[7]}, s, false);
+}
+
+// Sum over lanes n, n + 16, n + 32, n + 48 (the four 16-lane rows), equal in
+// all four: rows swapped pairwise, then halves (additions commute).
+__device__ inline float mm_rows_sum(float v) {
+    const uint32_t b = __float_as_uint(v);
+    const auto p = __builtin_amdgcn_permlane16_swap(b, b, false, false);
+    const float s = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    const uint32_t c = __float_as_uint(s);
+    const auto q = __builtin_amdgcn_permlane32_swap(c, c, false, false);
+    return __uint_as_float(q[0]) + __uint_as_float(q[1]);
+}
+
+__device__ inline float wave_max_f32(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// The FP32 key of the other matchers for B = 8: rows, then columns, one fmaf
+// per pixel.
+__device__ inline uint32_t mm_exact(const float* plane, size_t stride, int rx, int ry, int x, int y) {
+    const float* a = plane + static_cast<size_t>(ry) * stride + rx;
+    const float* b = plane + static_cast<size_t>(y) * stride + x;
+    float acc = 0.f;
+    for (int py = 0; py < 8; ++py) {
+#pragma unroll
+        for (int px = 0; px < 8; ++px) {
+            const float d = a[py * stride + px] - b[py * stride + px];
+            acc = fmaf(d, d, acc);
+        }
+    }
+    return dist_key(acc);
+}
+
+// Region row pitch in words: >= 8 nt + 5 (16 nt + 8 halves and the fifth word
+// of an odd-column read), = 16 mod 32 so that the four 16-lane rows of a
+// fragment read fall on disjoint banks.
+__host__ __device__ inline int mm_pitch(int nt) {
+    const int p = 8 * nt + 5;
+    return p + ((16 - p % 32) + 32) % 32;
+}
+
+struct MmWin {
+    int cx, cy, left, right, top, bottom;
+};
+
+__device__ inline MmWin mm_win(const MatchJob& jb, int i, int j) {
+    MmWin w;
+    w.cx = ref_x(jb.grid, i, jb.width, 8);
+    w.cy = ref_y(jb.grid, j, jb.height, 8);
+    w.left = max(w.cx - jb.range, 0);
+    w.right = min(w.cx + jb.range, jb.width - 8);
+    w.top = max(w.cy - jb.range, 0);
+    w.bottom = min(w.cy + jb.range, jb.height - 8);
+    return w;
+}
+
+// Minimum over the 16 lanes of each DPP row (rotations by 8, 4, 2, 1),
+// returned to every lane of the row.
+__device__ inline float row_min_f32(float v) {
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false)));
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false)));
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x122, 0xf, 0xf, false)));
+    return fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xf, 0xf, false)));
+}
+
+// d into the ascending four s (the largest of the five dropped).
+__device__ inline void top4_insert(float (&s)[4], float d) {
+    const float a = s[0], b = s[1], c = s[2], e = s[3];
+    s[0] = fminf(a, d);
+    s[1] = __builtin_amdgcn_fmed3f(a, b, d);
+    s[2] = __builtin_amdgcn_fmed3f(b, c, d);
+    s[3] = __builtin_amdgcn_fmed3f(c, e, d);
+}
+
+// bf16 bits of v rounded towards +inf (an upper bound survives the exchange).
+__device__ inline uint16_t bf16_up(float v) {
+    const uint32_t b = __float_as_uint(v);
+    const uint32_t hi = b >> 16;
+    return static_cast<uint16_t>(((b & 0xffffu) != 0u && (b >> 31) == 0u) ? hi + 1u : hi);
+}
+
+__global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
+    extern __shared__ __align__(16) uint32_t mm_lds[];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    const int lb = xcd_block();
+    const MatchJob& jb = tab.job[find_job(tab, lb)];
+    const int blk = lb - jb.wg_begin;
+    const int nbx = (jb.grid.nx + 3) >> 2;
+    const int by = blk / nbx, bx = blk - by * nbx;
+    const int i0 = 4 * bx, j0 = 4 * by;
+    const int ncol = min(4, jb.grid.nx - i0), nrow = min(4, jb.grid.ny - j0);
+    const float* plane = jb.plane;
+    const size_t stride = static_cast<size_t>(jb.stride);
+    const int G = jb.group;
+
+    const MmWin w00 = mm_win(jb, i0, j0), wlast = mm_win(jb, i0 + ncol - 1, j0 + nrow - 1);
+    const int ux0 = w00.left, ux1 = wlast.right, uy0 = w00.top, uy1 = wlast.bottom;
+    const int nt = (ux1 - ux0 + 16) >> 4;
+    const int UW = 16 * nt;
+    const int RH = uy1 - uy0 + 8;
+    const int pitch = mm_pitch(nt);
+    const int reg_words = max(RH * pitch, 16 * kMmList);
+    uint32_t* reg = mm_lds;
+    uint16_t* list = reinterpret_cast<uint16_t*>(mm_lds + reg_words);
+    int* cnt = reinterpret_cast<int*>(mm_lds + reg_words + 8 * kMmList);  // 16
+    int* pre = cnt + 16;                                                // 17 (+1 keeps 8-byte alignment)
+    int* kmv = pre + 18;                                                // 16
+    float* wl = reinterpret_cast<float*>(kmv + 16);                     // 16
+    uint32_t* xw = reinterpret_cast<uint32_t*>(wl + 16);                // 2048 words
+    uint16_t* xch = reinterpret_cast<uint16_t*>(xw);                    // [wave][j][lane][4] bf16
+
+    // Per-reference counts K = min(G - 1, window - 1); counters cleared.
+    if (tid < 16) {
+        const int c = tid >> 2, j = tid & 3;
+        int k = 0;
+        if (c < ncol && j < nrow) {
+            const MmWin w = mm_win(jb, i0 + c, j0 + j);
+            k = min(G - 1, (w.right - w.left + 1) * (w.bottom - w.top + 1) - 1);
+        }
+        kmv[tid] = k;
+        cnt[tid] = 0;
+    }
+
+    // mu: the mean of the first reference's patch (every wave alike); then
+    // the region as fp16.
+    const float mu =
+        wave_sum_f32(plane[static_cast<size_t>(w00.cy + (lane >> 3)) * stride + w00.cx + (lane & 7)]) *
+        (1.f / 64.f);
+    float err = 0.f;
+    bool bad = !(fabsf(mu) < kMmLimit);
+    {
+        const float inv_pitch = 1.f / static_cast<float>(pitch);
+        const int words = RH * pitch;
+#pragma unroll 4
+        for (int wd = tid; wd < words; wd += 256) {
+            const int yy = div_small(wd, inv_pitch), xw2 = wd - yy * pitch;
+            const float* row = plane + static_cast<size_t>(uy0 + yy) * stride;
+            const int x = ux0 + 2 * xw2;
+            const float a0 = x < jb.width ? row[x] - mu : 0.f;
+            const float a1 = x + 1 < jb.width ? row[x + 1] - mu : 0.f;
+            const uint32_t lo = mm_half(a0, err, bad), hi = mm_half(a1, err, bad);
+            reg[wd] = lo | (hi << 16);
+        }
+    }
+    {
+        const float we = wave_max_f32(err);
+        const bool wb = __builtin_amdgcn_ballot_w64(bad) != 0;
+        if (lane == 0) {
+            reinterpret_cast<float*>(xw)[wave] = we;
+            xw[4 + wave] = wb ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    const float E = 16.f *
+                    fmaxf(fmaxf(reinterpret_cast<float*>(xw)[0], reinterpret_cast<float*>(xw)[1]),
+                          fmaxf(reinterpret_cast<float*>(xw)[2], reinterpret_cast<float*>(xw)[3])) *
+                    1.00001f;
+    const bool anybad = (xw[4] | xw[5] | xw[6] | xw[7]) != 0u;
+    __syncthreads();
+
+    if (!anybad) {
+        // A operand: reference n (column n >> 2, row n & 3), patch rows g
+        // and g + 4.
+        const int acol = n >> 2, arow = n & 3;
+        mm_h16x8 A0 = {}, A1 = {};
+        if (acol < ncol && arow < nrow) {
+            const MmWin w = mm_win(jb, i0 + acol, j0 + arow);
+            A0 = mm_frag(reg, pitch, w.cy - uy0 + g, w.cx - ux0);
+            A1 = mm_frag(reg, pitch, w.cy - uy0 + 4 + g, w.cx - ux0);
+        }
+        const float nrA = mm_rows_sum(mm_sq(A0) + mm_sq(A1));  // |h_r|^2 of reference n
+        // C layout: lane (g, n) holds references 4 g + j (column g, row j)
+        // against candidate n of the tile.
+        float nr[4], nrk[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            nr[j] = __shfl(nrA, 4 * g + j, 64);
+            nrk[j] = nr[j] * (1.f - kMmC1);
+        }
+        const bool cval = g < ncol;
+        const MmWin cw = mm_win(jb, i0 + min(g, ncol - 1), j0);
+        const int ccx = cw.cx, cleft = cval ? cw.left : (1 << 30), cright = cval ? cw.right : -1;
+        int rtop[4], rbot[4], rcy[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const MmWin w = mm_win(jb, i0, j0 + min(j, nrow - 1));
+            rcy[j] = w.cy;
+            rtop[j] = j < nrow ? w.top : (1 << 30);
+            rbot[j] = j < nrow ? w.bottom : -1;
+        }
+
+        // Each wave takes the candidate rows y = uy0 + wave (mod 4).
+        float s[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s[j][k] = kMmInf;
+        float wv[4] = {0.f, 0.f, 0.f, 0.f};
+        auto pass = [&](auto surv_tag) {
+            constexpr bool SURV = decltype(surv_tag)::value;
+            for (int t = 0; t < nt; ++t) {
+                const int col = 16 * t + n;
+                const int x = ux0 + col;
+                const bool xv = x >= cleft && x <= cright;
+                const bool xs = x == ccx;
+                int y = uy0 + wave;
+                if (y > uy1) break;
+                mm_h16x8 F0 = mm_frag(reg, pitch, y - uy0 + g, col);
+                float T0 = mm_rows_sum(mm_sq(F0));
+                for (; y <= uy1; y += 4) {
+                    const mm_h16x8 F1 = mm_frag(reg, pitch, y - uy0 + 4 + g, col);
+                    const float T1 = mm_rows_sum(mm_sq(F1));
+                    mm_f32x4 c =
+                        __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, F0, mm_f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, F1, c, 0, 0, 0);
+                    const float ncx = xv ? T0 + T1 : kMmInf;
+                    bool sv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool use = y >= rtop[j] && y <= rbot[j] && !(y == rcy[j] && xs);
+                        const float base = use ? (SURV ? fmaf(ncx, 1.f - kMmC1, nrk[j]) : ncx + nr[j]) : kMmInf;
+                        const float d = fmaf(-2.f, c[j], base);
+                        if constexpr (SURV)
+                            sv[j] = d <= wv[j];
+                        else
+                            top4_insert(s[j], d);
+                    }
+                    if constexpr (SURV) {
+                        if (__builtin_amdgcn_ballot_w64(sv[0] || sv[1] || sv[2] || sv[3]) != 0) {
+                            const int u = (y - uy0) * UW + col;
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                if (sv[j]) {
+                                    const int m = 4 * g + j;
+                                    const int k = atomicAdd(&cnt[m], 1);
+                                    if (k < kMmList) list[m * kMmList + k] = static_cast<uint16_t>(u);
+                                }
+                            }
+                        }
+                    }
+                    F0 = F1;
+                    T0 = T1;
+                }
+            }
+        };
+
+        pass(std::false_type{});
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) xch[((wave * 4 + j) * 64 + lane) * 4 + k] = bf16_up(s[j][k]);
+        __syncthreads();
+        // Wave w: references of row j = w. Each lane merges the four waves'
+        // four smallest of its candidate column; the K-th smallest over the
+        // row's 16 lanes (64 distinct candidates) by rounds of a row minimum.
+        {
+            float m4[4] = {kMmInf, kMmInf, kMmInf, kMmInf};
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    top4_insert(m4, __uint_as_float(static_cast<uint32_t>(xch[((v * 4 + wave) * 64 + lane) * 4 + k])
+                                                    << 16));
+            const int K = kmv[4 * g + wave];
+            int taken = 0;
+            bool found = K == 0;
+            float td = kMmInf;
+            for (int round = 0; round < 16; ++round) {
+                if (__builtin_amdgcn_ballot_w64(!found) == 0) break;
+                const float mn = row_min_f32(m4[0]);
+                const bool eq = m4[0] == mn;
+                const unsigned long long bal = __builtin_amdgcn_ballot_w64(eq);
+                taken += __popcll((bal >> (16 * g)) & 0xffffull);
+                if (eq) {
+                    m4[0] = m4[1];
+                    m4[1] = m4[2];
+                    m4[2] = m4[3];
+                    m4[3] = kMmInf;
+                }
+                if (!found && taken >= K) {
+                    found = true;
+                    td = mn;
+                }
+            }
+            if (n == 0) {
+                const float nrm = wave == 0 ? nr[0] : wave == 1 ? nr[1] : wave == 2 ? nr[2] : nr[3];
+                float wt = -kMmInf;
+                if (K > 0) {
+                    if (!(td < kMmInf)) {
+                        wt = 3.402823466e38f;  // every finite d' (masked pairs are +inf)
+                    } else {
+                        const float d1 = kMmC1 * 1.0002f * (3.f * nrm + 2.f * fmaxf(td, 0.f));
+                        const float ru = sqrtf(fmaxf(td + d1, 0.f)) + E;
+                        const float U = ru * ru * 1.0001f;
+                        const float R = sqrtf(U * 1.0002f) + E;
+                        wt = R * R * 1.0001f;
+                    }
+                }
+                wl[4 * g + wave] = wt;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wv[j] = wl[4 * g + j];
+        pass(std::true_type{});
+    }
+    __syncthreads();
+
+    // Listed candidates: exact keys (into the region's words), then ranks.
+    if (tid == 0) {
+        int acc = 0;
+        for (int m = 0; m < 16; ++m) {
+            pre[m] = acc;
+            const int c = cnt[m];
+            acc += (!anybad && kmv[m] > 0 && c <= kMmList) ? c : 0;
+        }
+        pre[16] = acc;
+    }
+    __syncthreads();
+    const int total = pre[16];
+    const float inv_uw = 1.f / static_cast<float>(UW);
+    uint32_t* dk = reg;
+    auto item = [&](int q, int& m, int& sl) {
+        m = 0;
+#pragma unroll
+        for (int b = 8; b >= 1; b >>= 1)
+            if (m + b < 16 && q >= pre[m + b]) m += b;
+        sl = q - pre[m];
+    };
+    for (int q = tid; q < total; q += 256) {
+        int m, sl;
+        item(q, m, sl);
+        const int u = list[m * kMmList + sl];
+        const int uy = div_small(u, inv_uw), ux = u - uy * UW;
+        const MmWin w = mm_win(jb, i0 + (m >> 2), j0 + (m & 3));
+        dk[q] = mm_exact(plane, stride, w.cx, w.cy, ux0 + ux, uy0 + uy);
+    }
+    __syncthreads();
+    for (int q = tid; q < total; q += 256) {
+        int m, sl;
+        item(q, m, sl);
+        const int c = pre[m + 1] - pre[m];
+        const uint32_t d = dk[q];
+        const int u = list[m * kMmList + sl];
+        int rank = 0;
+        for (int s2 = 0; s2 < c; ++s2) {
+            const uint32_t d2 = dk[pre[m] + s2];
+            const int u2 = list[m * kMmList + s2];
+            rank += (d2 < d || (d2 == d && u2 < u)) ? 1 : 0;
+        }
+        if (rank < kmv[m]) {
+            const int uy = div_small(u, inv_uw), ux = u - uy * UW;
+            const int ref_id = (j0 + (m & 3)) * jb.grid.nx + i0 + (m >> 2);
+            jb.out[static_cast<size_t>(ref_id) * G + 1 + rank] = MatchRec{ux0 + ux, uy0 + uy, jb.t0, d};
+        }
+    }
+    if (tid < 16 && (tid >> 2) < ncol && (tid & 3) < nrow) {
+        const MmWin w = mm_win(jb, i0 + (tid >> 2), j0 + (tid & 3));
+        const int ref_id = (j0 + (tid & 3)) * jb.grid.nx + i0 + (tid >> 2);
+        jb.out[static_cast<size_t>(ref_id) * G] = MatchRec{w.cx, w.cy, jb.t0, 0u};
+        jb.counts[ref_id] = 1 + kmv[tid];
+    }
+
+    // Whole windows, exactly, for the references the bound could not serve
+    // (wave w takes references w, w + 4, ...): chunks of 64 candidates merged
+    // by rank into the running G - 1 best.
+    unsigned long long* ck = reinterpret_cast<unsigned long long*>(xw) + wave * 96;  // 64
+    unsigned long long* bk = ck + 64;                                                 // 16
+    unsigned long long* nbk = bk + 16;                                                // 16
+    for (int m = wave; m < 16; m += 4) {
+        const int K = kmv[m];
+        if (K == 0 || !(anybad || cnt[m] > kMmList)) continue;
+        const MmWin w = mm_win(jb, i0 + (m >> 2), j0 + (m & 3));
+        const int ww = w.right - w.left + 1, ncand = ww * (w.bottom - w.top + 1);
+        const int self = (w.cy - w.top) * ww + (w.cx - w.left);
+        const float inv_ww = 1.f / static_cast<float>(ww);
+        int nb = 0;
+        for (int base = 0; base < ncand; base += 64) {
+            const int c = base + lane;
+            unsigned long long key = ~0ull;
+            if (c < ncand && c != self) {
+                const int dy = div_small(c, inv_ww), dx = c - dy * ww;
+                key = (static_cast<unsigned long long>(mm_exact(plane, stride, w.cx, w.cy, w.left + dx, w.top + dy))
+                       << 32) |
+                      static_cast<uint32_t>(c);
+            }
+            ck[lane] = key;
+            wave_sync();
+            int rank = 0, brank = 0;
+            unsigned long long bkey = ~0ull;
+            for (int q = 0; q < 64; ++q) rank += ck[q] < key ? 1 : 0;
+            for (int j = 0; j < nb; ++j) rank += bk[j] < key ? 1 : 0;
+            if (lane < nb) {
+                bkey = bk[lane];
+                brank = lane;
+                for (int q = 0; q < 64; ++q) brank += ck[q] < bkey ? 1 : 0;
+            }
+            const int nvalid = __popcll(__builtin_amdgcn_ballot_w64(key != ~0ull));
+            wave_sync();
+            if (key != ~0ull && rank < K) nbk[rank] = key;
+            if (lane < nb && brank < K) nbk[brank] = bkey;
+            nb = min(K, nb + nvalid);
+            wave_sync();
+            if (lane < nb) bk[lane] = nbk[lane];
+            wave_sync();
+        }
+        if (lane < K) {
+            const unsigned long long key = bk[lane];
+            const int c = static_cast<int>(static_cast<uint32_t>(key));
+            const int dy = div_small(c, inv_ww), dx = c - dy * ww;
+            const int ref_id = (j0 + (m & 3)) * jb.grid.nx + i0 + (m >> 2);
+            jb.out[static_cast<size_t>(ref_id) * G + 1 + lane] =
+                MatchRec{w.left + dx, w.top + dy, jb.t0, static_cast<uint32_t>(key >> 32)};
+        }
+        wave_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1202,12 +1690,6 @@ __device__ inline void load_unit(const float* plane, int stride, const uint32_t*
             x[q] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
     }
-}
-
-__device__ inline float wave_max_f32(float v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    return v;
 }
 
 // Power-of-two scale taking max |x| of a group to [0.5, 1); 1 for a zero or
@@ -2197,6 +2679,26 @@ size_t match_wave_lds_bytes(int block, int range, bool stage) {
     return ((reg + 3) & ~static_cast<size_t>(3)) * 4 + cand;
 }
 
+size_t match_mfma_lds_bytes(int step, int range) {
+    // the union of a 4 x 4 block's windows spans at most 3 step + 2 range + 1
+    // candidate columns and rows
+    const int span = 3 * step + 2 * range + 1;
+    const int nt = (span + 15) >> 4;
+    const size_t reg = std::max(static_cast<size_t>(span + 7) * mm_pitch(nt), static_cast<size_t>(16 * kMmList));
+    return (reg + 8 * kMmList + 66 + 2048) * 4;
+}
+
+bool match_mfma_fits(int block, int group, int step, int range, int width, int height) {
+    return block == 8 && group >= 1 && group <= 16 && width >= 8 && height >= 8 &&
+           match_mfma_lds_bytes(step, range) <= 160 * 1024;
+}
+
+int match_workgroups(int kind, const Grid& g) {
+    if (kind == 3) return ((g.nx + 3) >> 2) * ((g.ny + 3) >> 2);
+    if (kind == 2) return g.ny * ((g.nx + kRunRefs - 1) / kRunRefs);
+    return g.nx * g.ny;
+}
+
 size_t match_run_lds_bytes(int block, int step, int range) {
     const size_t w = 2 * static_cast<size_t>(range) + 1;
     // windows' span + 6 padding columns, rounded up to a pitch of 2 mod 4;
@@ -2247,7 +2749,14 @@ static void allow_lds(K kernel, size_t lds) {
 
 template <int B>
 static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
-    if (kind == 2) {
+    if (kind == 3) {
+        if constexpr (B == 8) {
+            allow_lds(&match_mfma_kernel, lds);
+            match_mfma_kernel<<<total_wg, 256, lds, s>>>(t);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    } else if (kind == 2) {
         if constexpr (B <= 12) {
             allow_lds(&match_run_kernel<B>, lds);
             match_run_kernel<B><<<total_wg, 256, lds, s>>>(t);
@@ -2264,7 +2773,8 @@ static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, 
     return hipGetLastError();
 }
 
-// kind: 2 run matcher, 1 wave matcher with the window in LDS, 0 from global.
+// kind: 3 MFMA matcher (B = 8), 2 run matcher, 1 wave matcher with the
+// window in LDS, 0 from global.
 hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
     switch (block) {
         case 1: return launch_match_b<1>(t, total_wg, lds, kind, s);

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -74,6 +75,10 @@ TileGeom make_tiles(const Grid& g, int tw, int th, int block, int range) {
     return t;
 }
 
+std::atomic<int> g_match_path{0};
+
+int match_path() { return g_match_path.load(std::memory_order_relaxed); }
+
 bool allowed_block(int b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 12 || b == 16 || b == 32; }
 bool allowed_group(int g) { return g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64; }
 
@@ -125,13 +130,21 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
         for (int i = 0; i < h->nplanes; ++i)
             if (st.plane[i].sigma != 0.f && st.plane[i].block == b) bk.planes[bk.nplanes++] = i;
         if (!bk.nplanes) continue;
-        size_t run_lds = 0, wave_lds = 0;
+        size_t run_lds = 0, wave_lds = 0, mfma_lds = 0;
+        bool mfma = match_path() != 1;
         for (int k = 0; k < bk.nplanes; ++k) {
             const StagePlane& sp = st.plane[bk.planes[k]];
+            const int i = bk.planes[k];
             run_lds = std::max(run_lds, match_run_lds_bytes(b, sp.step, sp.range));
             wave_lds = std::max(wave_lds, match_wave_lds_bytes(b, sp.range, true));
+            mfma_lds = std::max(mfma_lds, match_mfma_lds_bytes(sp.step, sp.range));
+            mfma &= match_mfma_fits(b, sp.group, sp.step, sp.range, h->pw[i], h->ph[i]);
         }
-        if (b <= 12 && run_lds <= kRunLdsBudget) {
+        const bool run = b <= 12 && run_lds <= kRunLdsBudget;
+        if (mfma && (match_path() == 2 || !run)) {
+            bk.match_kind = 3;
+            bk.match_lds = mfma_lds;
+        } else if (run) {
             bk.match_kind = 2;
             bk.match_lds = run_lds;
         } else if (wave_lds <= kLdsLimit) {
@@ -441,7 +454,7 @@ void add_bucket_jobs(LaunchGroup& g, const FrameRef& f, const Stage& st, const B
         m.wg_begin = g.mwg;
         m.out = f.buf.matches[i];
         m.counts = f.buf.counts[i];
-        g.mwg += bk.match_kind == 2 ? gr.ny * ((gr.nx + kRunRefs - 1) / kRunRefs) : gr.nx * gr.ny;
+        g.mwg += match_workgroups(bk.match_kind, gr);
     }
     for (int k = 0; k < bk.nplanes; ++k) {
         const int i = bk.planes[k];
@@ -704,7 +717,7 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             m.wg_begin = mwg;
             m.out = h->buf.matches[i];
             m.counts = h->buf.counts[i];
-            mwg += bk.match_kind == 2 ? g.ny * ((g.nx + kRunRefs - 1) / kRunRefs) : g.nx * g.ny;
+            mwg += match_workgroups(bk.match_kind, g);
             // group 1 keeps the reference alone (src/cpu/bm/matcher.hpp:323).
             if (sp.group > 1) {
                 PredJob& p = pt.job[pt.n++];
@@ -1424,6 +1437,12 @@ int nss_hip_bm3d_destroy(nss_hip_bm3d* h) {
     return NSS_HIP_OK;
 }
 
+int nss_hip_set_match_path(int path) {
+    if (path < 0 || path > 2) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: match path must be 0, 1 or 2");
+    g_match_path.store(path, std::memory_order_relaxed);
+    return NSS_HIP_OK;
+}
+
 int nss_hip_spatial_match(const float* plane, int width, int height, int stride, int block, int step, int bm_range,
                           int group, int* out_rec, int* out_count, void* stream) {
     if (!plane || !out_rec || !out_count) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
@@ -1447,20 +1466,22 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
     if (nref == 0) return NSS_HIP_OK;
     int kind;
     size_t lds;
-    int wg;
-    if (block <= 12 && match_run_lds_bytes(block, step, bm_range) <= kRunLdsBudget) {
+    const bool run = block <= 12 && match_run_lds_bytes(block, step, bm_range) <= kRunLdsBudget;
+    if (match_path() != 1 && match_mfma_fits(block, group, step, bm_range, width, height) &&
+        (match_path() == 2 || !run)) {
+        kind = 3;
+        lds = match_mfma_lds_bytes(step, bm_range);
+    } else if (run) {
         kind = 2;
         lds = match_run_lds_bytes(block, step, bm_range);
-        wg = m.grid.ny * ((m.grid.nx + kRunRefs - 1) / kRunRefs);
     } else if (match_wave_lds_bytes(block, bm_range, true) <= kLdsLimit) {
         kind = 1;
         lds = match_wave_lds_bytes(block, bm_range, true);
-        wg = nref;
     } else {
         kind = 0;
         lds = match_wave_lds_bytes(block, bm_range, false);
-        wg = nref;
     }
+    const int wg = match_workgroups(kind, m.grid);
     const hipError_t e = launch_match(block, mt, wg, lds, kind, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "match launch");
     return NSS_HIP_OK;

@@ -22,6 +22,14 @@ hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s);
 hipError_t upload_dct_tables(const float* host, size_t count);
 size_t match_wave_lds_bytes(int block, int range, bool stage);
 size_t match_run_lds_bytes(int block, int step, int range);
+size_t match_mfma_lds_bytes(int step, int range);
+// The MFMA matcher serves B = 8, G <= 16 when its LDS fits.
+bool match_mfma_fits(int block, int group, int step, int range, int width, int height);
+// Workgroups of a match launch of `kind` over one plane's grid.
+int match_workgroups(int kind, const Grid& g);
+// Process-wide matcher choice (nss_hip_set_match_path): 0 automatic, 1 the
+// FP32 matchers only, 2 the MFMA matcher wherever it fits.
+int match_path();
 size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);
 size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t);

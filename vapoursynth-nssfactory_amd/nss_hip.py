@@ -95,8 +95,8 @@ _lib = None
 
 EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "nss_hip_bm3d_output_info",
            "nss_hip_bm3d_create", "nss_hip_bm3d_slots", "nss_hip_bm3d_process", "nss_hip_bm3d_destroy",
-           "nss_hip_spatial_match", "nss_hip_bm3d_set_tracing", "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push",
-           "nss_hip_bm3d_pop", "nss_hip_server_create", "nss_hip_server_open_stream", "nss_hip_server_submit",
+           "nss_hip_spatial_match", "nss_hip_set_match_path", "nss_hip_bm3d_set_tracing",
+           "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push", "nss_hip_bm3d_pop", "nss_hip_server_create", "nss_hip_server_open_stream", "nss_hip_server_submit",
            "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy",
            "nss_hip_bm3d_autotune", "nss_hip_vaggregate")
 # Contribution identity of legacy temporal frames (frame properties the
@@ -137,6 +137,7 @@ def lib():
         L.nss_hip_bm3d_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), ci, ci, vp]
         L.nss_hip_bm3d_destroy.argtypes = [vp]
         L.nss_hip_spatial_match.argtypes = [vp, ci, ci, ci, ci, ci, ci, ci, vp, vp, vp]
+        L.nss_hip_set_match_path.argtypes = [ci]
         L.nss_hip_bm3d_set_tracing.argtypes = [vp, ci]
         L.nss_hip_bm3d_kernel_times.argtypes = [vp, P(ci), P(ctypes.c_double), ci]
         L.nss_hip_bm3d_push.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), vp]
@@ -380,6 +381,16 @@ def grid_size(width, height, block, step):
     if width < block or height < block or step < 1:
         return 0, 0
     return (width - block + step - 1) // step + 1, (height - block + step - 1) // step + 1
+
+
+MATCH_AUTO, MATCH_FP32, MATCH_MFMA = 0, 1, 2
+
+
+def set_match_path(path):
+    """Matcher choice for the process: MATCH_AUTO, MATCH_FP32 (the FP32
+    matchers only) or MATCH_MFMA (the MFMA matcher wherever it applies: block
+    8, group <= 16). Every choice returns the same records."""
+    _check(lib().nss_hip_set_match_path(int(path)))
 
 
 def spatial_match(plane, block=8, step=8, bm_range=9, group=8, stream=None):
