@@ -560,11 +560,11 @@ __device__ inline float row_min_f32(float v) {
 
 // d into the ascending four s (the largest of the five dropped).
 __device__ inline void top4_insert(float (&s)[4], float d) {
-    const float a = s[0], b = s[1], c = s[2], e = s[3];
-    s[0] = fminf(a, d);
-    s[1] = __builtin_amdgcn_fmed3f(a, b, d);
-    s[2] = __builtin_amdgcn_fmed3f(b, c, d);
-    s[3] = __builtin_amdgcn_fmed3f(c, e, d);
+    // top down, so that each entry is overwritten after its last use
+    s[3] = __builtin_amdgcn_fmed3f(s[2], s[3], d);
+    s[2] = __builtin_amdgcn_fmed3f(s[1], s[2], d);
+    s[1] = __builtin_amdgcn_fmed3f(s[0], s[1], d);
+    s[0] = fminf(s[0], d);
 }
 
 // bf16 bits of v rounded towards +inf (an upper bound survives the exchange).
@@ -576,7 +576,8 @@ __device__ inline uint16_t bf16_up(float v) {
 
 __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
     extern __shared__ __align__(16) uint32_t mm_lds[];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: row loops branch on scalars
     const int lb = xcd_block();
     const MatchJob& jb = tab.job[find_job(tab, lb)];
     const int blk = lb - jb.wg_begin;
@@ -624,17 +625,32 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
     float err = 0.f;
     bool bad = !(fabsf(mu) < kMmLimit);
     {
+        // eight words per thread in flight: all loads first, then the packing
         const float inv_pitch = 1.f / static_cast<float>(pitch);
         const int words = RH * pitch;
-#pragma unroll 4
-        for (int wd = tid; wd < words; wd += 256) {
-            const int yy = div_small(wd, inv_pitch), xw2 = wd - yy * pitch;
-            const float* row = plane + static_cast<size_t>(uy0 + yy) * stride;
-            const int x = ux0 + 2 * xw2;
-            const float a0 = x < jb.width ? row[x] - mu : 0.f;
-            const float a1 = x + 1 < jb.width ? row[x + 1] - mu : 0.f;
-            const uint32_t lo = mm_half(a0, err, bad), hi = mm_half(a1, err, bad);
-            reg[wd] = lo | (hi << 16);
+        for (int base = 0; base < words; base += 8 * 256) {
+            float v0[8], v1[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int wd = base + tid + 256 * i;
+                v0[i] = mu;
+                v1[i] = mu;
+                if (wd < words) {
+                    const int yy = div_small(wd, inv_pitch), xw2 = wd - yy * pitch;
+                    const float* row = plane + static_cast<size_t>(uy0 + yy) * stride;
+                    const int x = ux0 + 2 * xw2;
+                    if (x < jb.width) v0[i] = row[x];
+                    if (x + 1 < jb.width) v1[i] = row[x + 1];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int wd = base + tid + 256 * i;
+                if (wd < words) {
+                    const uint32_t lo = mm_half(v0[i] - mu, err, bad), hi = mm_half(v1[i] - mu, err, bad);
+                    reg[wd] = lo | (hi << 16);
+                }
+            }
         }
     }
     {
@@ -709,16 +725,19 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                         __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, F0, mm_f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
                     c = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, F1, c, 0, 0, 0);
                     const float ncx = xv ? T0 + T1 : kMmInf;
-                    bool sv[4];
+                    bool sv[4] = {false, false, false, false};
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const bool use = y >= rtop[j] && y <= rbot[j] && !(y == rcy[j] && xs);
-                        const float base = use ? (SURV ? fmaf(ncx, 1.f - kMmC1, nrk[j]) : ncx + nr[j]) : kMmInf;
-                        const float d = fmaf(-2.f, c[j], base);
-                        if constexpr (SURV)
-                            sv[j] = d <= wv[j];
-                        else
-                            top4_insert(s[j], d);
+                        // reference row j's window holds row y: uniform
+                        if (y >= rtop[j] && y <= rbot[j]) {
+                            float base = SURV ? fmaf(ncx, 1.f - kMmC1, nrk[j]) : ncx + nr[j];
+                            if (y == rcy[j]) base = xs ? kMmInf : base;  // the reference itself
+                            const float d = fmaf(-2.f, c[j], base);
+                            if constexpr (SURV)
+                                sv[j] = d <= wv[j];
+                            else
+                                top4_insert(s[j], d);
+                        }
                     }
                     if constexpr (SURV) {
                         if (__builtin_amdgcn_ballot_w64(sv[0] || sv[1] || sv[2] || sv[3]) != 0) {
