@@ -115,6 +115,12 @@ bool apply_tile_shape(const nss_hip_bm3d* h, const Stage& st, Bucket& bk, int tw
     }
     const size_t budget = (tw == 1 && th == 1) ? kLdsLimit : kTileLdsBudget;
     if (lds > budget) return false;
+    // A tile of one reference writes (and the finish reads back) its whole
+    // accumulation rectangle: beyond four group cubes of pixels, the atomic
+    // path's direct sums move less.
+    if (tw == 1 && th == 1)
+        for (int k = 0; k < bk.nplanes; ++k)
+            if (static_cast<size_t>(g[k].aw) * g[k].ah > 4u * st.plane[bk.planes[k]].group * b * b) return false;
     bk.tiles = true;
     bk.filter_lds = lds;
     for (int k = 0; k < bk.nplanes; ++k) bk.geom[k] = g[k];
