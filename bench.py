@@ -73,7 +73,8 @@ def cpu_baseline(noisy, params, threads):
     for t in ts:
         t.join()
     dt = time.perf_counter() - t0
-    return threads * W * H / dt / 1e6, outs[0], dt
+    h0, w0 = noisy[0].shape
+    return threads * w0 * h0 / dt / 1e6, outs[0], dt
 
 
 FORMS = {"fp32": 0, "bf16x3": 1, "fp16": 2}
@@ -328,6 +329,106 @@ def single_stage_256_workload(args):
     print(json.dumps(res), flush=True)
 
 
+def large_window_workload(args):
+    """One 1920x1080 GRAYS frame per step (the config-0 generator at 1080p),
+    sigma 10, one hard-threshold stage with block 8, step 8, group 16,
+    range 30 (61 x 61 windows): the MFMA matcher's regime. The same frames run
+    on the FP32 matchers too (nss_hip_set_match_path) for the speedup and the
+    bitwise identity of the output; the CPU port runs a 640x360 crop."""
+    import numpy as np
+    import torch
+
+    import nss_hip
+    import oracle
+    import synth
+    w, h = 1920, 1080
+    kw = dict(sigma=10, block_size=8, block_step=8, group_size=16, bm_range=30)
+    clean, noisy = synth.noisy_frame("GRAYS", w, h, 10, seed=0)
+    dev = [torch.from_numpy(p).cuda() for p in noisy]
+    stream = torch.cuda.current_stream()
+
+    def run(path, steps, warmup):
+        nss_hip.set_match_path(path)
+        inst = nss_hip.BM3D("GRAYS", w, h, **kw)
+        out = [torch.empty_like(p) for p in dev]
+        for _ in range(warmup):
+            inst(dev, out=out)
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            starts[i].record(stream)
+            inst(dev, out=out)
+            ends[i].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
+        inst.set_tracing(True)
+        inst.kernel_times(reset=True)
+        for _ in range(max(steps // 2, 1)):
+            inst(dev, out=out)
+        torch.cuda.synchronize()
+        kt = inst.kernel_times(reset=True)
+        params = inst.params
+        inst.close()
+        nss_hip.set_match_path(nss_hip.MATCH_AUTO)
+        return wall, lat, kt, params, out[0].cpu().numpy()
+
+    wall, lat, kt, params, got = run(nss_hip.MATCH_AUTO, args.steps, args.warmup)
+    fp_steps = max(args.steps // 10, 2)
+    wall32, _, kt32, _, got32 = run(nss_hip.MATCH_FP32, fp_steps, 2)
+    ssd, tr, agg = stage_work(params, [(h, w)], False, directions=2)
+    work = {"match": (ssd, "TFLOP/s", PEAK_MFMA_TFLOPS), "filter": (tr, "TFLOP/s", PEAK_FP32_TFLOPS),
+            "finish": (w * h * 16, "GB/s", PEAK_HBM_GBS)}
+    kernels = {}
+    for kind, (n, ms) in kt.items():
+        if n and kind in work:
+            amount, unit, peak = work[kind]
+            avg_s = ms / n * 1e-3
+            ach = amount / avg_s / (1e12 if unit == "TFLOP/s" else 1e9)
+            kernels[kind] = {"launches": n, "avg_us": avg_s * 1e6, "achieved": ach, "unit": unit, "peak": peak,
+                             "frac": ach / peak, "algorithmic_per_launch": amount}
+    match32_us = kt32["match"][1] / kt32["match"][0] * 1e3 if kt32.get("match", (0, 0))[0] else None
+    print(f"# gpu {wall / args.steps * 1e3:.4f} ms/frame (FP32 matchers {wall32 / fp_steps * 1e3:.4f})",
+          file=sys.stderr, flush=True)
+    # parity and the CPU port on a crop (the whole frame would take the port
+    # minutes per thread at this window)
+    cw, ch = 640, 360
+    crop = [np.ascontiguousarray(noisy[0][:ch, :cw])]
+    inst = nss_hip.BM3D("GRAYS", cw, ch, **kw)
+    gcrop = inst([torch.from_numpy(crop[0]).cuda()])[0].cpu().numpy()
+    cparams = inst.params
+    inst.close()
+    threads = min(16, os.cpu_count() or 1) if not args.cpu_threads else args.cpu_threads
+    if args.no_cpu_baseline:
+        want = oracle.bm3d_frame(crop, cparams)
+    else:
+        cpu_value, want, dt = cpu_baseline(crop, cparams, threads)
+    dom = "match"
+    res = {"metric": "1080p one-stage BM3D Mpix/s, range 30 (block 8, step 8, group 16)",
+           "value": round(w * h * args.steps / wall / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 (matcher bound on f16 MFMA, f32 re-rank)",
+           "data": "synthetic",
+           "config": {"workload": "one-stage BM3D, sigma 10, block 8, step 8, group 16, range 30, one 1920x1080 "
+                                  "GRAYS frame per step"},
+           "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+           "roofline": {"kernel": dom, "bound": "mfma", "achieved": kernels[dom]["achieved"],
+                        "peak": kernels[dom]["peak"], "unit": kernels[dom]["unit"], "frac": kernels[dom]["frac"],
+                        "traffic": None},
+           "kernels": kernels,
+           "fp32_matchers": {"ms_per_step": round(wall32 / fp_steps * 1e3, 4),
+                             "match_avg_us": None if match32_us is None else round(match32_us, 2),
+                             "output_identical": bool(np.array_equal(got.view(np.uint32), got32.view(np.uint32)))},
+           "psnr_vs_cpu": round(oracle.psnr(gcrop, want[0]), 3)}
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = {"value": round(cpu_value, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                               "sample": f"{threads} copies of a 640x360 crop of the same frame, one per thread "
+                                         f"({dt:.1f} s wall)"}
+    print(json.dumps(res), flush=True)
+
+
 def serving_workload(args):
     """Three clip streams with Poisson arrivals at 24 fps each (720x480
     YUV420PS radius 1 sigma 5; 1920x1080 YUV420PS and 3840x2160 RGBS radius 0
@@ -499,7 +600,7 @@ def main():
                          "that scales rocprof's FETCH_SIZE / WRITE_SIZE readings")
     ap.add_argument("--seconds", type=float, default=5.0, help="serving: seconds of arrivals")
     ap.add_argument("--workload", choices=["two-stage-1080p", "temporal-1080p", "precision-4k", "serving",
-                                           "single-stage-256"],
+                                           "single-stage-256", "large-window-1080p"],
                     default="two-stage-1080p", help="two-stage-1080p is the headline; the others print their "
                                                     "own line")
     args = ap.parse_args()
@@ -509,7 +610,7 @@ def main():
         sys.exit(2)
 
     defaults = {"two-stage-1080p": (200, 500), "temporal-1080p": (2, 1), "precision-4k": (20, 50),
-                "serving": (0, 0), "single-stage-256": (500, 500)}[args.workload]
+                "serving": (0, 0), "single-stage-256": (500, 500), "large-window-1080p": (50, 50)}[args.workload]
     if args.steps is None:
         args.steps = defaults[0]
     if args.warmup is None:
@@ -529,6 +630,8 @@ def main():
         return serving_workload(args)
     if args.workload == "single-stage-256":
         return single_stage_256_workload(args)
+    if args.workload == "large-window-1080p":
+        return large_window_workload(args)
     if args.copy_calibration:
         n = args.copy_calibration * 1024 * 1024 // 4
         a = torch.ones(n, device="cuda")
