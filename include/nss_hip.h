@@ -212,6 +212,19 @@ int nss_hip_bm3d_autotune(nss_hip_bm3d* handle, int repeats, void* stream, doubl
 int nss_hip_bm3d_set_tracing(nss_hip_bm3d* handle, int on);
 int nss_hip_bm3d_kernel_times(nss_hip_bm3d* handle, int* launches, double* total_ms, int reset);
 
+/* Device memory the instance holds, in bytes, by the kinds of the
+ * reference's resource accounting (include/nss/resources.hpp:11
+ * ResourceKind: workspace, cached, pinned, inflight, output, framework):
+ * bytes[6]; the peak and the memory_limit_mb cap in bytes (0 when none).
+ * These are the values a VapourSynth glue stamps on each output frame as
+ * _NSSResourceBytes / _NSSResourcePeak / _NSSResourceLimit
+ * (src/frontend/ownership.hpp:103-115). Workspace: match lists, partials and
+ * sums of one frame; cached: the rolling source, ref and basic rings and the
+ * slot sums; output: the rolling output ring. Every buffer is allocated at
+ * creation, so the peak is their total; nss_hip_bm3d_create fails when it
+ * exceeds memory_limit_mb. */
+int nss_hip_bm3d_resources(nss_hip_bm3d* handle, long long* bytes, long long* peak, long long* limit);
+
 /* Frees the instance's device buffers (synchronises its work first). */
 int nss_hip_bm3d_destroy(nss_hip_bm3d* handle);
 

@@ -227,6 +227,42 @@ def test_final_equals_two_calls(gpu):
         assert np.array_equal(a, b)
 
 
+def test_memory_limit_enforced_and_reported(gpu):
+    """tests/cuda/test_cuda_bm3d_live.py:345-369 and
+    tests/test_plan01_plugin.py:63-73: _NSSResourceLimit is 0 without
+    memory_limit_mb and the limit in bytes with it; the owned bytes stay
+    within the limit; a limit too small for the clip fails at creation; the
+    smallest limit that creates gives the unlimited output."""
+    import synth
+    w, h = 640, 360
+    _, noisy = synth.noisy_frame("YUV420PS", w, h, 10, seed=14)
+    inst = gpu.BM3D("YUV420PS", w, h, sigma=10, final=1)
+    r = inst.resources()
+    free_out = [_host(o) for o in inst([_dev(p) for p in noisy])]
+    inst.close()
+    print("resources", r, flush=True)
+    assert r["_NSSResourceLimit"] == 0
+    assert r["_NSSResourcePeak"] == sum(r["_NSSResourceBytes"]) > (1 << 20)
+    assert r["_NSSResourceBytes"][1:] == [0, 0, 0, 0, 0]
+    need = (r["_NSSResourcePeak"] >> 20) + 1
+    with pytest.raises(gpu.Error) as e:
+        gpu.BM3D("YUV420PS", w, h, sigma=10, final=1, memory_limit_mb=need - 1)
+    assert "memory_limit_mb is too small for this clip" in str(e.value)
+    inst = gpu.BM3D("YUV420PS", w, h, sigma=10, final=1, memory_limit_mb=need)
+    r2 = inst.resources()
+    out = [_host(o) for o in inst([_dev(p) for p in noisy])]
+    inst.close()
+    assert r2["_NSSResourceLimit"] == need << 20
+    assert sum(r2["_NSSResourceBytes"][:5]) <= r2["_NSSResourceLimit"]
+    for x, y in zip(free_out, out):
+        assert np.array_equal(x, y)
+    roll = gpu.BM3D("YUV420PS", w, h, frames=6, sigma=10, radius=1)
+    rr = roll.resources()
+    roll.close()
+    assert rr["_NSSResourceBytes"][1] > 0 and rr["_NSSResourceBytes"][4] > 0
+    assert rr["_NSSResourcePeak"] == sum(rr["_NSSResourceBytes"])
+
+
 def test_memory_limit_does_not_change_output(gpu):
     """tests/cuda/test_cuda_bm3d_live.py:352-369: memory_limit_mb never
     changes the result."""

@@ -1185,7 +1185,45 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
             return hip_fail(e, "device allocation");
         }
     }
+    // memory_limit_mb caps the instance's device memory; a limit too small
+    // for the clip is an error at creation, never while frames are filtered
+    // (docs/api/cuda/README.md:141, src/cuda/common/group_driver.cpp:491-498).
+    if (p.memory_limit_mb > 0) {
+        const size_t need = instance_bytes(h);
+        if (need > (static_cast<size_t>(p.memory_limit_mb) << 20)) {
+            free_buffers(h);
+            delete h;
+            return fail(NSS_HIP_INVALID_INPUT, "nss_hip.BM3D: memory_limit_mb is too small for this clip: the "
+                                               "instance needs at least " +
+                                                   std::to_string((need >> 20) + 1) + " MiB");
+        }
+    }
     *out = h;
+    return NSS_HIP_OK;
+}
+
+int nss_hip_bm3d_resources(nss_hip_bm3d* h, long long* bytes, long long* peak, long long* limit) {
+    if (!h || !bytes || !peak || !limit) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    // kinds of include/nss/resources.hpp:11: workspace, cached, pinned,
+    // inflight, output, framework
+    size_t kind[6] = {frame_buffer_bytes(h), 0, 0, 0, 0, 0};
+    const auto& R = h->roll;
+    for (int i = 0; i < h->nplanes; ++i) {
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        kind[1] += npx * 4 * static_cast<size_t>(R.src_n + R.ref_n + R.basic_n);
+        kind[4] += npx * 4 * static_cast<size_t>(R.out_n) * R.out_rows;
+        for (int s = 0; s < h->nstages; ++s) {
+            if (R.num[s][i]) kind[1] += npx * 16 * static_cast<size_t>(R.acc_n);
+            if (R.part[s][i]) kind[1] += R.part_center[s][i] * (2 * static_cast<size_t>(R.r) + 1) * sizeof(float);
+        }
+    }
+    size_t total = 0;
+    for (int k = 0; k < 6; ++k) {
+        bytes[k] = static_cast<long long>(kind[k]);
+        total += kind[k];
+    }
+    *peak = static_cast<long long>(total);  // every buffer lives from creation to destruction
+    *limit = h->p.memory_limit_mb > 0 ? static_cast<long long>(h->p.memory_limit_mb) << 20 : 0;
     return NSS_HIP_OK;
 }
 

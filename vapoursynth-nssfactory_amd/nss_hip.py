@@ -98,7 +98,7 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_spatial_match", "nss_hip_set_match_path", "nss_hip_bm3d_set_tracing",
            "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push", "nss_hip_bm3d_pop", "nss_hip_server_create", "nss_hip_server_open_stream", "nss_hip_server_submit",
            "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy",
-           "nss_hip_bm3d_autotune", "nss_hip_vaggregate")
+           "nss_hip_bm3d_autotune", "nss_hip_vaggregate", "nss_hip_bm3d_resources")
 # Contribution identity of legacy temporal frames (frame properties the
 # VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
 # src/frontend/contribution.hpp:12-27).
@@ -144,6 +144,8 @@ def lib():
         L.nss_hip_bm3d_pop.argtypes = [vp, P(vp), P(ci), P(ci), vp]
         L.nss_hip_bm3d_autotune.argtypes = [vp, ci, vp, P(ctypes.c_double)]
         L.nss_hip_vaggregate.argtypes = [vp, ci, P(vp), P(vp), P(ci), ci, vp, ci, ci, ci, vp]
+        L.nss_hip_bm3d_resources.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_longlong),
+                                             P(ctypes.c_longlong)]
         L.nss_hip_server_create.argtypes = [P(ServerConfig), P(vp)]
         L.nss_hip_server_open_stream.argtypes = [vp, P(Params), P(VideoInfo), P(ci)]
         L.nss_hip_server_submit.argtypes = [vp, ci, P(vp), P(ci), P(vp), P(ci), ctypes.c_double,
@@ -287,6 +289,16 @@ class BM3D:
         ms = (ctypes.c_double * 4)()
         _check(lib().nss_hip_bm3d_kernel_times(self._h, n, ms, int(reset)))
         return {k: (n[i], ms[i]) for i, k in enumerate(KERNEL_KINDS)}
+
+    def resources(self):
+        """The frame properties a VapourSynth glue stamps for this instance
+        (src/frontend/ownership.hpp:103-115): _NSSResourceBytes (workspace,
+        cached, pinned, inflight, output, framework), _NSSResourcePeak and
+        _NSSResourceLimit (0 without memory_limit_mb)."""
+        b = (ctypes.c_longlong * 6)()
+        peak, limit = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().nss_hip_bm3d_resources(self._h, b, ctypes.byref(peak), ctypes.byref(limit)))
+        return {"_NSSResourceBytes": list(b), "_NSSResourcePeak": peak.value, "_NSSResourceLimit": limit.value}
 
     def _check_planes(self, planes, what="plane"):
         import torch
