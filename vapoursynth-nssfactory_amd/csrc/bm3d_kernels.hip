@@ -441,7 +441,7 @@ __global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
 // matchers (rows, then columns, one fmaf per pixel) and ranked by
 // (dist, y, x): the records are those of match_run_kernel bit for bit. A
 // reference whose list overflows, or a block holding a value fp16 cannot
-// carry (non-finite, or |v - mu| >= 32768), scores its whole window exactly.
+// carry (non-finite, or |v - mu| >= 16384), scores its whole window exactly.
 //
 // Error bound (all quantities per pair, u = 2^-24):
 //   |v - mu - h| <= e per element, e measured while staging (the fp16
@@ -462,7 +462,7 @@ typedef uint32_t mm_u32x4 __attribute__((ext_vector_type(4)));
 inline constexpr int kMmList = 128;
 inline constexpr float kMmC1 = 3.0517578125e-05f;    // 512 * 2^-24
 inline constexpr float kMmTiny = 6.103515625e-05f;   // 2^-14, fp16's smallest normal
-inline constexpr float kMmLimit = 32768.f;
+inline constexpr float kMmLimit = 16384.f;  // -2 h stays within fp16's range
 inline constexpr float kMmInf = __builtin_inff();
 
 // fp16 bits of a (0 below 2^-14); err = max(err, bound of |a - h| plus the
@@ -680,14 +680,15 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
             A1 = mm_frag(reg, pitch, w.cy - uy0 + 4 + g, w.cx - ux0);
         }
         const float nrA = mm_rows_sum(mm_sq(A0) + mm_sq(A1));  // |h_r|^2 of reference n
+        // The reference operand enters as -2 h_r (exact in fp16) and the
+        // accumulator starts at |h_r|^2: a tile then holds |h_r|^2 - 2 h_r.h_c.
+        A0 *= static_cast<_Float16>(-2.f);
+        A1 *= static_cast<_Float16>(-2.f);
         // C layout: lane (g, n) holds references 4 g + j (column g, row j)
         // against candidate n of the tile.
-        float nr[4], nrk[4];
+        mm_f32x4 nr;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            nr[j] = __shfl(nrA, 4 * g + j, 64);
-            nrk[j] = nr[j] * (1.f - kMmC1);
-        }
+        for (int j = 0; j < 4; ++j) nr[j] = __shfl(nrA, 4 * g + j, 64);
         const bool cval = g < ncol;
         const MmWin cw = mm_win(jb, i0 + min(g, ncol - 1), j0);
         const int ccx = cw.cx, cleft = cval ? cw.left : (1 << 30), cright = cval ? cw.right : -1;
@@ -721,8 +722,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                 for (; y <= uy1; y += 4) {
                     const mm_h16x8 F1 = mm_frag(reg, pitch, y - uy0 + 4 + g, col);
                     const float T1 = mm_rows_sum(mm_sq(F1));
-                    mm_f32x4 c =
-                        __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, F0, mm_f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    mm_f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, F0, nr, 0, 0, 0);
                     c = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, F1, c, 0, 0, 0);
                     const float ncx = xv ? T0 + T1 : kMmInf;
                     bool sv[4] = {false, false, false, false};
@@ -730,9 +730,11 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                     for (int j = 0; j < 4; ++j) {
                         // reference row j's window holds row y: uniform
                         if (y >= rtop[j] && y <= rbot[j]) {
-                            float base = SURV ? fmaf(ncx, 1.f - kMmC1, nrk[j]) : ncx + nr[j];
-                            if (y == rcy[j]) base = xs ? kMmInf : base;  // the reference itself
-                            const float d = fmaf(-2.f, c[j], base);
+                            // pass A: d' = |h_c|^2 + c; pass B:
+                            // d' - C1 (|h_r|^2 + |h_c|^2) <= W as
+                            // (1 - C1) |h_c|^2 + c <= W + C1 |h_r|^2
+                            float d = SURV ? fmaf(ncx, 1.f - kMmC1, c[j]) : ncx + c[j];
+                            if (y == rcy[j]) d = xs ? kMmInf : d;  // the reference itself
                             if constexpr (SURV)
                                 sv[j] = d <= wv[j];
                             else
@@ -814,7 +816,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 4; ++j) wv[j] = wl[4 * g + j];
+        for (int j = 0; j < 4; ++j) wv[j] = wl[4 * g + j] + kMmC1 * nr[j];
         pass(std::true_type{});
     }
     __syncthreads();
