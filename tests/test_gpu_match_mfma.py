@@ -144,3 +144,26 @@ def test_mfma_filter_output_identical(paths, fmt, kw):
         inst.close()
     for i, (a, b) in enumerate(zip(*outs)):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (fmt, kw, i)
+
+
+@pytest.mark.parametrize("kw", [dict(sigma=10, radius=1, bm_range=20, group_size=16, final=1),
+                                dict(sigma=10, radius=1, bm_range=20, group_size=8, temporal_mode="legacy")])
+def test_mfma_temporal_identical(paths, kw):
+    """The temporal path's spatial matches come from the same matcher: the
+    clip's frames (rolling) or contribution slices (legacy) are the same bits
+    on either matcher."""
+    import synth
+    gpu = paths
+    w, h, n = 160, 120, 5
+    _, clip = synth.shifted_sequence("YUV420PS", w, h, n, kw["sigma"], seed=2, max_step=2)
+    outs = []
+    for path in (gpu.MATCH_FP32, gpu.MATCH_MFMA):
+        gpu.set_match_path(path)
+        inst = gpu.BM3D("YUV420PS", w, h, n, **kw)
+        res = inst.filter_clip([[_dev(p) for p in f] for f in clip])
+        outs.append([[_host(p) for p in f] for f in res])
+        inst.close()
+    assert len(outs[0]) == len(outs[1]) == n
+    for fa, fb in zip(*outs):
+        for a, b in zip(fa, fb):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), kw
