@@ -77,6 +77,15 @@ TileGeom make_tiles(const Grid& g, int tw, int th, int block, int range) {
 
 std::atomic<int> g_match_path{0};
 
+// The MFMA matcher scores every candidate of a 4 x 4 block's union window
+// (side 3 step + 2 range + 1) for all 16 references; the run matcher scores
+// each reference's own window. Measured on 1080p planes (block 8, group 8
+// and 16), the MFMA matcher is faster once the window side exceeds about
+// 3.3 steps (range >= 13 at step 8, >= 10 at step 4 or 3) and the frame has
+// enough references to fill the device (a 256 x 256 frame at step 3 does
+// not).
+bool match_mfma_pays(int step, int range) { return range >= 10 && 3 * (2 * range + 1) > 10 * step; }
+
 int match_path() { return g_match_path.load(std::memory_order_relaxed); }
 
 bool allowed_block(int b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 12 || b == 16 || b == 32; }
@@ -137,7 +146,8 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
             if (st.plane[i].sigma != 0.f && st.plane[i].block == b) bk.planes[bk.nplanes++] = i;
         if (!bk.nplanes) continue;
         size_t run_lds = 0, wave_lds = 0, mfma_lds = 0;
-        bool mfma = match_path() != 1;
+        bool mfma = match_path() != 1, mfma_pays = true;
+        long long nref = 0;
         for (int k = 0; k < bk.nplanes; ++k) {
             const StagePlane& sp = st.plane[bk.planes[k]];
             const int i = bk.planes[k];
@@ -145,9 +155,12 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
             wave_lds = std::max(wave_lds, match_wave_lds_bytes(b, sp.range, true));
             mfma_lds = std::max(mfma_lds, match_mfma_lds_bytes(sp.step, sp.range));
             mfma &= match_mfma_fits(b, sp.group, sp.step, sp.range, h->pw[i], h->ph[i]);
+            mfma_pays &= match_mfma_pays(sp.step, sp.range);
+            const Grid gr = make_grid(h->pw[i], h->ph[i], b, sp.step);
+            nref += static_cast<long long>(gr.nx) * gr.ny;
         }
         const bool run = b <= 12 && run_lds <= kRunLdsBudget;
-        if (mfma && (match_path() == 2 || !run)) {
+        if (mfma && (match_path() == 2 || !run || (mfma_pays && nref >= kMfmaMinRefs))) {
             bk.match_kind = 3;
             bk.match_lds = mfma_lds;
         } else if (run) {
@@ -1512,7 +1525,7 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
     size_t lds;
     const bool run = block <= 12 && match_run_lds_bytes(block, step, bm_range) <= kRunLdsBudget;
     if (match_path() != 1 && match_mfma_fits(block, group, step, bm_range, width, height) &&
-        (match_path() == 2 || !run)) {
+        (match_path() == 2 || !run || (match_mfma_pays(step, bm_range) && nref >= kMfmaMinRefs))) {
         kind = 3;
         lds = match_mfma_lds_bytes(step, bm_range);
     } else if (run) {

@@ -30,6 +30,11 @@ int match_workgroups(int kind, const Grid& g);
 // Process-wide matcher choice (nss_hip_set_match_path): 0 automatic, 1 the
 // FP32 matchers only, 2 the MFMA matcher wherever it fits.
 int match_path();
+// Automatic choice: the MFMA matcher also where the run matcher fits, when
+// the window is wide against the step and the stage has at least
+// kMfmaMinRefs references.
+bool match_mfma_pays(int step, int range);
+constexpr long long kMfmaMinRefs = 16384;
 size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);
 size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t);
