@@ -23,6 +23,9 @@ HIPFLAGS = [
     "-fPIC",
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in VGPRs (gfx950's unified register file) rather than
+    # AGPRs: no v_accvgpr_read before the epilogues, fewer registers in all.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall",
 ]
 
