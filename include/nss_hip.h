@@ -239,7 +239,7 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
 
 /* Matcher choice for this process (instances created afterwards, and
  * nss_hip_spatial_match): 0 automatic, 1 the FP32 matchers only, 2 the MFMA
- * matcher wherever it applies (block 8, group <= 16). Every choice returns
+ * matcher wherever it applies (block 8, group <= 32). Every choice returns
  * the same records; 1 and 2 exist to compare them. The reference picks its
  * tensor-core matcher by window side (src/cuda/common/match.cu:1609-1615
  * spatial_match_tensor) and exposes the forced form to its tests

@@ -1,4 +1,4 @@
-"""The MFMA matcher (block 8, group <= 16) through the C ABI.
+"""The MFMA matcher (block 8, group <= 32) through the C ABI.
 
 Its records must equal the FP32 matchers' bit for bit (same count, same
 x, y, t and distance bits per entry): the MFMA pass only chooses which
@@ -83,7 +83,8 @@ def _assert_same(a, b, label):
 @pytest.mark.parametrize("kind", ["noise", "lownoise", "flat"])
 @pytest.mark.parametrize("width,height,step,rng_,group", [
     (160, 120, 8, 9, 8), (131, 97, 3, 16, 16), (150, 110, 8, 20, 16), (170, 130, 6, 30, 8),
-    (200, 180, 8, 64, 16), (96, 80, 5, 12, 4), (64, 48, 2, 9, 2), (45, 37, 7, 9, 16)])
+    (200, 180, 8, 64, 16), (96, 80, 5, 12, 4), (64, 48, 2, 9, 2), (45, 37, 7, 9, 16),
+    (150, 110, 3, 19, 32), (120, 90, 8, 20, 32), (40, 30, 1, 6, 32)])
 def test_mfma_equals_fp32(paths, kind, width, height, step, rng_, group):
     gpu = paths
     label = f"{kind} {width}x{height} s{step} r{rng_} g{group}"
@@ -93,13 +94,14 @@ def test_mfma_equals_fp32(paths, kind, width, height, step, rng_, group):
     _assert_same(a, b, label)
 
 
+@pytest.mark.parametrize("group", [16, 32])
 @pytest.mark.parametrize("kind", ["const", "bright", "huge", "nan"])
-def test_mfma_fallbacks(paths, kind):
+def test_mfma_fallbacks(paths, kind, group):
     gpu = paths
-    label = f"{kind} 120x90 s4 r18 g16"
+    label = f"{kind} 120x90 s4 r18 g{group}"
     print("mfma match", label, flush=True)
     plane = _plane(kind, 120, 90, seed=3)
-    a, b = _both(gpu, plane, block=8, step=4, bm_range=18, group=16)
+    a, b = _both(gpu, plane, block=8, step=4, bm_range=18, group=group)
     _assert_same(a, b, label)
 
 
@@ -129,7 +131,9 @@ def test_mfma_against_oracle(paths, oracle):
 
 @pytest.mark.parametrize("fmt,kw", [("YUV420PS", dict(sigma=10, final=1)),
                                     ("GRAYS", dict(sigma=5, block_step=3, group_size=16, bm_range=16)),
-                                    ("GRAYS", dict(sigma=20, bm_range=24, group_size=16, final=1))])
+                                    ("GRAYS", dict(sigma=20, bm_range=24, group_size=16, final=1)),
+                                    ("YUV420PS", dict(sigma=3, final=1, block_step=3, group_size=32,
+                                                      group_size_basic=16, bm_range=19))])
 def test_mfma_filter_output_identical(paths, fmt, kw):
     """A whole filter call gives the same output bits on either matcher."""
     import synth

@@ -424,7 +424,7 @@ __global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
 }
 
 // ---------------------------------------------------------------------------
-// MFMA matcher (B = 8, G <= 16): the role of the reference's tensor-core
+// MFMA matcher (B = 8, G <= 32): the role of the reference's tensor-core
 // matcher (src/cuda/common/match.cu:1165-1620, spatial_match_tensor), built
 // for CDNA4. A workgroup of four waves takes a block of 4 x 4 references.
 // The union of their windows is staged in LDS as fp16 values
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
 //      (16 lanes hold disjoint columns): the K-th smallest of those 64
 //      distinct candidates bounds the K-th exact key from above;
 //   B. every candidate whose d' does not rule it out against that bound is
-//      listed (at most kMmList per reference).
+//      listed (at most 128 per reference for G <= 16, 256 for G <= 32).
 // The listed candidates are then scored in the FP32 order of the other
 // matchers (rows, then columns, one fmaf per pixel) and ranked by
 // (dist, y, x): the records are those of match_run_kernel bit for bit. A
@@ -459,7 +459,6 @@ typedef _Float16 mm_h16x2 __attribute__((ext_vector_type(2)));
 typedef float mm_f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t mm_u32x4 __attribute__((ext_vector_type(4)));
 
-inline constexpr int kMmList = 128;
 inline constexpr float kMmC1 = 3.0517578125e-05f;    // 512 * 2^-24
 inline constexpr float kMmTiny = 6.103515625e-05f;   // 2^-14, fp16's smallest normal
 inline constexpr float kMmLimit = 16384.f;  // -2 h stays within fp16's range
@@ -574,6 +573,8 @@ __device__ inline uint16_t bf16_up(float v) {
     return static_cast<uint16_t>(((b & 0xffffu) != 0u && (b >> 31) == 0u) ? hi + 1u : hi);
 }
 
+// LIST: listed candidates per reference (128 for G <= 16, 256 for G <= 32).
+template <int LIST>
 __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
     extern __shared__ __align__(16) uint32_t mm_lds[];
     const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, n = lane & 15;
@@ -595,10 +596,10 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
     const int UW = 16 * nt;
     const int RH = uy1 - uy0 + 8;
     const int pitch = mm_pitch(nt);
-    const int reg_words = max(RH * pitch, 16 * kMmList);
+    const int reg_words = max(RH * pitch, 16 * LIST);
     uint32_t* reg = mm_lds;
     uint16_t* list = reinterpret_cast<uint16_t*>(mm_lds + reg_words);
-    int* cnt = reinterpret_cast<int*>(mm_lds + reg_words + 8 * kMmList);  // 16
+    int* cnt = reinterpret_cast<int*>(mm_lds + reg_words + 8 * LIST);  // 16
     int* pre = cnt + 16;                                                // 17 (+1 keeps 8-byte alignment)
     int* kmv = pre + 18;                                                // 16
     float* wl = reinterpret_cast<float*>(kmv + 16);                     // 16
@@ -749,7 +750,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                                 if (sv[j]) {
                                     const int m = 4 * g + j;
                                     const int k = atomicAdd(&cnt[m], 1);
-                                    if (k < kMmList) list[m * kMmList + k] = static_cast<uint16_t>(u);
+                                    if (k < LIST) list[m * LIST + k] = static_cast<uint16_t>(u);
                                 }
                             }
                         }
@@ -780,7 +781,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
             int taken = 0;
             bool found = K == 0;
             float td = kMmInf;
-            for (int round = 0; round < 16; ++round) {
+            for (int round = 0; round < 32; ++round) {
                 if (__builtin_amdgcn_ballot_w64(!found) == 0) break;
                 const float mn = row_min_f32(m4[0]);
                 const bool eq = m4[0] == mn;
@@ -827,7 +828,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
         for (int m = 0; m < 16; ++m) {
             pre[m] = acc;
             const int c = cnt[m];
-            acc += (!anybad && kmv[m] > 0 && c <= kMmList) ? c : 0;
+            acc += (!anybad && kmv[m] > 0 && c <= LIST) ? c : 0;
         }
         pre[16] = acc;
     }
@@ -845,7 +846,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
     for (int q = tid; q < total; q += 256) {
         int m, sl;
         item(q, m, sl);
-        const int u = list[m * kMmList + sl];
+        const int u = list[m * LIST + sl];
         const int uy = div_small(u, inv_uw), ux = u - uy * UW;
         const MmWin w = mm_win(jb, i0 + (m >> 2), j0 + (m & 3));
         dk[q] = mm_exact(plane, stride, w.cx, w.cy, ux0 + ux, uy0 + uy);
@@ -856,11 +857,11 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
         item(q, m, sl);
         const int c = pre[m + 1] - pre[m];
         const uint32_t d = dk[q];
-        const int u = list[m * kMmList + sl];
+        const int u = list[m * LIST + sl];
         int rank = 0;
         for (int s2 = 0; s2 < c; ++s2) {
             const uint32_t d2 = dk[pre[m] + s2];
-            const int u2 = list[m * kMmList + s2];
+            const int u2 = list[m * LIST + s2];
             rank += (d2 < d || (d2 == d && u2 < u)) ? 1 : 0;
         }
         if (rank < kmv[m]) {
@@ -879,12 +880,12 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
     // Whole windows, exactly, for the references the bound could not serve
     // (wave w takes references w, w + 4, ...): chunks of 64 candidates merged
     // by rank into the running G - 1 best.
-    unsigned long long* ck = reinterpret_cast<unsigned long long*>(xw) + wave * 96;  // 64
-    unsigned long long* bk = ck + 64;                                                 // 16
-    unsigned long long* nbk = bk + 16;                                                // 16
+    unsigned long long* ck = reinterpret_cast<unsigned long long*>(xw) + wave * 128;  // 64
+    unsigned long long* bk = ck + 64;                                                  // 32
+    unsigned long long* nbk = bk + 32;                                                 // 32
     for (int m = wave; m < 16; m += 4) {
         const int K = kmv[m];
-        if (K == 0 || !(anybad || cnt[m] > kMmList)) continue;
+        if (K == 0 || !(anybad || cnt[m] > LIST)) continue;
         const MmWin w = mm_win(jb, i0 + (m >> 2), j0 + (m & 3));
         const int ww = w.right - w.left + 1, ncand = ww * (w.bottom - w.top + 1);
         const int self = (w.cy - w.top) * ww + (w.cx - w.left);
@@ -2700,18 +2701,21 @@ size_t match_wave_lds_bytes(int block, int range, bool stage) {
     return ((reg + 3) & ~static_cast<size_t>(3)) * 4 + cand;
 }
 
-size_t match_mfma_lds_bytes(int step, int range) {
+static int mm_list(int group) { return group > 16 ? 256 : 128; }
+
+size_t match_mfma_lds_bytes(int step, int range, int group) {
     // the union of a 4 x 4 block's windows spans at most 3 step + 2 range + 1
     // candidate columns and rows
     const int span = 3 * step + 2 * range + 1;
     const int nt = (span + 15) >> 4;
-    const size_t reg = std::max(static_cast<size_t>(span + 7) * mm_pitch(nt), static_cast<size_t>(16 * kMmList));
-    return (reg + 8 * kMmList + 66 + 2048) * 4;
+    const size_t list = static_cast<size_t>(mm_list(group));
+    const size_t reg = std::max(static_cast<size_t>(span + 7) * mm_pitch(nt), 16 * list);
+    return (reg + 8 * list + 66 + 2048) * 4;
 }
 
 bool match_mfma_fits(int block, int group, int step, int range, int width, int height) {
-    return block == 8 && group >= 1 && group <= 16 && width >= 8 && height >= 8 &&
-           match_mfma_lds_bytes(step, range) <= 160 * 1024;
+    return block == 8 && group >= 1 && group <= 32 && width >= 8 && height >= 8 &&
+           match_mfma_lds_bytes(step, range, group) <= 160 * 1024;
 }
 
 int match_workgroups(int kind, const Grid& g) {
@@ -2772,8 +2776,15 @@ template <int B>
 static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
     if (kind == 3) {
         if constexpr (B == 8) {
-            allow_lds(&match_mfma_kernel, lds);
-            match_mfma_kernel<<<total_wg, 256, lds, s>>>(t);
+            int gmax = 0;
+            for (int j = 0; j < t.n; ++j) gmax = std::max(gmax, t.job[j].group);
+            if (mm_list(gmax) == 256) {
+                allow_lds(&match_mfma_kernel<256>, lds);
+                match_mfma_kernel<256><<<total_wg, 256, lds, s>>>(t);
+            } else {
+                allow_lds(&match_mfma_kernel<128>, lds);
+                match_mfma_kernel<128><<<total_wg, 256, lds, s>>>(t);
+            }
         } else {
             return hipErrorInvalidValue;
         }

@@ -83,8 +83,12 @@ std::atomic<int> g_match_path{0};
 // and 16), the MFMA matcher is faster once the window side exceeds about
 // 3.3 steps (range >= 13 at step 8, >= 10 at step 4 or 3) and the frame has
 // enough references to fill the device (a 256 x 256 frame at step 3 does
-// not).
-bool match_mfma_pays(int step, int range) { return range >= 10 && 3 * (2 * range + 1) > 10 * step; }
+// not). At group 32 the longer candidate lists make it slower than the run
+// matcher (1080p YUV420, step 3, range 19: 6.4 against 5.1 ms per frame for
+// both stages' matching), so there it only replaces the wave matcher.
+bool match_mfma_pays(int step, int range, int group) {
+    return group <= 16 && range >= 10 && 3 * (2 * range + 1) > 10 * step;
+}
 
 int match_path() { return g_match_path.load(std::memory_order_relaxed); }
 
@@ -153,9 +157,9 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
             const int i = bk.planes[k];
             run_lds = std::max(run_lds, match_run_lds_bytes(b, sp.step, sp.range));
             wave_lds = std::max(wave_lds, match_wave_lds_bytes(b, sp.range, true));
-            mfma_lds = std::max(mfma_lds, match_mfma_lds_bytes(sp.step, sp.range));
+            mfma_lds = std::max(mfma_lds, match_mfma_lds_bytes(sp.step, sp.range, sp.group));
             mfma &= match_mfma_fits(b, sp.group, sp.step, sp.range, h->pw[i], h->ph[i]);
-            mfma_pays &= match_mfma_pays(sp.step, sp.range);
+            mfma_pays &= match_mfma_pays(sp.step, sp.range, sp.group);
             const Grid gr = make_grid(h->pw[i], h->ph[i], b, sp.step);
             nref += static_cast<long long>(gr.nx) * gr.ny;
         }
@@ -1525,9 +1529,9 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
     size_t lds;
     const bool run = block <= 12 && match_run_lds_bytes(block, step, bm_range) <= kRunLdsBudget;
     if (match_path() != 1 && match_mfma_fits(block, group, step, bm_range, width, height) &&
-        (match_path() == 2 || !run || (match_mfma_pays(step, bm_range) && nref >= kMfmaMinRefs))) {
+        (match_path() == 2 || !run || (match_mfma_pays(step, bm_range, group) && nref >= kMfmaMinRefs))) {
         kind = 3;
-        lds = match_mfma_lds_bytes(step, bm_range);
+        lds = match_mfma_lds_bytes(step, bm_range, group);
     } else if (run) {
         kind = 2;
         lds = match_run_lds_bytes(block, step, bm_range);

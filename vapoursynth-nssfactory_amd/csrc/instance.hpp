@@ -22,8 +22,8 @@ hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s);
 hipError_t upload_dct_tables(const float* host, size_t count);
 size_t match_wave_lds_bytes(int block, int range, bool stage);
 size_t match_run_lds_bytes(int block, int step, int range);
-size_t match_mfma_lds_bytes(int step, int range);
-// The MFMA matcher serves B = 8, G <= 16 when its LDS fits.
+size_t match_mfma_lds_bytes(int step, int range, int group);
+// The MFMA matcher serves B = 8, G <= 32 when its LDS fits.
 bool match_mfma_fits(int block, int group, int step, int range, int width, int height);
 // Workgroups of a match launch of `kind` over one plane's grid.
 int match_workgroups(int kind, const Grid& g);
@@ -33,7 +33,7 @@ int match_path();
 // Automatic choice: the MFMA matcher also where the run matcher fits, when
 // the window is wide against the step and the stage has at least
 // kMfmaMinRefs references.
-bool match_mfma_pays(int step, int range);
+bool match_mfma_pays(int step, int range, int group);
 constexpr long long kMfmaMinRefs = 16384;
 size_t filter_atomic_lds_bytes(int block, int group, bool wiener);
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t);

@@ -401,7 +401,7 @@ MATCH_AUTO, MATCH_FP32, MATCH_MFMA = 0, 1, 2
 def set_match_path(path):
     """Matcher choice for the process: MATCH_AUTO, MATCH_FP32 (the FP32
     matchers only) or MATCH_MFMA (the MFMA matcher wherever it applies: block
-    8, group <= 16). Every choice returns the same records."""
+    8, group <= 32). Every choice returns the same records."""
     _check(lib().nss_hip_set_match_path(int(path)))
 
 
