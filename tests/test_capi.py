@@ -60,3 +60,18 @@ def test_server_config_validation_without_device():
     assert lib.nss_hip_server_create(None, ctypes.byref(h)) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_server_destroy(None) == nss_hip.OK
     assert lib.nss_hip_server_drain(None) == nss_hip.INVALID_INPUT
+
+
+def test_match_path_and_resources_argument_checks():
+    """nss_hip_set_match_path takes 0, 1 or 2 and nothing else; the resource
+    query rejects null arguments. Neither touches the device."""
+    import nss_hip
+    lib = nss_hip.lib()
+    for bad in (-1, 3):
+        assert lib.nss_hip_set_match_path(bad) == nss_hip.INVALID_INPUT
+        assert "match path must be 0, 1 or 2" in nss_hip.last_error()
+    for good in (nss_hip.MATCH_FP32, nss_hip.MATCH_MFMA, nss_hip.MATCH_AUTO):
+        assert lib.nss_hip_set_match_path(good) == nss_hip.OK
+    b = (ctypes.c_longlong * 6)()
+    v = ctypes.c_longlong()
+    assert lib.nss_hip_bm3d_resources(None, b, ctypes.byref(v), ctypes.byref(v)) == nss_hip.INVALID_INPUT
