@@ -2014,14 +2014,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void pr
                     const int cw = centers[w];
                     const int y0 = (cw >> 13) - ps, x0 = (cw & 8191) - ps;
                     float* dst = region + w * (R * P);
-                    if (lr < rows_per)
-                        for (int j = lr; j < R; j += rows_per)
-                            for (int i = li; i < P; i += (P <= 64 ? P : 64)) {
-                                const int y = y0 + j, x = x0 + i;
-                                dst[j * P + i] = (i < R && y >= 0 && y < jb.height && x >= 0 && x < jb.width)
-                                                     ? pt[static_cast<size_t>(y) * jb.stride + x]
-                                                     : 0.f;
-                            }
+                    if (lr >= rows_per) continue;
+                    if (P <= 64 && R <= 8 * rows_per) {
+                        // one column per lane, up to 8 rows: every load in
+                        // flight before the first store
+                        float v[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int j = lr + q * rows_per, y = y0 + j, x = x0 + li;
+                            v[q] = (j < R && li < R && y >= 0 && y < jb.height && x >= 0 && x < jb.width)
+                                       ? pt[static_cast<size_t>(y) * jb.stride + x]
+                                       : 0.f;
+                        }
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int j = lr + q * rows_per;
+                            if (j < R) dst[j * P + li] = v[q];
+                        }
+                        continue;
+                    }
+                    for (int j = lr; j < R; j += rows_per)
+                        for (int i = li; i < P; i += (P <= 64 ? P : 64)) {
+                            const int y = y0 + j, x = x0 + i;
+                            dst[j * P + i] = (i < R && y >= 0 && y < jb.height && x >= 0 && x < jb.width)
+                                                 ? pt[static_cast<size_t>(y) * jb.stride + x]
+                                                 : 0.f;
+                        }
                 }
                 __syncthreads();
                 const int strips = nc * W * SW;
