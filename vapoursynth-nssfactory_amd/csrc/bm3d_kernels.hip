@@ -215,14 +215,29 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int RW = need + (6 - need % 4) % 4;
     const int RH = bottom - top + B + 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int yy = wave; yy < RH; yy += 4) {
-        const int gy = top + yy;
-        const float* src = jb.plane + static_cast<size_t>(min(gy, jb.height - 1)) * jb.stride;
-        for (int xx = lane; xx < RW; xx += 64) {
-            const int gx = rx0 + xx;
-            lds[yy * RW + xx] = (gy < jb.height && gx >= 0 && gx < jb.width) ? src[gx] : 0.f;
+    // Rows wave, wave + 4, ...: four rows by two 64-column strips per step,
+    // the eight loads in flight before their stores.
+    for (int y4 = wave; y4 < RH; y4 += 16)
+        for (int c0 = 0; c0 < RW; c0 += 128) {
+            float v[4][2];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int yy = y4 + 4 * q, xx = c0 + 64 * c + lane;
+                    const int gy = top + yy, gx = rx0 + xx;
+                    v[q][c] = (yy < RH && xx < RW && gy < jb.height && gx >= 0 && gx < jb.width)
+                                  ? jb.plane[static_cast<size_t>(gy) * jb.stride + gx]
+                                  : 0.f;
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int yy = y4 + 4 * q, xx = c0 + 64 * c + lane;
+                    if (yy < RH && xx < RW) lds[yy * RW + xx] = v[q][c];
+                }
         }
-    }
     const int wmax = 2 * r + 1;
     uint32_t* dist = reinterpret_cast<uint32_t*>(lds + ((RW * RH + 3) & ~3));
     __syncthreads();
