@@ -133,7 +133,10 @@ def test_mfma_against_oracle(paths, oracle):
                                     ("GRAYS", dict(sigma=5, block_step=3, group_size=16, bm_range=16)),
                                     ("GRAYS", dict(sigma=20, bm_range=24, group_size=16, final=1)),
                                     ("YUV420PS", dict(sigma=3, final=1, block_step=3, group_size=32,
-                                                      group_size_basic=16, bm_range=19))])
+                                                      group_size_basic=16, bm_range=19)),
+                                    # planes with different groups and ranges in one launch: the
+                                    # launch's list length (group 32) sized for every plane
+                                    ("YUV420PS", dict(sigma=10, group_size=[32, 16, 16], bm_range=[8, 30, 30]))])
 def test_mfma_filter_output_identical(paths, fmt, kw):
     """A whole filter call gives the same output bits on either matcher."""
     import synth

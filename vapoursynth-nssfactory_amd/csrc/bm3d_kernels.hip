@@ -2809,8 +2809,13 @@ template <int B>
 static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
     if (kind == 3) {
         if constexpr (B == 8) {
+            // One list length for the whole launch (the largest group's), so
+            // the LDS is sized here for every job at that length.
             int gmax = 0;
             for (int j = 0; j < t.n; ++j) gmax = std::max(gmax, t.job[j].group);
+            for (int j = 0; j < t.n; ++j)
+                lds = std::max(lds, match_mfma_lds_bytes(t.job[j].grid.step, t.job[j].range, gmax));
+            if (lds > 160 * 1024) return hipErrorInvalidValue;
             if (mm_list(gmax) == 256) {
                 allow_lds(&match_mfma_kernel<256>, lds);
                 match_mfma_kernel<256><<<total_wg, 256, lds, s>>>(t);
