@@ -2433,8 +2433,8 @@ __global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable ta
             const TileGeom& g = jb.tiles[t];
             total += g.aw * g.ah + (g.aw - B + 1) * (g.ah - B + 1);
         }
-        if (threadIdx.x < ns) {
-            const int t = threadIdx.x;
+        if (static_cast<int>(threadIdx.x) < ns) {
+            const int t = static_cast<int>(threadIdx.x);
             int o = 0;
             for (int q = 0; q < t; ++q)
                 o += jb.tiles[q].aw * jb.tiles[q].ah + (jb.tiles[q].aw - B + 1) * (jb.tiles[q].ah - B + 1);
