@@ -443,30 +443,37 @@ __global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
 // matcher (src/cuda/common/match.cu:1165-1620, spatial_match_tensor), built
 // for CDNA4. A workgroup of four waves takes a block of 4 x 4 references.
 // The union of their windows is staged in LDS as fp16 values
-// h = fp16(v - mu). Dot products of reference and candidate patches run on
+// h = fp16((v - mu) 2^S), S the block's power-of-two scale (the largest
+// |v - mu| then sits in [2^12, 2^13)). Dot products of reference and
+// candidate patches run on
 // v_mfma_f32_16x16x32_f16 (16 references x 16 candidates of one row per
 // tile), giving d' = |h_r|^2 + |h_c|^2 - 2 h_r.h_c for every pair; wave w
 // takes the candidate rows w mod 4. Two passes over the tiles:
-//   A. per reference, the four smallest d' of each lane's candidate column
-//      (16 lanes hold disjoint columns): the K-th smallest of those 64
-//      distinct candidates bounds the K-th exact key from above;
-//   B. every candidate whose d' does not rule it out against that bound is
-//      listed (at most 128 per reference for G <= 16, 256 for G <= 32).
+//   A. per reference, the four smallest upper bounds U of each lane's
+//      candidate column (16 lanes hold disjoint columns): the K-th smallest
+//      of those 64 distinct candidates bounds the K-th exact key from above;
+//   B. every candidate whose lower bound L does not exceed it is listed (at
+//      most 128 per reference for G <= 16, 256 for G <= 32).
 // The listed candidates are then scored in the FP32 order of the other
 // matchers (rows, then columns, one fmaf per pixel) and ranked by
 // (dist, y, x): the records are those of match_run_kernel bit for bit. A
 // reference whose list overflows, or a block holding a value fp16 cannot
-// carry (non-finite, or |v - mu| >= 16384), scores its whole window exactly.
+// carry (non-finite, or |v - mu| >= 8192 so that S would be negative),
+// scores its whole window exactly.
 //
-// Error bound (all quantities per pair, u = 2^-24):
-//   |v - mu - h| <= e per element, e measured while staging (the fp16
-//     rounding, values below 2^-14 flushed to 0, plus the FP32 rounding of
-//     v - mu); over a patch ||.||_2 <= 8 e_max, so by the triangle inequality
-//     |sqrt(d*) - sqrt(d'')| <= E = 16 e_max (d* the real SSD, d'' the exact
-//     SSD of the fp16 patches);
-//   |d' - d''| <= 200 u (|h_r|^2 + |h_c|^2) (squares exact, 64-term FP32 sums
-//     and the MFMA's FP32 accumulation), taken as kMmC1 = 512 u;
-//   the FP32 key differs from d* by at most 66 u d* (taken as 2e-4).
+// Error bound, in the scaled units (every distance times 2^2S; u = 2^-24):
+//   per element, |(v - mu) 2^S - h| <= 2^-11 |h| (1 + 2^-11) for a normal h,
+//     and < 2^-14 where h was flushed to 0 (below fp16's smallest normal);
+//     over a patch, E_c^2 <= kappa (|h_c|^2 + 1), kappa = 1.002 2^-22;
+//   |d' - d''| <= C1 (|h_r|^2 + |h_c|^2), d'' the exact SSD of the fp16
+//     patches (squares exact, 64-term FP32 sums and the MFMA's FP32
+//     accumulation, about 200 u), C1 = 512 u;
+//   by the triangle inequality sqrt(d*) lies within E_r + E_c of
+//     sqrt(d''), d* the real SSD; with (a + b)^2 <= (1 + t) a^2 +
+//     (1 + 1/t) b^2, (a - b)^2 >= (1 - t) a^2 - (1/t - 1) b^2 (t = 2^-6)
+//     and E^2 <= 2 E_r^2 + 2 E_c^2 both bounds are linear in (c, |h_c|^2),
+//     c = |h_r|^2 - 2 h_r.h_c the MFMA tile: one FMA per pair;
+//   the FP32 key differs from d* by at most 66 u d* (taken as 1e-4).
 // ---------------------------------------------------------------------------
 
 typedef _Float16 mm_h16x8 __attribute__((ext_vector_type(8)));
@@ -476,16 +483,19 @@ typedef uint32_t mm_u32x4 __attribute__((ext_vector_type(4)));
 
 inline constexpr float kMmC1 = 3.0517578125e-05f;    // 512 * 2^-24
 inline constexpr float kMmTiny = 6.103515625e-05f;   // 2^-14, fp16's smallest normal
-inline constexpr float kMmLimit = 16384.f;  // -2 h stays within fp16's range
 inline constexpr float kMmInf = __builtin_inff();
+inline constexpr float kMmT = 0.015625f;              // t = 2^-6
+inline constexpr float kMmKappa = 2.3889e-07f;        // 1.002 * 2^-22, rounded up
+inline constexpr float kMmGammaU = 2.f * kMmKappa * (1.f + 1.f / kMmT);  // E^2 weight, upper bound
+inline constexpr float kMmGammaL = 2.f * kMmKappa / kMmT;                // E^2 weight, lower bound
+// Per-pair slopes in |h_c|^2 of the bounds U / (1 + t) and L / (1 - t) (c has
+// slope 1): (1 + C1) + gamma_U / (1 + t) and (1 - C1) - gamma_L / (1 - t).
+inline constexpr float kMmSlopeU = 1.f + kMmC1 + kMmGammaU / (1.f + kMmT) + 1e-6f;
+inline constexpr float kMmSlopeL = 1.f - kMmC1 - kMmGammaL / (1.f - kMmT) - 1e-6f;
 
-// fp16 bits of a (0 below 2^-14); err = max(err, bound of |a - h| plus the
-// rounding of a = v - mu); bad when a cannot be carried.
-__device__ inline uint32_t mm_half(float a, float& err, bool& bad) {
-    const float m = fabsf(a);
-    bad |= !(m < kMmLimit);
-    const _Float16 h = m < kMmTiny ? static_cast<_Float16>(0.f) : static_cast<_Float16>(a);
-    err = fmaxf(err, fabsf(static_cast<float>(h) - a) + 1.2e-7f * m);
+// fp16 bits of a scaled value (0 below 2^-14).
+__device__ inline uint32_t mm_half(float a) {
+    const _Float16 h = fabsf(a) < kMmTiny ? static_cast<_Float16>(0.f) : static_cast<_Float16>(a);
     return __builtin_bit_cast(uint16_t, h);
 }
 
@@ -633,56 +643,70 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
         cnt[tid] = 0;
     }
 
-    // mu: the mean of the first reference's patch (every wave alike); then
-    // the region as fp16.
+    // mu: the mean of the first reference's patch (every wave alike). Two
+    // passes over the region, eight words per thread in flight each: the
+    // largest |v - mu| (non-finite values make it NaN), then the scaled
+    // fp16 values.
     const float mu =
         wave_sum_f32(plane[static_cast<size_t>(w00.cy + (lane >> 3)) * stride + w00.cx + (lane & 7)]) *
         (1.f / 64.f);
-    float err = 0.f;
-    bool bad = !(fabsf(mu) < kMmLimit);
-    {
-        // eight words per thread in flight: all loads first, then the packing
-        const float inv_pitch = 1.f / static_cast<float>(pitch);
-        const int words = RH * pitch;
-        for (int base = 0; base < words; base += 8 * 256) {
-            float v0[8], v1[8];
+    const float inv_pitch = 1.f / static_cast<float>(pitch);
+    const int words = RH * pitch;
+    auto load8 = [&](int base, float (&v0)[8], float (&v1)[8]) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int wd = base + tid + 256 * i;
-                v0[i] = mu;
-                v1[i] = mu;
-                if (wd < words) {
-                    const int yy = div_small(wd, inv_pitch), xw2 = wd - yy * pitch;
-                    const float* row = plane + static_cast<size_t>(uy0 + yy) * stride;
-                    const int x = ux0 + 2 * xw2;
-                    if (x < jb.width) v0[i] = row[x];
-                    if (x + 1 < jb.width) v1[i] = row[x + 1];
-                }
+        for (int i = 0; i < 8; ++i) {
+            const int wd = base + tid + 256 * i;
+            v0[i] = mu;
+            v1[i] = mu;
+            if (wd < words) {
+                const int yy = div_small(wd, inv_pitch), xw2 = wd - yy * pitch;
+                const float* row = plane + static_cast<size_t>(uy0 + yy) * stride;
+                const int x = ux0 + 2 * xw2;
+                if (x < jb.width) v0[i] = row[x];
+                if (x + 1 < jb.width) v1[i] = row[x + 1];
             }
+        }
+    };
+    float amax = fabsf(mu - mu);  // NaN when mu is
+    for (int base = 0; base < words; base += 8 * 256) {
+        float v0[8], v1[8];
+        load8(base, v0, v1);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int wd = base + tid + 256 * i;
-                if (wd < words) {
-                    const uint32_t lo = mm_half(v0[i] - mu, err, bad), hi = mm_half(v1[i] - mu, err, bad);
-                    reg[wd] = lo | (hi << 16);
-                }
-            }
+        for (int i = 0; i < 8; ++i) {
+            const float a = fabsf(v0[i] - mu), b = fabsf(v1[i] - mu);
+            amax = (a > amax || a != a) ? a : amax;
+            amax = (b > amax || b != b) ? b : amax;
         }
     }
     {
-        const float we = wave_max_f32(err);
-        const bool wb = __builtin_amdgcn_ballot_w64(bad) != 0;
+        const float wm = wave_max_f32(amax);
+        const bool wn = __builtin_amdgcn_ballot_w64(amax != amax) != 0;
         if (lane == 0) {
-            reinterpret_cast<float*>(xw)[wave] = we;
-            xw[4 + wave] = wb ? 1u : 0u;
+            reinterpret_cast<float*>(xw)[wave] = wm;
+            xw[4 + wave] = wn ? 1u : 0u;
         }
     }
     __syncthreads();
-    const float E = 16.f *
-                    fmaxf(fmaxf(reinterpret_cast<float*>(xw)[0], reinterpret_cast<float*>(xw)[1]),
-                          fmaxf(reinterpret_cast<float*>(xw)[2], reinterpret_cast<float*>(xw)[3])) *
-                    1.00001f;
-    const bool anybad = (xw[4] | xw[5] | xw[6] | xw[7]) != 0u;
+    const float bmax = fmaxf(fmaxf(reinterpret_cast<float*>(xw)[0], reinterpret_cast<float*>(xw)[1]),
+                             fmaxf(reinterpret_cast<float*>(xw)[2], reinterpret_cast<float*>(xw)[3]));
+    // bmax < 2^e: S = 13 - e puts it below 2^13 (|-2 h| < 2^15 fits fp16)
+    int ex = 0;
+    (void)frexpf(bmax, &ex);
+    const int S = bmax > 0.f ? min(13 - ex, 40) : 40;
+    const bool anybad = (xw[4] | xw[5] | xw[6] | xw[7]) != 0u || !(bmax < 8192.f) || S < 0;
+    __syncthreads();
+    if (!anybad) {
+        const float sc = ldexpf(1.f, S);
+        for (int base = 0; base < words; base += 8 * 256) {
+            float v0[8], v1[8];
+            load8(base, v0, v1);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int wd = base + tid + 256 * i;
+                if (wd < words) reg[wd] = mm_half((v0[i] - mu) * sc) | (mm_half((v1[i] - mu) * sc) << 16);
+            }
+        }
+    }
     __syncthreads();
 
     if (!anybad) {
@@ -711,10 +735,11 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
         int rtop[4], rbot[4], rcy[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
+            // uniform: the row tests below branch on scalars
             const MmWin w = mm_win(jb, i0, j0 + min(j, nrow - 1));
-            rcy[j] = w.cy;
-            rtop[j] = j < nrow ? w.top : (1 << 30);
-            rbot[j] = j < nrow ? w.bottom : -1;
+            rcy[j] = __builtin_amdgcn_readfirstlane(w.cy);
+            rtop[j] = __builtin_amdgcn_readfirstlane(j < nrow ? w.top : (1 << 30));
+            rbot[j] = __builtin_amdgcn_readfirstlane(j < nrow ? w.bottom : -1);
         }
 
         // Each wave takes the candidate rows y = uy0 + wave (mod 4).
@@ -746,10 +771,9 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                     for (int j = 0; j < 4; ++j) {
                         // reference row j's window holds row y: uniform
                         if (y >= rtop[j] && y <= rbot[j]) {
-                            // pass A: d' = |h_c|^2 + c; pass B:
-                            // d' - C1 (|h_r|^2 + |h_c|^2) <= W as
-                            // (1 - C1) |h_c|^2 + c <= W + C1 |h_r|^2
-                            float d = SURV ? fmaf(ncx, 1.f - kMmC1, c[j]) : ncx + c[j];
+                            // pass A: U / (1 + t) up to a per-reference
+                            // constant; pass B: L / (1 - t) likewise
+                            float d = fmaf(ncx, SURV ? kMmSlopeL : kMmSlopeU, c[j]);
                             if (y == rcy[j]) d = xs ? kMmInf : d;  // the reference itself
                             if constexpr (SURV)
                                 sv[j] = d <= wv[j];
@@ -820,11 +844,13 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                     if (!(td < kMmInf)) {
                         wt = 3.402823466e38f;  // every finite d' (masked pairs are +inf)
                     } else {
-                        const float d1 = kMmC1 * 1.0002f * (3.f * nrm + 2.f * fmaxf(td, 0.f));
-                        const float ru = sqrtf(fmaxf(td + d1, 0.f)) + E;
-                        const float U = ru * ru * 1.0001f;
-                        const float R = sqrtf(U * 1.0002f) + E;
-                        wt = R * R * 1.0001f;
+                        // U = (1 + t) td + beta_U bounds the K-th key;
+                        // a candidate survives while its L = (1 - t) v -
+                        // beta_L does not exceed U.
+                        const float betaU = (1.f + kMmT) * kMmC1 * nrm + kMmGammaU * (nrm + 2.f);
+                        const float betaL = (1.f - kMmT) * kMmC1 * nrm + kMmGammaL * (nrm + 2.f);
+                        const float U = ((1.f + kMmT) * td + betaU) * 1.0001f;
+                        wt = (U * 1.0001f + betaL) / (1.f - kMmT) * 1.00001f;
                     }
                 }
                 wl[4 * g + wave] = wt;
@@ -832,7 +858,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 4; ++j) wv[j] = wl[4 * g + j] + kMmC1 * nr[j];
+        for (int j = 0; j < 4; ++j) wv[j] = wl[4 * g + j];
         pass(std::true_type{});
     }
     __syncthreads();
