@@ -749,23 +749,44 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) s[j][k] = kMmInf;
         float wv[4] = {0.f, 0.f, 0.f, 0.f};
-        auto pass = [&](auto surv_tag) {
+        // One row step over NC tile columns at once: independent chains
+        // (fragment loads, norms, two MFMAs, epilogue) for the scheduler to
+        // interleave.
+        auto pass_cols = [&](auto surv_tag, auto nc_tag, int t0) {
             constexpr bool SURV = decltype(surv_tag)::value;
-            for (int t = 0; t < nt; ++t) {
-                const int col = 16 * t + n;
-                const int x = ux0 + col;
-                const bool xv = x >= cleft && x <= cright;
-                const bool xs = x == ccx;
-                int y = uy0 + wave;
-                if (y > uy1) break;
-                mm_h16x8 F0 = mm_frag(reg, pitch, y - uy0 + g, col);
-                float T0 = mm_rows_sum(mm_sq(F0));
-                for (; y <= uy1; y += 4) {
-                    const mm_h16x8 F1 = mm_frag(reg, pitch, y - uy0 + 4 + g, col);
-                    const float T1 = mm_rows_sum(mm_sq(F1));
-                    mm_f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, F0, nr, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, F1, c, 0, 0, 0);
-                    const float ncx = xv ? T0 + T1 : kMmInf;
+            constexpr int NC = decltype(nc_tag)::value;
+            int col[NC];
+            bool xv[NC], xs[NC];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                col[q] = 16 * (t0 + q) + n;
+                const int x = ux0 + col[q];
+                xv[q] = x >= cleft && x <= cright;
+                xs[q] = x == ccx;
+            }
+            int y = uy0 + wave;
+            if (y > uy1) return;
+            mm_h16x8 F0[NC];
+            float T0[NC];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                F0[q] = mm_frag(reg, pitch, y - uy0 + g, col[q]);
+                T0[q] = mm_rows_sum(mm_sq(F0[q]));
+            }
+            for (; y <= uy1; y += 4) {
+                mm_h16x8 F1[NC];
+                float T1[NC];
+                mm_f32x4 c[NC];
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    F1[q] = mm_frag(reg, pitch, y - uy0 + 4 + g, col[q]);
+                    T1[q] = mm_rows_sum(mm_sq(F1[q]));
+                    c[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, F0[q], nr, 0, 0, 0);
+                    c[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, F1[q], c[q], 0, 0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    const float ncx = xv[q] ? T0[q] + T1[q] : kMmInf;
                     bool sv[4] = {false, false, false, false};
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
@@ -773,8 +794,8 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                         if (y >= rtop[j] && y <= rbot[j]) {
                             // pass A: U / (1 + t) up to a per-reference
                             // constant; pass B: L / (1 - t) likewise
-                            float d = fmaf(ncx, SURV ? kMmSlopeL : kMmSlopeU, c[j]);
-                            if (y == rcy[j]) d = xs ? kMmInf : d;  // the reference itself
+                            float d = fmaf(ncx, SURV ? kMmSlopeL : kMmSlopeU, c[q][j]);
+                            if (y == rcy[j]) d = xs[q] ? kMmInf : d;  // the reference itself
                             if constexpr (SURV)
                                 sv[j] = d <= wv[j];
                             else
@@ -783,7 +804,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                     }
                     if constexpr (SURV) {
                         if (__builtin_amdgcn_ballot_w64(sv[0] || sv[1] || sv[2] || sv[3]) != 0) {
-                            const int u = (y - uy0) * UW + col;
+                            const int u = (y - uy0) * UW + col[q];
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
                                 if (sv[j]) {
@@ -794,10 +815,15 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
                             }
                         }
                     }
-                    F0 = F1;
-                    T0 = T1;
+                    F0[q] = F1[q];
+                    T0[q] = T1[q];
                 }
             }
+        };
+        auto pass = [&](auto surv_tag) {
+            int t = 0;
+            for (; t + 2 <= nt; t += 2) pass_cols(surv_tag, std::integral_constant<int, 2>{}, t);
+            if (t < nt) pass_cols(surv_tag, std::integral_constant<int, 1>{}, t);
         };
 
         pass(std::false_type{});
