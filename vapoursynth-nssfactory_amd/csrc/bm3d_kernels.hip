@@ -550,12 +550,23 @@ __device__ inline uint32_t mm_exact(const float* plane, size_t stride, int rx, i
     return dist_key(acc);
 }
 
-// Region row pitch in words: >= 8 nt + 5 (16 nt + 8 halves and the fifth word
-// of an odd-column read), = 16 mod 32 so that the four 16-lane rows of a
-// fragment read fall on disjoint banks.
+// Region row pitch in words: the smallest p >= 8 nt + 5 (16 nt + 8 halves
+// and the fifth word of an odd-column read) at which the four 16-lane rows
+// of a fragment read (each within 9 consecutive words, rows p apart) fall on
+// disjoint banks, for 32 banks (two rows per half-wave) and for 64.
 __host__ __device__ inline int mm_pitch(int nt) {
-    const int p = 8 * nt + 5;
-    return p + ((16 - p % 32) + 32) % 32;
+    for (int p = 8 * nt + 5;; ++p) {
+        const int a = p % 32;
+        if (a < 9 || a > 23) continue;
+        const int o[4] = {0, p % 64, (2 * p) % 64, (3 * p) % 64};
+        bool ok = true;
+        for (int i = 0; i < 4; ++i)
+            for (int j = i + 1; j < 4; ++j) {
+                const int d = ((o[i] - o[j]) % 64 + 64) % 64;
+                ok = ok && d >= 9 && d <= 55;
+            }
+        if (ok) return p;  // p = 16 mod 64 always qualifies
+    }
 }
 
 struct MmWin {
