@@ -634,13 +634,17 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
     const int pitch = mm_pitch(nt);
     const int reg_words = max(RH * pitch, 16 * LIST);
     uint32_t* reg = mm_lds;
-    uint16_t* list = reinterpret_cast<uint16_t*>(mm_lds + reg_words);
-    int* cnt = reinterpret_cast<int*>(mm_lds + reg_words + 8 * LIST);  // 16
+    // One area serves, in turn, the staging reductions and the pass-A
+    // exchange (xw), the candidate lists (pass B, ranking), and the
+    // fallback's keys.
+    constexpr int kUnion = 8 * LIST > 2048 ? 8 * LIST : 2048;           // words
+    uint32_t* xw = mm_lds + reg_words;
+    uint16_t* xch = reinterpret_cast<uint16_t*>(xw);                    // [wave][j][lane][4] bf16
+    uint16_t* list = reinterpret_cast<uint16_t*>(xw);                   // [16][LIST]
+    int* cnt = reinterpret_cast<int*>(mm_lds + reg_words + kUnion);     // 16
     int* pre = cnt + 16;                                                // 17 (+1 keeps 8-byte alignment)
     int* kmv = pre + 18;                                                // 16
     float* wl = reinterpret_cast<float*>(kmv + 16);                     // 16
-    uint32_t* xw = reinterpret_cast<uint32_t*>(wl + 16);                // 2048 words
-    uint16_t* xch = reinterpret_cast<uint16_t*>(xw);                    // [wave][j][lane][4] bf16
 
     // Per-reference counts K = min(G - 1, window - 1); counters cleared.
     if (tid < 16) {
@@ -955,6 +959,7 @@ __global__ __launch_bounds__(256) void match_mfma_kernel(MatchTable tab) {
         jb.counts[ref_id] = 1 + kmv[tid];
     }
 
+    __syncthreads();  // the lists' area becomes the fallback's keys
     // Whole windows, exactly, for the references the bound could not serve
     // (wave w takes references w, w + 4, ...): chunks of 64 candidates merged
     // by rank into the running G - 1 best.
@@ -2806,7 +2811,7 @@ size_t match_mfma_lds_bytes(int step, int range, int group) {
     const int nt = (span + 15) >> 4;
     const size_t list = static_cast<size_t>(mm_list(group));
     const size_t reg = std::max(static_cast<size_t>(span + 7) * mm_pitch(nt), 16 * list);
-    return (reg + 8 * list + 66 + 2048) * 4;
+    return (reg + std::max(8 * list, static_cast<size_t>(2048)) + 66) * 4;
 }
 
 bool match_mfma_fits(int block, int group, int step, int range, int width, int height) {
