@@ -174,3 +174,20 @@ def test_mfma_temporal_identical(paths, kw):
     for fa, fb in zip(*outs):
         for a, b in zip(fa, fb):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), kw
+
+
+def test_mfma_strided_plane(paths):
+    """A plane whose row stride exceeds its width (a view into a wider
+    buffer) gives the same records on both matchers."""
+    import torch
+    gpu = paths
+    wide = _plane("noise", 200, 110, seed=11)
+    full = torch.from_numpy(wide).cuda()
+    view = full[:, 7:7 + 150]  # width 150, stride 200
+    assert view.stride(0) == 200
+    out = []
+    for path in (gpu.MATCH_FP32, gpu.MATCH_MFMA):
+        gpu.set_match_path(path)
+        rec, cnt = gpu.spatial_match(view, block=8, step=4, bm_range=20, group=16)
+        out.append((_host(rec), _host(cnt)))
+    _assert_same(out[0], out[1], "strided 150x110 (stride 200) s4 r20 g16")
