@@ -246,6 +246,52 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
  * (:1690 spatial_match_tensor_only). */
 int nss_hip_set_match_path(int path);
 
+/* ---- WNNM ---------------------------------------------------------------
+ * nss.WNNM, the sibling group filter on the same matcher
+ * (src/host/filter_wnnm.cpp; the device form src/cuda/wnnm/filter.cpp:16-70):
+ * each reference's group is matched as for BM3D, its singular values shrunk
+ * by weighted nuclear-norm minimisation, and the groups aggregated. */
+
+/* Resolved arguments (include/nss/params/wnnm.hpp:9-20; defaults
+ * include/nss/params.hpp:61-75). sigma is the user value on the 0-255 scale. */
+typedef struct nss_hip_wnnm_params {
+    float sigma[3];
+    int block_size;
+    int block_step;
+    int group_size;
+    int bm_range;
+    int radius;
+    int ps_num;
+    int ps_range;
+    int residual;        /* centre each group's rows before the shrinkage */
+    int adaptive;        /* adaptive_aggregation: weight 1 / kept values */
+    int has_rclip;       /* an rclip was given: matching runs on it */
+    int memory_limit_mb; /* 0 = not given */
+    int device_id;
+} nss_hip_wnnm_params;
+
+typedef struct nss_hip_wnnm nss_hip_wnnm;
+
+/* Argument resolution of <ns>.WNNM (src/frontend/wnnm_args.cpp:8-29
+ * parse_wnnm, then src/cuda/wnnm/filter.cpp:26-33: rclip must match clip,
+ * validate_group_planes of src/frontend/validate.hpp:19-25), with the first
+ * error's text in nss_hip_last_error(). This build runs radius 0 (spatial)
+ * and the matcher's block sizes 1, 2, 4, 8, 12 and 16; other values the
+ * reference accepts are rejected here with a message that says so. */
+int nss_hip_wnnm_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
+                       const nss_hip_arg* args, int nargs, const char* ns, nss_hip_wnnm_params* out);
+/* Instance for a clip of `info` (the reference's create,
+ * src/cuda/wnnm/filter.cpp:16-70): device buffers for every plane. */
+int nss_hip_wnnm_create(const nss_hip_wnnm_params* params, const nss_hip_video_info* info, int device,
+                        nss_hip_wnnm** out);
+/* One frame (src/host/filter_wnnm.cpp:163-260 wnnmGetFrame at radius 0):
+ * device planes src[p] (stride in floats), rclip[p] or NULL (match on src),
+ * dst[p]. A plane with sigma 0 is copied. Asynchronous on stream. */
+int nss_hip_wnnm_process(nss_hip_wnnm* handle, const float* const* src, const int* src_stride,
+                         const float* const* rclip, const int* rclip_stride, float* const* dst,
+                         const int* dst_stride, void* stream);
+int nss_hip_wnnm_destroy(nss_hip_wnnm* handle);
+
 /* ---- Serving loop ------------------------------------------------------
  * No counterpart in the reference: its filters answer one VapourSynth frame
  * request at a time (src/host/filter_bm3d.cpp bm3dGetFrame). This loop is

@@ -499,6 +499,26 @@ int orc_bm3d_plane(const float* const* match, const float* const* src, const flo
                              group, step, range, ps_num, ps_range, radius, wiener, center, frame_count, match_dump);
 }
 
+// Every reference of one plane's raster grid (raster_grid above): per
+// reference `group` slots of (x, y) in match order and its count, as
+// spatial_match_batch writes them for WNNM (src/host/filter_wnnm.cpp:96-104).
+int orc_spatial_match_plane(const float* plane, int stride, int width, int height, int block, int step, int range,
+                            int group, int fp64, int* xy, int* counts) {
+    std::vector<int> xs, ys;
+    raster_grid(width, height, block, step, xs, ys);
+    Match ms[kMaxGroup];
+    for (std::size_t r = 0; r < xs.size(); ++r) {
+        const int n = fp64 ? spatial_match<double>(plane, stride, width, height, xs[r], ys[r], block, range, group, ms)
+                           : spatial_match<float>(plane, stride, width, height, xs[r], ys[r], block, range, group, ms);
+        counts[r] = n;
+        for (int i = 0; i < n; ++i) {
+            xy[(r * group + i) * 2] = ms[i].x;
+            xy[(r * group + i) * 2 + 1] = ms[i].y;
+        }
+    }
+    return static_cast<int>(xs.size());
+}
+
 int orc_max_group(void) { return kMaxGroup; }
 
 }  // extern "C"

@@ -376,3 +376,112 @@ def psnr(a, b, peak=1.0):
     b = np.asarray(b, np.float64)
     mse = float(np.mean((a - b) ** 2))
     return float("inf") if mse == 0 else 10.0 * np.log10(peak * peak / mse)
+
+
+# ---------------------------------------------------------------------------
+# WNNM (the sibling group filter on the same matcher), restated in float64.
+# Parity pins: the reference's zero-rank plugin test (its known answer, sigma
+# 255 with residual 1), sigma 0 (every singular value kept whole, so the
+# output is the input), hand-computed shrink values, and a per-group scalar
+# restatement through the Gram eigenproblem instead of the SVD.
+# ---------------------------------------------------------------------------
+WNNM_DEFAULTS = dict(block=8, step=8, group=8, bm_range=7, residual=0, adaptive=1)  # include/nss/params.hpp:61-75
+
+
+def match_plane(plane, block, step, bm_range, group, fp64=False):
+    """Every raster reference's matches: (xy[nref, group, 2], counts[nref])
+    (src/host/filter_wnnm.cpp:96-104 spatial_match_batch on the match plane)."""
+    plane = np.ascontiguousarray(plane, np.float32)
+    h, w = plane.shape
+    L = lib()
+    if not getattr(L, "_plane_types", False):
+        P = ctypes.POINTER
+        ci = ctypes.c_int
+        L.orc_spatial_match_plane.argtypes = [P(ctypes.c_float), ci, ci, ci, ci, ci, ci, ci, ci, P(ci), P(ci)]
+        L.orc_spatial_match_plane.restype = ci
+        L._plane_types = True
+    nref = len(raster_grid(w, h, block, step))
+    xy = np.zeros((max(nref, 1), group, 2), np.int32)
+    counts = np.zeros(max(nref, 1), np.int32)
+    n = L.orc_spatial_match_plane(_fp(plane), w, w, h, block, step, bm_range, group, int(fp64), _ip(xy), _ip(counts))
+    assert n == nref
+    return xy[:nref], counts[:nref]
+
+
+def sv_shrink(s, constant, start):
+    """src/cpu/common/sv_shrink.cpp:9-24 on a descending spectrum: values
+    before `start` stay; each later value s with s^2 - constant > 0 becomes
+    (s + sqrt(s^2 - constant)) / 2, and the walk stops at the first that is
+    not. Returns (shrunk values, kept count)."""
+    s = np.array(s, np.float64)
+    k = min(max(start, 0), len(s))
+    while k < len(s):
+        tmp = s[k] * s[k] - constant
+        if not tmp > 0.0:
+            break
+        s[k] = (s[k] + np.sqrt(tmp)) * 0.5
+        k += 1
+    s[k:] = 0.0
+    return s, k
+
+
+def wnnm_group(A, sigma, residual, adaptive):
+    """One group matrix A (m = block^2 rows, n = matched patches as columns):
+    src/cpu/wnnm/shrink.cpp:13-49 wnnm_shrink with src/cpu/finishers.hpp:12-36
+    finish_wnnm. Optional row centering (src/cpu/common/group_center.cpp:15-58),
+    economy SVD, sv_shrink with constant 8 sqrt(2n) sigma^2 from place
+    residual ? 0 : 1, X = U S' V^T, centre added back. Returns (X, weight):
+    1 / kept when adaptive and kept > 0, else 1."""
+    A = np.array(A, np.float64)
+    m, n = A.shape
+    mean = A.mean(axis=1, keepdims=True) if residual else np.zeros((m, 1))
+    U, s, Vt = np.linalg.svd(A - mean, full_matrices=False)
+    constant = 8.0 * np.sqrt(2.0 * n) * sigma * sigma
+    shrunk, kept = sv_shrink(s, constant, 0 if residual else 1)
+    X = (U * shrunk) @ Vt + mean
+    weight = 1.0 / kept if adaptive and kept > 0 else 1.0
+    return X, weight
+
+
+def wnnm_plane(src, sigma_user, block=8, step=8, group=8, bm_range=7, residual=0, adaptive=1, match=None,
+               want_matches=False):
+    """One spatial plane of nss.WNNM (src/host/filter_wnnm.cpp:44-160
+    process_plane_batched, radius 0): match on `match` (rclip) or `src`,
+    pack each reference's k matches from `src` (src/cpu/bm/pack.cpp:23-40),
+    shrink, add weight * X into num and weight into den over each patch
+    (aggregate_add), then dst = den > 1e-12 ? num / den : src
+    (src/cpu/bm/agg.cpp:17-42). sigma is the user value (divided by 255,
+    src/cuda/wnnm/filter.cpp:51)."""
+    src = np.ascontiguousarray(src, np.float32)
+    h, w = src.shape
+    sigma = float(np.float32(sigma_user) / np.float32(255.0))
+    mplane = src if match is None else np.ascontiguousarray(match, np.float32)
+    xy, counts = match_plane(mplane, block, step, bm_range, group)
+    num = np.zeros((h, w), np.float64)
+    den = np.zeros((h, w), np.float64)
+    dy, dx = np.divmod(np.arange(block * block), block)
+    s64 = src.astype(np.float64)
+    for r in range(len(counts)):
+        k = int(counts[r])
+        if k <= 0:
+            continue
+        cols = [s64[xy[r, j, 1] + dy, xy[r, j, 0] + dx] for j in range(k)]
+        X, wt = wnnm_group(np.stack(cols, axis=1), sigma, residual, adaptive)
+        for j in range(k):
+            num[xy[r, j, 1] + dy, xy[r, j, 0] + dx] += wt * X[:, j]
+            den[xy[r, j, 1]:xy[r, j, 1] + block, xy[r, j, 0]:xy[r, j, 0] + block] += wt
+    out = np.where(den > 1e-12, num / np.where(den > 1e-12, den, 1.0), s64).astype(np.float32)
+    return (out, xy, counts) if want_matches else out
+
+
+def wnnm_frame(planes, sigma, block=8, step=8, group=8, bm_range=7, residual=0, adaptive=1, rclip=None):
+    """All planes of one frame; a plane with sigma 0 is copied
+    (src/host/filter_wnnm.cpp:200-260)."""
+    out = []
+    for i, p in enumerate(planes):
+        if sigma[i] == 0:
+            out.append(np.array(p, np.float32))
+            continue
+        out.append(wnnm_plane(p, sigma[i], block, step, group, bm_range, residual, adaptive,
+                              None if rclip is None else rclip[i]))
+    return out

@@ -75,3 +75,34 @@ def test_match_path_and_resources_argument_checks():
     b = (ctypes.c_longlong * 6)()
     v = ctypes.c_longlong()
     assert lib.nss_hip_bm3d_resources(None, b, ctypes.byref(v), ctypes.byref(v)) == nss_hip.INVALID_INPUT
+
+
+def test_wnnm_parse_messages_and_null_checks():
+    """nss_hip.WNNM's argument errors carry the reference's text
+    (src/frontend/wnnm_args.cpp:8-29, src/frontend/validate.hpp:19-25);
+    the entry points reject null handles without touching a device."""
+    import nss_hip
+    lib = nss_hip.lib()
+    p = nss_hip.parse_wnnm("YUV420PS", 64, 48)
+    assert (p.block_size, p.block_step, p.group_size, p.bm_range, p.residual, p.adaptive) == (8, 8, 8, 7, 0, 1)
+    assert list(p.sigma) == [3.0, 3.0, 3.0]
+    cases = [
+        (dict(block_size=17), "nss_hip.WNNM: invalid block_size/group_size/block_step/radius"),
+        (dict(group_size=33), "nss_hip.WNNM: invalid block_size/group_size/block_step/radius"),
+        (dict(block_step=9), "nss_hip.WNNM: invalid block_size/group_size/block_step/radius"),
+        (dict(bm_range=65), "nss_hip.WNNM: bm_range must be in [1, 64]"),
+        (dict(ps_num=9), "nss_hip.WNNM: invalid ps_num/ps_range"),
+        (dict(sigma=[-1.0]), "nss: sigma must be non-negative"),
+        (dict(temporal_mode="fused"), "nss_hip.WNNM: temporal_mode=fused is not supported; use rolling or legacy"),
+    ]
+    for kw, text in cases:
+        with pytest.raises(nss_hip.Error) as e:
+            nss_hip.parse_wnnm("GRAYS", 64, 48, **kw)
+        assert str(e.value) == text
+    with pytest.raises(nss_hip.Error, match="selected plane is smaller than block_size"):
+        nss_hip.parse_wnnm("GRAYS", 6, 6)
+    with pytest.raises(nss_hip.Error, match="not available in this build"):
+        nss_hip.parse_wnnm("GRAYS", 64, 48, radius=1)
+    assert lib.nss_hip_wnnm_create(None, None, 0, None) == nss_hip.INVALID_INPUT
+    assert lib.nss_hip_wnnm_process(None, None, None, None, None, None, None, None) == nss_hip.INVALID_INPUT
+    assert lib.nss_hip_wnnm_destroy(None) == nss_hip.OK

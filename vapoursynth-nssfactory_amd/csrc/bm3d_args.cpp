@@ -366,6 +366,36 @@ void parse_bm3d(const ArgMap& in, const nss_hip_video_info& vi, const nss_hip_vi
     check_planes(vi, ns, p);
 }
 
+// ---- WNNM (src/frontend/wnnm_args.cpp:8-29, include/nss/params.hpp:61-75) ----
+constexpr int kWnnmMaxBlock = 16, kWnnmMaxGroup = 32, kWnnmDefaultBlock = 8, kWnnmDefaultStep = 8;
+constexpr int kWnnmDefaultGroup = 8, kWnnmDefaultRange = 7, kWnnmDefaultResidual = 0, kWnnmDefaultAdaptive = 1;
+constexpr float kWnnmDefaultSigma = 3.0f;
+
+[[noreturn]] void fail_wnnm(const std::string& ns, const std::string& message) {
+    throw Invalid(ns + ".WNNM: " + message);
+}
+
+void parse_wnnm(const ArgMap& in, const nss_hip_video_info& vi, const std::string& ns, nss_hip_wnnm_params& p) {
+    if (!is_const_32f(vi)) fail_wnnm(ns, "constant Gray/YUV/RGB 32-bit float required");
+    map_float_array(in, "sigma", p.sigma, vi.num_planes, kWnnmDefaultSigma);
+    for (int i = vi.num_planes; i < 3; ++i) p.sigma[i] = 0.f;
+    p.block_size = map_int(in, "block_size", kWnnmDefaultBlock);
+    p.block_step = map_int(in, "block_step", kWnnmDefaultStep);
+    p.group_size = map_int(in, "group_size", kWnnmDefaultGroup);
+    p.bm_range = map_int(in, "bm_range", kWnnmDefaultRange);
+    p.radius = map_int(in, "radius", 0);
+    p.ps_num = map_int(in, "ps_num", kBmDefaultPsNum);
+    p.ps_range = map_int(in, "ps_range", kBmDefaultPsRange);
+    p.residual = map_int(in, "residual", kWnnmDefaultResidual);
+    p.adaptive = map_int(in, "adaptive_aggregation", kWnnmDefaultAdaptive);
+    if (p.block_size < 1 || p.block_size > kWnnmMaxBlock || p.group_size < 1 || p.group_size > kWnnmMaxGroup ||
+        p.block_step < 1 || p.block_step > p.block_size || p.radius < 0 || p.radius > kBmMaxRadius)
+        fail_wnnm(ns, "invalid block_size/group_size/block_step/radius");
+    if (p.bm_range < 1 || p.bm_range > kBmMaxRange) fail_wnnm(ns, "bm_range must be in [1, 64]");
+    if (p.ps_num < 1 || p.ps_num > p.group_size || p.ps_range < 1 || p.ps_range > kBmMaxRange)
+        fail_wnnm(ns, "invalid ps_num/ps_range");
+}
+
 }  // namespace
 
 extern "C" {
@@ -433,6 +463,59 @@ int nss_hip_bm3d_output_info(const nss_hip_bm3d_params* p, const nss_hip_video_i
         out->height = static_cast<int>(h);
     }
     return NSS_HIP_OK;
+}
+
+// <ns>.WNNM's create-time checks in the reference's order: numeric
+// representability and memory_limit_mb (src/frontend/validate.hpp:33-45),
+// temporal_mode (src/frontend/temporal_args.cpp:11-28), parse_wnnm, rclip
+// against clip, validate_group_planes (src/frontend/validate.hpp:19-25); then
+// this build's limits (radius 0; the matcher's block sizes).
+int nss_hip_wnnm_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
+                       const nss_hip_arg* args, int nargs, const char* ns_c, nss_hip_wnnm_params* out) {
+    const std::string ns = ns_c ? ns_c : "nss_hip";
+    if (!clip_info || !out || (nargs > 0 && !args)) {
+        nsship::g_last_error = ns + ".WNNM: null argument";
+        return NSS_HIP_INVALID_INPUT;
+    }
+    try {
+        const ArgMap in(args, nargs);
+        validate_numeric(in, ns);
+        nss_hip_wnnm_params p{};
+        int err = 0;
+        const std::int64_t mb = in.get_int("memory_limit_mb", 0, &err);
+        if (!err && (mb < 1 || mb > 1048576)) throw Invalid(ns + ": memory_limit_mb must be in [1, 1048576]");
+        p.memory_limit_mb = err ? 0 : static_cast<int>(mb);
+        const char* mode = in.get_data("temporal_mode", &err);
+        std::string m;
+        if (!err && mode) {
+            m = mode;
+            for (char& c : m)
+                if (c >= 'A' && c <= 'Z') c = static_cast<char>(c - 'A' + 'a');
+        }
+        if (m == "fused") fail_wnnm(ns, "temporal_mode=fused is not supported; use rolling or legacy");
+        if (!m.empty() && m != "rolling" && m != "legacy") fail_wnnm(ns, "temporal_mode must be rolling or legacy");
+        parse_wnnm(in, *clip_info, ns, p);
+        if (rclip_info && !same_video(*clip_info, *rclip_info)) fail_wnnm(ns, "rclip must match clip");
+        p.has_rclip = rclip_info != nullptr;
+        for (int i = 0; i < clip_info->num_planes; ++i) {
+            if (p.sigma[i] < 0.f) throw Invalid("nss: sigma must be non-negative");
+            if (p.sigma[i] > 0.f &&
+                (plane_width(*clip_info, i) < p.block_size || plane_height(*clip_info, i) < p.block_size))
+                throw Invalid("nss: selected plane is smaller than block_size");
+        }
+        if (p.radius > 0) fail_wnnm(ns, "radius > 0 (temporal WNNM) is not available in this build; use radius=0");
+        if (!allowed_block(p.block_size) || p.block_size > kWnnmMaxBlock)
+            fail_wnnm(ns, "block_size must be one of 1, 2, 4, 8, 12, 16 in this build");
+        const std::int64_t device = in.get_int("device_id", 0, &err);
+        if (!err && (device < 0 || device > INT32_MAX)) fail_wnnm(ns, "device_id must be >= 0");
+        p.device_id = err ? 0 : static_cast<int>(device);
+        *out = p;
+        nsship::g_last_error.clear();
+        return NSS_HIP_OK;
+    } catch (const std::exception& e) {
+        nsship::g_last_error = e.what();
+        return NSS_HIP_INVALID_INPUT;
+    }
 }
 
 }  // extern "C"

@@ -260,6 +260,27 @@ struct TFinishTable {
     TFinishJob job[3];
 };
 
+// WNNM group shrinkage (src/cuda/wnnm/kernels.hpp:23-36): one wave per
+// reference of one plane; patches from src at the matched positions, sums in
+// the atomic path's frame-sized fixed-point layout (den at patch corners).
+struct WnnmJob {
+    const float* src;
+    int width, height, stride;
+    int nref, group, block;
+    float sigma;        // user sigma / 255
+    int residual, adaptive;
+    int wg_begin;
+    const MatchRec* matches;  // nref * group slots
+    const int* counts;
+    unsigned long long* num;
+    unsigned long long* den;
+};
+
+struct WnnmTable {
+    int n;
+    WnnmJob job[kMaxJobs];
+};
+
 struct ZeroJob {
     unsigned long long* p;
     long long count;

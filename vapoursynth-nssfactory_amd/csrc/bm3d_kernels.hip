@@ -2791,6 +2791,230 @@ __global__ __launch_bounds__(256) void zero_kernel(ZeroTable tab) {
 }
 
 // ---------------------------------------------------------------------------
+// WNNM: weighted nuclear-norm shrinkage of each group (src/cpu/wnnm/shrink.cpp:13-49,
+// src/cpu/finishers.hpp:12-36, src/cpu/common/sv_shrink.cpp:9-24). The group
+// matrix A has m = B^2 rows and k matched patches as columns, optionally
+// centred row by row. Its spectrum comes from the k x k Gram matrix
+// A^T A = V diag(s^2) V^T by cyclic Jacobi in round-robin order: every round
+// rotates N/2 disjoint index pairs, so the whole N x N matrix updates at once,
+// one entry per lane and step. The shrunk group is X = A V diag(s'/s) V^T
+// (the product src/cpu/batch.cpp:490-520 forms), added back to the centre.
+// ---------------------------------------------------------------------------
+inline constexpr int kWnnmSweeps = 16;
+// A pair rotates while |g_pq| > tol sqrt(g_pp g_qq) (src/cpu/wnnm/numerics.hpp:10).
+inline constexpr float kWnnmTol = 1e-6f;
+
+// Round r of the round-robin pairing of N indices (N even): N - 1 is paired
+// with r, and i with (2r - i) mod (N - 1) otherwise; over r = 0 .. N - 2
+// every pair meets once.
+template <int N>
+__device__ inline int rr_partner(int i, int r) {
+    if (i == N - 1) return r;
+    if (i == r) return N - 1;
+    int j = 2 * r - i;
+    if (j < 0) j += N - 1;
+    else if (j >= N - 1) j -= N - 1;
+    return j;
+}
+
+// One wave per reference. N = 8, 16 or 32 >= group; indices k .. N - 1 are
+// zero columns that never rotate. LDS: A (N columns of m + 1) | Gram, later
+// the shrink operator W (N x N + 1) | V (N x N + 1) | per-index rotation
+// coefficients, diagonal steps (later the singular values) and gains.
+template <int N>
+__global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
+    extern __shared__ __align__(16) float wn[];
+    constexpr int GP = N + 1;
+    constexpr int E = N * N / 64;
+    constexpr int LOG_N = N == 8 ? 3 : N == 16 ? 4 : 5;
+    const int lane = threadIdx.x;
+    const int lb = xcd_block();
+    const WnnmJob& jb = tab.job[find_job(tab, lb)];
+    const int g = lb - jb.wg_begin;
+    if (g >= jb.nref) return;
+    const int k = min(jb.counts[g], N);
+    if (k <= 0) return;
+    const int B = jb.block, m = B * B, P = m + 1;
+    const MatchRec* rec = jb.matches + static_cast<size_t>(g) * jb.group;
+    float* A = wn;
+    float* G = A + N * P;
+    float* V = G + N * GP;
+    float* cself = V + N * GP;
+    float* cpart = cself + N;
+    float* tstep = cpart + N;
+    float* gain = tstep + N;
+
+    // The lane's pixels e = lane + 64 q of a patch (m <= 256).
+    int soff[4], noff[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int e = lane + 64 * q, ey = e / B, ex = e - ey * B;
+        soff[q] = ey * jb.stride + ex;
+        noff[q] = ey * jb.width + ex;
+    }
+    for (int j = 0; j < N; ++j) {
+        if (j < k) {
+            const float* base = jb.src + static_cast<size_t>(rec[j].y) * jb.stride + rec[j].x;
+            float v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = lane + 64 * q < m ? base[soff[q]] : 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (lane + 64 * q < m) A[j * P + lane + 64 * q] = v[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (lane + 64 * q < m) A[j * P + lane + 64 * q] = 0.f;
+        }
+    }
+    // Row centring (src/cpu/common/group_center.cpp:15-58): the mean of each
+    // pixel over the k patches, subtracted; only the lane's own rows.
+    float mean[4] = {0.f, 0.f, 0.f, 0.f};
+    if (jb.residual) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = lane + 64 * q;
+            if (e < m) {
+                float sum = 0.f;
+                for (int j = 0; j < k; ++j) sum += A[j * P + e];
+                mean[q] = sum / static_cast<float>(k);
+                for (int j = 0; j < k; ++j) A[j * P + e] -= mean[q];
+            }
+        }
+    }
+    __syncthreads();
+
+    // Gram matrix and V = I. G[a][b] and G[b][a] take the same fmaf chain,
+    // so the matrix starts exactly symmetric.
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
+        float acc = 0.f;
+        if (a < k && b < k)
+            for (int r = 0; r < m; ++r) acc = fmaf(A[a * P + r], A[b * P + r], acc);
+        G[a * GP + b] = acc;
+        V[a * GP + b] = a == b ? 1.f : 0.f;
+    }
+    __syncthreads();
+
+    for (int sweep = 0; sweep < kWnnmSweeps; ++sweep) {
+        bool rotated = false;
+        for (int r = 0; r < N - 1; ++r) {
+            // Rotation of pair (p, q), p < q, zeroing g_pq: J = [c s; -s c],
+            // G' = J^T G J, V' = V J (the classical Jacobi angle, with the
+            // tangent formed from scaled coordinates).
+            if (lane < N) {
+                const int i = lane, o = rr_partner<N>(i, r);
+                const int p0 = min(i, o), q0 = max(i, o);
+                float c = 1.f, sn = 0.f, tn = 0.f;
+                if (q0 < k) {
+                    const float app = G[p0 * GP + p0], aqq = G[q0 * GP + q0], apq = G[p0 * GP + q0];
+                    if (apq != 0.f && fabsf(apq) > kWnnmTol * sqrtf(fabsf(app)) * sqrtf(fabsf(aqq))) {
+                        const float delta = (aqq - app) * 0.5f;
+                        const float scale = fmaxf(fabsf(delta), fabsf(apq));
+                        const float x = delta / scale, y = apq / scale;
+                        tn = y / (x + copysignf(sqrtf(x * x + y * y), x));
+                        c = 1.f / sqrtf(1.f + tn * tn);
+                        sn = tn * c;
+                        rotated = true;
+                    }
+                }
+                cself[i] = c;
+                cpart[i] = i == p0 ? -sn : sn;
+                tstep[i] = i == p0 ? -tn : tn;
+            }
+            __syncthreads();
+            float ng[E], nv[E];
+#pragma unroll
+            for (int t = 0; t < E; ++t) {
+                const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
+                const int pa = rr_partner<N>(a, r), pb = rr_partner<N>(b, r);
+                const float ca = cself[a], sa = cpart[a], cb = cself[b], sb = cpart[b];
+                const float gab = G[a * GP + b];
+                float v;
+                if (a == b) {
+                    v = fmaf(tstep[a], G[a * GP + pa], gab);
+                } else if (b == pa) {
+                    v = sa != 0.f ? 0.f : gab;
+                } else {
+                    v = ca * fmaf(cb, gab, sb * G[a * GP + pb]) + sa * fmaf(cb, G[pa * GP + b], sb * G[pa * GP + pb]);
+                }
+                ng[t] = v;
+                nv[t] = fmaf(cb, V[a * GP + b], sb * V[a * GP + pb]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < E; ++t) {
+                const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
+                G[a * GP + b] = ng[t];
+                V[a * GP + b] = nv[t];
+            }
+            __syncthreads();
+        }
+        if (!__any(rotated)) break;
+    }
+
+    // Spectrum and gains: place = rank of s_j in descending order (ties by
+    // index); places below `start` stay, later ones shrink to
+    // (s + sqrt(s^2 - c)) / 2 while s^2 > c, with c = 8 sqrt(2k) sigma^2.
+    if (lane < N) tstep[lane] = lane < k ? sqrtf(fmaxf(G[lane * GP + lane], 0.f)) : 0.f;
+    __syncthreads();
+    bool keep = false;
+    float gj = 0.f;
+    if (lane < k) {
+        const float sj = tstep[lane];
+        int place = 0;
+        for (int i = 0; i < k; ++i) {
+            const float si = tstep[i];
+            place += (si > sj || (si == sj && i < lane)) ? 1 : 0;
+        }
+        const int rank = min(m, k), start = min(jb.residual ? 0 : 1, rank);
+        const float constant = 8.f * sqrtf(2.f * static_cast<float>(k)) * jb.sigma * jb.sigma;
+        if (place < start) {
+            gj = 1.f;
+            keep = true;
+        } else if (place < rank) {
+            const float tmp = sj * sj - constant;
+            if (tmp > 0.f) {
+                gj = ((sj + sqrtf(tmp)) * 0.5f) / sj;
+                keep = true;
+            }
+        }
+    }
+    if (lane < N) gain[lane] = gj;
+    const int kept = static_cast<int>(__popcll(__ballot(keep)));
+    __syncthreads();
+    // W = V diag(gain) V^T over the Gram area (G is no longer read).
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
+        float w = 0.f;
+        if (a < k && b < k)
+            for (int i = 0; i < k; ++i) w = fmaf(V[a * GP + i] * gain[i], V[b * GP + i], w);
+        G[a * GP + b] = w;
+    }
+    __syncthreads();
+
+    // X = A W (+ the centre), weighted into the sums: 1 / kept when adaptive
+    // and kept > 0, else 1 (src/cuda/wnnm/kernels.cu:84-105).
+    const float wt = jb.adaptive && kept > 0 ? 1.f / static_cast<float>(kept) : 1.f;
+    const unsigned long long wfix = to_fixed(wt);
+    for (int j = 0; j < k; ++j) {
+        unsigned long long* cell = jb.num + static_cast<size_t>(rec[j].y) * jb.width + rec[j].x;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = lane + 64 * q;
+            if (e < m) {
+                float x = 0.f;
+                for (int i = 0; i < k; ++i) x = fmaf(A[i * P + e], G[i * GP + j], x);
+                atomicAdd(cell + noff[q], to_fixed(wt * (x + mean[q])));
+            }
+        }
+        if (lane == 0) atomicAdd(jb.den + static_cast<size_t>(rec[j].y) * jb.width + rec[j].x, wfix);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host-side sizing and launchers.
 // ---------------------------------------------------------------------------
 
@@ -3159,6 +3383,39 @@ hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg
         case 32: return launch_tfilter_b<32>(t, total_wg, lds, wiener, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+static int wnnm_pad(int group) { return group <= 8 ? 8 : group <= 16 ? 16 : 32; }
+
+size_t wnnm_lds_bytes(int block, int group) {
+    const size_t n = wnnm_pad(group), m = static_cast<size_t>(block) * block;
+    return 4 * (n * (m + 1) + 2 * n * (n + 1) + 4 * n);
+}
+
+hipError_t launch_wnnm(const WnnmTable& t, int total_wg, hipStream_t s) {
+    int gmax = 0, bmax = 0;
+    for (int j = 0; j < t.n; ++j) {
+        if (t.job[j].group < 1 || t.job[j].group > 32 || t.job[j].block < 1 || t.job[j].block > 16)
+            return hipErrorInvalidValue;
+        gmax = std::max(gmax, t.job[j].group);
+        bmax = std::max(bmax, t.job[j].block);
+    }
+    const size_t lds = wnnm_lds_bytes(bmax, gmax);
+    switch (wnnm_pad(gmax)) {
+        case 8:
+            allow_lds(&wnnm_group_kernel<8>, lds);
+            wnnm_group_kernel<8><<<total_wg, 64, lds, s>>>(t);
+            break;
+        case 16:
+            allow_lds(&wnnm_group_kernel<16>, lds);
+            wnnm_group_kernel<16><<<total_wg, 64, lds, s>>>(t);
+            break;
+        default:
+            allow_lds(&wnnm_group_kernel<32>, lds);
+            wnnm_group_kernel<32><<<total_wg, 64, lds, s>>>(t);
+            break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t upload_dct_tables(const float* host, size_t count) {

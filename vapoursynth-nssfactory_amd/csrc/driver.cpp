@@ -55,6 +55,26 @@ int ensure_tables(int device) {
     return NSS_HIP_OK;
 }
 
+int choose_match_kind(int block, int step, int range, int group, int width, int height, long long nref,
+                      size_t* lds) {
+    const bool run = block <= 12 && match_run_lds_bytes(block, step, range) <= kRunLdsBudget;
+    if (match_path() != 1 && match_mfma_fits(block, group, step, range, width, height) &&
+        (match_path() == 2 || !run || (match_mfma_pays(step, range, group) && nref >= kMfmaMinRefs))) {
+        *lds = match_mfma_lds_bytes(step, range, group);
+        return 3;
+    }
+    if (run) {
+        *lds = match_run_lds_bytes(block, step, range);
+        return 2;
+    }
+    if (match_wave_lds_bytes(block, range, true) <= kLdsLimit) {
+        *lds = match_wave_lds_bytes(block, range, true);
+        return 1;
+    }
+    *lds = match_wave_lds_bytes(block, range, false);
+    return 0;
+}
+
 Grid make_grid(int width, int height, int block, int step) {
     Grid g{0, 0, step};
     if (width < block || height < block || step < 1) return g;
@@ -1525,23 +1545,8 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
     m.counts = out_count;
     const int nref = m.grid.nx * m.grid.ny;
     if (nref == 0) return NSS_HIP_OK;
-    int kind;
     size_t lds;
-    const bool run = block <= 12 && match_run_lds_bytes(block, step, bm_range) <= kRunLdsBudget;
-    if (match_path() != 1 && match_mfma_fits(block, group, step, bm_range, width, height) &&
-        (match_path() == 2 || !run || (match_mfma_pays(step, bm_range, group) && nref >= kMfmaMinRefs))) {
-        kind = 3;
-        lds = match_mfma_lds_bytes(step, bm_range, group);
-    } else if (run) {
-        kind = 2;
-        lds = match_run_lds_bytes(block, step, bm_range);
-    } else if (match_wave_lds_bytes(block, bm_range, true) <= kLdsLimit) {
-        kind = 1;
-        lds = match_wave_lds_bytes(block, bm_range, true);
-    } else {
-        kind = 0;
-        lds = match_wave_lds_bytes(block, bm_range, false);
-    }
+    const int kind = choose_match_kind(block, step, bm_range, group, width, height, nref, &lds);
     const int wg = match_workgroups(kind, m.grid);
     const hipError_t e = launch_match(block, mt, wg, lds, kind, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "match launch");

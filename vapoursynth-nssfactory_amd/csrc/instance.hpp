@@ -51,6 +51,13 @@ hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int tota
                                         hipStream_t s);
 hipError_t launch_finish_temporal_tiles(const TFinishTable& t, int total_wg, hipStream_t s);
 hipError_t launch_vaggregate(const VAggJob& jb, hipStream_t s);
+// WNNM group shrinkage (group <= 32, block <= 16).
+size_t wnnm_lds_bytes(int block, int group);
+hipError_t launch_wnnm(const WnnmTable& t, int total_wg, hipStream_t s);
+// The matcher kind for one plane (3 MFMA, 2 run, 1 wave with the window in
+// LDS, 0 wave from global) and its LDS bytes.
+int choose_match_kind(int block, int step, int range, int group, int width, int height, long long nref,
+                      size_t* lds);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);

@@ -91,6 +91,21 @@ class Params(ctypes.Structure):
         return out
 
 
+class WnnmParams(ctypes.Structure):
+    _fields_ = [("sigma", _arr(ctypes.c_float, 3)), ("block_size", ctypes.c_int), ("block_step", ctypes.c_int),
+                ("group_size", ctypes.c_int), ("bm_range", ctypes.c_int), ("radius", ctypes.c_int),
+                ("ps_num", ctypes.c_int), ("ps_range", ctypes.c_int), ("residual", ctypes.c_int),
+                ("adaptive", ctypes.c_int), ("has_rclip", ctypes.c_int), ("memory_limit_mb", ctypes.c_int),
+                ("device_id", ctypes.c_int)]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
 _lib = None
 
 EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "nss_hip_bm3d_output_info",
@@ -98,7 +113,8 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_spatial_match", "nss_hip_set_match_path", "nss_hip_bm3d_set_tracing",
            "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push", "nss_hip_bm3d_pop", "nss_hip_server_create", "nss_hip_server_open_stream", "nss_hip_server_submit",
            "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy",
-           "nss_hip_bm3d_autotune", "nss_hip_vaggregate", "nss_hip_bm3d_resources")
+           "nss_hip_bm3d_autotune", "nss_hip_vaggregate", "nss_hip_bm3d_resources", "nss_hip_wnnm_parse",
+           "nss_hip_wnnm_create", "nss_hip_wnnm_process", "nss_hip_wnnm_destroy")
 # Contribution identity of legacy temporal frames (frame properties the
 # VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
 # src/frontend/contribution.hpp:12-27).
@@ -154,6 +170,10 @@ def lib():
         L.nss_hip_server_drain.argtypes = [vp]
         L.nss_hip_server_get_stats.argtypes = [vp, P(ServerStats)]
         L.nss_hip_server_destroy.argtypes = [vp]
+        L.nss_hip_wnnm_parse.argtypes = [P(VideoInfo), P(VideoInfo), P(Arg), ci, ctypes.c_char_p, P(WnnmParams)]
+        L.nss_hip_wnnm_create.argtypes = [P(WnnmParams), P(VideoInfo), ci, P(vp)]
+        L.nss_hip_wnnm_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), vp]
+        L.nss_hip_wnnm_destroy.argtypes = [vp]
         for name in EXPORTS[1:]:
             getattr(L, name).restype = ci
         _lib = L
@@ -174,28 +194,38 @@ def video_info(fmt: str, width: int, height: int, frames: int = 1) -> VideoInfo:
     return VideoInfo(cf, 1, 32, sw, sh, np_, width, height, frames)
 
 
-def _signature_types():
+def _signature_types(signature=None):
     types = {}
-    for item in filter(None, BM3D_SIGNATURE.split(";")):
+    for item in filter(None, (signature or BM3D_SIGNATURE).split(";")):
         name, kind, *_ = item.split(":")
         types[name] = kind.rstrip("[]")
     return types
 
 
 _TYPES = _signature_types()
+# nss.WNNM's argument list (src/frontend/wnnm_args.hpp:10-13 kWnnmSignature
+# with NSS_TEMPORAL_SIGNATURE, src/frontend/temporal_args.hpp:20-21) and the
+# device backend's device_id.
+WNNM_SIGNATURE = ("clip:vnode;sigma:float[]:opt;block_size:int:opt;block_step:int:opt;group_size:int:opt;"
+                  "bm_range:int:opt;radius:int:opt;ps_num:int:opt;ps_range:int:opt;residual:int:opt;"
+                  "adaptive_aggregation:int:opt;rclip:vnode:opt;memory_limit_mb:int:opt;"
+                  "temporal_mode:data:opt;rolling_chunk:int:opt;rolling_cache_chunks:int:opt;"
+                  "rolling_cache_limit:int:opt;device_id:int:opt;")
+_WNNM_TYPES = _signature_types(WNNM_SIGNATURE)
 
 
-def _build_args(kwargs):
+def _build_args(kwargs, types=None, name="BM3D"):
     """kwargs -> Arg array, values converted to the signature's type (as
     VapourSynth's invoke does); the buffers are kept alive by the return."""
+    types = types or _TYPES
     keep = []
     args = []
     for key, value in kwargs.items():
         if value is None:
             continue
-        kind = _TYPES.get(key)
+        kind = types.get(key)
         if kind is None:
-            raise Error(f"{NAMESPACE}.BM3D: no argument named {key}")
+            raise Error(f"{NAMESPACE}.{name}: no argument named {key}")
         vals = list(value) if isinstance(value, (list, tuple)) else [value]
         if kind == "int":
             buf = (ctypes.c_int64 * len(vals))(*[int(v) for v in vals])
@@ -210,7 +240,7 @@ def _build_args(kwargs):
             keep.append(data)
             args.append(Arg(key.encode(), b"s", 1, None, None, data, None))
         else:
-            raise Error(f"{NAMESPACE}.BM3D: argument {key} is a clip")
+            raise Error(f"{NAMESPACE}.{name}: argument {key} is a clip")
     arr = (Arg * max(1, len(args)))(*args)
     return arr, len(args), keep
 
@@ -386,6 +416,79 @@ class BM3D:
         _check(lib().nss_hip_bm3d_process(self._h, src, sst, rp, rst, dp, dst, frame, frame_count,
                                           _stream_handle(stream)))
         return out
+
+
+def parse_wnnm(fmt, width, height, frames=1, rclip=False, info=None, **kwargs) -> WnnmParams:
+    """Resolve nss.WNNM arguments; raises Error with the reference's text."""
+    vi = info if info is not None else video_info(fmt, width, height, frames)
+    arr, n, keep = _build_args(kwargs, _WNNM_TYPES, "WNNM")
+    out = WnnmParams()
+    rc = lib().nss_hip_wnnm_parse(ctypes.byref(vi), ctypes.byref(vi) if rclip else None, arr, n,
+                                  NAMESPACE.encode(), ctypes.byref(out))
+    del keep
+    _check(rc)
+    return out
+
+
+class WNNM:
+    """nss.WNNM on the device (src/cuda/wnnm/filter.cpp:16-70): one instance
+    per format and size. Calling it filters one frame: ``planes`` is a list
+    of 2-D float32 CUDA tensors; ``rclip`` (instances made with
+    ``rclip=True``) guides the matching only. Radius 0 (spatial)."""
+
+    def __init__(self, fmt, width, height, frames=1, rclip=False, **kwargs):
+        self.fmt = fmt
+        self.info = video_info(fmt, width, height, frames)
+        self.params = parse_wnnm(fmt, width, height, frames, rclip=rclip, info=self.info, **kwargs)
+        self.device = self.params.device_id
+        h = ctypes.c_void_p()
+        _check(lib().nss_hip_wnnm_create(ctypes.byref(self.params), ctypes.byref(self.info), self.device,
+                                         ctypes.byref(h)))
+        self._h = h
+        cf, sw, sh, n = FORMATS[fmt]
+        self.plane_dims = [(height >> (sh if i else 0), width >> (sw if i else 0)) for i in range(n)]
+
+    def _check_planes(self, planes, what="plane"):
+        import torch
+        n = len(self.plane_dims)
+        if len(planes) != n:
+            raise Error(f"{NAMESPACE}.WNNM: expected {n} {what}s, got {len(planes)}")
+        for i, p in enumerate(planes):
+            if p.dtype != torch.float32 or not p.is_cuda or p.dim() != 2 or p.stride(1) != 1:
+                raise Error(f"{NAMESPACE}.WNNM: {what} {i} must be a row-contiguous 2-D float32 CUDA tensor")
+            if tuple(p.shape) != self.plane_dims[i]:
+                raise Error(f"{NAMESPACE}.WNNM: {what} {i} has shape {tuple(p.shape)}, expected {self.plane_dims[i]}")
+
+    def __call__(self, planes, rclip=None, out=None, stream=None):
+        import torch
+        n = len(self.plane_dims)
+        self._check_planes(planes)
+        if rclip is not None:
+            if not self.params.has_rclip:
+                raise Error(f"{NAMESPACE}.WNNM: this instance was made without rclip")
+            self._check_planes(rclip, "rclip plane")
+        if out is None:
+            out = [torch.empty(self.plane_dims[i], dtype=torch.float32, device=planes[i].device) for i in range(n)]
+        VP, IP = ctypes.c_void_p * n, ctypes.c_int * n
+        src = VP(*[p.data_ptr() for p in planes])
+        sst = IP(*[p.stride(0) for p in planes])
+        rp = VP(*[p.data_ptr() for p in rclip]) if rclip is not None else None
+        rst = IP(*[p.stride(0) for p in rclip]) if rclip is not None else None
+        dp = VP(*[p.data_ptr() for p in out])
+        dst = IP(*[p.stride(0) for p in out])
+        _check(lib().nss_hip_wnnm_process(self._h, src, sst, rp, rst, dp, dst, _stream_handle(stream)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nss_hip_wnnm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def grid_size(width, height, block, step):
