@@ -429,6 +429,83 @@ def large_window_workload(args):
     print(json.dumps(res), flush=True)
 
 
+def wnnm_workload(args):
+    """nss.WNNM (the sibling group filter on the same matchers) on one
+    1920x1080 GRAYS frame per step, sigma 10: the reference's defaults
+    (block 8, step 8, group 8, range 7; include/nss/params.hpp:61-75), and a
+    denser form (step 4, group 16, range 9, residual 1) beside it. The WNNM
+    oracle (float64 SVD, one thread) runs a 640x360 crop as the CPU
+    baseline and the parity check."""
+    import numpy as np
+    import torch
+
+    import nss_hip
+    import oracle
+    import synth
+    w, h = 1920, 1080
+    clean, noisy = synth.noisy_frame("GRAYS", w, h, 10, seed=0)
+    dev = [torch.from_numpy(p).cuda() for p in noisy]
+    stream = torch.cuda.current_stream()
+    forms = {"defaults": dict(sigma=[10.0]),
+             "dense": dict(sigma=[10.0], block_step=4, group_size=16, bm_range=9, residual=1)}
+
+    def run(kw, steps, warmup):
+        inst = nss_hip.WNNM("GRAYS", w, h, **kw)
+        out = [torch.empty_like(p) for p in dev]
+        for _ in range(warmup):
+            inst(dev, out=out)
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            starts[i].record(stream)
+            inst(dev, out=out)
+            ends[i].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
+        got = out[0].cpu().numpy()
+        again = inst(dev)[0].cpu().numpy()
+        inst.close()
+        return wall, lat, got, bool(np.array_equal(got.view(np.uint32), again.view(np.uint32)))
+
+    results = {}
+    for name, kw in forms.items():
+        wall, lat, got, same = run(kw, args.steps, args.warmup)
+        results[name] = {"ms_per_step": round(wall / args.steps * 1e3, 4),
+                         "value": round(w * h * args.steps / wall / 1e6, 3),
+                         "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                         "run_to_run_identical": same,
+                         "psnr_vs_clean": round(oracle.psnr(got, clean[0]), 3)}
+        print(f"# wnnm {name}: {results[name]['ms_per_step']} ms/frame", file=sys.stderr, flush=True)
+    # Parity and the CPU leg on a crop.
+    cw, ch = 640, 360
+    crop = np.ascontiguousarray(noisy[0][:ch, :cw])
+    for name, kw in forms.items():
+        inst = nss_hip.WNNM("GRAYS", cw, ch, **kw)
+        g = inst([torch.from_numpy(crop).cuda()])[0].cpu().numpy()
+        inst.close()
+        t0 = time.perf_counter()
+        want = oracle.wnnm_plane(crop, 10.0, step=kw.get("block_step", 8), group=kw.get("group_size", 8),
+                                 bm_range=kw.get("bm_range", 7), residual=kw.get("residual", 0))
+        dt = time.perf_counter() - t0
+        results[name]["psnr_vs_cpu"] = round(oracle.psnr(g, want), 3)
+        results[name]["cpu_mpix_s"] = round(cw * ch / dt / 1e6, 4)
+    d = results["defaults"]
+    res = {"metric": "1080p WNNM Mpix/s (reference defaults: block 8, step 8, group 8, range 7)",
+           "value": d["value"], "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": d["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "WNNM, sigma 10, radius 0, one 1920x1080 GRAYS frame per step"},
+           "p50_ms": d["p50_ms"], "p99_ms": d["p99_ms"], "forms": results,
+           "cpu_baseline": {"value": d["cpu_mpix_s"], "unit": "Mpix/s", "cores": 1, "kind": "port",
+                            "sample": "the WNNM oracle (numpy, float64 SVD per group) on a 640x360 crop of the "
+                                      "same frame, one thread"}}
+    print(json.dumps(res), flush=True)
+
+
 def reference_profiles_workload(args):
     """The reference's own speed table (docs/api/cuda/README.md:10-22):
     1920x1080 YUV420, sigma 3, frames already on the device, BM3D with its
@@ -665,7 +742,8 @@ def main():
                          "that scales rocprof's FETCH_SIZE / WRITE_SIZE readings")
     ap.add_argument("--seconds", type=float, default=5.0, help="serving: seconds of arrivals")
     ap.add_argument("--workload", choices=["two-stage-1080p", "temporal-1080p", "precision-4k", "serving",
-                                           "single-stage-256", "large-window-1080p", "reference-profiles"],
+                                           "single-stage-256", "large-window-1080p", "reference-profiles",
+                                           "wnnm-1080p"],
                     default="two-stage-1080p", help="two-stage-1080p is the headline; the others print their "
                                                     "own line")
     args = ap.parse_args()
@@ -676,7 +754,7 @@ def main():
 
     defaults = {"two-stage-1080p": (200, 500), "temporal-1080p": (2, 1), "precision-4k": (20, 50),
                 "serving": (0, 0), "single-stage-256": (500, 500), "large-window-1080p": (50, 50),
-                "reference-profiles": (200, 300)}[args.workload]
+                "reference-profiles": (200, 300), "wnnm-1080p": (50, 20)}[args.workload]
     if args.steps is None:
         args.steps = defaults[0]
     if args.warmup is None:
@@ -698,6 +776,8 @@ def main():
         return single_stage_256_workload(args)
     if args.workload == "large-window-1080p":
         return large_window_workload(args)
+    if args.workload == "wnnm-1080p":
+        return wnnm_workload(args)
     if args.workload == "reference-profiles":
         return reference_profiles_workload(args)
     if args.copy_calibration:

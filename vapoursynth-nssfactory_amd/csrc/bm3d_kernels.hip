@@ -2801,8 +2801,14 @@ __global__ __launch_bounds__(256) void zero_kernel(ZeroTable tab) {
 // (the product src/cpu/batch.cpp:490-520 forms), added back to the centre.
 // ---------------------------------------------------------------------------
 inline constexpr int kWnnmSweeps = 16;
-// A pair rotates while |g_pq| > tol sqrt(g_pp g_qq) (src/cpu/wnnm/numerics.hpp:10).
+// A pair rotates while |g_pq| > tol sqrt(g_pp g_qq) (src/cpu/wnnm/numerics.hpp:10)
+// and |g_pq| is above FP32 rounding noise of the matrix, floor * trace: a
+// rotation leaves g_pq at a few ulps of the largest entries, which the
+// relative test alone can keep seeing for a pair of a large and a tiny
+// eigenvalue. An off-diagonal at the floor moves the eigenvalues by about
+// g_pq^2 / gap.
 inline constexpr float kWnnmTol = 1e-6f;
+inline constexpr float kWnnmFloor = 0x1p-21f;
 
 // Round r of the round-robin pairing of N indices (N even): N - 1 is paired
 // with r, and i with (2r - i) mod (N - 1) otherwise; over r = 0 .. N - 2
@@ -2896,34 +2902,45 @@ __global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
         V[a * GP + b] = a == b ? 1.f : 0.f;
     }
     __syncthreads();
+    float trace = 0.f;
+    for (int j = 0; j < k; ++j) trace += G[j * GP + j];
+    const float floor_abs = kWnnmFloor * trace;
 
     for (int sweep = 0; sweep < kWnnmSweeps; ++sweep) {
         bool rotated = false;
         for (int r = 0; r < N - 1; ++r) {
             // Rotation of pair (p, q), p < q, zeroing g_pq: J = [c s; -s c],
-            // G' = J^T G J, V' = V J (the classical Jacobi angle, with the
-            // tangent formed from scaled coordinates).
+            // G' = J^T G J, V' = V J, t = tan(theta) the smaller root of
+            // t^2 + 2 zeta t - 1 = 0 with zeta = (g_qq - g_pp) / (2 g_pq).
+            bool rot = false;
             if (lane < N) {
                 const int i = lane, o = rr_partner<N>(i, r);
                 const int p0 = min(i, o), q0 = max(i, o);
                 float c = 1.f, sn = 0.f, tn = 0.f;
                 if (q0 < k) {
                     const float app = G[p0 * GP + p0], aqq = G[q0 * GP + q0], apq = G[p0 * GP + q0];
-                    if (apq != 0.f && fabsf(apq) > kWnnmTol * sqrtf(fabsf(app)) * sqrtf(fabsf(aqq))) {
+                    // |g_pq| > tol sqrt(g_pp g_qq), squared. Gram entries of
+                    // plane values stay far from overflow in the squares, so
+                    // the tangent needs no scaling; v_rcp / v_sqrt (1 ulp)
+                    // keep the rotation orthogonal to a few ulps.
+                    if (fabsf(apq) > floor_abs && apq * apq > kWnnmTol * kWnnmTol * fabsf(app * aqq)) {
                         const float delta = (aqq - app) * 0.5f;
-                        const float scale = fmaxf(fabsf(delta), fabsf(apq));
-                        const float x = delta / scale, y = apq / scale;
-                        tn = y / (x + copysignf(sqrtf(x * x + y * y), x));
-                        c = 1.f / sqrtf(1.f + tn * tn);
+                        const float root = __builtin_amdgcn_sqrtf(fmaf(delta, delta, apq * apq));
+                        tn = apq * __builtin_amdgcn_rcpf(delta + copysignf(root, delta));
+                        c = __builtin_amdgcn_rsqf(fmaf(tn, tn, 1.f));
                         sn = tn * c;
-                        rotated = true;
+                        rot = true;
                     }
                 }
                 cself[i] = c;
                 cpart[i] = i == p0 ? -sn : sn;
                 tstep[i] = i == p0 ? -tn : tn;
+                rotated |= rot;
             }
             __syncthreads();
+            // A round where no pair rotates leaves G and V as they are; the
+            // next round's coefficients overwrite these unread.
+            if (!__any(rot)) continue;
             float ng[E], nv[E];
 #pragma unroll
             for (int t = 0; t < E; ++t) {
@@ -2931,15 +2948,12 @@ __global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
                 const int pa = rr_partner<N>(a, r), pb = rr_partner<N>(b, r);
                 const float ca = cself[a], sa = cpart[a], cb = cself[b], sb = cpart[b];
                 const float gab = G[a * GP + b];
-                float v;
-                if (a == b) {
-                    v = fmaf(tstep[a], G[a * GP + pa], gab);
-                } else if (b == pa) {
-                    v = sa != 0.f ? 0.f : gab;
-                } else {
-                    v = ca * fmaf(cb, gab, sb * G[a * GP + pb]) + sa * fmaf(cb, G[pa * GP + b], sb * G[pa * GP + pb]);
-                }
-                ng[t] = v;
+                // Selected, not branched: the three cases share the wave.
+                const float vgen =
+                    ca * fmaf(cb, gab, sb * G[a * GP + pb]) + sa * fmaf(cb, G[pa * GP + b], sb * G[pa * GP + pb]);
+                const float vdiag = fmaf(tstep[a], G[a * GP + pa], gab);
+                const float vpair = sa != 0.f ? 0.f : gab;
+                ng[t] = a == b ? vdiag : (b == pa ? vpair : vgen);
                 nv[t] = fmaf(cb, V[a * GP + b], sb * V[a * GP + pb]);
             }
             __syncthreads();
