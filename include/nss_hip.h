@@ -292,6 +292,48 @@ int nss_hip_wnnm_process(nss_hip_wnnm* handle, const float* const* src, const in
                          const int* dst_stride, void* stream);
 int nss_hip_wnnm_destroy(nss_hip_wnnm* handle);
 
+/* ---- NCSR ---------------------------------------------------------------
+ * nss.NCSR (src/host/filter_ncsr.cpp; the device form src/cuda/ncsr/filter.cpp:16-80,
+ * kernels.cu:48-190): the same matcher and eigenbasis as WNNM; each code row
+ * of the centred group's PCA is soft-thresholded about its match-weighted
+ * mean, over `iters` rounds on an estimate relaxed toward the noisy frame by
+ * `delta`. */
+
+/* Resolved arguments (include/nss/params/ncsr.hpp:9-20; defaults
+ * include/nss/params.hpp:101-107). sigma on the 0-255 scale. */
+typedef struct nss_hip_ncsr_params {
+    float sigma[3];
+    int block_size;
+    int block_step;
+    int group_size;
+    int bm_range;
+    int radius;
+    int ps_num;
+    int ps_range;
+    int iters;
+    float delta;
+    int has_rclip;
+    int memory_limit_mb;
+    int device_id;
+} nss_hip_ncsr_params;
+
+typedef struct nss_hip_ncsr nss_hip_ncsr;
+
+/* <ns>.NCSR's argument resolution (src/frontend/ncsr_args.cpp:10-32 parse_ncsr
+ * after parse_temporal_mode and the memory limit; rclip and plane checks of
+ * src/cuda/ncsr/filter.cpp:26-34). Radius 0 and block sizes 1, 2, 4, 8, 12, 16
+ * in this build, as for WNNM. */
+int nss_hip_ncsr_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
+                       const nss_hip_arg* args, int nargs, const char* ns, nss_hip_ncsr_params* out);
+int nss_hip_ncsr_create(const nss_hip_ncsr_params* params, const nss_hip_video_info* info, int device,
+                        nss_hip_ncsr** out);
+/* One frame (ncsrGetFrame at radius 0, src/host/filter_ncsr.cpp:34-190):
+ * device planes in and out, rclip (or NULL) guiding round 0's matching. */
+int nss_hip_ncsr_process(nss_hip_ncsr* handle, const float* const* src, const int* src_stride,
+                         const float* const* rclip, const int* rclip_stride, float* const* dst,
+                         const int* dst_stride, void* stream);
+int nss_hip_ncsr_destroy(nss_hip_ncsr* handle);
+
 /* ---- Serving loop ------------------------------------------------------
  * No counterpart in the reference: its filters answer one VapourSynth frame
  * request at a time (src/host/filter_bm3d.cpp bm3dGetFrame). This loop is

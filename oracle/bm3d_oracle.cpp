@@ -500,10 +500,10 @@ int orc_bm3d_plane(const float* const* match, const float* const* src, const flo
 }
 
 // Every reference of one plane's raster grid (raster_grid above): per
-// reference `group` slots of (x, y) in match order and its count, as
+// reference `group` slots of (x, y) and distance in match order and its count, as
 // spatial_match_batch writes them for WNNM (src/host/filter_wnnm.cpp:96-104).
 int orc_spatial_match_plane(const float* plane, int stride, int width, int height, int block, int step, int range,
-                            int group, int fp64, int* xy, int* counts) {
+                            int group, int fp64, int* xy, int* counts, float* dist) {
     std::vector<int> xs, ys;
     raster_grid(width, height, block, step, xs, ys);
     Match ms[kMaxGroup];
@@ -514,6 +514,7 @@ int orc_spatial_match_plane(const float* plane, int stride, int width, int heigh
         for (int i = 0; i < n; ++i) {
             xy[(r * group + i) * 2] = ms[i].x;
             xy[(r * group + i) * 2 + 1] = ms[i].y;
+            dist[r * group + i] = ms[i].dist;
         }
     }
     return static_cast<int>(xs.size());

@@ -388,8 +388,9 @@ def psnr(a, b, peak=1.0):
 WNNM_DEFAULTS = dict(block=8, step=8, group=8, bm_range=7, residual=0, adaptive=1)  # include/nss/params.hpp:61-75
 
 
-def match_plane(plane, block, step, bm_range, group, fp64=False):
-    """Every raster reference's matches: (xy[nref, group, 2], counts[nref])
+def match_plane(plane, block, step, bm_range, group, fp64=False, want_dist=False):
+    """Every raster reference's matches: (xy[nref, group, 2], counts[nref]),
+    and their distances [nref, group] with want_dist
     (src/host/filter_wnnm.cpp:96-104 spatial_match_batch on the match plane)."""
     plane = np.ascontiguousarray(plane, np.float32)
     h, w = plane.shape
@@ -397,15 +398,18 @@ def match_plane(plane, block, step, bm_range, group, fp64=False):
     if not getattr(L, "_plane_types", False):
         P = ctypes.POINTER
         ci = ctypes.c_int
-        L.orc_spatial_match_plane.argtypes = [P(ctypes.c_float), ci, ci, ci, ci, ci, ci, ci, ci, P(ci), P(ci)]
+        L.orc_spatial_match_plane.argtypes = [P(ctypes.c_float), ci, ci, ci, ci, ci, ci, ci, ci, P(ci), P(ci),
+                                              P(ctypes.c_float)]
         L.orc_spatial_match_plane.restype = ci
         L._plane_types = True
     nref = len(raster_grid(w, h, block, step))
     xy = np.zeros((max(nref, 1), group, 2), np.int32)
     counts = np.zeros(max(nref, 1), np.int32)
-    n = L.orc_spatial_match_plane(_fp(plane), w, w, h, block, step, bm_range, group, int(fp64), _ip(xy), _ip(counts))
+    dist = np.zeros((max(nref, 1), group), np.float32)
+    n = L.orc_spatial_match_plane(_fp(plane), w, w, h, block, step, bm_range, group, int(fp64), _ip(xy), _ip(counts),
+                                  _fp(dist))
     assert n == nref
-    return xy[:nref], counts[:nref]
+    return (xy[:nref], counts[:nref], dist[:nref]) if want_dist else (xy[:nref], counts[:nref])
 
 
 def sv_shrink(s, constant, start):
@@ -485,3 +489,80 @@ def wnnm_frame(planes, sigma, block=8, step=8, group=8, bm_range=7, residual=0, 
         out.append(wnnm_plane(p, sigma[i], block, step, group, bm_range, residual, adaptive,
                               None if rclip is None else rclip[i]))
     return out
+
+
+# ---------------------------------------------------------------------------
+# NCSR (the second sibling group filter), restated in float64.
+# ---------------------------------------------------------------------------
+NCSR_DEFAULTS = dict(block=8, step=8, group=8, bm_range=7, iters=2, delta=0.1)  # include/nss/params.hpp:101-107
+
+
+def ncsr_group(A, dist, sigma):
+    """One group (m rows, n = k columns) and its match distances (self
+    first): src/cpu/ncsr/centralize.cpp:143-176 NcsrFilterGroup with
+    src/cpu/finishers.hpp:53-80 finish_ncsr_codes. PCA of the row-centred
+    group (U, codes B = U^T (A - mean), the min(m, n) leading directions),
+    column weights exp(-dist / h) with h = max(2 m sigma^2, 1e-12) and the
+    self column at distance 0 (centralize.cpp:60-88), per code row the
+    weighted mean beta, the second moment over the actual columns about it,
+    tau = 2 sqrt(2) sigma^2 / (sqrt(moment / n) + 1e-12), the row
+    soft-thresholded about beta (centralize.cpp:90-141), then
+    X = U B' + mean. sigma 0 returns the group unchanged. Weight 1."""
+    A = np.array(A, np.float64)
+    m, n = A.shape
+    if not sigma > 0:
+        return A.copy()
+    mean = A.mean(axis=1, keepdims=True)
+    C = A - mean
+    U, s, Vt = np.linalg.svd(C, full_matrices=False)
+    r = min(m, n)
+    B = U[:, :r].T @ C
+    h = max(2.0 * m * sigma * sigma, 1e-12)
+    d = np.array(dist, np.float64)[:n].copy()
+    d[0] = 0.0
+    w = np.exp(-d / h)
+    wsum = w.sum()
+    if not wsum > 0:
+        w, wsum = np.ones(n), float(n)
+    for i in range(r):
+        if not s[i] > 0:
+            B[i] = 0.0
+            continue
+        row = B[i]
+        beta = float(np.dot(w, row)) / wsum
+        moment = float(np.sum((row - beta) ** 2))
+        tau = 2.8284271247461903 * sigma * sigma / (np.sqrt(moment / n) + 1e-12)
+        e = row - beta
+        B[i] = np.sign(e) * np.maximum(np.abs(e) - tau, 0.0) + beta
+    return U[:, :r] @ B + mean
+
+
+def ncsr_plane(src, sigma_user, block=8, step=8, group=8, bm_range=7, iters=2, delta=0.1, match=None):
+    """One spatial plane of nss.NCSR (src/host/filter_ncsr.cpp:34-190 at
+    radius 0): the estimate starts as the noisy plane; round i > 0 first
+    relaxes it toward the noisy plane, est += delta (noisy - est)
+    (src/cpu/common/iter_regularize.cpp:15-39); each round matches on the
+    estimate (on rclip in round 0), filters the estimate's groups
+    (src/cpu/ncsr/centralize.cpp:423-540 ncsr_run_groups), aggregates with
+    weight 1 and finishes est = den > 0 ? num / den : est."""
+    noisy = np.ascontiguousarray(src, np.float32).astype(np.float64)
+    h, w = noisy.shape
+    sigma = float(np.float32(sigma_user) / np.float32(255.0))
+    est = noisy.copy()
+    dy, dx = np.divmod(np.arange(block * block), block)
+    for it in range(iters):
+        if it > 0:
+            est = est + delta * (noisy - est)
+        mplane = match if (it == 0 and match is not None) else est
+        xy, counts, dist = match_plane(np.asarray(mplane, np.float32), block, step, bm_range, group, want_dist=True)
+        num = np.zeros((h, w))
+        den = np.zeros((h, w))
+        for r in range(len(counts)):
+            k = int(counts[r])
+            cols = [est[xy[r, j, 1] + dy, xy[r, j, 0] + dx] for j in range(k)]
+            X = ncsr_group(np.stack(cols, axis=1), dist[r, :k], sigma)
+            for j in range(k):
+                num[xy[r, j, 1] + dy, xy[r, j, 0] + dx] += X[:, j]
+                den[xy[r, j, 1]:xy[r, j, 1] + block, xy[r, j, 0]:xy[r, j, 0] + block] += 1.0
+        est = np.where(den > 1e-12, num / np.where(den > 1e-12, den, 1.0), est)
+    return est.astype(np.float32)

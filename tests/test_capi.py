@@ -106,3 +106,27 @@ def test_wnnm_parse_messages_and_null_checks():
     assert lib.nss_hip_wnnm_create(None, None, 0, None) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_wnnm_process(None, None, None, None, None, None, None, None) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_wnnm_destroy(None) == nss_hip.OK
+
+
+def test_ncsr_parse_messages():
+    """nss_hip.NCSR's argument errors carry the reference's text
+    (src/frontend/ncsr_args.cpp:10-32)."""
+    import nss_hip
+    lib = nss_hip.lib()
+    p = nss_hip.parse_ncsr("GRAYS", 64, 48)
+    assert (p.block_size, p.block_step, p.group_size, p.bm_range, p.iters) == (8, 8, 8, 7, 2)
+    assert abs(p.delta - 0.1) < 1e-7
+    cases = [
+        (dict(block_size=17), "nss_hip.NCSR: invalid block_size/group_size/block_step/radius"),
+        (dict(iters=65), "nss_hip.NCSR: invalid iters/delta (iters in [1, 64], delta in [0, 1])"),
+        (dict(delta=-0.5), "nss_hip.NCSR: invalid iters/delta (iters in [1, 64], delta in [0, 1])"),
+        (dict(ps_num=9), "nss_hip.NCSR: invalid ps_num/ps_range"),
+        (dict(bm_range=65), "nss_hip.NCSR: bm_range must be in [1, 64]"),
+    ]
+    for kw, text in cases:
+        with pytest.raises(nss_hip.Error) as e:
+            nss_hip.parse_ncsr("GRAYS", 64, 48, **kw)
+        assert str(e.value) == text
+    assert lib.nss_hip_ncsr_create(None, None, 0, None) == nss_hip.INVALID_INPUT
+    assert lib.nss_hip_ncsr_process(None, None, None, None, None, None, None, None) == nss_hip.INVALID_INPUT
+    assert lib.nss_hip_ncsr_destroy(None) == nss_hip.OK

@@ -1,0 +1,111 @@
+"""nss_hip.NCSR on the device against the NCSR oracle (float64 SVD route).
+
+Bar: at least 60 dB PSNR per plane against the oracle, bitwise identity
+between runs, sigma-0 planes copied bit for bit, and the reference's own
+NCSR invariants (tests/test_ncsr_variance_plugin.py): a noiseless stroke
+image at sigma 0.1 moves no pixel by 0.5/255 or more, and groups of 9, 16
+and 32 at bm_range 1 agree within rtol 2e-4, atol 2e-5.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PSNR_MIN = 60.0
+
+
+def _plane(h, w, seed, amp=0.3, noise=10 / 255):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    clean = 0.5 + amp * np.sin(xx / 6.0 + seed) * np.cos(yy / 4.5) + 0.1 * ((xx // 9 + yy // 7) % 2)
+    return (clean + rng.normal(0, noise, (h, w))).astype(np.float32), clean.astype(np.float32)
+
+
+def _stroke_image(size=32):
+    y, x = np.mgrid[:size, :size]
+    base = (0.12 + ((x * 17 + y * 31) % 97) / 400).astype(np.float32)
+    base[((x % 11 == 3) & (y > 3)) | ((y == 9) & (x > 5))] = 0.9
+    return base
+
+
+def _run(gpu, fmt, planes, rclip=None, **kw):
+    import torch
+    h, w = planes[0].shape
+    f = gpu.NCSR(fmt, w, h, rclip=rclip is not None, **kw)
+    src = [torch.from_numpy(np.ascontiguousarray(p)).cuda() for p in planes]
+    rc = [torch.from_numpy(p).cuda() for p in rclip] if rclip is not None else None
+    out = f(src, rclip=rc)
+    torch.cuda.synchronize()
+    res = [o.cpu().numpy() for o in out]
+    f.close()
+    return res
+
+
+CASES = [
+    dict(),
+    dict(iters=1),
+    dict(iters=3, delta=0.5),
+    dict(block_size=8, block_step=4, group_size=16, bm_range=9),
+    dict(block_size=4, block_step=2, group_size=32, bm_range=6, delta=0.0),
+    dict(block_size=16, block_step=8, group_size=8, bm_range=5),
+    dict(block_size=12, block_step=6, group_size=12, bm_range=4),
+    dict(block_size=2, block_step=1, group_size=4, bm_range=3),
+]
+
+
+@pytest.mark.parametrize("kw", CASES, ids=[",".join(f"{k}={v}" for k, v in c.items()) or "defaults" for c in CASES])
+def test_ncsr_against_oracle(gpu, oracle, kw):
+    src, _ = _plane(80, 96, 5)
+    out = _run(gpu, "GRAYS", [src], sigma=[10.0], **kw)[0]
+    ref = oracle.ncsr_plane(src, 10.0, block=kw.get("block_size", 8), step=kw.get("block_step", 8),
+                            group=kw.get("group_size", 8), bm_range=kw.get("bm_range", 7),
+                            iters=kw.get("iters", 2), delta=kw.get("delta", 0.1))
+    assert np.isfinite(out).all()
+    p = oracle.psnr(out, ref)
+    print(f"{kw}: PSNR vs oracle {p:.1f} dB")
+    assert p >= PSNR_MIN
+
+
+def test_ncsr_reference_stroke_invariants(gpu):
+    src = _stroke_image()
+    for sigma in (0.0, 0.1):
+        out = _run(gpu, "GRAYS", [src], sigma=[sigma])[0]
+        again = _run(gpu, "GRAYS", [src], sigma=[sigma])[0]
+        assert np.isfinite(out).all()
+        if sigma == 0:
+            assert np.array_equal(out, src)
+        else:
+            assert float(np.max(np.abs(out - src))) < 0.5 / 255
+        assert np.array_equal(out, again)
+    for block in (1, 4, 8):
+        kw = dict(sigma=[10.0], block_size=block, block_step=block, bm_range=1, iters=1)
+        ref = _run(gpu, "GRAYS", [src], group_size=9, **kw)[0]
+        for capacity in (16, 32):
+            out = _run(gpu, "GRAYS", [src], group_size=capacity, **kw)[0]
+            np.testing.assert_allclose(out, ref, rtol=2e-4, atol=2e-5)
+
+
+def test_ncsr_yuv_and_rclip(gpu, oracle):
+    y, cy = _plane(72, 96, 1)
+    u, _ = _plane(36, 48, 2, amp=0.1)
+    v, _ = _plane(36, 48, 3, amp=0.2)
+    out = _run(gpu, "YUV420PS", [y, u, v], sigma=[10.0, 0.0, 6.0], block_step=4)
+    assert np.array_equal(out[1], u)
+    assert oracle.psnr(out[0], oracle.ncsr_plane(y, 10.0, step=4)) >= PSNR_MIN
+    assert oracle.psnr(out[2], oracle.ncsr_plane(v, 6.0, step=4)) >= PSNR_MIN
+    guide = (cy + np.random.default_rng(8).normal(0, 2 / 255, cy.shape)).astype(np.float32)
+    g = _run(gpu, "GRAYS", [y], rclip=[guide], sigma=[10.0], block_step=4)[0]
+    assert oracle.psnr(g, oracle.ncsr_plane(y, 10.0, step=4, match=guide)) >= PSNR_MIN
+
+
+def test_ncsr_1080p_defaults_against_oracle(gpu, oracle):
+    src, clean = _plane(1080, 1920, 12)
+    out = _run(gpu, "GRAYS", [src], sigma=[10.0])[0]
+    again = _run(gpu, "GRAYS", [src], sigma=[10.0])[0]
+    ref = oracle.ncsr_plane(src, 10.0)
+    p = oracle.psnr(out, ref)
+    print(f"1080p defaults: PSNR vs oracle {p:.1f} dB, vs clean {oracle.psnr(out, clean):.2f} "
+          f"(noisy {oracle.psnr(src, clean):.2f})")
+    assert p >= PSNR_MIN
+    assert np.array_equal(out, again)
+    assert oracle.psnr(out, clean) > oracle.psnr(src, clean)

@@ -396,6 +396,39 @@ void parse_wnnm(const ArgMap& in, const nss_hip_video_info& vi, const std::strin
         fail_wnnm(ns, "invalid ps_num/ps_range");
 }
 
+// ---- NCSR (src/frontend/ncsr_args.cpp:10-32, include/nss/params.hpp:101-107) ----
+constexpr int kNcsrDefaultIters = 2;
+constexpr float kNcsrDefaultDelta = 0.1f;
+
+[[noreturn]] void fail_ncsr(const std::string& ns, const std::string& message) {
+    throw Invalid(ns + ".NCSR: " + message);
+}
+
+void parse_ncsr(const ArgMap& in, const nss_hip_video_info& vi, const std::string& ns, nss_hip_ncsr_params& p) {
+    if (!is_const_32f(vi)) fail_ncsr(ns, "constant Gray/YUV/RGB 32-bit float required");
+    map_float_array(in, "sigma", p.sigma, vi.num_planes, kWnnmDefaultSigma);
+    for (int i = vi.num_planes; i < 3; ++i) p.sigma[i] = 0.f;
+    p.block_size = map_int(in, "block_size", kWnnmDefaultBlock);
+    p.block_step = map_int(in, "block_step", kWnnmDefaultStep);
+    p.group_size = map_int(in, "group_size", kWnnmDefaultGroup);
+    p.bm_range = map_int(in, "bm_range", kWnnmDefaultRange);
+    p.radius = map_int(in, "radius", 0);
+    p.ps_num = map_int(in, "ps_num", kBmDefaultPsNum);
+    p.ps_range = map_int(in, "ps_range", kBmDefaultPsRange);
+    p.iters = map_int(in, "iters", kNcsrDefaultIters);
+    int err = 0;
+    const double delta = in.get_float("delta", 0, &err);
+    p.delta = err ? kNcsrDefaultDelta : static_cast<float>(delta);
+    if (p.block_size < 1 || p.block_size > kWnnmMaxBlock || p.group_size < 1 || p.group_size > kWnnmMaxGroup ||
+        p.block_step < 1 || p.block_step > p.block_size || p.radius < 0 || p.radius > kBmMaxRadius)
+        fail_ncsr(ns, "invalid block_size/group_size/block_step/radius");
+    if (p.iters < 1 || p.iters > 64 || !finite_bits(p.delta) || p.delta < 0.f || p.delta > 1.f)
+        fail_ncsr(ns, "invalid iters/delta (iters in [1, 64], delta in [0, 1])");
+    if (p.ps_num < 1 || p.ps_num > p.group_size || p.ps_range < 1 || p.ps_range > kBmMaxRange)
+        fail_ncsr(ns, "invalid ps_num/ps_range");
+    if (p.bm_range < 1 || p.bm_range > kBmMaxRange) fail_ncsr(ns, "bm_range must be in [1, 64]");
+}
+
 }  // namespace
 
 extern "C" {
@@ -508,6 +541,55 @@ int nss_hip_wnnm_parse(const nss_hip_video_info* clip_info, const nss_hip_video_
             fail_wnnm(ns, "block_size must be one of 1, 2, 4, 8, 12, 16 in this build");
         const std::int64_t device = in.get_int("device_id", 0, &err);
         if (!err && (device < 0 || device > INT32_MAX)) fail_wnnm(ns, "device_id must be >= 0");
+        p.device_id = err ? 0 : static_cast<int>(device);
+        *out = p;
+        nsship::g_last_error.clear();
+        return NSS_HIP_OK;
+    } catch (const std::exception& e) {
+        nsship::g_last_error = e.what();
+        return NSS_HIP_INVALID_INPUT;
+    }
+}
+
+// <ns>.NCSR's create-time checks, in the order of nss_hip_wnnm_parse.
+int nss_hip_ncsr_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
+                       const nss_hip_arg* args, int nargs, const char* ns_c, nss_hip_ncsr_params* out) {
+    const std::string ns = ns_c ? ns_c : "nss_hip";
+    if (!clip_info || !out || (nargs > 0 && !args)) {
+        nsship::g_last_error = ns + ".NCSR: null argument";
+        return NSS_HIP_INVALID_INPUT;
+    }
+    try {
+        const ArgMap in(args, nargs);
+        validate_numeric(in, ns);
+        nss_hip_ncsr_params p{};
+        int err = 0;
+        const std::int64_t mb = in.get_int("memory_limit_mb", 0, &err);
+        if (!err && (mb < 1 || mb > 1048576)) throw Invalid(ns + ": memory_limit_mb must be in [1, 1048576]");
+        p.memory_limit_mb = err ? 0 : static_cast<int>(mb);
+        const char* mode = in.get_data("temporal_mode", &err);
+        std::string m;
+        if (!err && mode) {
+            m = mode;
+            for (char& c : m)
+                if (c >= 'A' && c <= 'Z') c = static_cast<char>(c - 'A' + 'a');
+        }
+        if (m == "fused") fail_ncsr(ns, "temporal_mode=fused is not supported; use rolling or legacy");
+        if (!m.empty() && m != "rolling" && m != "legacy") fail_ncsr(ns, "temporal_mode must be rolling or legacy");
+        parse_ncsr(in, *clip_info, ns, p);
+        if (rclip_info && !same_video(*clip_info, *rclip_info)) fail_ncsr(ns, "rclip must match clip");
+        p.has_rclip = rclip_info != nullptr;
+        for (int i = 0; i < clip_info->num_planes; ++i) {
+            if (p.sigma[i] < 0.f) throw Invalid("nss: sigma must be non-negative");
+            if (p.sigma[i] > 0.f &&
+                (plane_width(*clip_info, i) < p.block_size || plane_height(*clip_info, i) < p.block_size))
+                throw Invalid("nss: selected plane is smaller than block_size");
+        }
+        if (p.radius > 0) fail_ncsr(ns, "radius > 0 (temporal NCSR) is not available in this build; use radius=0");
+        if (!allowed_block(p.block_size) || p.block_size > kWnnmMaxBlock)
+            fail_ncsr(ns, "block_size must be one of 1, 2, 4, 8, 12, 16 in this build");
+        const std::int64_t device = in.get_int("device_id", 0, &err);
+        if (!err && (device < 0 || device > INT32_MAX)) fail_ncsr(ns, "device_id must be >= 0");
         p.device_id = err ? 0 : static_cast<int>(device);
         *out = p;
         nsship::g_last_error.clear();

@@ -2810,16 +2810,18 @@ inline constexpr int kWnnmSweeps = 16;
 inline constexpr float kWnnmTol = 1e-6f;
 inline constexpr float kWnnmFloor = 0x1p-21f;
 
-// Round r of the round-robin pairing of N indices (N even): N - 1 is paired
-// with r, and i with (2r - i) mod (N - 1) otherwise; over r = 0 .. N - 2
-// every pair meets once.
-template <int N>
-__device__ inline int rr_partner(int i, int r) {
-    if (i == N - 1) return r;
-    if (i == r) return N - 1;
+// Round r of the round-robin pairing of the first n indices (n even):
+// n - 1 is paired with r, and i with (2r - i) mod (n - 1) otherwise; over
+// r = 0 .. n - 2 every pair meets once. Indices from n on pair with
+// themselves. The schedule depends on the group's own column count only, so
+// a group gives the same bits whatever size it is padded to.
+__device__ inline int rr_partner(int i, int r, int n) {
+    if (i >= n) return i;
+    if (i == n - 1) return r;
+    if (i == r) return n - 1;
     int j = 2 * r - i;
-    if (j < 0) j += N - 1;
-    else if (j >= N - 1) j -= N - 1;
+    if (j < 0) j += n - 1;
+    else if (j >= n - 1) j -= n - 1;
     return j;
 }
 
@@ -2827,8 +2829,8 @@ __device__ inline int rr_partner(int i, int r) {
 // zero columns that never rotate. LDS: A (N columns of m + 1) | Gram, later
 // the shrink operator W (N x N + 1) | V (N x N + 1) | per-index rotation
 // coefficients, diagonal steps (later the singular values) and gains.
-template <int N>
-__global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
+template <int N, bool NCSR>
+__device__ __forceinline__ void gram_group_body(const WnnmTable& tab) {
     extern __shared__ __align__(16) float wn[];
     constexpr int GP = N + 1;
     constexpr int E = N * N / 64;
@@ -2906,15 +2908,16 @@ __global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
     for (int j = 0; j < k; ++j) trace += G[j * GP + j];
     const float floor_abs = kWnnmFloor * trace;
 
+    const int ke = k + (k & 1);  // the schedule's size: k rounded up to even
     for (int sweep = 0; sweep < kWnnmSweeps; ++sweep) {
         bool rotated = false;
-        for (int r = 0; r < N - 1; ++r) {
+        for (int r = 0; r < ke - 1; ++r) {
             // Rotation of pair (p, q), p < q, zeroing g_pq: J = [c s; -s c],
             // G' = J^T G J, V' = V J, t = tan(theta) the smaller root of
             // t^2 + 2 zeta t - 1 = 0 with zeta = (g_qq - g_pp) / (2 g_pq).
             bool rot = false;
             if (lane < N) {
-                const int i = lane, o = rr_partner<N>(i, r);
+                const int i = lane, o = rr_partner(i, r, ke);
                 const int p0 = min(i, o), q0 = max(i, o);
                 float c = 1.f, sn = 0.f, tn = 0.f;
                 if (q0 < k) {
@@ -2945,7 +2948,7 @@ __global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
 #pragma unroll
             for (int t = 0; t < E; ++t) {
                 const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
-                const int pa = rr_partner<N>(a, r), pb = rr_partner<N>(b, r);
+                const int pa = rr_partner(a, r, ke), pb = rr_partner(b, r, ke);
                 const float ca = cself[a], sa = cpart[a], cb = cself[b], sb = cpart[b];
                 const float gab = G[a * GP + b];
                 // Selected, not branched: the three cases share the wave.
@@ -2968,50 +2971,120 @@ __global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
         if (!__any(rotated)) break;
     }
 
-    // Spectrum and gains: place = rank of s_j in descending order (ties by
-    // index); places below `start` stay, later ones shrink to
-    // (s + sqrt(s^2 - c)) / 2 while s^2 > c, with c = 8 sqrt(2k) sigma^2.
+    // Spectrum: s_j and its place in descending order (ties by index).
     if (lane < N) tstep[lane] = lane < k ? sqrtf(fmaxf(G[lane * GP + lane], 0.f)) : 0.f;
+    if constexpr (NCSR) {
+        // Column weights exp(-dist / h), h = max(2 m sigma^2, 1e-12), the
+        // reference column at distance 0 (src/cpu/ncsr/centralize.cpp:60-88).
+        const float sg = jb.sigma;
+        const float h = fmaxf(2.f * static_cast<float>(m) * sg * sg, 1e-12f);
+        if (lane < N) cpart[lane] = lane < k ? expf(-(lane > 0 ? __uint_as_float(rec[lane].dist) : 0.f) / h) : 0.f;
+    }
     __syncthreads();
-    bool keep = false;
-    float gj = 0.f;
+    int place = 0;
     if (lane < k) {
         const float sj = tstep[lane];
-        int place = 0;
         for (int i = 0; i < k; ++i) {
             const float si = tstep[i];
             place += (si > sj || (si == sj && i < lane)) ? 1 : 0;
         }
-        const int rank = min(m, k), start = min(jb.residual ? 0 : 1, rank);
-        const float constant = 8.f * sqrtf(2.f * static_cast<float>(k)) * jb.sigma * jb.sigma;
-        if (place < start) {
-            gj = 1.f;
-            keep = true;
-        } else if (place < rank) {
-            const float tmp = sj * sj - constant;
-            if (tmp > 0.f) {
-                gj = ((sj + sqrtf(tmp)) * 0.5f) / sj;
-                keep = true;
+    }
+    const int rank = min(m, k);
+    int kept = 0;
+    if constexpr (NCSR) {
+        // Code row i of the centred group's PCA, b_ij = s_i V[j][i]: its
+        // weighted mean beta, the second moment over the k columns about it,
+        // tau = 2 sqrt(2) sigma^2 / (sqrt(moment / k) + 1e-12), the row
+        // soft-thresholded about beta (src/cpu/finishers.hpp:53-80,
+        // src/cpu/ncsr/centralize.cpp:90-141), kept for the rank largest
+        // values with s > 0 and stored over s_i: C[i][j]. The Gram area is
+        // free (the spectrum is in tstep).
+        const float sg = jb.sigma;
+        float wsum = 0.f;
+        for (int j = 0; j < k; ++j) wsum += cpart[j];
+        const bool unit = !(wsum > 0.f);
+        const float inverse = 1.f / (unit ? static_cast<float>(k) : wsum);
+        if (lane < k) {
+            const int i = lane;
+            const float si = tstep[i];
+            if (!(sg > 0.f)) {
+                for (int j = 0; j < k; ++j) G[i * GP + j] = V[j * GP + i];
+            } else if (place < rank && si > 0.f) {
+                float beta = 0.f;
+                for (int j = 0; j < k; ++j) beta = fmaf(unit ? 1.f : cpart[j], si * V[j * GP + i], beta);
+                beta *= inverse;
+                float moment = 0.f;
+                for (int j = 0; j < k; ++j) {
+                    const float e = si * V[j * GP + i] - beta;
+                    moment = fmaf(e, e, moment);
+                }
+                const float tau = 2.8284271247461903f * sg * sg / (sqrtf(moment / static_cast<float>(k)) + 1e-12f);
+                const float inv_s = 1.f / si;
+                for (int j = 0; j < k; ++j) {
+                    const float d = si * V[j * GP + i] - beta, a = fabsf(d);
+                    G[i * GP + j] = ((a > tau ? copysignf(a - tau, d) : 0.f) + beta) * inv_s;
+                }
+            } else {
+                for (int j = 0; j < k; ++j) G[i * GP + j] = 0.f;
             }
         }
-    }
-    if (lane < N) gain[lane] = gj;
-    const int kept = static_cast<int>(__popcll(__ballot(keep)));
-    __syncthreads();
-    // W = V diag(gain) V^T over the Gram area (G is no longer read).
+        __syncthreads();
+        // M = V C (M[a][b] = sum_i V[a][i] C[i][b]), then over the Gram area.
+        float mv[E];
 #pragma unroll
-    for (int t = 0; t < E; ++t) {
-        const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
-        float w = 0.f;
-        if (a < k && b < k)
-            for (int i = 0; i < k; ++i) w = fmaf(V[a * GP + i] * gain[i], V[b * GP + i], w);
-        G[a * GP + b] = w;
+        for (int t = 0; t < E; ++t) {
+            const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
+            float w = 0.f;
+            if (a < k && b < k)
+                for (int i = 0; i < k; ++i) w = fmaf(V[a * GP + i], G[i * GP + b], w);
+            mv[t] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < E; ++t) {
+            const int idx = lane + 64 * t;
+            G[(idx >> LOG_N) * GP + (idx & (N - 1))] = mv[t];
+        }
+        __syncthreads();
+    } else {
+        // Places below `start` stay; later ones shrink to
+        // (s + sqrt(s^2 - c)) / 2 while s^2 > c, with c = 8 sqrt(2k) sigma^2.
+        bool keep = false;
+        float gj = 0.f;
+        if (lane < k) {
+            const float sj = tstep[lane];
+            const int start = min(jb.residual ? 0 : 1, rank);
+            const float constant = 8.f * sqrtf(2.f * static_cast<float>(k)) * jb.sigma * jb.sigma;
+            if (place < start) {
+                gj = 1.f;
+                keep = true;
+            } else if (place < rank) {
+                const float tmp = sj * sj - constant;
+                if (tmp > 0.f) {
+                    gj = ((sj + sqrtf(tmp)) * 0.5f) / sj;
+                    keep = true;
+                }
+            }
+        }
+        if (lane < N) gain[lane] = gj;
+        kept = static_cast<int>(__popcll(__ballot(keep)));
+        __syncthreads();
+        // W = V diag(gain) V^T over the Gram area (G is no longer read).
+#pragma unroll
+        for (int t = 0; t < E; ++t) {
+            const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
+            float w = 0.f;
+            if (a < k && b < k)
+                for (int i = 0; i < k; ++i) w = fmaf(V[a * GP + i] * gain[i], V[b * GP + i], w);
+            G[a * GP + b] = w;
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
-    // X = A W (+ the centre), weighted into the sums: 1 / kept when adaptive
-    // and kept > 0, else 1 (src/cuda/wnnm/kernels.cu:84-105).
-    const float wt = jb.adaptive && kept > 0 ? 1.f / static_cast<float>(kept) : 1.f;
+    // X = A W (A M for NCSR) plus the centre, weighted into the sums: WNNM
+    // 1 / kept when adaptive and kept > 0, else 1 (src/cuda/wnnm/kernels.cu:84-105);
+    // NCSR 1.
+    const float wt = !NCSR && jb.adaptive && kept > 0 ? 1.f / static_cast<float>(kept) : 1.f;
     const unsigned long long wfix = to_fixed(wt);
     for (int j = 0; j < k; ++j) {
         unsigned long long* cell = jb.num + static_cast<size_t>(rec[j].y) * jb.width + rec[j].x;
@@ -3026,6 +3099,30 @@ __global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
         }
         if (lane == 0) atomicAdd(jb.den + static_cast<size_t>(rec[j].y) * jb.width + rec[j].x, wfix);
     }
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void wnnm_group_kernel(WnnmTable tab) {
+    gram_group_body<N, false>(tab);
+}
+
+// NCSR on the same eigenbasis (src/cuda/ncsr/kernels.cu:48-190): the rows
+// are always centred and the shrink works on the code rows.
+template <int N>
+__global__ __launch_bounds__(64) void ncsr_group_kernel(WnnmTable tab) {
+    gram_group_body<N, true>(tab);
+}
+
+// NCSR rounds after the first: est += delta (noisy - est)
+// (src/cpu/common/iter_regularize.cpp:15-39). 64 x 4 pixels per block.
+__global__ __launch_bounds__(256) void regularize_kernel(float* est, int est_stride, const float* noisy,
+                                                         int noisy_stride, int width, int height, float delta) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= width || y >= height) return;
+    float* e = est + static_cast<size_t>(y) * est_stride + x;
+    const float n = noisy[static_cast<size_t>(y) * noisy_stride + x];
+    *e = fmaf(delta, n - *e, *e);
 }
 
 // ---------------------------------------------------------------------------
@@ -3406,7 +3503,26 @@ size_t wnnm_lds_bytes(int block, int group) {
     return 4 * (n * (m + 1) + 2 * n * (n + 1) + 4 * n);
 }
 
-hipError_t launch_wnnm(const WnnmTable& t, int total_wg, hipStream_t s) {
+hipError_t launch_regularize(float* est, int est_stride, const float* noisy, int noisy_stride, int width, int height,
+                             float delta, hipStream_t s) {
+    const dim3 grid((width + 63) / 64, (height + 3) / 4);
+    regularize_kernel<<<grid, 256, 0, s>>>(est, est_stride, noisy, noisy_stride, width, height, delta);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_gram_groups(const WnnmTable& t, int total_wg, size_t lds, bool ncsr, hipStream_t s) {
+    if (ncsr) {
+        allow_lds(&ncsr_group_kernel<N>, lds);
+        ncsr_group_kernel<N><<<total_wg, 64, lds, s>>>(t);
+    } else {
+        allow_lds(&wnnm_group_kernel<N>, lds);
+        wnnm_group_kernel<N><<<total_wg, 64, lds, s>>>(t);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_wnnm(const WnnmTable& t, int total_wg, bool ncsr, hipStream_t s) {
     int gmax = 0, bmax = 0;
     for (int j = 0; j < t.n; ++j) {
         if (t.job[j].group < 1 || t.job[j].group > 32 || t.job[j].block < 1 || t.job[j].block > 16)
@@ -3416,20 +3532,10 @@ hipError_t launch_wnnm(const WnnmTable& t, int total_wg, hipStream_t s) {
     }
     const size_t lds = wnnm_lds_bytes(bmax, gmax);
     switch (wnnm_pad(gmax)) {
-        case 8:
-            allow_lds(&wnnm_group_kernel<8>, lds);
-            wnnm_group_kernel<8><<<total_wg, 64, lds, s>>>(t);
-            break;
-        case 16:
-            allow_lds(&wnnm_group_kernel<16>, lds);
-            wnnm_group_kernel<16><<<total_wg, 64, lds, s>>>(t);
-            break;
-        default:
-            allow_lds(&wnnm_group_kernel<32>, lds);
-            wnnm_group_kernel<32><<<total_wg, 64, lds, s>>>(t);
-            break;
+        case 8: return launch_gram_groups<8>(t, total_wg, lds, ncsr, s);
+        case 16: return launch_gram_groups<16>(t, total_wg, lds, ncsr, s);
+        default: return launch_gram_groups<32>(t, total_wg, lds, ncsr, s);
     }
-    return hipGetLastError();
 }
 
 hipError_t upload_dct_tables(const float* host, size_t count) {

@@ -53,7 +53,10 @@ hipError_t launch_finish_temporal_tiles(const TFinishTable& t, int total_wg, hip
 hipError_t launch_vaggregate(const VAggJob& jb, hipStream_t s);
 // WNNM group shrinkage (group <= 32, block <= 16).
 size_t wnnm_lds_bytes(int block, int group);
-hipError_t launch_wnnm(const WnnmTable& t, int total_wg, hipStream_t s);
+// ncsr: the NCSR code shrink on the same eigenbasis instead of WNNM's.
+hipError_t launch_wnnm(const WnnmTable& t, int total_wg, bool ncsr, hipStream_t s);
+hipError_t launch_regularize(float* est, int est_stride, const float* noisy, int noisy_stride, int width, int height,
+                             float delta, hipStream_t s);
 // The matcher kind for one plane (3 MFMA, 2 run, 1 wave with the window in
 // LDS, 0 wave from global) and its LDS bytes.
 int choose_match_kind(int block, int step, int range, int group, int width, int height, long long nref,

@@ -1,5 +1,5 @@
-// nss_hip.WNNM behind the C ABI: one spatial frame per call, every step on
-// the caller's stream.
+// nss_hip.WNNM and nss_hip.NCSR behind the C ABI: one spatial frame per
+// call, every step on the caller's stream.
 //
 // Per active plane (src/host/filter_wnnm.cpp:44-160 process_plane_batched at
 // radius 0): clear the plane's fixed-point sums, match every raster reference
@@ -21,6 +21,9 @@ struct nss_hip_wnnm {
     nss_hip_wnnm_params p{};
     nss_hip_video_info vi{};
     int device = 0;
+    bool ncsr = false;  // NCSR: iterations on the estimate, the code shrink
+    int iters = 1;
+    float delta = 0.f;
     struct Plane {
         int width = 0, height = 0;
         Grid grid{};
@@ -30,9 +33,13 @@ struct nss_hip_wnnm {
         int* counts = nullptr;
         unsigned long long* num = nullptr;
         unsigned long long* den = nullptr;
+        float* est = nullptr;  // NCSR estimate, width floats per row
     } pl[3];
     long long bytes = 0;
 };
+
+// The NCSR instance is the same state with the NCSR fields set.
+struct nss_hip_ncsr : nss_hip_wnnm {};
 
 namespace {
 
@@ -42,6 +49,8 @@ void release(nss_hip_wnnm* h) {
         if (q.counts) (void)hipFree(q.counts);
         if (q.num) (void)hipFree(q.num);
         if (q.den) (void)hipFree(q.den);
+        if (q.est) (void)hipFree(q.est);
+        q.est = nullptr;
         q.recs = nullptr;
         q.counts = nullptr;
         q.num = nullptr;
@@ -49,28 +58,21 @@ void release(nss_hip_wnnm* h) {
     }
 }
 
-}  // namespace
-
-extern "C" {
-
-int nss_hip_wnnm_create(const nss_hip_wnnm_params* params, const nss_hip_video_info* info, int device,
-                        nss_hip_wnnm** out) {
-    if (!params || !info || !out) return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: null argument");
-    *out = nullptr;
-    const nss_hip_wnnm_params& p = *params;
+// Buffers of every active plane; h->ncsr and the parameters are set.
+int init_instance(nss_hip_wnnm* h, const nss_hip_wnnm_params& p, const nss_hip_video_info* info, int device,
+                  const char* name) {
+    const std::string ns = std::string("nss_hip.") + name;
     if (info->num_planes < 1 || info->num_planes > 3 || info->width < 1 || info->height < 1)
-        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: invalid video info");
+        return fail(NSS_HIP_INVALID_INPUT, ns + ": invalid video info");
     if (p.radius != 0 || p.block_size < 1 || p.block_size > 16 || p.group_size < 1 || p.group_size > 32 ||
         p.block_step < 1 || p.block_step > p.block_size || p.bm_range < 1 || p.bm_range > 64)
-        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: parameters not resolved by nss_hip_wnnm_parse");
+        return fail(NSS_HIP_INVALID_INPUT, ns + ": parameters not resolved by the parse call");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess) return hip_fail(e, "device count");
-    if (device < 0 || device >= ndev) return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: device_id out of range");
+    if (device < 0 || device >= ndev) return fail(NSS_HIP_INVALID_INPUT, ns + ": device_id out of range");
     DeviceGuard dg(device);
     if (dg.err != hipSuccess) return hip_fail(dg.err, "set device");
-    auto* h = new (std::nothrow) nss_hip_wnnm;
-    if (!h) return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip.WNNM: out of host memory");
     h->p = p;
     h->vi = *info;
     h->device = device;
@@ -81,7 +83,7 @@ int nss_hip_wnnm_create(const nss_hip_wnnm_params* params, const nss_hip_video_i
         q.active = p.sigma[i] != 0.f;
         if (!q.active) continue;
         if (q.width < p.block_size || q.height < p.block_size) {
-            delete h;
+            release(h);
             return fail(NSS_HIP_INVALID_INPUT, "nss: selected plane is smaller than block_size");
         }
         q.grid = make_grid(q.width, q.height, p.block_size, p.block_step);
@@ -90,56 +92,38 @@ int nss_hip_wnnm_create(const nss_hip_wnnm_params* params, const nss_hip_video_i
         const size_t rec_bytes = static_cast<size_t>(q.nref) * p.group_size * sizeof(MatchRec);
         const size_t cnt_bytes = static_cast<size_t>(q.nref) * sizeof(int);
         const size_t sum_bytes = pixels * sizeof(unsigned long long);
-        h->bytes += static_cast<long long>(rec_bytes + cnt_bytes + 2 * sum_bytes);
+        const size_t est_bytes = h->ncsr ? pixels * sizeof(float) : 0;
+        h->bytes += static_cast<long long>(rec_bytes + cnt_bytes + 2 * sum_bytes + est_bytes);
         if ((e = hipMalloc(&q.recs, rec_bytes)) != hipSuccess || (e = hipMalloc(&q.counts, cnt_bytes)) != hipSuccess ||
-            (e = hipMalloc(&q.num, sum_bytes)) != hipSuccess || (e = hipMalloc(&q.den, sum_bytes)) != hipSuccess) {
+            (e = hipMalloc(&q.num, sum_bytes)) != hipSuccess || (e = hipMalloc(&q.den, sum_bytes)) != hipSuccess ||
+            (h->ncsr && (e = hipMalloc(&q.est, est_bytes)) != hipSuccess)) {
             release(h);
-            delete h;
-            return hip_fail(e, "WNNM buffers");
+            return hip_fail(e, "group filter buffers");
         }
     }
     if (p.memory_limit_mb > 0 && h->bytes > static_cast<long long>(p.memory_limit_mb) << 20) {
         const long long need = (h->bytes + (1ll << 20) - 1) >> 20;
         release(h);
-        delete h;
-        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: memory_limit_mb is too small for this clip: the instance "
-                                           "needs at least " + std::to_string(need) + " MiB");
+        return fail(NSS_HIP_INVALID_INPUT, ns + ": memory_limit_mb is too small for this clip: the instance needs at "
+                                                "least " + std::to_string(need) + " MiB");
     }
-    *out = h;
     return NSS_HIP_OK;
 }
 
-int nss_hip_wnnm_process(nss_hip_wnnm* h, const float* const* src, const int* src_stride,
-                         const float* const* rclip, const int* rclip_stride, float* const* dst,
-                         const int* dst_stride, void* stream) {
-    if (!h || !src || !src_stride || !dst || !dst_stride || (rclip && !rclip_stride))
-        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: null argument");
-    const int np = h->vi.num_planes;
-    for (int i = 0; i < np; ++i) {
-        const auto& q = h->pl[i];
-        if (!src[i] || !dst[i] || (rclip && !rclip[i]))
-            return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: null plane pointer");
-        if (src_stride[i] < q.width || dst_stride[i] < q.width || (rclip && rclip_stride[i] < q.width))
-            return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: plane stride smaller than its width");
-    }
-    DeviceGuard dg(h->device);
-    if (dg.err != hipSuccess) return hip_fail(dg.err, "set device");
-    const hipStream_t s = static_cast<hipStream_t>(stream);
+// One round over every active plane: clear the sums, match on mplane[p],
+// shrink the groups of gsrc[p], finish into fdst[p] (fallback fsrc[p]).
+int group_round(nss_hip_wnnm* h, const float* const* mplane, const int* mstride, const float* const* gsrc,
+                const int* gstride, const float* const* fsrc, const int* fsstride, float* const* fdst,
+                const int* fdstride, hipStream_t s) {
     const nss_hip_wnnm_params& p = h->p;
     ZeroTable zt{};
     WnnmTable wt{};
     FinishTable ft{};
     int zero_wg = 0, wnnm_wg = 0, finish_wg = 0;
     hipError_t e;
-    for (int i = 0; i < np; ++i) {
+    for (int i = 0; i < h->vi.num_planes; ++i) {
         const auto& q = h->pl[i];
-        if (!q.active) {
-            e = hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, src[i],
-                                 static_cast<size_t>(src_stride[i]) * 4, static_cast<size_t>(q.width) * 4, q.height,
-                                 hipMemcpyDeviceToDevice, s);
-            if (e != hipSuccess) return hip_fail(e, "WNNM plane copy");
-            continue;
-        }
+        if (!q.active) continue;
         const long long pixels = static_cast<long long>(q.width) * q.height;
         for (unsigned long long* buf : {q.num, q.den}) {
             ZeroJob& z = zt.job[zt.n++];
@@ -149,15 +133,15 @@ int nss_hip_wnnm_process(nss_hip_wnnm* h, const float* const* src, const int* sr
             zero_wg += static_cast<int>((pixels + 1023) / 1024);
         }
         WnnmJob& w = wt.job[wt.n++];
-        w.src = src[i];
+        w.src = gsrc[i];
         w.width = q.width;
         w.height = q.height;
-        w.stride = src_stride[i];
+        w.stride = gstride[i];
         w.nref = q.nref;
         w.group = p.group_size;
         w.block = p.block_size;
         w.sigma = p.sigma[i] / 255.f;
-        w.residual = p.residual;
+        w.residual = h->ncsr ? 1 : p.residual;
         w.adaptive = p.adaptive;
         w.wg_begin = wnnm_wg;
         w.matches = q.recs;
@@ -169,26 +153,26 @@ int nss_hip_wnnm_process(nss_hip_wnnm* h, const float* const* src, const int* sr
         f.num = q.num;
         f.den = q.den;
         f.block = p.block_size;
-        f.src = src[i];
-        f.dst = dst[i];
+        f.src = fsrc[i];
+        f.dst = fdst[i];
         f.dst_den = nullptr;
         f.width = q.width;
         f.height = q.height;
-        f.sstride = src_stride[i];
-        f.dstride = dst_stride[i];
+        f.sstride = fsstride[i];
+        f.dstride = fdstride[i];
         f.tiles_x = (q.width + 15) / 16;
         f.wg_begin = finish_wg;
         finish_wg += f.tiles_x * ((q.height + 15) / 16);
     }
     if (wt.n == 0) return NSS_HIP_OK;
-    if ((e = launch_zero(zt, zero_wg, s)) != hipSuccess) return hip_fail(e, "WNNM clear");
-    for (int i = 0; i < np; ++i) {
+    if ((e = launch_zero(zt, zero_wg, s)) != hipSuccess) return hip_fail(e, "group filter clear");
+    for (int i = 0; i < h->vi.num_planes; ++i) {
         const auto& q = h->pl[i];
         if (!q.active) continue;
         MatchTable mt{};
         MatchJob& m = mt.job[mt.n++];
-        m.plane = rclip ? rclip[i] : src[i];
-        m.stride = rclip ? rclip_stride[i] : src_stride[i];
+        m.plane = mplane[i];
+        m.stride = mstride[i];
         m.width = q.width;
         m.height = q.height;
         m.grid = q.grid;
@@ -202,15 +186,167 @@ int nss_hip_wnnm_process(nss_hip_wnnm* h, const float* const* src, const int* sr
         const int kind =
             choose_match_kind(p.block_size, p.block_step, p.bm_range, p.group_size, q.width, q.height, q.nref, &lds);
         if ((e = launch_match(p.block_size, mt, match_workgroups(kind, q.grid), lds, kind, s)) != hipSuccess)
-            return hip_fail(e, "WNNM match launch");
+            return hip_fail(e, "group filter match launch");
     }
-    if ((e = launch_wnnm(wt, wnnm_wg, s)) != hipSuccess) return hip_fail(e, "WNNM group launch");
+    if ((e = launch_wnnm(wt, wnnm_wg, h->ncsr, s)) != hipSuccess) return hip_fail(e, "group filter launch");
     if ((e = launch_finish(p.block_size, ft, finish_wg, false, s)) != hipSuccess)
-        return hip_fail(e, "WNNM finish launch");
+        return hip_fail(e, "group filter finish launch");
     return NSS_HIP_OK;
 }
 
+int check_planes(const nss_hip_wnnm* h, const float* const* src, const int* src_stride, const float* const* rclip,
+                 const int* rclip_stride, float* const* dst, const int* dst_stride, const char* name) {
+    const std::string ns = std::string("nss_hip.") + name;
+    if (!src || !src_stride || !dst || !dst_stride || (rclip && !rclip_stride))
+        return fail(NSS_HIP_INVALID_INPUT, ns + ": null argument");
+    for (int i = 0; i < h->vi.num_planes; ++i) {
+        const auto& q = h->pl[i];
+        if (!src[i] || !dst[i] || (rclip && !rclip[i])) return fail(NSS_HIP_INVALID_INPUT, ns + ": null plane pointer");
+        if (src_stride[i] < q.width || dst_stride[i] < q.width || (rclip && rclip_stride[i] < q.width))
+            return fail(NSS_HIP_INVALID_INPUT, ns + ": plane stride smaller than its width");
+    }
+    return NSS_HIP_OK;
+}
+
+int copy_inactive(const nss_hip_wnnm* h, const float* const* src, const int* src_stride, float* const* dst,
+                  const int* dst_stride, hipStream_t s) {
+    for (int i = 0; i < h->vi.num_planes; ++i) {
+        const auto& q = h->pl[i];
+        if (q.active) continue;
+        const hipError_t e = hipMemcpy2DAsync(dst[i], static_cast<size_t>(dst_stride[i]) * 4, src[i],
+                                              static_cast<size_t>(src_stride[i]) * 4,
+                                              static_cast<size_t>(q.width) * 4, q.height, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "plane copy");
+    }
+    return NSS_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nss_hip_wnnm_create(const nss_hip_wnnm_params* params, const nss_hip_video_info* info, int device,
+                        nss_hip_wnnm** out) {
+    if (!params || !info || !out) return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: null argument");
+    *out = nullptr;
+    auto* h = new (std::nothrow) nss_hip_wnnm;
+    if (!h) return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip.WNNM: out of host memory");
+    const int rc = init_instance(h, *params, info, device, "WNNM");
+    if (rc != NSS_HIP_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return NSS_HIP_OK;
+}
+
+int nss_hip_wnnm_process(nss_hip_wnnm* h, const float* const* src, const int* src_stride,
+                         const float* const* rclip, const int* rclip_stride, float* const* dst,
+                         const int* dst_stride, void* stream) {
+    if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: null argument");
+    int rc = check_planes(h, src, src_stride, rclip, rclip_stride, dst, dst_stride, "WNNM");
+    if (rc != NSS_HIP_OK) return rc;
+    DeviceGuard dg(h->device);
+    if (dg.err != hipSuccess) return hip_fail(dg.err, "set device");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((rc = copy_inactive(h, src, src_stride, dst, dst_stride, s)) != NSS_HIP_OK) return rc;
+    return group_round(h, rclip ? rclip : src, rclip ? rclip_stride : src_stride, src, src_stride, src, src_stride,
+                       dst, dst_stride, s);
+}
+
 int nss_hip_wnnm_destroy(nss_hip_wnnm* h) {
+    if (!h) return NSS_HIP_OK;
+    {
+        DeviceGuard dg(h->device);
+        (void)hipDeviceSynchronize();
+        release(h);
+    }
+    delete h;
+    return NSS_HIP_OK;
+}
+
+int nss_hip_ncsr_create(const nss_hip_ncsr_params* params, const nss_hip_video_info* info, int device,
+                        nss_hip_ncsr** out) {
+    if (!params || !info || !out) return fail(NSS_HIP_INVALID_INPUT, "nss_hip.NCSR: null argument");
+    *out = nullptr;
+    if (params->iters < 1 || params->iters > 64 || !(params->delta >= 0.f && params->delta <= 1.f))
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.NCSR: parameters not resolved by the parse call");
+    auto* h = new (std::nothrow) nss_hip_ncsr;
+    if (!h) return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip.NCSR: out of host memory");
+    nss_hip_wnnm_params p{};
+    for (int i = 0; i < 3; ++i) p.sigma[i] = params->sigma[i];
+    p.block_size = params->block_size;
+    p.block_step = params->block_step;
+    p.group_size = params->group_size;
+    p.bm_range = params->bm_range;
+    p.radius = params->radius;
+    p.ps_num = params->ps_num;
+    p.ps_range = params->ps_range;
+    p.residual = 1;
+    p.adaptive = 0;
+    p.has_rclip = params->has_rclip;
+    p.memory_limit_mb = params->memory_limit_mb;
+    p.device_id = params->device_id;
+    h->ncsr = true;
+    h->iters = params->iters;
+    h->delta = params->delta;
+    const int rc = init_instance(h, p, info, device, "NCSR");
+    if (rc != NSS_HIP_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return NSS_HIP_OK;
+}
+
+// src/host/filter_ncsr.cpp:34-190 at radius 0: the estimate starts as the
+// noisy plane; round i > 0 relaxes it toward the noisy plane by delta; each
+// round matches on the estimate (on rclip in round 0), filters the
+// estimate's groups and finishes into the estimate (the last round into dst),
+// with the estimate where no patch landed.
+int nss_hip_ncsr_process(nss_hip_ncsr* h, const float* const* src, const int* src_stride,
+                         const float* const* rclip, const int* rclip_stride, float* const* dst,
+                         const int* dst_stride, void* stream) {
+    if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip.NCSR: null argument");
+    int rc = check_planes(h, src, src_stride, rclip, rclip_stride, dst, dst_stride, "NCSR");
+    if (rc != NSS_HIP_OK) return rc;
+    DeviceGuard dg(h->device);
+    if (dg.err != hipSuccess) return hip_fail(dg.err, "set device");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((rc = copy_inactive(h, src, src_stride, dst, dst_stride, s)) != NSS_HIP_OK) return rc;
+    const float* est_c[3] = {};
+    float* est_m[3] = {};
+    int est_stride[3] = {};
+    hipError_t e;
+    for (int i = 0; i < h->vi.num_planes; ++i) {
+        const auto& q = h->pl[i];
+        if (!q.active) continue;
+        est_c[i] = est_m[i] = q.est;
+        est_stride[i] = q.width;
+        e = hipMemcpy2DAsync(q.est, static_cast<size_t>(q.width) * 4, src[i], static_cast<size_t>(src_stride[i]) * 4,
+                             static_cast<size_t>(q.width) * 4, q.height, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "NCSR estimate copy");
+    }
+    for (int it = 0; it < h->iters; ++it) {
+        if (it > 0) {
+            for (int i = 0; i < h->vi.num_planes; ++i) {
+                const auto& q = h->pl[i];
+                if (!q.active) continue;
+                if ((e = launch_regularize(q.est, q.width, src[i], src_stride[i], q.width, q.height, h->delta, s)) !=
+                    hipSuccess)
+                    return hip_fail(e, "NCSR regularize launch");
+            }
+        }
+        const bool guide = it == 0 && rclip;
+        const bool last = it + 1 == h->iters;
+        rc = group_round(h, guide ? rclip : est_c, guide ? rclip_stride : est_stride, est_c, est_stride, est_c,
+                         est_stride, last ? dst : est_m, last ? dst_stride : est_stride, s);
+        if (rc != NSS_HIP_OK) return rc;
+    }
+    return NSS_HIP_OK;
+}
+
+int nss_hip_ncsr_destroy(nss_hip_ncsr* h) {
     if (!h) return NSS_HIP_OK;
     {
         DeviceGuard dg(h->device);

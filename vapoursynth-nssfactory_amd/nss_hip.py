@@ -106,6 +106,16 @@ class WnnmParams(ctypes.Structure):
         return out
 
 
+class NcsrParams(ctypes.Structure):
+    _fields_ = [("sigma", _arr(ctypes.c_float, 3)), ("block_size", ctypes.c_int), ("block_step", ctypes.c_int),
+                ("group_size", ctypes.c_int), ("bm_range", ctypes.c_int), ("radius", ctypes.c_int),
+                ("ps_num", ctypes.c_int), ("ps_range", ctypes.c_int), ("iters", ctypes.c_int),
+                ("delta", ctypes.c_float), ("has_rclip", ctypes.c_int), ("memory_limit_mb", ctypes.c_int),
+                ("device_id", ctypes.c_int)]
+
+    as_dict = WnnmParams.as_dict
+
+
 _lib = None
 
 EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "nss_hip_bm3d_output_info",
@@ -114,7 +124,8 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_bm3d_kernel_times", "nss_hip_bm3d_push", "nss_hip_bm3d_pop", "nss_hip_server_create", "nss_hip_server_open_stream", "nss_hip_server_submit",
            "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy",
            "nss_hip_bm3d_autotune", "nss_hip_vaggregate", "nss_hip_bm3d_resources", "nss_hip_wnnm_parse",
-           "nss_hip_wnnm_create", "nss_hip_wnnm_process", "nss_hip_wnnm_destroy")
+           "nss_hip_wnnm_create", "nss_hip_wnnm_process", "nss_hip_wnnm_destroy", "nss_hip_ncsr_parse",
+           "nss_hip_ncsr_create", "nss_hip_ncsr_process", "nss_hip_ncsr_destroy")
 # Contribution identity of legacy temporal frames (frame properties the
 # VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
 # src/frontend/contribution.hpp:12-27).
@@ -174,6 +185,10 @@ def lib():
         L.nss_hip_wnnm_create.argtypes = [P(WnnmParams), P(VideoInfo), ci, P(vp)]
         L.nss_hip_wnnm_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_wnnm_destroy.argtypes = [vp]
+        L.nss_hip_ncsr_parse.argtypes = [P(VideoInfo), P(VideoInfo), P(Arg), ci, ctypes.c_char_p, P(NcsrParams)]
+        L.nss_hip_ncsr_create.argtypes = [P(NcsrParams), P(VideoInfo), ci, P(vp)]
+        L.nss_hip_ncsr_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), vp]
+        L.nss_hip_ncsr_destroy.argtypes = [vp]
         for name in EXPORTS[1:]:
             getattr(L, name).restype = ci
         _lib = L
@@ -212,6 +227,14 @@ WNNM_SIGNATURE = ("clip:vnode;sigma:float[]:opt;block_size:int:opt;block_step:in
                   "temporal_mode:data:opt;rolling_chunk:int:opt;rolling_cache_chunks:int:opt;"
                   "rolling_cache_limit:int:opt;device_id:int:opt;")
 _WNNM_TYPES = _signature_types(WNNM_SIGNATURE)
+# nss.NCSR's (src/frontend/ncsr_args.hpp signature: WNNM's group arguments
+# with iters and delta in place of residual and adaptive_aggregation).
+NCSR_SIGNATURE = ("clip:vnode;sigma:float[]:opt;block_size:int:opt;block_step:int:opt;group_size:int:opt;"
+                  "bm_range:int:opt;radius:int:opt;ps_num:int:opt;ps_range:int:opt;iters:int:opt;delta:float:opt;"
+                  "rclip:vnode:opt;memory_limit_mb:int:opt;"
+                  "temporal_mode:data:opt;rolling_chunk:int:opt;rolling_cache_chunks:int:opt;"
+                  "rolling_cache_limit:int:opt;device_id:int:opt;")
+_NCSR_TYPES = _signature_types(NCSR_SIGNATURE)
 
 
 def _build_args(kwargs, types=None, name="BM3D"):
@@ -430,20 +453,40 @@ def parse_wnnm(fmt, width, height, frames=1, rclip=False, info=None, **kwargs) -
     return out
 
 
+def parse_ncsr(fmt, width, height, frames=1, rclip=False, info=None, **kwargs) -> NcsrParams:
+    """Resolve nss.NCSR arguments; raises Error with the reference's text."""
+    vi = info if info is not None else video_info(fmt, width, height, frames)
+    arr, n, keep = _build_args(kwargs, _NCSR_TYPES, "NCSR")
+    out = NcsrParams()
+    rc = lib().nss_hip_ncsr_parse(ctypes.byref(vi), ctypes.byref(vi) if rclip else None, arr, n,
+                                  NAMESPACE.encode(), ctypes.byref(out))
+    del keep
+    _check(rc)
+    return out
+
+
 class WNNM:
     """nss.WNNM on the device (src/cuda/wnnm/filter.cpp:16-70): one instance
     per format and size. Calling it filters one frame: ``planes`` is a list
     of 2-D float32 CUDA tensors; ``rclip`` (instances made with
     ``rclip=True``) guides the matching only. Radius 0 (spatial)."""
 
+    _NAME = "WNNM"
+
+    def _parse(self, fmt, width, height, frames, rclip, **kwargs):
+        return parse_wnnm(fmt, width, height, frames, rclip=rclip, info=self.info, **kwargs)
+
+    def _call(self, what, *args):
+        return getattr(lib(), f"nss_hip_{self._NAME.lower()}_{what}")(*args)
+
     def __init__(self, fmt, width, height, frames=1, rclip=False, **kwargs):
         self.fmt = fmt
         self.info = video_info(fmt, width, height, frames)
-        self.params = parse_wnnm(fmt, width, height, frames, rclip=rclip, info=self.info, **kwargs)
+        self.params = self._parse(fmt, width, height, frames, rclip, **kwargs)
         self.device = self.params.device_id
         h = ctypes.c_void_p()
-        _check(lib().nss_hip_wnnm_create(ctypes.byref(self.params), ctypes.byref(self.info), self.device,
-                                         ctypes.byref(h)))
+        _check(self._call("create", ctypes.byref(self.params), ctypes.byref(self.info), self.device,
+                          ctypes.byref(h)))
         self._h = h
         cf, sw, sh, n = FORMATS[fmt]
         self.plane_dims = [(height >> (sh if i else 0), width >> (sw if i else 0)) for i in range(n)]
@@ -452,12 +495,13 @@ class WNNM:
         import torch
         n = len(self.plane_dims)
         if len(planes) != n:
-            raise Error(f"{NAMESPACE}.WNNM: expected {n} {what}s, got {len(planes)}")
+            raise Error(f"{NAMESPACE}.{self._NAME}: expected {n} {what}s, got {len(planes)}")
         for i, p in enumerate(planes):
             if p.dtype != torch.float32 or not p.is_cuda or p.dim() != 2 or p.stride(1) != 1:
-                raise Error(f"{NAMESPACE}.WNNM: {what} {i} must be a row-contiguous 2-D float32 CUDA tensor")
+                raise Error(f"{NAMESPACE}.{self._NAME}: {what} {i} must be a row-contiguous 2-D float32 CUDA tensor")
             if tuple(p.shape) != self.plane_dims[i]:
-                raise Error(f"{NAMESPACE}.WNNM: {what} {i} has shape {tuple(p.shape)}, expected {self.plane_dims[i]}")
+                raise Error(f"{NAMESPACE}.{self._NAME}: {what} {i} has shape {tuple(p.shape)}, "
+                            f"expected {self.plane_dims[i]}")
 
     def __call__(self, planes, rclip=None, out=None, stream=None):
         import torch
@@ -465,7 +509,7 @@ class WNNM:
         self._check_planes(planes)
         if rclip is not None:
             if not self.params.has_rclip:
-                raise Error(f"{NAMESPACE}.WNNM: this instance was made without rclip")
+                raise Error(f"{NAMESPACE}.{self._NAME}: this instance was made without rclip")
             self._check_planes(rclip, "rclip plane")
         if out is None:
             out = [torch.empty(self.plane_dims[i], dtype=torch.float32, device=planes[i].device) for i in range(n)]
@@ -476,12 +520,12 @@ class WNNM:
         rst = IP(*[p.stride(0) for p in rclip]) if rclip is not None else None
         dp = VP(*[p.data_ptr() for p in out])
         dst = IP(*[p.stride(0) for p in out])
-        _check(lib().nss_hip_wnnm_process(self._h, src, sst, rp, rst, dp, dst, _stream_handle(stream)))
+        _check(self._call("process", self._h, src, sst, rp, rst, dp, dst, _stream_handle(stream)))
         return out
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().nss_hip_wnnm_destroy(self._h)
+            self._call("destroy", self._h)
             self._h = None
 
     def __del__(self):
@@ -489,6 +533,16 @@ class WNNM:
             self.close()
         except Exception:
             pass
+
+
+class NCSR(WNNM):
+    """nss.NCSR on the device (src/cuda/ncsr/filter.cpp:16-80): the WNNM
+    instance's interface with NCSR's arguments (iters, delta)."""
+
+    _NAME = "NCSR"
+
+    def _parse(self, fmt, width, height, frames, rclip, **kwargs):
+        return parse_ncsr(fmt, width, height, frames, rclip=rclip, info=self.info, **kwargs)
 
 
 def grid_size(width, height, block, step):
