@@ -433,9 +433,10 @@ def wnnm_workload(args):
     """nss.WNNM (the sibling group filter on the same matchers) on one
     1920x1080 GRAYS frame per step, sigma 10: the reference's defaults
     (block 8, step 8, group 8, range 7; include/nss/params.hpp:61-75), and a
-    denser form (step 4, group 16, range 9, residual 1) beside it. The WNNM
-    oracle (float64 SVD, one thread) runs a 640x360 crop as the CPU
-    baseline and the parity check."""
+    denser form (step 4, group 16, range 9, residual 1) beside it, and NCSR
+    at its defaults (the same shape, 2 rounds, delta 0.1;
+    include/nss/params.hpp:101-107). The oracles (float64 SVD, one thread)
+    run a 640x360 crop as the CPU baseline and the parity check."""
     import numpy as np
     import torch
 
@@ -447,10 +448,15 @@ def wnnm_workload(args):
     dev = [torch.from_numpy(p).cuda() for p in noisy]
     stream = torch.cuda.current_stream()
     forms = {"defaults": dict(sigma=[10.0]),
-             "dense": dict(sigma=[10.0], block_step=4, group_size=16, bm_range=9, residual=1)}
+             "dense": dict(sigma=[10.0], block_step=4, group_size=16, bm_range=9, residual=1),
+             "ncsr_defaults": dict(sigma=[10.0])}
 
-    def run(kw, steps, warmup):
-        inst = nss_hip.WNNM("GRAYS", w, h, **kw)
+    def make(name, kw, width, height):
+        cls = nss_hip.NCSR if name.startswith("ncsr") else nss_hip.WNNM
+        return cls("GRAYS", width, height, **kw)
+
+    def run(name, kw, steps, warmup):
+        inst = make(name, kw, w, h)
         out = [torch.empty_like(p) for p in dev]
         for _ in range(warmup):
             inst(dev, out=out)
@@ -472,7 +478,7 @@ def wnnm_workload(args):
 
     results = {}
     for name, kw in forms.items():
-        wall, lat, got, same = run(kw, args.steps, args.warmup)
+        wall, lat, got, same = run(name, kw, args.steps, args.warmup)
         results[name] = {"ms_per_step": round(wall / args.steps * 1e3, 4),
                          "value": round(w * h * args.steps / wall / 1e6, 3),
                          "p50_ms": round(float(np.percentile(lat, 50)), 4),
@@ -484,12 +490,15 @@ def wnnm_workload(args):
     cw, ch = 640, 360
     crop = np.ascontiguousarray(noisy[0][:ch, :cw])
     for name, kw in forms.items():
-        inst = nss_hip.WNNM("GRAYS", cw, ch, **kw)
+        inst = make(name, kw, cw, ch)
         g = inst([torch.from_numpy(crop).cuda()])[0].cpu().numpy()
         inst.close()
         t0 = time.perf_counter()
-        want = oracle.wnnm_plane(crop, 10.0, step=kw.get("block_step", 8), group=kw.get("group_size", 8),
-                                 bm_range=kw.get("bm_range", 7), residual=kw.get("residual", 0))
+        if name.startswith("ncsr"):
+            want = oracle.ncsr_plane(crop, 10.0)
+        else:
+            want = oracle.wnnm_plane(crop, 10.0, step=kw.get("block_step", 8), group=kw.get("group_size", 8),
+                                     bm_range=kw.get("bm_range", 7), residual=kw.get("residual", 0))
         dt = time.perf_counter() - t0
         results[name]["psnr_vs_cpu"] = round(oracle.psnr(g, want), 3)
         results[name]["cpu_mpix_s"] = round(cw * ch / dt / 1e6, 4)
