@@ -147,3 +147,15 @@ def test_wnnm_1080p_defaults_against_oracle(gpu, oracle):
 def test_wnnm_memory_limit(gpu):
     with pytest.raises(gpu.Error, match="memory_limit_mb is too small"):
         gpu.WNNM("GRAYS", 1920, 1080, sigma=[5.0], memory_limit_mb=1)
+
+
+def test_wnnm_group_capacity_changes_no_bit(gpu):
+    # With bm_range 1 a reference has at most nine matches; the Jacobi
+    # schedule follows the group's own column count, so groups of 9, 16 and
+    # 32 (padded to 16 and 32 lanes of the eigen step) give the same bits.
+    src, _ = _plane(40, 48, 9)
+    for block in (1, 4, 8):
+        kw = dict(sigma=[10.0], block_size=block, block_step=block, bm_range=1, residual=1)
+        ref = _run(gpu, "GRAYS", [src], group_size=9, **kw)[0]
+        for capacity in (16, 32):
+            assert np.array_equal(_run(gpu, "GRAYS", [src], group_size=capacity, **kw)[0], ref)

@@ -2945,11 +2945,14 @@ __device__ __forceinline__ void gram_group_body(const WnnmTable& tab) {
             // next round's coefficients overwrite these unread.
             if (!__any(rot)) continue;
             float ng[E], nv[E];
+            // A lane's entries share their column (64 is a multiple of N).
+            const int b = lane & (N - 1), pb = rr_partner(b, r, ke);
+            const float cb = cself[b], sb = cpart[b];
 #pragma unroll
             for (int t = 0; t < E; ++t) {
-                const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
-                const int pa = rr_partner(a, r, ke), pb = rr_partner(b, r, ke);
-                const float ca = cself[a], sa = cpart[a], cb = cself[b], sb = cpart[b];
+                const int a = (lane + 64 * t) >> LOG_N;
+                const int pa = rr_partner(a, r, ke);
+                const float ca = cself[a], sa = cpart[a];
                 const float gab = G[a * GP + b];
                 // Selected, not branched: the three cases share the wave.
                 const float vgen =
@@ -2962,7 +2965,7 @@ __device__ __forceinline__ void gram_group_body(const WnnmTable& tab) {
             __syncthreads();
 #pragma unroll
             for (int t = 0; t < E; ++t) {
-                const int idx = lane + 64 * t, a = idx >> LOG_N, b = idx & (N - 1);
+                const int a = (lane + 64 * t) >> LOG_N;
                 G[a * GP + b] = ng[t];
                 V[a * GP + b] = nv[t];
             }
