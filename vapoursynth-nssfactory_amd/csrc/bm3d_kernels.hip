@@ -2909,6 +2909,49 @@ __device__ __forceinline__ void gram_group_body(const WnnmTable& tab) {
     const float floor_abs = kWnnmFloor * trace;
 
     const int ke = k + (k & 1);  // the schedule's size: k rounded up to even
+    if constexpr (N == 8) {
+        // One entry per lane, in registers: lane (a, b) = 8a + b holds G[a][b]
+        // and V[a][b]; the same rotations as the LDS form below, with the
+        // neighbours' entries read across lanes (ds_bpermute) and no barriers.
+        // Each lane forms its row index's rotation; column b's is row b's.
+        const int a = lane >> 3, b = lane & 7;
+        float g = G[a * GP + b], v = a == b ? 1.f : 0.f;
+        for (int sweep = 0; sweep < kWnnmSweeps; ++sweep) {
+            bool rotated = false;
+            for (int r = 0; r < ke - 1; ++r) {
+                const int pa = rr_partner(a, r, ke), pb = rr_partner(b, r, ke);
+                const int p0 = min(a, pa), q0 = max(a, pa);
+                const float app = __shfl(g, p0 * 9, 64), aqq = __shfl(g, q0 * 9, 64);
+                const float apq = __shfl(g, p0 * 8 + q0, 64);
+                float c = 1.f, sn = 0.f, tn = 0.f;
+                bool rot = false;
+                if (q0 < k && fabsf(apq) > floor_abs && apq * apq > kWnnmTol * kWnnmTol * fabsf(app * aqq)) {
+                    const float delta = (aqq - app) * 0.5f;
+                    const float root = __builtin_amdgcn_sqrtf(fmaf(delta, delta, apq * apq));
+                    tn = apq * __builtin_amdgcn_rcpf(delta + copysignf(root, delta));
+                    c = __builtin_amdgcn_rsqf(fmaf(tn, tn, 1.f));
+                    sn = tn * c;
+                    rot = true;
+                }
+                if (!__any(rot)) continue;
+                rotated = true;
+                const float ca = c, sa = a == p0 ? -sn : sn, ta = a == p0 ? -tn : tn;
+                const float cb = __shfl(ca, b * 8, 64), sb = __shfl(sa, b * 8, 64);
+                const float g_apb = __shfl(g, a * 8 + pb, 64), v_apb = __shfl(v, a * 8 + pb, 64);
+                const float g_pab = __shfl(g, pa * 8 + b, 64), g_papb = __shfl(g, pa * 8 + pb, 64);
+                const float vgen = ca * fmaf(cb, g, sb * g_apb) + sa * fmaf(cb, g_pab, sb * g_papb);
+                const float vdiag = fmaf(ta, g_apb, g);  // at a == b, pb == pa
+                const float vpair = sa != 0.f ? 0.f : g;
+                const float gn = a == b ? vdiag : (b == pa ? vpair : vgen);
+                v = fmaf(cb, v, sb * v_apb);
+                g = gn;
+            }
+            if (!rotated) break;
+        }
+        G[a * GP + b] = g;
+        V[a * GP + b] = v;
+        __syncthreads();
+    } else {
     for (int sweep = 0; sweep < kWnnmSweeps; ++sweep) {
         bool rotated = false;
         for (int r = 0; r < ke - 1; ++r) {
@@ -2972,6 +3015,7 @@ __device__ __forceinline__ void gram_group_body(const WnnmTable& tab) {
             __syncthreads();
         }
         if (!__any(rotated)) break;
+    }
     }
 
     // Spectrum: s_j and its place in descending order (ties by index).
