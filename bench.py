@@ -757,8 +757,8 @@ def main():
                                                     "own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; each frame is independent, so N GPUs would each run this "
-              "workload on their own frames (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: measures one GPU; each frame is independent, so N GPUs would each run this "
+              "workload on their own frames; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     defaults = {"two-stage-1080p": (200, 500), "temporal-1080p": (2, 1), "precision-4k": (20, 50),

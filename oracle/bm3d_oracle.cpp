@@ -1,6 +1,7 @@
 // CPU oracle for the BM3D hot path. TEST INFRASTRUCTURE ONLY: the parity
 // tests, the smoke check and the CPU-baseline leg of the benchmark call it as
-// the checker; the product library never links, loads or calls it.
+// the reference results are compared against; the product library never
+// links, loads or calls it.
 //
 // A from-scratch restatement of the reference CPU plugin's nss.BM3D
 // algorithm (numeric build NSS_BM_EXPERIMENT=0), in two arithmetic modes:

@@ -1,8 +1,8 @@
 """CPU oracle for the BM3D hot path: ctypes wrapper over ``bm3d_oracle.cpp``.
 
 TEST INFRASTRUCTURE ONLY. Only the tests, the smoke check and the benchmark's
-CPU-baseline leg use this module, as the checker. The product library never
-imports it.
+CPU-baseline leg use this module, as the reference that results are compared
+against. The product library never imports it.
 
 Frame-level semantics follow the reference CPU plugin:
 - per plane, sigma == 0 copies the plane (src/host/filter_bm3d.cpp:402-412);

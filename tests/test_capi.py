@@ -101,7 +101,7 @@ def test_wnnm_parse_messages_and_null_checks():
         assert str(e.value) == text
     with pytest.raises(nss_hip.Error, match="selected plane is smaller than block_size"):
         nss_hip.parse_wnnm("GRAYS", 6, 6)
-    with pytest.raises(nss_hip.Error, match="not available in this build"):
+    with pytest.raises(nss_hip.Error, match="not available in nss_hip"):
         nss_hip.parse_wnnm("GRAYS", 64, 48, radius=1)
     assert lib.nss_hip_wnnm_create(None, None, 0, None) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_wnnm_process(None, None, None, None, None, None, None, None) == nss_hip.INVALID_INPUT
