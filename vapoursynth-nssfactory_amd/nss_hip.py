@@ -293,6 +293,20 @@ def backend_info(device: int = 0) -> str:
     return buf.value.decode()
 
 
+def _check_out(name, out, dims):
+    """Output tensors: one per plane, row-contiguous 2-D float32 on the
+    device, of the plane's shape (a smaller buffer would be written past
+    its end)."""
+    import torch
+    if len(out) != len(dims):
+        raise Error(f"{NAMESPACE}.{name}: expected {len(dims)} output planes, got {len(out)}")
+    for i, p in enumerate(out):
+        if p.dtype != torch.float32 or not p.is_cuda or p.dim() != 2 or p.stride(1) != 1:
+            raise Error(f"{NAMESPACE}.{name}: output plane {i} must be a row-contiguous 2-D float32 CUDA tensor")
+        if tuple(p.shape) != tuple(dims[i]):
+            raise Error(f"{NAMESPACE}.{name}: output plane {i} has shape {tuple(p.shape)}, expected {tuple(dims[i])}")
+
+
 def _stream_handle(stream):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -383,6 +397,7 @@ class BM3D:
         if out is None:
             out = [torch.empty(d, dtype=torch.float32, device=torch.device("cuda", self.device))
                    for d in self.out_dims]
+        _check_out("BM3D", out, self.out_dims)
         dp = (ctypes.c_void_p * n)(*[p.data_ptr() for p in out])
         dst = (ctypes.c_int * n)(*[p.stride(0) for p in out])
         frame = ctypes.c_int(-1)
@@ -425,6 +440,7 @@ class BM3D:
             self._check_planes(ref, "ref plane")
         if out is None:
             out = [torch.empty(self.plane_dims[i], dtype=torch.float32, device=planes[i].device) for i in range(n)]
+        _check_out("BM3D", out, self.plane_dims)
         VP = ctypes.c_void_p * len(planes)
         IP = ctypes.c_int * len(planes)
         src = VP(*[p.data_ptr() for p in planes])
@@ -513,6 +529,7 @@ class WNNM:
             self._check_planes(rclip, "rclip plane")
         if out is None:
             out = [torch.empty(self.plane_dims[i], dtype=torch.float32, device=planes[i].device) for i in range(n)]
+        _check_out(self._NAME, out, self.plane_dims)
         VP, IP = ctypes.c_void_p * n, ctypes.c_int * n
         src = VP(*[p.data_ptr() for p in planes])
         sst = IP(*[p.stride(0) for p in planes])
