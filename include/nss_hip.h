@@ -275,9 +275,8 @@ typedef struct nss_hip_wnnm nss_hip_wnnm;
 /* Argument resolution of <ns>.WNNM (src/frontend/wnnm_args.cpp:8-29
  * parse_wnnm, then src/cuda/wnnm/filter.cpp:26-33: rclip must match clip,
  * validate_group_planes of src/frontend/validate.hpp:19-25), with the first
- * error's text in nss_hip_last_error(). nss_hip runs radius 0 (spatial)
- * and the matcher's block sizes 1, 2, 4, 8, 12 and 16; other values the
- * reference accepts are rejected here with a message that says so. */
+ * error's text in nss_hip_last_error(). nss_hip runs radius 0 (spatial);
+ * radius > 0 is rejected with a message that says so. */
 int nss_hip_wnnm_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
                        const nss_hip_arg* args, int nargs, const char* ns, nss_hip_wnnm_params* out);
 /* Instance for a clip of `info` (the reference's create,
@@ -321,8 +320,7 @@ typedef struct nss_hip_ncsr nss_hip_ncsr;
 
 /* <ns>.NCSR's argument resolution (src/frontend/ncsr_args.cpp:10-32 parse_ncsr
  * after parse_temporal_mode and the memory limit; rclip and plane checks of
- * src/cuda/ncsr/filter.cpp:26-34). Radius 0 and block sizes 1, 2, 4, 8, 12, 16
- * in nss_hip, as for WNNM. */
+ * src/cuda/ncsr/filter.cpp:26-34). Radius 0, as for WNNM. */
 int nss_hip_ncsr_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
                        const nss_hip_arg* args, int nargs, const char* ns, nss_hip_ncsr_params* out);
 int nss_hip_ncsr_create(const nss_hip_ncsr_params* params, const nss_hip_video_info* info, int device,

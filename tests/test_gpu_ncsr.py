@@ -50,6 +50,9 @@ CASES = [
     dict(block_size=16, block_step=8, group_size=8, bm_range=5),
     dict(block_size=12, block_step=6, group_size=12, bm_range=4),
     dict(block_size=2, block_step=1, group_size=4, bm_range=3),
+    dict(block_size=3, block_step=2, group_size=8, bm_range=4),
+    dict(block_size=6, block_step=3, group_size=12, bm_range=5),
+    dict(block_size=11, block_step=5, group_size=8, bm_range=3),
 ]
 
 

@@ -43,6 +43,9 @@ CASES = [
     dict(block_size=2, block_step=1, group_size=4, bm_range=3, residual=1),
     dict(block_size=1, block_step=1, group_size=5, bm_range=2),
     dict(block_size=16, block_step=4, group_size=32, bm_range=4, residual=1),
+    dict(block_size=5, block_step=3, group_size=8, bm_range=4),
+    dict(block_size=7, block_step=7, group_size=16, bm_range=5, residual=1),
+    dict(block_size=15, block_step=6, group_size=8, bm_range=3),
 ]
 
 

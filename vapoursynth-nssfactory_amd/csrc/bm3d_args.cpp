@@ -502,7 +502,7 @@ int nss_hip_bm3d_output_info(const nss_hip_bm3d_params* p, const nss_hip_video_i
 // representability and memory_limit_mb (src/frontend/validate.hpp:33-45),
 // temporal_mode (src/frontend/temporal_args.cpp:11-28), parse_wnnm, rclip
 // against clip, validate_group_planes (src/frontend/validate.hpp:19-25); then
-// nss_hip's own limits (radius 0; the matchers' block sizes).
+// nss_hip's own limit (radius 0).
 int nss_hip_wnnm_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
                        const nss_hip_arg* args, int nargs, const char* ns_c, nss_hip_wnnm_params* out) {
     const std::string ns = ns_c ? ns_c : "nss_hip";
@@ -537,8 +537,6 @@ int nss_hip_wnnm_parse(const nss_hip_video_info* clip_info, const nss_hip_video_
                 throw Invalid("nss: selected plane is smaller than block_size");
         }
         if (p.radius > 0) fail_wnnm(ns, "radius > 0 (temporal WNNM) is not available in nss_hip; use radius=0");
-        if (!allowed_block(p.block_size) || p.block_size > kWnnmMaxBlock)
-            fail_wnnm(ns, "block_size must be one of 1, 2, 4, 8, 12, 16 in nss_hip");
         const std::int64_t device = in.get_int("device_id", 0, &err);
         if (!err && (device < 0 || device > INT32_MAX)) fail_wnnm(ns, "device_id must be >= 0");
         p.device_id = err ? 0 : static_cast<int>(device);
@@ -586,8 +584,6 @@ int nss_hip_ncsr_parse(const nss_hip_video_info* clip_info, const nss_hip_video_
                 throw Invalid("nss: selected plane is smaller than block_size");
         }
         if (p.radius > 0) fail_ncsr(ns, "radius > 0 (temporal NCSR) is not available in nss_hip; use radius=0");
-        if (!allowed_block(p.block_size) || p.block_size > kWnnmMaxBlock)
-            fail_ncsr(ns, "block_size must be one of 1, 2, 4, 8, 12, 16 in nss_hip");
         const std::int64_t device = in.get_int("device_id", 0, &err);
         if (!err && (device < 0 || device > INT32_MAX)) fail_ncsr(ns, "device_id must be >= 0");
         p.device_id = err ? 0 : static_cast<int>(device);

@@ -3293,10 +3293,36 @@ static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, 
     return hipGetLastError();
 }
 
+// Block sizes outside the BM3D set (the group filters take any block from 1
+// to 16): the wave matcher only, window in LDS (kind 1) or from global (0).
+template <int B>
+static hipError_t launch_match_wave(const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
+    if (kind == 1) {
+        allow_lds(&match_wave_kernel<B, true>, lds);
+        match_wave_kernel<B, true><<<total_wg, 64, lds, s>>>(t);
+    } else if (kind == 0) {
+        allow_lds(&match_wave_kernel<B, false>, lds);
+        match_wave_kernel<B, false><<<total_wg, 64, lds, s>>>(t);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // kind: 3 MFMA matcher (B = 8), 2 run matcher, 1 wave matcher with the
 // window in LDS, 0 from global.
 hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
     switch (block) {
+        case 3: return launch_match_wave<3>(t, total_wg, lds, kind, s);
+        case 5: return launch_match_wave<5>(t, total_wg, lds, kind, s);
+        case 6: return launch_match_wave<6>(t, total_wg, lds, kind, s);
+        case 7: return launch_match_wave<7>(t, total_wg, lds, kind, s);
+        case 9: return launch_match_wave<9>(t, total_wg, lds, kind, s);
+        case 10: return launch_match_wave<10>(t, total_wg, lds, kind, s);
+        case 11: return launch_match_wave<11>(t, total_wg, lds, kind, s);
+        case 13: return launch_match_wave<13>(t, total_wg, lds, kind, s);
+        case 14: return launch_match_wave<14>(t, total_wg, lds, kind, s);
+        case 15: return launch_match_wave<15>(t, total_wg, lds, kind, s);
         case 1: return launch_match_b<1>(t, total_wg, lds, kind, s);
         case 2: return launch_match_b<2>(t, total_wg, lds, kind, s);
         case 4: return launch_match_b<4>(t, total_wg, lds, kind, s);
@@ -3397,6 +3423,16 @@ hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool til
         case 12: finish_corner_kernel<12><<<total_wg, 256, 0, s>>>(t); break;
         case 16: finish_corner_kernel<16><<<total_wg, 256, 0, s>>>(t); break;
         case 32: finish_corner_kernel<32><<<total_wg, 256, 0, s>>>(t); break;
+        case 3: finish_corner_kernel<3><<<total_wg, 256, 0, s>>>(t); break;
+        case 5: finish_corner_kernel<5><<<total_wg, 256, 0, s>>>(t); break;
+        case 6: finish_corner_kernel<6><<<total_wg, 256, 0, s>>>(t); break;
+        case 7: finish_corner_kernel<7><<<total_wg, 256, 0, s>>>(t); break;
+        case 9: finish_corner_kernel<9><<<total_wg, 256, 0, s>>>(t); break;
+        case 10: finish_corner_kernel<10><<<total_wg, 256, 0, s>>>(t); break;
+        case 11: finish_corner_kernel<11><<<total_wg, 256, 0, s>>>(t); break;
+        case 13: finish_corner_kernel<13><<<total_wg, 256, 0, s>>>(t); break;
+        case 14: finish_corner_kernel<14><<<total_wg, 256, 0, s>>>(t); break;
+        case 15: finish_corner_kernel<15><<<total_wg, 256, 0, s>>>(t); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

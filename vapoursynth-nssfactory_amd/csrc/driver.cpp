@@ -57,7 +57,10 @@ int ensure_tables(int device) {
 
 int choose_match_kind(int block, int step, int range, int group, int width, int height, long long nref,
                       size_t* lds) {
-    const bool run = block <= 12 && match_run_lds_bytes(block, step, range) <= kRunLdsBudget;
+    // The run matcher is built for the BM3D block sizes up to 12; other sizes
+    // (the group filters' 3, 5-7, 9-11, 13-15) take the wave matcher.
+    const bool run_block = block == 1 || block == 2 || block == 4 || block == 8 || block == 12;
+    const bool run = run_block && match_run_lds_bytes(block, step, range) <= kRunLdsBudget;
     if (match_path() != 1 && match_mfma_fits(block, group, step, range, width, height) &&
         (match_path() == 2 || !run || (match_mfma_pays(step, range, group) && nref >= kMfmaMinRefs))) {
         *lds = match_mfma_lds_bytes(step, range, group);
