@@ -208,6 +208,13 @@ int nss_hip_vaggregate(float* dst, int dst_stride, const float* const* nums, con
  * Radius > 0 and ref-only instances have no tile plan (OK, nothing done). */
 int nss_hip_bm3d_autotune(nss_hip_bm3d* handle, int repeats, void* stream, double* best_ms);
 
+/* Tile shape of a stage's filter (no counterpart in the reference): every
+ * tile-path bucket of stage (0 first) takes tiles of tw x th references;
+ * NSS_HIP_UNSUPPORTED when they do not fit its LDS budget. The output does not
+ * depend on the shape (tile partials are exact integer sums); the shape only
+ * changes the speed. Waits for the instance's queued work. */
+int nss_hip_bm3d_set_tiles(nss_hip_bm3d* handle, int stage, int tw, int th);
+
 #define NSS_HIP_KERNEL_KINDS 4
 int nss_hip_bm3d_set_tracing(nss_hip_bm3d* handle, int on);
 int nss_hip_bm3d_kernel_times(nss_hip_bm3d* handle, int* launches, double* total_ms, int reset);
@@ -236,6 +243,23 @@ int nss_hip_bm3d_destroy(nss_hip_bm3d* handle);
  * nref = nx * ny with nx = (width - block + step - 1) / step + 1, ny alike. */
 int nss_hip_spatial_match(const float* plane, int width, int height, int stride, int block, int step, int bm_range,
                           int group, int* out_rec, int* out_count, void* stream);
+
+/* Batched temporal block matching over one plane's reference grid
+ * (predictive_match for every reference: src/cpu/bm/match.cpp:10-33,
+ * collect_temporal src/cpu/bm/matcher.hpp:319-373; the batch of
+ * predictive_match_batch, src/cpu/batch.cpp:204-241). frames: device pointer
+ * to nframes planes of the clip, plane f at frames + f * stride * height.
+ * Centre frame `center`, slot t in [0, 2 radius] reads frame
+ * center + clamp(t - radius, -center, nframes - 1 - center), and only slots
+ * in [max(0, radius - center), radius + min(radius + 1, nframes - center))
+ * are searched (src/frontend/temporal.hpp:11-21, src/host/filter_bm3d.cpp:96-97).
+ * The spatial search on the centre frame (slot radius) seeds ps_num
+ * predictive searches of (2 ps_range + 1)^2 windows per layer. out_rec: device
+ * int32 [nref][group][4] = (x, y, slot, bits of the float SSD) in match order
+ * (reference first, then ascending distance, slot, y, x); out_count: [nref]. */
+int nss_hip_predictive_match(const float* frames, int nframes, int center, int width, int height, int stride,
+                             int block, int step, int bm_range, int group, int radius, int ps_num, int ps_range,
+                             int* out_rec, int* out_count, void* stream);
 
 /* Matcher choice for this process (instances created afterwards, and
  * nss_hip_spatial_match): 0 automatic, 1 the FP32 matchers only, 2 the MFMA
@@ -356,6 +380,7 @@ typedef struct nss_hip_server_config {
 typedef struct nss_hip_server_stats {
     long long submitted, completed, batches, batched_frames, graph_replays, graph_captures;
     long long bytes_in_use, peak_bytes, admissions_deferred;
+    long long slots_released; /* idle frame slots released to admit another stream's frame */
 } nss_hip_server_stats;
 
 int nss_hip_server_create(const nss_hip_server_config* config, nss_hip_server** out);
@@ -365,11 +390,15 @@ int nss_hip_server_create(const nss_hip_server_config* config, nss_hip_server** 
 int nss_hip_server_open_stream(nss_hip_server* server, const nss_hip_bm3d_params* params,
                                const nss_hip_video_info* info, int* stream_id);
 /* Queues one frame (device planes, read until the request completes; dst
- * written when it completes). sigma_user > 0 overrides the stream's sigma
- * for this frame on every filtered plane and both stages (radius-0 streams);
- * 0 keeps it. */
+ * written when it completes: the caller keeps both allocated and untouched
+ * until nss_hip_server_query reports the request done). sigma_user > 0
+ * overrides the stream's sigma for this frame on every filtered plane and
+ * both stages (radius-0 streams); 0 keeps it. ready_stream: the HIP stream
+ * on which the caller produced the planes; the request's work waits for what
+ * was queued on it before this call (NULL: the planes are complete now). */
 int nss_hip_server_submit(nss_hip_server* server, int stream_id, const float* const* src, const int* src_stride,
-                          float* const* dst, const int* dst_stride, double sigma_user, long long* request_id);
+                          float* const* dst, const int* dst_stride, double sigma_user, void* ready_stream,
+                          long long* request_id);
 /* NSS_HIP_OK when the request's output is complete on the device (latency_ms:
  * submit to completion seen by the scheduler), NSS_HIP_BYPASS while pending. */
 int nss_hip_server_query(nss_hip_server* server, long long request_id, double* latency_ms);

@@ -93,3 +93,38 @@ def brute_force_matches(plane, cx, cy, block, rng, group, ts=None):
             cands.append((d, y, x))
     cands.sort()
     return [(cx, cy, 0.0)] + [(x, y, d) for d, y, x in cands[:group - 1]]
+
+
+def temporal_single_block(frames, group, sigma, route_to_center=False):
+    """Rolling radius-1 output of a clip whose frames are each exactly one
+    block (block_step = block): centre c's group is its own patch (slot 1),
+    then the patches of frames c - 1 (slot 0) and c + 1 (slot 2) that exist,
+    in (distance, slot) order (src/cpu/bm/matcher.hpp:27-57, :319-373); every
+    filtered patch goes into the sums of the frame it came from, with the
+    centre's weight, and frame n is the ratio of its sums over the centres
+    n - 1 .. n + 1 (src/host/filter_bm3d.cpp:730-851). One hard-threshold
+    stage. Returns the output frames and each centre's group as frame indices.
+    route_to_center sums every patch into the centre's frame instead (a
+    routing mistake, for checking that a test would catch one)."""
+    n = len(frames)
+    b = frames[0].shape[0]
+    num = [np.zeros((b, b)) for _ in range(n)]
+    den = [0.0] * n
+    groups = []
+    for c in range(n):
+        members = [(0.0, 1, c)]
+        for t, f in ((0, c - 1), (2, c + 1)):
+            if 0 <= f < n:
+                members.append((float(np.sum((frames[c] - frames[f]) ** 2)), t, f))
+        members = [members[0]] + sorted(members[1:])
+        members = members[:group]
+        cube = np.zeros((group, b, b))
+        for j, (_, _, f) in enumerate(members):
+            cube[j] = frames[f]
+        out, w = group_filter(cube, len(members), sigma)
+        for j, (_, _, f) in enumerate(members):
+            dst = c if route_to_center else f
+            num[dst] += w * out[j]
+            den[dst] += w
+        groups.append([f for _, _, f in members])
+    return [num[f] / den[f] for f in range(n)], groups

@@ -180,3 +180,71 @@ def test_autotune_keeps_parity(gpu, oracle):
         ps = [oracle.psnr(g, w_) for g, w_ in zip(got, want)]
         print(f"autotune {fmt}: {ms:.3f} ms/frame, PSNR {ps}", flush=True)
         assert ms > 0 and min(ps) >= 60.0
+
+
+def test_budget_evicts_idle_slots_under_busy_ones(gpu):
+    """One stream grows to several frame slots; a second stream then needs a
+    slot under a full budget and releases the first stream's idle extras
+    while others of them are still busy (one frame per batch, four batches in
+    flight, so the slots finish in any order). Every output equals the alone
+    result and the eviction path ran."""
+    import torch
+    probe = gpu.Server()
+    a = probe.open_stream("YUV420PS", 160, 120, sigma=10, final=1)
+    base_a = probe.stats()["bytes_in_use"]
+    b = probe.open_stream("GRAYS", 128, 96, sigma=10)
+    base = probe.stats()["bytes_in_use"]
+    probe.close()
+    budget = base + 3 * base_a  # own slots plus three extra slots of A's size
+    srv = gpu.Server(hbm_budget_bytes=budget, max_batch_frames=1, max_inflight=4, use_graphs=False)
+    a = srv.open_stream("YUV420PS", 160, 120, sigma=10, final=1)
+    b = srv.open_stream("GRAYS", 128, 96, sigma=10)
+    specs = {a: ("YUV420PS", 160, 120, {"final": 1}), b: ("GRAYS", 128, 96, {})}
+    order = [a] * 6 + [b] * 6 + [a, b] * 4 + [a] * 4 + [b] * 4
+    reqs = []
+    for k, sid in enumerate(order):
+        fmt, w, h, kw = specs[sid]
+        planes = _frames(fmt, w, h, 1, seed=100 + k)[0]
+        src = [_dev(p) for p in planes]
+        out = [torch.empty_like(p) for p in src]
+        torch.cuda.synchronize()
+        reqs.append((srv.submit(sid, src, out), sid, planes, out))
+    srv.drain()
+    st = srv.stats()
+    print(f"eviction stats {st}", flush=True)
+    assert st["completed"] == len(reqs) and st["peak_bytes"] <= budget
+    assert st["slots_released"] > 0
+    for rid, sid, planes, out in reqs:
+        fmt, w, h, kw = specs[sid]
+        want = _alone(gpu, fmt, w, h, planes, 10, **kw)
+        for g, w_ in zip(out, want):
+            assert np.array_equal(g.cpu().numpy(), w_), sid
+    srv.close()
+
+
+def test_submit_waits_for_the_producing_stream(gpu):
+    """Planes written on a side stream and submitted at once, without a host
+    synchronisation: the request waits for that stream's work."""
+    import torch
+    fmt, w, h, kw = STREAMS[1]
+    planes = _frames(fmt, w, h, 1, seed=77)[0]
+    host = [torch.from_numpy(np.ascontiguousarray(p, np.float32)).pin_memory() for p in planes]
+    want = _alone(gpu, fmt, w, h, planes, 10, **kw)
+    srv = gpu.Server(max_batch_frames=2, max_inflight=1)
+    sid = srv.open_stream(fmt, w, h, sigma=10, **kw)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        src = [torch.empty(p.shape, dtype=torch.float32, device="cuda") for p in host]
+        out = [torch.empty_like(p) for p in src]
+        # a long kernel first, so the copies land well after the submit
+        x = torch.randn(1 << 24, device="cuda")
+        for _ in range(20):
+            x = x * 1.0001
+        for d, p in zip(src, host):
+            d.copy_(p, non_blocking=True)
+        rid = srv.submit(sid, src, out, ready_stream=side)
+    srv.drain()
+    assert srv.query(rid) is not None
+    for g, w_ in zip(out, want):
+        assert np.array_equal(g.cpu().numpy(), w_)
+    srv.close()

@@ -289,3 +289,35 @@ def test_vaggregate_oracle_known_answers(oracle):
     # frame 1: centres 0, 1, 2 -> num (6, 12), den (3, 6)
     np.testing.assert_array_equal(out[1][0], [[2., 2., 7.]])
     assert np.array_equal(oracle.vaggregate(fat, src, r, planes=[])[2][0], src[2][0])
+
+
+@pytest.mark.parametrize("tie", [False, True])
+@pytest.mark.parametrize("group", [4, 8])
+def test_temporal_slot_routing_closed_form(oracle, group, tie):
+    """Rolling radius 1 on three single-block frames: the oracle's groups and
+    the frame each filtered patch is summed into match the closed form
+    (tests/closed_form.py temporal_single_block) within 3e-5."""
+    import closed_form as cf
+    b = 8
+    rng = np.random.default_rng(11 + group)
+    # values on a 1/64 grid: the patches, their differences and the squared
+    # distances are exact in float32, so a tie is an exact tie
+    p1 = rng.integers(16, 48, (b, b)) / 64.0
+    d = rng.integers(-5, 6, (b, b)) / 64.0
+    p0 = p1 + d
+    p2 = p1 - d if tie else p1 + rng.integers(-8, 9, (b, b)) / 64.0
+    frames = [np.asarray(p, np.float32) for p in (p0, p1, p2)]
+    sigma = oracle.effective_sigma(20.0)
+    want, groups = cf.temporal_single_block([f.astype(np.float64) for f in frames], group, sigma)
+    if tie:
+        assert groups[1] == [1, 0, 2]
+    import nss_hip
+    params = nss_hip.parse("GRAYS", b, b, 3, sigma=20.0, radius=1, block_size=b, block_step=b, group_size=group,
+                           bm_range=1, ps_num=1, ps_range=0)
+    got = [f[0] for f in oracle.bm3d_clip([[f] for f in frames], params)]
+    for n in range(3):
+        assert np.abs(got[n] - want[n]).max() <= 3e-5, (n, np.abs(got[n] - want[n]).max())
+    # a routing mistake (each patch summed into the centre's own frame) is
+    # far outside that tolerance
+    wrong, _ = cf.temporal_single_block([f.astype(np.float64) for f in frames], group, sigma, route_to_center=True)
+    assert max(np.abs(got[n] - wrong[n]).max() for n in range(3)) > 1e-3

@@ -84,10 +84,11 @@ struct FilterJob {
     // pixel, weight at each patch's top-left corner, 2^32 units.
     unsigned long long* num;
     unsigned long long* den;
-    // Per-tile partial sums (the tile path): [tile][ah][aw] floats each.
+    // Per-tile partial sums (the tile path): [tile][ah][pitch] int64 each,
+    // 2^32 units, exact (the finish adds them exactly).
     TileGeom tiles;
-    float* part_num;
-    float* part_den;
+    unsigned long long* part_num;
+    unsigned long long* part_den;
 };
 
 struct FilterTable {
@@ -98,7 +99,9 @@ struct FilterTable {
 struct FinishJob {
     const unsigned long long* num;  // atomic path
     const unsigned long long* den;
-    const float* part_num;          // tile path
+    const long long* exact_num;     // spatial tile path: exact int64 partials
+    const long long* exact_den;
+    const float* part_num;          // temporal tile path: float slot partials
     const float* part_den;
     TileGeom tiles;
     int block, step, range;

@@ -1200,14 +1200,68 @@ __device__ inline void lds_add(unsigned long long* p, unsigned long long v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Tile epilogue: den = B x B box sum of the corner weights (horizontal pass
-// into scratch, then vertical), and num, as the tile's float partials (rows of
+// Tile epilogue: the tile's exact int64 sums as its partials (2^32 units), so
+// that the finish adds every pixel's sums exactly and the output does not
+// depend on how the grid was cut into tiles. num goes out first; its LDS area
+// then holds the horizontal B-wide sums of the corner weights, and den is
+// their vertical B-high sum. pn / pd point at column ax0 & 3 of the tile's
+// first row (rows of `pitch` entries). Called by every thread of the block
+// after the accumulation's barrier.
+template <int B>
+__device__ inline void partials_out(int aw, int ah, int pitch, unsigned long long* num,
+                                    const unsigned long long* corner, unsigned long long* pn,
+                                    unsigned long long* pd) {
+    const int cw = aw - B + 1, ch = ah - B + 1;
+    const int nt = static_cast<int>(blockDim.x);
+    for (int e = threadIdx.x; e < aw * ah; e += nt) {
+        const int y = e / aw, x = e - y * aw;
+        pn[y * pitch + x] = num[e];
+    }
+    __syncthreads();
+    unsigned long long* hsum = num;  // ch rows of aw
+    // Horizontal: a thread slides the B-wide window over 8 outputs of a row.
+    const int segs = (aw + 7) >> 3;
+    for (int e = threadIdx.x; e < ch * segs; e += nt) {
+        const int y = e / segs, x0 = (e - y * segs) * 8;
+        const unsigned long long* row = corner + y * cw;
+        unsigned long long w[B + 7];  // corners x0 - B + 1 .. x0 + 7
+#pragma unroll
+        for (int i = 0; i < B + 7; ++i) {
+            const int cx = x0 - B + 1 + i;
+            w[i] = (cx >= 0 && cx < cw) ? row[cx] : 0ull;
+        }
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int i = 0; i < B - 1; ++i) sum += w[i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            sum += w[B - 1 + j];
+            if (x0 + j < aw) hsum[y * aw + x0 + j] = sum;
+            sum -= w[j];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < aw * ah; e += nt) {
+        const int y = e / aw, x = e - y * aw;
+        unsigned long long d = 0;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int cy = y - i;
+            if (cy >= 0 && cy < ch) d += hsum[cy * aw + x];
+        }
+        pd[y * pitch + x] = d;
+    }
+}
+
+// Temporal tile epilogue: den = B x B box sum of the corner weights
+// (horizontal pass into scratch, then vertical), and num, as the tile's float
+// partials (rows of
 // `pitch` floats; pn / pd point at column ax0 & 3 of the tile's first row, so
 // that x = 0 mod 4 is 16-byte aligned in every tile). Called by every thread
 // of the block after the accumulation's barrier; scratch holds
 // (ah - B + 1) * aw floats.
 template <int B>
-__device__ inline void partials_out(int aw, int ah, int pitch, const unsigned long long* num,
+__device__ inline void partials_out_float(int aw, int ah, int pitch, const unsigned long long* num,
                                     const unsigned long long* corner, float* scratch, float* pn, float* pd) {
     const int cw = aw - B + 1, ch = ah - B + 1;
     const int nt = static_cast<int>(blockDim.x);
@@ -1250,14 +1304,13 @@ __device__ inline void partials_out(int aw, int ah, int pitch, const unsigned lo
 }
 
 template <int B>
-__device__ inline void tile_partials(const FilterJob& jb, int tile, int ax0, const unsigned long long* num,
-                                     const unsigned long long* corner, float* scratch) {
+__device__ inline void tile_partials(const FilterJob& jb, int tile, int ax0, unsigned long long* num,
+                                     const unsigned long long* corner) {
     const size_t base = static_cast<size_t>(tile) * jb.tiles.pitch * jb.tiles.ah + (ax0 & 3);
-    partials_out<B>(jb.tiles.aw, jb.tiles.ah, jb.tiles.pitch, num, corner, scratch, jb.part_num + base,
-                    jb.part_den + base);
+    partials_out<B>(jb.tiles.aw, jb.tiles.ah, jb.tiles.pitch, num, corner, jb.part_num + base, jb.part_den + base);
 }
 
-// Tile path: sums of a tile of groups in LDS, then float partials.
+// Tile path: sums of a tile of groups in LDS, then exact partials.
 // LDS: num [ah][aw] int64 | den corners [ah-B+1][aw-B+1] int64 | per-wave
 // cubes (reused after filtering for the den box sum).
 template <int B, bool WIENER>
@@ -1305,7 +1358,7 @@ __global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
         wave_sync();
     }
     __syncthreads();
-    tile_partials<B>(jb, tile, ax0, num, corner, scratch);
+    tile_partials<B>(jb, tile, ax0, num, corner);
 }
 
 // ---------------------------------------------------------------------------
@@ -1578,7 +1631,7 @@ __global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
         }
     }
     __syncthreads();
-    tile_partials<B>(jb, tile, ax0, num, corner, scratch);
+    tile_partials<B>(jb, tile, ax0, num, corner);
 }
 
 // ---------------------------------------------------------------------------
@@ -1965,7 +2018,7 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
         }
     }
     __syncthreads();
-    tile_partials<B>(jb, tile, ax0, num, corner, scratch);
+    tile_partials<B>(jb, tile, ax0, num, corner);
 }
 
 // Atomic path: one wave per group, frame-sized fixed-point sums.
@@ -2590,7 +2643,7 @@ __global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable ta
             const size_t area = static_cast<size_t>(g.pitch) * g.ah;
             float* pn = jb.part + jb.slot_off[t] + static_cast<size_t>(cur.tile) * area + (q.ax0 & 3);
             float* pd = pn + static_cast<size_t>(g.ntx) * g.nty * area;
-            partials_out<B>(q.aw, q.ah, g.pitch, acc + q.off, acc + q.off + q.aw * q.ah, scratch, pn, pd);
+            partials_out_float<B>(q.aw, q.ah, g.pitch, acc + q.off, acc + q.off + q.aw * q.ah, scratch, pn, pd);
             if (t == 0 && more) publish(nxt, nf);
             __syncthreads();
             if (t == 0 && more && wave < nxt.ngroups) gather(nxt, wave, v, vr);
@@ -2653,8 +2706,12 @@ __global__ __launch_bounds__(256) void finish_temporal_tiles_kernel(TFinishTable
 
 // Tile path: 64 x 16 pixels per block, four consecutive pixels of a row per
 // thread; per pixel, the partials of every tile whose rectangle covers it
-// (tiles in descending (ty, tx) order), read as one aligned 16-byte vector
-// per tile for the four pixels.
+// (tiles in descending (ty, tx) order), read as aligned vectors per tile for
+// the four pixels. EXACT: the spatial path's int64 partials, summed exactly
+// and converted once, num / den in double rounded to float (the atomic path's
+// formula, finish_corner_kernel); otherwise the temporal slot partials
+// (floats) in slice mode.
+template <bool EXACT>
 __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
     // XCD-contiguous blocks: neighbouring blocks read the same tile partial
     // lines through one L2.
@@ -2671,7 +2728,8 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
     while (txh + 1 < tg.ntx && tg.ax0(txh + 1, jb.width, B, s, r) <= xr) ++txh;
     int tyh = min(tg.nty - 1, (y + r) / (tg.th * s));
     while (tyh + 1 < tg.nty && tg.ay0(tyh + 1, jb.height, B, s, r) <= y) ++tyh;
-    f32x4 n = {0.f, 0.f, 0.f, 0.f}, d = {0.f, 0.f, 0.f, 0.f};
+    using Acc = typename std::conditional<EXACT, long long, float>::type;
+    Acc n[4] = {0, 0, 0, 0}, d[4] = {0, 0, 0, 0};
     const size_t area = static_cast<size_t>(tg.pitch) * tg.ah;
     for (int ty = tyh; ty >= 0; --ty) {
         const int oy = y - tg.ay0(ty, jb.height, B, s, r);
@@ -2682,13 +2740,24 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
             // column of x0: a multiple of 4 in [0, pitch - 4]
             const size_t o = (static_cast<size_t>(ty) * tg.ntx + tx) * area + static_cast<size_t>(oy) * tg.pitch +
                              (x0 - ax0 + (ax0 & 3));
-            const f32x4 pn = *reinterpret_cast<const f32x4*>(jb.part_num + o);
-            const f32x4 pd = *reinterpret_cast<const f32x4*>(jb.part_den + o);
+            Acc pn[4], pd[4];
+            if constexpr (EXACT) {
+                const longlong2* qn = reinterpret_cast<const longlong2*>(jb.exact_num + o);
+                const longlong2* qd = reinterpret_cast<const longlong2*>(jb.exact_den + o);
+                const longlong2 n0 = qn[0], n1 = qn[1], d0 = qd[0], d1 = qd[1];
+                pn[0] = n0.x, pn[1] = n0.y, pn[2] = n1.x, pn[3] = n1.y;
+                pd[0] = d0.x, pd[1] = d0.y, pd[2] = d1.x, pd[3] = d1.y;
+            } else {
+                const f32x4 vn = *reinterpret_cast<const f32x4*>(jb.part_num + o);
+                const f32x4 vd = *reinterpret_cast<const f32x4*>(jb.part_den + o);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) pn[i] = vn[i], pd[i] = vd[i];
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const bool in = static_cast<unsigned>(x0 + i - ax0) < static_cast<unsigned>(tg.aw);
-                n[i] += in ? pn[i] : 0.f;
-                d[i] += in ? pd[i] : 0.f;
+                n[i] += in ? pn[i] : Acc(0);
+                d[i] += in ? pd[i] : Acc(0);
             }
         }
     }
@@ -2698,15 +2767,26 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             if (x0 + i < jb.width) {
-                dst[x0 + i] = n[i];
-                dd[x0 + i] = d[i];
+                if constexpr (EXACT) {
+                    dst[x0 + i] = static_cast<float>(static_cast<double>(n[i]) * (1.0 / 4294967296.0));
+                    dd[x0 + i] = static_cast<float>(static_cast<double>(d[i]) * (1.0 / 4294967296.0));
+                } else {
+                    dst[x0 + i] = n[i];
+                    dd[x0 + i] = d[i];
+                }
             }
         return;
     }
     const float* src = jb.src + static_cast<size_t>(y) * jb.sstride;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        if (x0 + i < jb.width) dst[x0 + i] = d[i] > 1e-12f ? n[i] / d[i] : src[x0 + i];
+        if (x0 + i < jb.width) {
+            if constexpr (EXACT)
+                dst[x0 + i] = d[i] > 0 ? static_cast<float>(static_cast<double>(n[i]) / static_cast<double>(d[i]))
+                                       : src[x0 + i];
+            else
+                dst[x0 + i] = d[i] > 1e-12f ? n[i] / d[i] : src[x0 + i];
+        }
 }
 
 // Atomic path: den kept at each patch's top-left corner, box-summed here.
@@ -3226,8 +3306,7 @@ size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t) {
     const size_t recs = static_cast<size_t>(t.tw) * t.th * group * sizeof(MatchRec) +
                         ((static_cast<size_t>(t.tw) * t.th + 3) & ~static_cast<size_t>(3)) * 4;
     const size_t tbuf = 4 * static_cast<size_t>(group) * kTs * 4;
-    const size_t boxsum = (ah - 7) * aw * 4;
-    return acc + recs + (tbuf > boxsum ? tbuf : boxsum);
+    return acc + recs + tbuf;  // the den box sum reuses num's area
 }
 
 size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t) {
@@ -3236,16 +3315,13 @@ size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t) {
     const size_t recs = ((static_cast<size_t>(t.tw) * t.th * group + 3) & ~static_cast<size_t>(3)) * 4 +
                         ((static_cast<size_t>(t.tw) * t.th + 3) & ~static_cast<size_t>(3)) * 4;
     const size_t tbuf = 4 * static_cast<size_t>(kMfmaBuf) * 4;
-    const size_t boxsum = (ah - 7) * aw * 4;
-    return acc + recs + (tbuf > boxsum ? tbuf : boxsum);
+    return acc + recs + tbuf;  // the den box sum reuses num's area
 }
 
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t) {
     const size_t aw = t.aw, ah = t.ah;
     const size_t acc = aw * ah * 8 + (aw - block + 1) * (ah - block + 1) * 8;
-    const size_t cubes = 4 * filter_atomic_lds_bytes(block, group, wiener);
-    const size_t boxsum = (ah - block + 1) * aw * 4;
-    return acc + (cubes > boxsum ? cubes : boxsum);
+    return acc + 4 * filter_atomic_lds_bytes(block, group, wiener);  // the den box sum reuses num's area
 }
 
 template <typename K>
@@ -3412,7 +3488,10 @@ hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t l
 
 hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s) {
     if (tiles) {
-        finish_tiles_kernel<<<total_wg, 256, 0, s>>>(t);
+        if (t.n && t.job[0].exact_num)
+            finish_tiles_kernel<true><<<total_wg, 256, 0, s>>>(t);
+        else
+            finish_tiles_kernel<false><<<total_wg, 256, 0, s>>>(t);
         return hipGetLastError();
     }
     switch (block) {

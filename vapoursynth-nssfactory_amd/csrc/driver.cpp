@@ -238,7 +238,14 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
 }
 
 void free_buffers(nss_hip_bm3d* h) {
-    free_frame_buffers(h->buf);
+    for (auto& sl : h->slots) {
+        free_frame_buffers(sl.buf);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        sl.done = nullptr;
+    }
+    h->slots.clear();
+    if (h->roll_done) (void)hipEventDestroy(h->roll_done);
+    h->roll_done = nullptr;
     auto& R = h->roll;
     for (int i = 0; i < 3; ++i) {
         for (void* p : {static_cast<void*>(R.src[i]), static_cast<void*>(R.ref[i]), static_cast<void*>(R.basic[i]),
@@ -297,28 +304,39 @@ PlaneNeeds plane_needs(const nss_hip_bm3d* h, int i) {
 
 namespace nsship {
 
+int frame_slot_count(const nss_hip_bm3d* h) { return h->p.radius > 0 ? 1 : std::max(1, h->p.num_streams); }
+
 size_t frame_buffer_bytes(const nss_hip_bm3d* h) {
     size_t total = 0;
     for (int i = 0; i < h->nplanes; ++i) {
         const PlaneNeeds n = plane_needs(h, i);
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        total += n.recs * sizeof(MatchRec) + n.nref * sizeof(int) + 2 * h->part_floats[i] * sizeof(float) +
+        total += n.recs * sizeof(MatchRec) + n.nref * sizeof(int) + 2 * h->part_elems[i] * sizeof(unsigned long long) +
                  (n.atomic ? 16 * npx : 0) + (n.basic ? 4 * npx : 0);
     }
     return total;
 }
 
-size_t instance_bytes(const nss_hip_bm3d* h) {
-    size_t total = frame_buffer_bytes(h);
+// Rolling state of plane i (rings and sums), from the plan.
+size_t rolling_bytes(const nss_hip_bm3d* h, int i, size_t* cached, size_t* output) {
     const auto& R = h->roll;
-    for (int i = 0; i < h->nplanes; ++i) {
-        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        total += npx * 4 * static_cast<size_t>(R.src_n + R.ref_n + R.basic_n + R.out_n * R.out_rows);
-        for (int s = 0; s < h->nstages; ++s) {
-            if (R.num[s][i]) total += npx * 16 * static_cast<size_t>(R.acc_n);
-            if (R.part[s][i]) total += R.part_center[s][i] * (2 * static_cast<size_t>(R.r) + 1) * sizeof(float);
-        }
+    if (h->p.radius == 0) return 0;
+    const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+    size_t c = npx * 4 * static_cast<size_t>(R.src_n + R.ref_n + R.basic_n);
+    const size_t o = npx * 4 * static_cast<size_t>(R.out_n) * R.out_rows;
+    for (int s = 0; s < h->nstages; ++s) {
+        if (h->stages[s].plane[i].sigma == 0.f) continue;
+        c += R.tiles[s] ? R.part_center[s][i] * (2 * static_cast<size_t>(R.r) + 1) * sizeof(float)
+                        : npx * 16 * static_cast<size_t>(R.acc_n);
     }
+    if (cached) *cached += c;
+    if (output) *output += o;
+    return c + o;
+}
+
+size_t instance_bytes(const nss_hip_bm3d* h) {
+    size_t total = frame_buffer_bytes(h) * frame_slot_count(h);
+    for (int i = 0; i < h->nplanes; ++i) total += rolling_bytes(h, i, nullptr, nullptr);
     return total;
 }
 
@@ -346,7 +364,7 @@ hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb) {
             e = hipMalloc(&fb.num[i], npx * 8);
             if (e == hipSuccess) e = hipMalloc(&fb.den[i], npx * 8);
         }
-        if (e == hipSuccess && h->part_floats[i]) e = hipMalloc(&fb.part[i], 2 * h->part_floats[i] * sizeof(float));
+        if (e == hipSuccess && h->part_elems[i]) e = hipMalloc(&fb.part[i], 2 * h->part_elems[i] * sizeof(unsigned long long));
         if (e == hipSuccess && n.basic) e = hipMalloc(&fb.basic[i], npx * 4);
     }
     if (e != hipSuccess) free_frame_buffers(fb);
@@ -360,12 +378,10 @@ namespace {
 // Event-traced launch on stream s (kind: 0 clear, 1 match, 2 filter, 3 finish).
 template <typename F>
 hipError_t traced(Tracer* tr, int kind, hipStream_t s, F&& launch) {
-    hipEvent_t a = nullptr, z = nullptr;
-    if (tr && tr->on) {
-        a = tr->take();
-        z = tr->take();
-        if (a) (void)hipEventRecord(a, s);
-    }
+    if (!tr || !tr->on) return launch();
+    std::lock_guard<std::mutex> lock(tr->mu);
+    hipEvent_t a = tr->take(), z = tr->take();
+    if (a) (void)hipEventRecord(a, s);
     const hipError_t err = launch();
     if (a && z) {
         (void)hipEventRecord(z, s);
@@ -538,14 +554,14 @@ void add_bucket_jobs(LaunchGroup& g, const FrameRef& f, const Stage& st, const B
         fj.den = f.buf.den[i];
         fj.tiles = bk.geom[k];
         fj.part_num = f.buf.part[i];
-        fj.part_den = f.buf.part[i] ? f.buf.part[i] + h->part_floats[i] : nullptr;
+        fj.part_den = f.buf.part[i] ? f.buf.part[i] + h->part_elems[i] : nullptr;
         g.fwg += bk.tiles ? bk.geom[k].ntx * bk.geom[k].nty : nref;
 
         FinishJob& n = g.fn.job[g.fn.n++];
         n.num = f.buf.num[i];
         n.den = f.buf.den[i];
-        n.part_num = fj.part_num;
-        n.part_den = fj.part_den;
+        n.exact_num = reinterpret_cast<const long long*>(fj.part_num);
+        n.exact_den = reinterpret_cast<const long long*>(fj.part_den);
         n.tiles = bk.geom[k];
         n.block = b;
         n.step = sp.step;
@@ -621,7 +637,8 @@ int run_batch(const FrameRef* frames, int nframes, Tracer* tr, hipStream_t s) {
     return NSS_HIP_OK;
 }
 
-// The instance's own buffers and sigma, for frame planes src / ref / dst.
+// The instance's sigmas for frame planes src / ref / dst (the caller sets
+// the frame's buffers).
 FrameRef frame_ref(nss_hip_bm3d* h, const float* const* src, const int* src_stride, const float* const* ref,
                    const int* ref_stride, float* const* dst, const int* dst_stride) {
     FrameRef f{};
@@ -634,7 +651,6 @@ FrameRef frame_ref(nss_hip_bm3d* h, const float* const* src, const int* src_stri
         f.dst[i] = dst[i];
         f.dst_stride[i] = dst_stride[i];
     }
-    f.buf = h->buf;
     for (int k = 0; k < h->nstages; ++k)
         for (int i = 0; i < 3; ++i) f.sigma[k][i] = h->stages[k].plane[i].sigma;
     return f;
@@ -761,8 +777,8 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             m.group = sp.group;
             m.t0 = R.r;
             m.wg_begin = mwg;
-            m.out = h->buf.matches[i];
-            m.counts = h->buf.counts[i];
+            m.out = h->slots[0].buf.matches[i];
+            m.counts = h->slots[0].buf.counts[i];
             mwg += match_workgroups(bk.match_kind, g);
             // group 1 keeps the reference alone (src/cpu/bm/matcher.hpp:323).
             if (sp.group > 1) {
@@ -777,8 +793,8 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
                 p.ps_range = sp.ps_range;
                 p.slots = slots;
                 p.wg_begin = pwg;
-                p.recs = h->buf.matches[i];
-                p.counts = h->buf.counts[i];
+                p.recs = h->slots[0].buf.matches[i];
+                p.counts = h->slots[0].buf.counts[i];
                 pwg += p.nref;
                 predictive = true;
             }
@@ -809,8 +825,8 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             f.sigma = sp.sigma;
             f.slots = slots;
             f.wg_begin = fwg;
-            f.matches = h->buf.matches[mi];
-            f.counts = h->buf.counts[mi];
+            f.matches = h->slots[0].buf.matches[mi];
+            f.counts = h->slots[0].buf.counts[mi];
             f.num = R.num[s][i];
             f.den = R.den[s][i];
             f.acc_ring = R.acc_n;
@@ -1094,6 +1110,103 @@ int temporal_advance(nss_hip_bm3d* h, hipStream_t st) {
     return NSS_HIP_OK;
 }
 
+// A lease of one frame slot for the time a call queues its frame: the slot
+// free the longest (src/cuda/runtime/stream_pool.hpp SlotPool::acquire, FIFO
+// so that frames rotate over the slots), waiting asleep while all are taken.
+struct SlotLease {
+    nss_hip_bm3d* h;
+    int index;
+    explicit SlotLease(nss_hip_bm3d* hh) : h(hh) {
+        std::unique_lock<std::mutex> lock(h->slot_mu);
+        h->slot_cv.wait(lock, [&] { return !h->free_slots.empty(); });
+        index = h->free_slots.front();
+        h->free_slots.pop_front();
+    }
+    ~SlotLease() {
+        {
+            std::lock_guard<std::mutex> lock(h->slot_mu);
+            h->free_slots.push_back(index);
+        }
+        h->slot_cv.notify_one();
+    }
+    SlotLease(const SlotLease&) = delete;
+    SlotLease& operator=(const SlotLease&) = delete;
+};
+
+// Every slot at once (the autotuner changes the plan and the buffers): waits
+// until no call holds a slot, then holds them all.
+struct AllSlotsLease {
+    nss_hip_bm3d* h;
+    explicit AllSlotsLease(nss_hip_bm3d* hh) : h(hh) {
+        std::unique_lock<std::mutex> lock(h->slot_mu);
+        h->slot_cv.wait(lock, [&] { return h->free_slots.size() == h->slots.size(); });
+        h->free_slots.clear();
+    }
+    ~AllSlotsLease() {
+        {
+            std::lock_guard<std::mutex> lock(h->slot_mu);
+            for (size_t k = 0; k < h->slots.size(); ++k) h->free_slots.push_back(static_cast<int>(k));
+        }
+        h->slot_cv.notify_all();
+    }
+    AllSlotsLease(const AllSlotsLease&) = delete;
+    AllSlotsLease& operator=(const AllSlotsLease&) = delete;
+};
+
+// One radius-0 frame on slot k, queued on stream s after the slot's previous
+// work (which may have been queued on another stream).
+int process_on_slot(nss_hip_bm3d* h, int k, const float* const* src, const int* src_stride, const float* const* ref,
+                    const int* ref_stride, float* const* dst, const int* dst_stride, hipStream_t s) {
+    FrameSlot& sl = h->slots[static_cast<size_t>(k)];
+    if (sl.used) {
+        const hipError_t e = hipStreamWaitEvent(s, sl.done, 0);
+        if (e != hipSuccess) return hip_fail(e, "stream wait");
+    }
+    FrameRef f = frame_ref(h, src, src_stride, ref, ref_stride, dst, dst_stride);
+    f.buf = sl.buf;
+    const int rc = run_batch(&f, 1, &h->tracer, s);
+    // Recorded whatever run_batch returned: the next lessee orders after
+    // everything this call queued.
+    const hipError_t e = hipEventRecord(sl.done, s);
+    if (e != hipSuccess) return hip_fail(e, "event record");
+    sl.used = true;
+    return rc;
+}
+
+// Temporal calls: one at a time, each ordered on the device after the last.
+struct RollTurn {
+    nss_hip_bm3d* h;
+    hipStream_t s;
+    std::lock_guard<std::mutex> lock;
+    hipError_t err = hipSuccess;
+    RollTurn(nss_hip_bm3d* hh, hipStream_t st) : h(hh), s(st), lock(hh->roll_mu) {
+        if (h->roll_used) err = hipStreamWaitEvent(s, h->roll_done, 0);
+    }
+    ~RollTurn() {
+        if (hipEventRecord(h->roll_done, s) == hipSuccess) h->roll_used = true;
+    }
+};
+
+// Partials of every slot grown to need[i] int64 entries per half, after the
+// slots' earlier work (the caller holds all slots).
+int grow_partials(nss_hip_bm3d* h, const size_t* need) {
+    hipError_t e = hipSuccess;
+    for (auto& sl : h->slots)
+        if (e == hipSuccess && sl.used) e = hipEventSynchronize(sl.done);
+    if (e != hipSuccess) return hip_fail(e, "synchronize");
+    for (int i = 0; i < h->nplanes; ++i) {
+        if (need[i] <= h->part_elems[i]) continue;
+        h->part_elems[i] = need[i];
+        for (auto& sl : h->slots) {
+            if (sl.buf.part[i]) (void)hipFree(sl.buf.part[i]);
+            sl.buf.part[i] = nullptr;
+            e = hipMalloc(&sl.buf.part[i], 2 * need[i] * sizeof(unsigned long long));
+            if (e != hipSuccess) return hip_fail(e, "device allocation");
+        }
+    }
+    return NSS_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1171,46 +1284,72 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
         }
     }
     const bool rolling = p.radius > 0;
-    for (int i = 0; i < h->nplanes; ++i) h->part_floats[i] = plane_needs(h, i).part;
-    hipError_t ea = alloc_frame_buffers(h, h->buf);
-    if (ea != hipSuccess) {
-        delete h;
-        return hip_fail(ea, "device allocation");
+    for (int i = 0; i < h->nplanes; ++i) h->part_elems[i] = plane_needs(h, i).part;
+    if (rolling) {
+        // Ring depths: the oldest frame a pending centre or finish reads is
+        // 4r (final=1) or 2r (one stage) behind the newest pushed; the Wiener
+        // stage reads basic frames up to 3r behind the newest finished one;
+        // 4r + 1 finished frames can wait for pop at the clip's end.
+        auto& R = h->roll;
+        const int r = p.radius;
+        R.N = info->num_frames;
+        R.r = r;
+        R.src_n = p.final_stage ? 4 * r + 2 : 2 * r + 2;
+        R.ref_n = (p.has_ref && !p.final_stage) ? 2 * r + 2 : 0;
+        R.basic_n = p.final_stage ? 3 * r + 2 : 0;
+        R.out_n = 4 * r + 2;
+        R.acc_n = 2 * r + 1;
+        R.legacy = !p.temporal_mode;
+        R.out_rows = R.legacy ? 2 * (2 * r + 1) : 1;
+        for (int s = 0; s < h->nstages; ++s) plan_temporal_tiles(h, s);
+        for (int i = 0; i < h->nplanes; ++i)
+            for (int s = 0; s < h->nstages; ++s) {
+                if (h->stages[s].plane[i].sigma == 0.f || !R.tiles[s]) continue;
+                TileGeom g[kMaxTileSlots];
+                int range[kMaxTileSlots];
+                slot_geoms(h, s, i, R.tw[s], R.th[s], g, range);
+                R.part_center[s][i] = centre_partials(g, 2 * r + 1, nullptr);
+            }
     }
-    for (int i = 0; i < h->nplanes; ++i) {
-        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        hipError_t e = hipSuccess;
-        if (e == hipSuccess && rolling) {
-            // Ring depths: the oldest frame a pending centre or finish reads
-            // is 4r (final=1) or 2r (one stage) behind the newest pushed; the
-            // Wiener stage reads basic frames up to 3r behind the newest
-            // finished one; 4r + 1 finished frames can wait for pop at the
-            // clip's end.
-            auto& R = h->roll;
-            const int r = p.radius;
-            R.N = info->num_frames;
-            R.r = r;
-            R.src_n = p.final_stage ? 4 * r + 2 : 2 * r + 2;
-            R.ref_n = (p.has_ref && !p.final_stage) ? 2 * r + 2 : 0;
-            R.basic_n = p.final_stage ? 3 * r + 2 : 0;
-            R.out_n = 4 * r + 2;
-            R.acc_n = 2 * r + 1;
-            R.legacy = !p.temporal_mode;
-            R.out_rows = R.legacy ? 2 * (2 * r + 1) : 1;
+    // memory_limit_mb caps the instance's device memory; a limit too small
+    // for the clip is an error at creation, never while frames are filtered
+    // (docs/api/cuda/README.md:141, src/cuda/common/group_driver.cpp:491-498).
+    // Checked before anything is allocated.
+    if (p.memory_limit_mb > 0) {
+        const size_t need = instance_bytes(h);
+        if (need > (static_cast<size_t>(p.memory_limit_mb) << 20)) {
+            const int n = frame_slot_count(h);
+            const size_t each = need / static_cast<size_t>(n);
+            delete h;
+            return fail(NSS_HIP_INVALID_INPUT, "nss_hip.BM3D: memory_limit_mb is too small for this clip: each of the " +
+                                                   std::to_string(n) + " stream(s) needs at least " +
+                                                   std::to_string((each >> 20) + 1) + " MiB");
+        }
+    }
+    h->slots.resize(static_cast<size_t>(frame_slot_count(h)));
+    for (size_t k = 0; k < h->slots.size(); ++k) {
+        hipError_t ea = alloc_frame_buffers(h, h->slots[k].buf);
+        if (ea == hipSuccess) ea = hipEventCreateWithFlags(&h->slots[k].done, hipEventDisableTiming);
+        if (ea != hipSuccess) {
+            free_buffers(h);
+            delete h;
+            return hip_fail(ea, "device allocation");
+        }
+        h->free_slots.push_back(static_cast<int>(k));
+    }
+    if (rolling) {
+        auto& R = h->roll;
+        hipError_t e = hipEventCreateWithFlags(&h->roll_done, hipEventDisableTiming);
+        for (int i = 0; i < h->nplanes && e == hipSuccess; ++i) {
+            const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
             e = hipMalloc(&R.src[i], npx * 4 * R.src_n);
             if (e == hipSuccess && R.ref_n) e = hipMalloc(&R.ref[i], npx * 4 * R.ref_n);
             if (e == hipSuccess && R.basic_n) e = hipMalloc(&R.basic[i], npx * 4 * R.basic_n);
             if (e == hipSuccess) e = hipMalloc(&R.out[i], npx * 4 * R.out_n * R.out_rows);
-            if (i == 0)
-                for (int s = 0; s < h->nstages; ++s) plan_temporal_tiles(h, s);
             for (int s = 0; s < h->nstages && e == hipSuccess; ++s) {
                 if (h->stages[s].plane[i].sigma == 0.f) continue;
                 if (R.tiles[s]) {
-                    TileGeom g[kMaxTileSlots];
-                    int range[kMaxTileSlots];
-                    slot_geoms(h, s, i, R.tw[s], R.th[s], g, range);
-                    R.part_center[s][i] = centre_partials(g, 2 * r + 1, nullptr);
-                    e = hipMalloc(&R.part[s][i], R.part_center[s][i] * (2 * r + 1) * sizeof(float));
+                    e = hipMalloc(&R.part[s][i], R.part_center[s][i] * (2 * R.r + 1) * sizeof(float));
                     continue;
                 }
                 e = hipMalloc(&R.num[s][i], npx * 8 * R.acc_n);
@@ -1225,19 +1364,6 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
             return hip_fail(e, "device allocation");
         }
     }
-    // memory_limit_mb caps the instance's device memory; a limit too small
-    // for the clip is an error at creation, never while frames are filtered
-    // (docs/api/cuda/README.md:141, src/cuda/common/group_driver.cpp:491-498).
-    if (p.memory_limit_mb > 0) {
-        const size_t need = instance_bytes(h);
-        if (need > (static_cast<size_t>(p.memory_limit_mb) << 20)) {
-            free_buffers(h);
-            delete h;
-            return fail(NSS_HIP_INVALID_INPUT, "nss_hip.BM3D: memory_limit_mb is too small for this clip: the "
-                                               "instance needs at least " +
-                                                   std::to_string((need >> 20) + 1) + " MiB");
-        }
-    }
     *out = h;
     return NSS_HIP_OK;
 }
@@ -1246,17 +1372,8 @@ int nss_hip_bm3d_resources(nss_hip_bm3d* h, long long* bytes, long long* peak, l
     if (!h || !bytes || !peak || !limit) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
     // kinds of include/nss/resources.hpp:11: workspace, cached, pinned,
     // inflight, output, framework
-    size_t kind[6] = {frame_buffer_bytes(h), 0, 0, 0, 0, 0};
-    const auto& R = h->roll;
-    for (int i = 0; i < h->nplanes; ++i) {
-        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        kind[1] += npx * 4 * static_cast<size_t>(R.src_n + R.ref_n + R.basic_n);
-        kind[4] += npx * 4 * static_cast<size_t>(R.out_n) * R.out_rows;
-        for (int s = 0; s < h->nstages; ++s) {
-            if (R.num[s][i]) kind[1] += npx * 16 * static_cast<size_t>(R.acc_n);
-            if (R.part[s][i]) kind[1] += R.part_center[s][i] * (2 * static_cast<size_t>(R.r) + 1) * sizeof(float);
-        }
-    }
+    size_t kind[6] = {frame_buffer_bytes(h) * h->slots.size(), 0, 0, 0, 0, 0};
+    for (int i = 0; i < h->nplanes; ++i) rolling_bytes(h, i, &kind[1], &kind[4]);
     size_t total = 0;
     for (int k = 0; k < 6; ++k) {
         bytes[k] = static_cast<long long>(kind[k]);
@@ -1272,12 +1389,6 @@ int nss_hip_bm3d_push(nss_hip_bm3d* h, const float* const* src, const int* src_s
     if (!h || !src || !src_stride) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
     auto& R = h->roll;
     if (h->p.radius == 0) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: push/pop serve radius > 0; use process");
-    if (R.pushed >= R.N) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: every frame of the clip was already pushed");
-    const int last = h->nstages - 1;
-    const int pending = R.finished[last] - R.out_head;
-    const int may_finish = R.pushed + 1 == R.N ? R.N - R.finished[last] : 2;
-    if (pending + may_finish > R.out_n)
-        return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: pop the finished frames before pushing more");
     const bool need_ref = h->p.has_ref && !h->p.final_stage;
     for (int i = 0; i < h->nplanes; ++i) {
         if (!src[i] || src_stride[i] < h->pw[i])
@@ -1288,6 +1399,14 @@ int nss_hip_bm3d_push(nss_hip_bm3d* h, const float* const* src, const int* src_s
     DeviceGuard guard(h->device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     hipStream_t s = static_cast<hipStream_t>(stream);
+    RollTurn turn(h, s);
+    if (turn.err != hipSuccess) return hip_fail(turn.err, "stream wait");
+    if (R.pushed >= R.N) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: every frame of the clip was already pushed");
+    const int last = h->nstages - 1;
+    const int pending = R.finished[last] - R.out_head;
+    const int may_finish = R.pushed + 1 == R.N ? R.N - R.finished[last] : 2;
+    if (pending + may_finish > R.out_n)
+        return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: pop the finished frames before pushing more");
     const int f = R.pushed;
     for (int i = 0; i < h->nplanes; ++i) {
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
@@ -1308,10 +1427,12 @@ int nss_hip_bm3d_pop(nss_hip_bm3d* h, float* const* dst, const int* dst_stride, 
     if (!h || !dst || !dst_stride) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
     auto& R = h->roll;
     if (h->p.radius == 0) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: push/pop serve radius > 0; use process");
-    if (R.out_head >= R.finished[h->nstages - 1]) return NSS_HIP_BYPASS;
     DeviceGuard guard(h->device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     hipStream_t s = static_cast<hipStream_t>(stream);
+    RollTurn turn(h, s);
+    if (turn.err != hipSuccess) return hip_fail(turn.err, "stream wait");
+    if (R.out_head >= R.finished[h->nstages - 1]) return NSS_HIP_BYPASS;
     const int n = R.out_head;
     for (int i = 0; i < h->nplanes; ++i) {
         if (!dst[i] || dst_stride[i] < h->pw[i])
@@ -1375,8 +1496,9 @@ int nss_hip_bm3d_process(nss_hip_bm3d* h, const float* const* src, const int* sr
     }
     DeviceGuard guard(h->device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
-    const FrameRef f = frame_ref(h, src, src_stride, ref, ref_stride, dst, dst_stride);
-    return run_batch(&f, 1, &h->tracer, static_cast<hipStream_t>(stream));
+    SlotLease lease(h);
+    return process_on_slot(h, lease.index, src, src_stride, ref, ref_stride, dst, dst_stride,
+                           static_cast<hipStream_t>(stream));
 }
 
 int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* best_ms) {
@@ -1386,6 +1508,7 @@ int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* be
     DeviceGuard guard(h->device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     hipStream_t s = static_cast<hipStream_t>(stream);
+    AllSlotsLease all(h);
     // Partials large enough for every candidate shape.
     size_t need[3]{};
     for (int k = 0; k < h->nstages; ++k)
@@ -1402,22 +1525,14 @@ int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* be
         }
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "synchronize");
-    for (int i = 0; i < h->nplanes; ++i) {
-        if (need[i] <= h->part_floats[i]) continue;
-        if (h->buf.part[i]) (void)hipFree(h->buf.part[i]);
-        h->buf.part[i] = nullptr;
-        h->part_floats[i] = 0;
-        e = hipMalloc(&h->buf.part[i], 2 * need[i] * sizeof(float));
-        if (e != hipSuccess) return hip_fail(e, "device allocation");
-        h->part_floats[i] = need[i];
-    }
+    int rc = grow_partials(h, need);
+    if (rc != NSS_HIP_OK) return rc;
     // A synthetic frame: smooth pattern plus pseudo-random noise (sigma 10/255).
     float* src[3]{};
     float* dst[3]{};
     int stride[3]{};
     std::vector<float> host;
     unsigned lcg = 12345u;
-    int rc = NSS_HIP_OK;
     for (int i = 0; i < h->nplanes && rc == NSS_HIP_OK; ++i) {
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
         host.resize(npx);
@@ -1438,11 +1553,11 @@ int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* be
         rc = fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: event create");
     const float* csrc[3] = {src[0], src[1], src[2]};
     auto time_frame = [&](double* ms) -> int {
-        int r = nss_hip_bm3d_process(h, csrc, stride, nullptr, nullptr, dst, stride, 0, 1, s);
+        int r = process_on_slot(h, 0, csrc, stride, nullptr, nullptr, dst, stride, s);
         if (r != NSS_HIP_OK) return r;
         (void)hipEventRecord(ev[0], s);
         for (int k = 0; k < repeats && r == NSS_HIP_OK; ++k)
-            r = nss_hip_bm3d_process(h, csrc, stride, nullptr, nullptr, dst, stride, 0, 1, s);
+            r = process_on_slot(h, 0, csrc, stride, nullptr, nullptr, dst, stride, s);
         (void)hipEventRecord(ev[1], s);
         if (r != NSS_HIP_OK) return r;
         const hipError_t x = hipEventSynchronize(ev[1]);
@@ -1485,6 +1600,34 @@ int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* be
     return rc;
 }
 
+int nss_hip_bm3d_set_tiles(nss_hip_bm3d* h, int stage, int tw, int th) {
+    if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null handle");
+    if (stage < 0 || stage >= h->nstages) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: no such stage");
+    if (tw < 1 || th < 1 || tw * th > 64) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: tile of 1 to 64 references");
+    if (h->p.radius > 0) return fail(NSS_HIP_UNSUPPORTED, "nss_hip: temporal instances keep their tile plan");
+    DeviceGuard guard(h->device);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
+    AllSlotsLease all(h);
+    Stage& st = h->stages[stage];
+    Bucket trial[7];
+    size_t need[3]{};
+    for (int bi = 0; bi < st.nbuckets; ++bi) {
+        trial[bi] = st.bucket[bi];
+        if (!trial[bi].tiles) continue;
+        if (!apply_tile_shape(h, st, trial[bi], tw, th))
+            return fail(NSS_HIP_UNSUPPORTED, "nss_hip: the tile shape does not fit this stage");
+        for (int q = 0; q < trial[bi].nplanes; ++q) {
+            const TileGeom& tg = trial[bi].geom[q];
+            need[trial[bi].planes[q]] = std::max(need[trial[bi].planes[q]],
+                                                 static_cast<size_t>(tg.ntx) * tg.nty * tg.pitch * tg.ah);
+        }
+    }
+    const int rc = grow_partials(h, need);
+    if (rc != NSS_HIP_OK) return rc;
+    for (int bi = 0; bi < st.nbuckets; ++bi) st.bucket[bi] = trial[bi];
+    return NSS_HIP_OK;
+}
+
 int nss_hip_bm3d_set_tracing(nss_hip_bm3d* h, int on) {
     if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null handle");
     h->tracer.on = on != 0;
@@ -1494,6 +1637,7 @@ int nss_hip_bm3d_set_tracing(nss_hip_bm3d* h, int on) {
 int nss_hip_bm3d_kernel_times(nss_hip_bm3d* h, int* launches, double* total_ms, int reset) {
     if (!h || !launches || !total_ms) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
     DeviceGuard guard(h->device);
+    std::lock_guard<std::mutex> lock(h->tracer.mu);
     for (int k = 0; k < NSS_HIP_KERNEL_KINDS; ++k) {
         launches[k] = static_cast<int>(h->tracer.spans[k].size());
         double sum = 0;
@@ -1554,6 +1698,60 @@ int nss_hip_spatial_match(const float* plane, int width, int height, int stride,
     const hipError_t e = launch_match(block, mt, wg, lds, kind, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "match launch");
     return NSS_HIP_OK;
+}
+
+int nss_hip_predictive_match(const float* frames, int nframes, int center, int width, int height, int stride,
+                             int block, int step, int bm_range, int group, int radius, int ps_num, int ps_range,
+                             int* out_rec, int* out_count, void* stream) {
+    if (!frames || !out_rec || !out_count) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    if (!allowed_block(block) || group < 1 || group > kMaxGroup || width < block || height < block || step < 1 ||
+        step > block || stride < width || bm_range < 0 || bm_range > 64 || radius < 1 || radius > kMaxRadius ||
+        ps_num < 1 || ps_num > group || ps_range < 0 || ps_range > 64 || nframes < 1 || center < 0 ||
+        center >= nframes)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip: invalid predictive match arguments");
+    if (width > 8191 || height > 8191)
+        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: radius > 0 supports planes up to 8191 x 8191");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const FrameRing ring{frames, static_cast<long long>(stride) * height, nframes};
+    const Slots slots{center, nframes, radius};
+    MatchTable mt{};
+    MatchJob& m = mt.job[mt.n++];
+    m.plane = ring.frame(slots.frame(radius));
+    m.width = width;
+    m.height = height;
+    m.stride = stride;
+    m.grid = make_grid(width, height, block, step);
+    m.range = bm_range;
+    m.group = group;
+    m.t0 = radius;
+    m.wg_begin = 0;
+    m.out = reinterpret_cast<MatchRec*>(out_rec);
+    m.counts = out_count;
+    const int nref = m.grid.nx * m.grid.ny;
+    if (nref == 0) return NSS_HIP_OK;
+    size_t lds;
+    const int kind = choose_match_kind(block, step, bm_range, group, width, height, nref, &lds);
+    hipError_t e = launch_match(block, mt, match_workgroups(kind, m.grid), lds, kind, s);
+    if (e != hipSuccess) return hip_fail(e, "match launch");
+    // group 1 keeps the reference alone (src/cpu/bm/matcher.hpp:323).
+    if (group == 1) return NSS_HIP_OK;
+    PredTable pt{};
+    PredJob& pj = pt.job[pt.n++];
+    pj.match = ring;
+    pj.width = width;
+    pj.height = height;
+    pj.stride = stride;
+    pj.nref = nref;
+    pj.group = group;
+    pj.ps_num = ps_num;
+    pj.ps_range = ps_range;
+    pj.slots = slots;
+    pj.wg_begin = 0;
+    pj.recs = m.out;
+    pj.counts = out_count;
+    const size_t plds = predictive_table_lds(block, pt, radius);
+    e = launch_predictive(block, pt, nref, plds, s);
+    return e == hipSuccess ? NSS_HIP_OK : hip_fail(e, "predictive match launch");
 }
 
 }  // extern "C"

@@ -5,6 +5,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -118,7 +121,10 @@ struct Stage {
 };
 
 // Event pairs around launches while tracing is on (one vector per kind).
+// Calls on one instance may come from several threads: mu guards the pool
+// and the spans.
 struct Tracer {
+    std::mutex mu;
     bool on = false;
     std::vector<hipEvent_t> pool;
     size_t used = 0;
@@ -147,10 +153,20 @@ struct Tracer {
 struct FrameBuffers {
     unsigned long long* num[3]{};  // atomic path sums
     unsigned long long* den[3]{};
-    float* part[3]{};              // tile path partials: num then den
+    unsigned long long* part[3]{};  // tile path partials: num then den (int64, exact)
     MatchRec* matches[3]{};
     int* counts[3]{};
     float* basic[3]{};
+};
+
+// One of an instance's num_streams frame slots: a frame's buffers and the
+// event recorded after the last work queued on them, which the next lessee's
+// stream waits for (so calls on different HIP streams never overlap on one
+// slot's buffers).
+struct FrameSlot {
+    FrameBuffers buf;
+    hipEvent_t done = nullptr;
+    bool used = false;  // done has been recorded
 };
 
 }  // namespace nsship
@@ -164,8 +180,20 @@ struct nss_hip_bm3d {
     int pw[3]{}, ph[3]{};
     nsship::Stage stages[2]{};
     int nstages = 0;
-    size_t part_floats[3]{};  // per half (num or den) of a plane's tile partials
-    nsship::FrameBuffers buf;  // the instance's own frame (process)
+    size_t part_elems[3]{};  // per half (num or den) of a plane's tile partials, int64 entries
+    // num_streams frame slots (radius 0; one for temporal instances). A
+    // process call leases the slot free the longest for the time it takes to
+    // queue the frame, sleeping while all are leased
+    // (src/cuda/runtime/stream_pool.hpp:1-4 SlotPool::acquire).
+    std::vector<nsship::FrameSlot> slots;
+    std::mutex slot_mu;
+    std::condition_variable slot_cv;
+    std::deque<int> free_slots;
+    // Temporal push / pop: one call at a time, each ordered after the last on
+    // the device (roll_done), whatever stream it comes on.
+    std::mutex roll_mu;
+    hipEvent_t roll_done = nullptr;
+    bool roll_used = false;
     // Rolling temporal state (radius > 0): frames enter in clip order and
     // stay on the device in rings (frame f at slot f % ring); each stage keeps
     // frame-sized int64 sums for the 2r+1 frames its current centre touches.
@@ -216,6 +244,8 @@ struct FrameRef {
 // Device bytes of one frame's buffers for instance h (radius 0), and of all
 // the instance holds (frame buffers plus rolling rings and sums).
 size_t frame_buffer_bytes(const nss_hip_bm3d* h);
+// Frame slots an instance holds: num_streams at radius 0, one otherwise.
+int frame_slot_count(const nss_hip_bm3d* h);
 size_t instance_bytes(const nss_hip_bm3d* h);
 hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb);
 void free_frame_buffers(FrameBuffers& fb);

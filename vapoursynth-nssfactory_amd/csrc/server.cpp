@@ -25,6 +25,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -46,6 +47,15 @@ constexpr size_t kGraphCacheLimit = 64;
 
 enum ReqState { kQueued, kLaunched, kDone, kFailed };
 
+// A frame slot: one radius-0 frame's device buffers. Slots live behind
+// unique_ptr, so a request keeps a stable pointer to its slot while idle
+// slots of the same or another stream are released around it.
+struct Slot {
+    FrameBuffers buf;
+    bool own = false;  // the instance's own buffers (never released)
+    bool busy = false;
+};
+
 struct Request {
     long long id = 0;
     int stream = 0;
@@ -55,22 +65,17 @@ struct Request {
     int dst_stride[3]{};
     float sigma_user = 0.f;
     int frame = 0;  // radius > 0: index in the clip
-    int slot = -1;
+    Slot* slot = nullptr;
+    hipEvent_t ready = nullptr;  // recorded on the client's stream at submit
     ReqState state = kQueued;
     Clock::time_point t_submit, t_done;
-};
-
-struct Slot {
-    FrameBuffers buf;
-    bool own = false;  // the instance's own buffers (never released)
-    bool busy = false;
 };
 
 struct ClipStream {
     nss_hip_bm3d* h = nullptr;
     int radius = 0;
     size_t slot_bytes = 0;
-    std::vector<Slot> slots;
+    std::vector<std::unique_ptr<Slot>> slots;
     hipStream_t hs = nullptr;                  // radius > 0
     int submitted = 0;                          // radius > 0: frames queued so far
     std::map<int, long long> frame_request;    // radius > 0: frame -> request
@@ -149,7 +154,7 @@ struct nss_hip_server {
                 Request& r = requests[id];
                 r.state = q == hipSuccess ? kDone : kFailed;
                 r.t_done = now;
-                if (r.slot >= 0) streams[r.stream].slots[r.slot].busy = false;
+                if (r.slot) r.slot->busy = false;
                 ++st.completed;
             }
             event_pool.push_back(inflight[k].done);
@@ -159,37 +164,49 @@ struct nss_hip_server {
     }
 
     // A free slot of stream c, allocating one within the budget (releasing
-    // idle extra slots of other streams when needed); -1 if none.
-    int free_slot(int c) {
+    // idle extra slots of other streams when needed); null if none.
+    Slot* free_slot(int c) {
         ClipStream& cs = streams[c];
-        for (size_t i = 0; i < cs.slots.size(); ++i)
-            if (!cs.slots[i].busy) return static_cast<int>(i);
+        for (auto& sl : cs.slots)
+            if (!sl->busy) return sl.get();
         const size_t budget = static_cast<size_t>(cfg.hbm_budget_bytes);
         if (bytes_in_use() + cs.slot_bytes > budget) {
             for (size_t o = 0; o < streams.size() && bytes_in_use() + cs.slot_bytes > budget; ++o) {
                 if (o == static_cast<size_t>(c) || streams[o].radius > 0) continue;
                 auto& sl = streams[o].slots;
                 for (size_t i = sl.size(); i-- > 0;) {
-                    if (sl[i].own || sl[i].busy) continue;
-                    free_frame_buffers(sl[i].buf);
+                    if (sl[i]->own || sl[i]->busy) continue;
+                    free_frame_buffers(sl[i]->buf);
                     sl.erase(sl.begin() + static_cast<long>(i));
+                    ++st.slots_released;
                     if (bytes_in_use() + cs.slot_bytes <= budget) break;
                 }
             }
-            if (bytes_in_use() + cs.slot_bytes > budget) return -1;
+            if (bytes_in_use() + cs.slot_bytes > budget) return nullptr;
         }
-        Slot s;
-        if (alloc_frame_buffers(cs.h, s.buf) != hipSuccess) return -1;
-        cs.slots.push_back(s);
+        auto s = std::make_unique<Slot>();
+        if (alloc_frame_buffers(cs.h, s->buf) != hipSuccess) return nullptr;
+        cs.slots.push_back(std::move(s));
         st.bytes_in_use = static_cast<long long>(bytes_in_use());
         st.peak_bytes = std::max(st.peak_bytes, st.bytes_in_use);
-        return static_cast<int>(cs.slots.size()) - 1;
+        return cs.slots.back().get();
+    }
+
+    // Orders stream s after the client work that produced request r's
+    // planes, then returns r's readiness event to the pool (a wait takes
+    // the event's state at the time of the call).
+    bool wait_ready(Request& r, hipStream_t s) {
+        if (!r.ready) return true;
+        const hipError_t e = hipStreamWaitEvent(s, r.ready, 0);
+        event_pool.push_back(r.ready);
+        r.ready = nullptr;
+        return e == hipSuccess;
     }
 
     bool launch_temporal(long long id) {
         Request& r = requests[id];
         ClipStream& cs = streams[r.stream];
-        if (nss_hip_bm3d_push(cs.h, r.src, r.src_stride, nullptr, nullptr, cs.hs) != NSS_HIP_OK) {
+        if (!wait_ready(r, cs.hs) || nss_hip_bm3d_push(cs.h, r.src, r.src_stride, nullptr, nullptr, cs.hs) != NSS_HIP_OK) {
             r.state = kFailed;
             r.t_done = Clock::now();
             ++st.completed;
@@ -311,17 +328,17 @@ struct nss_hip_server {
         for (size_t q = 0; q < queue.size() && static_cast<int>(frames.size()) < cfg.max_batch_frames;) {
             const long long id = queue[q];
             Request& r = requests[id];
-            const int slot = free_slot(r.stream);
-            if (slot < 0) {
+            Slot* slot = free_slot(r.stream);
+            if (!slot) {
                 ++st.admissions_deferred;
                 ++q;
                 continue;
             }
             ClipStream& cs = streams[r.stream];
-            cs.slots[slot].busy = true;
+            slot->busy = true;
             r.slot = slot;
             FrameRef f = frame_ref(cs.h, r.src, r.src_stride, nullptr, nullptr, r.dst, r.dst_stride);
-            f.buf = cs.slots[slot].buf;
+            f.buf = slot->buf;
             if (r.sigma_user > 0.f) {
                 const float eff = effective_sigma(r.sigma_user);
                 for (int k = 0; k < cs.h->nstages; ++k)
@@ -335,7 +352,10 @@ struct nss_hip_server {
         if (frames.empty()) return false;
         hipStream_t s = hs[static_cast<size_t>(next_hs)];
         next_hs = (next_hs + 1) % static_cast<int>(hs.size());
-        const int rc = cfg.use_graphs ? run_graph(frames, s)
+        bool ready = true;
+        for (long long id : ids) ready &= wait_ready(requests[id], s);
+        const int rc = !ready ? fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: server stream wait failed")
+                     : cfg.use_graphs ? run_graph(frames, s)
                                       : run_batch(frames.data(), static_cast<int>(frames.size()), nullptr, s);
         Batch b;
         b.reqs = ids;
@@ -347,7 +367,7 @@ struct nss_hip_server {
                 Request& r = requests[id];
                 r.state = kFailed;
                 r.t_done = now;
-                streams[r.stream].slots[r.slot].busy = false;
+                r.slot->busy = false;
                 ++st.completed;
             }
             if (b.done) event_pool.push_back(b.done);
@@ -441,7 +461,11 @@ int nss_hip_server_open_stream(nss_hip_server* s, const nss_hip_bm3d_params* par
     ClipStream cs;
     cs.radius = params->radius;
     nss_hip_bm3d* h = nullptr;
-    int rc = nss_hip_bm3d_create(params, info, s->cfg.device_id, &h);
+    // The scheduler gives each frame a slot of its own; the instance keeps
+    // one (the stream's first slot).
+    nss_hip_bm3d_params one = *params;
+    one.num_streams = 1;
+    int rc = nss_hip_bm3d_create(&one, info, s->cfg.device_id, &h);
     if (rc != NSS_HIP_OK) return rc;
     cs.h = h;
     cs.slot_bytes = cs.radius > 0 ? instance_bytes(h) : frame_buffer_bytes(h);
@@ -467,12 +491,12 @@ int nss_hip_server_open_stream(nss_hip_server* s, const nss_hip_bm3d_params* par
             }
             cs.slot_bytes = frame_buffer_bytes(h);
         }
-        Slot own;
-        own.buf = h->buf;
-        own.own = true;
-        cs.slots.push_back(own);
+        auto own = std::make_unique<Slot>();
+        own->buf = h->slots[0].buf;
+        own->own = true;
+        cs.slots.push_back(std::move(own));
     }
-    s->streams.push_back(cs);
+    s->streams.push_back(std::move(cs));
     s->st.bytes_in_use = static_cast<long long>(s->bytes_in_use());
     s->st.peak_bytes = std::max(s->st.peak_bytes, s->st.bytes_in_use);
     *stream_id = static_cast<int>(s->streams.size()) - 1;
@@ -480,7 +504,8 @@ int nss_hip_server_open_stream(nss_hip_server* s, const nss_hip_bm3d_params* par
 }
 
 int nss_hip_server_submit(nss_hip_server* s, int stream_id, const float* const* src, const int* src_stride,
-                          float* const* dst, const int* dst_stride, double sigma_user, long long* request_id) {
+                          float* const* dst, const int* dst_stride, double sigma_user, void* ready_stream,
+                          long long* request_id) {
     if (!s || !src || !src_stride || !dst || !dst_stride || !request_id)
         return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
     if (!(sigma_user >= 0.0) || sigma_user > 1e30) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: invalid sigma_user");
@@ -505,6 +530,15 @@ int nss_hip_server_submit(nss_hip_server* s, int stream_id, const float* const* 
         r.dst_stride[i] = dst_stride[i];
     }
     r.sigma_user = static_cast<float>(sigma_user);
+    if (ready_stream) {
+        DeviceGuard guard(s->cfg.device_id);
+        r.ready = s->take_event();
+        if (!r.ready || guard.err != hipSuccess ||
+            hipEventRecord(r.ready, static_cast<hipStream_t>(ready_stream)) != hipSuccess) {
+            if (r.ready) s->event_pool.push_back(r.ready);
+            return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: readiness event record failed");
+        }
+    }
     r.t_submit = Clock::now();
     if (cs.radius > 0) {
         r.frame = cs.submitted++;
@@ -567,7 +601,7 @@ int nss_hip_server_destroy(nss_hip_server* s) {
             (void)hipStreamDestroy(c.hs);
         }
         for (auto& sl : c.slots)
-            if (!sl.own) free_frame_buffers(sl.buf);
+            if (!sl->own) free_frame_buffers(sl->buf);
         nss_hip_bm3d_destroy(c.h);
     }
     for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.second.exec);

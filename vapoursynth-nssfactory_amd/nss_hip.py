@@ -125,7 +125,8 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_server_query", "nss_hip_server_drain", "nss_hip_server_get_stats", "nss_hip_server_destroy",
            "nss_hip_bm3d_autotune", "nss_hip_vaggregate", "nss_hip_bm3d_resources", "nss_hip_wnnm_parse",
            "nss_hip_wnnm_create", "nss_hip_wnnm_process", "nss_hip_wnnm_destroy", "nss_hip_ncsr_parse",
-           "nss_hip_ncsr_create", "nss_hip_ncsr_process", "nss_hip_ncsr_destroy")
+           "nss_hip_ncsr_create", "nss_hip_ncsr_process", "nss_hip_ncsr_destroy", "nss_hip_bm3d_set_tiles",
+           "nss_hip_predictive_match")
 # Contribution identity of legacy temporal frames (frame properties the
 # VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
 # src/frontend/contribution.hpp:12-27).
@@ -142,7 +143,7 @@ class ServerConfig(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_longlong) for n in ("submitted", "completed", "batches", "batched_frames",
                                                  "graph_replays", "graph_captures", "bytes_in_use", "peak_bytes",
-                                                 "admissions_deferred")]
+                                                 "admissions_deferred", "slots_released")]
 
 
 def lib():
@@ -165,17 +166,19 @@ def lib():
         L.nss_hip_bm3d_destroy.argtypes = [vp]
         L.nss_hip_spatial_match.argtypes = [vp, ci, ci, ci, ci, ci, ci, ci, vp, vp, vp]
         L.nss_hip_set_match_path.argtypes = [ci]
+        L.nss_hip_predictive_match.argtypes = [vp, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, vp, vp, vp]
         L.nss_hip_bm3d_set_tracing.argtypes = [vp, ci]
         L.nss_hip_bm3d_kernel_times.argtypes = [vp, P(ci), P(ctypes.c_double), ci]
         L.nss_hip_bm3d_push.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_bm3d_pop.argtypes = [vp, P(vp), P(ci), P(ci), vp]
         L.nss_hip_bm3d_autotune.argtypes = [vp, ci, vp, P(ctypes.c_double)]
+        L.nss_hip_bm3d_set_tiles.argtypes = [vp, ci, ci, ci]
         L.nss_hip_vaggregate.argtypes = [vp, ci, P(vp), P(vp), P(ci), ci, vp, ci, ci, ci, vp]
         L.nss_hip_bm3d_resources.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_longlong),
                                              P(ctypes.c_longlong)]
         L.nss_hip_server_create.argtypes = [P(ServerConfig), P(vp)]
         L.nss_hip_server_open_stream.argtypes = [vp, P(Params), P(VideoInfo), P(ci)]
-        L.nss_hip_server_submit.argtypes = [vp, ci, P(vp), P(ci), P(vp), P(ci), ctypes.c_double,
+        L.nss_hip_server_submit.argtypes = [vp, ci, P(vp), P(ci), P(vp), P(ci), ctypes.c_double, vp,
                                             P(ctypes.c_longlong)]
         L.nss_hip_server_query.argtypes = [vp, ctypes.c_longlong, P(ctypes.c_double)]
         L.nss_hip_server_drain.argtypes = [vp]
@@ -346,6 +349,15 @@ class BM3D:
         ms = ctypes.c_double(0)
         _check(lib().nss_hip_bm3d_autotune(self._h, repeats, _stream_handle(stream), ctypes.byref(ms)))
         return ms.value
+
+    def set_tiles(self, stage, tw, th):
+        """Filter tiles of tw x th references for stage (0 first); False when
+        the shape does not fit. The output does not change with the shape."""
+        rc = lib().nss_hip_bm3d_set_tiles(self._h, stage, tw, th)
+        if rc == UNSUPPORTED:
+            return False
+        _check(rc)
+        return True
 
     def set_tracing(self, on=True):
         _check(lib().nss_hip_bm3d_set_tracing(self._h, int(on)))
@@ -593,6 +605,27 @@ def spatial_match(plane, block=8, step=8, bm_range=9, group=8, stream=None):
     return rec, cnt
 
 
+def predictive_match(frames, center, radius, block=8, step=8, bm_range=9, group=8, ps_num=2, ps_range=4,
+                     stream=None):
+    """Batched temporal matcher over one plane's grid. frames: 3-D float32
+    CUDA tensor [nframes, height, width] (the clip's planes), centre frame
+    `center`. Returns (rec int32 [nref, group, 4] = x, y, slot, dist bits;
+    count int32 [nref])."""
+    import torch
+    n, h, w = frames.shape
+    if frames.dtype != torch.float32 or not frames.is_cuda or frames.stride(2) != 1 or \
+            frames.stride(0) != frames.stride(1) * h:
+        raise Error(f"{NAMESPACE}: frames must be a contiguous [frames, height, width] float32 CUDA tensor")
+    nx, ny = grid_size(w, h, block, step)
+    rec = torch.zeros((nx * ny, group, 4), dtype=torch.int32, device=frames.device)
+    cnt = torch.zeros((nx * ny,), dtype=torch.int32, device=frames.device)
+    _check(lib().nss_hip_predictive_match(ctypes.c_void_p(frames.data_ptr()), n, center, w, h, frames.stride(1),
+                                          block, step, bm_range, group, radius, ps_num, ps_range,
+                                          ctypes.c_void_p(rec.data_ptr()), ctypes.c_void_p(cnt.data_ptr()),
+                                          _stream_handle(stream)))
+    return rec, cnt
+
+
 def VAggregate(clip, src, radius, planes=None, stream=None):
     """nss.VAggregate on the device (frame logic src/host/filter_vaggregate.cpp:29-100,
     argument checks src/frontend/vaggregate_args.cpp:13-45): ``clip`` holds the
@@ -668,6 +701,10 @@ class Server:
         self._h = h
         self.device = device
         self._dims = []
+        # request id -> (planes, out): the tensors stay referenced until the
+        # request is reported done, so the caching allocator cannot hand
+        # their memory to another tensor while the scheduler still uses it.
+        self._held = {}
 
     def open_stream(self, fmt, width, height, frames=1, **kwargs):
         """A clip stream with nss.BM3D arguments; returns its id."""
@@ -680,9 +717,11 @@ class Server:
         self._dims.append([(height >> (sh if i else 0), width >> (sw if i else 0)) for i in range(n)])
         return sid.value
 
-    def submit(self, stream, planes, out, sigma=0.0):
-        """Queues one frame (device planes, ready before the call); `out`
-        planes are written when the request completes. Returns its id."""
+    def submit(self, stream, planes, out, sigma=0.0, ready_stream=None):
+        """Queues one frame (device planes); `out` planes are written when
+        the request completes. The request waits for the work queued so far
+        on `ready_stream` (default: torch's current stream), where the planes
+        were produced. Returns its id."""
         import torch
         dims = self._dims[stream]
         for what, ps in (("plane", planes), ("output plane", out)):
@@ -698,7 +737,9 @@ class Server:
         dst = (ctypes.c_void_p * n)(*[p.data_ptr() for p in out])
         dstr = (ctypes.c_int * n)(*[p.stride(0) for p in out])
         rid = ctypes.c_longlong(-1)
-        _check(lib().nss_hip_server_submit(self._h, stream, src, sst, dst, dstr, float(sigma), ctypes.byref(rid)))
+        _check(lib().nss_hip_server_submit(self._h, stream, src, sst, dst, dstr, float(sigma),
+                                           _stream_handle(ready_stream), ctypes.byref(rid)))
+        self._held[rid.value] = (list(planes), list(out))
         return rid.value
 
     def query(self, rid):
@@ -707,11 +748,13 @@ class Server:
         rc = lib().nss_hip_server_query(self._h, rid, ctypes.byref(lat))
         if rc == BYPASS:
             return None
+        self._held.pop(rid, None)
         _check(rc)
         return lat.value
 
     def drain(self):
         _check(lib().nss_hip_server_drain(self._h))
+        self._held.clear()
 
     def stats(self):
         st = ServerStats()
