@@ -201,14 +201,23 @@ def test_budget_evicts_idle_slots_under_busy_ones(gpu):
     b = srv.open_stream("GRAYS", 128, 96, sigma=10)
     specs = {a: ("YUV420PS", 160, 120, {"final": 1}), b: ("GRAYS", 128, 96, {})}
     order = [a] * 6 + [b] * 6 + [a, b] * 4 + [a] * 4 + [b] * 4
-    reqs = []
+    work = []
     for k, sid in enumerate(order):
         fmt, w, h, kw = specs[sid]
         planes = _frames(fmt, w, h, 1, seed=100 + k)[0]
         src = [_dev(p) for p in planes]
         out = [torch.empty_like(p) for p in src]
-        torch.cuda.synchronize()
-        reqs.append((srv.submit(sid, src, out), sid, planes, out))
+        work.append((sid, planes, src, out))
+    torch.cuda.synchronize()
+    # Every request waits for a long kernel on a side stream, so the first
+    # four batches hold A's four slots while the rest of the queue arrives;
+    # then B's second frame needs a slot that only A's idle extras can free.
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        x = torch.randn(1 << 24, device="cuda")
+        for _ in range(40):
+            x = x * 1.0001
+    reqs = [(srv.submit(sid, src, out, ready_stream=side), sid, planes, out) for sid, planes, src, out in work]
     srv.drain()
     st = srv.stats()
     print(f"eviction stats {st}", flush=True)
