@@ -155,6 +155,25 @@ int nss_hip_bm3d_process(nss_hip_bm3d* handle, const float* const* src, const in
                          const float* const* ref, const int* ref_stride, float* const* dst, const int* dst_stride,
                          int frame, int frame_count, void* stream);
 
+/* No counterpart in the reference: the frame filter of nss_hip_bm3d_process
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It takes
+ * the resolved parameters and format instead of an instance and allocates
+ * nothing: the frame's working buffers (match lists, tile partials, the basic
+ * estimate) live in one fixed device array of 192 MiB per GPU, held from the
+ * library's load until the process ends. A frame that needs more (a
+ * 1920 x 1080 YUV420 two-stage frame at the default arguments needs about
+ * 158 MiB; 3840 x 2160 does not fit) fails with NSS_HIP_UNSUPPORTED before
+ * any work is queued. Radius 0 only. Every call on a GPU shares that GPU's
+ * array and keeps nothing between calls: calls that overlap on one GPU
+ * (different streams, or threads without ordering) race. The first call on
+ * a device also loads the DCT tables there (a constant-memory copy), so make
+ * it outside a graph capture. Output bits equal nss_hip_bm3d_process's for
+ * the same parameters, format and planes. Asynchronous on stream. */
+int nss_hip_bm3d_process_noalloc(const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device,
+                                 const float* const* src, const int* src_stride, const float* const* ref,
+                                 const int* ref_stride, float* const* dst, const int* dst_stride, void* stream);
+
 /* Temporal instances (radius > 0; rolling, or legacy below): the rolling
  * computation of rollingGetFrame (src/host/filter_bm3d.cpp:730-851,
  * :985-1072) with all state on the device. Push the clip's frames in order

@@ -371,6 +371,32 @@ hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb) {
     return e;
 }
 
+size_t carve_frame_buffers(const nss_hip_bm3d* h, char* base, FrameBuffers& fb) {
+    fb = FrameBuffers{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~static_cast<size_t>(255);
+        return p;
+    };
+    for (int i = 0; i < h->nplanes; ++i) {
+        const PlaneNeeds n = plane_needs(h, i);
+        const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+        if (n.recs) {
+            fb.matches[i] = reinterpret_cast<MatchRec*>(take(n.recs * sizeof(MatchRec)));
+            fb.counts[i] = reinterpret_cast<int*>(take(n.nref * sizeof(int)));
+        }
+        if (n.atomic) {
+            fb.num[i] = reinterpret_cast<unsigned long long*>(take(npx * 8));
+            fb.den[i] = reinterpret_cast<unsigned long long*>(take(npx * 8));
+        }
+        if (h->part_elems[i])
+            fb.part[i] = reinterpret_cast<unsigned long long*>(take(2 * h->part_elems[i] * sizeof(unsigned long long)));
+        if (n.basic) fb.basic[i] = reinterpret_cast<float*>(take(npx * 4));
+    }
+    return off;
+}
+
 }  // namespace nsship
 
 namespace {
@@ -1209,28 +1235,13 @@ int grow_partials(nss_hip_bm3d* h, const size_t* need) {
 
 }  // namespace
 
-extern "C" {
+namespace nsship {
 
-const char* nss_hip_last_error(void) { return g_last_error.c_str(); }
-
-int nss_hip_backend_info(int device, char* text, int len) {
-    if (!text || len < 1) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: no output buffer");
-    int count = 0;
-    hipError_t e = hipGetDeviceCount(&count);
-    if (e != hipSuccess) return hip_fail(e, "device count");
-    if (device < 0 || device >= count) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: device index out of range");
-    hipDeviceProp_t prop;
-    e = hipGetDeviceProperties(&prop, device);
-    if (e != hipSuccess) return hip_fail(e, "device properties");
-    std::snprintf(text, static_cast<size_t>(len), "nss_hip 1.0 device=%d name=%s arch=%s cus=%d hbm_bytes=%zu",
-                  device, prop.name, prop.gcnArchName, prop.multiProcessorCount, prop.totalGlobalMem);
-    return NSS_HIP_OK;
-}
-
-int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device,
-                        nss_hip_bm3d** out) {
-    if (!params || !info || !out) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
-    *out = nullptr;
+// The host part of an instance (the node creation of
+// src/host/filter_bm3d.cpp:1143-1163 without its buffers): format checks,
+// plane sizes, the stages and their launch plans, the tile partial sizes and,
+// for radius > 0, the rolling plan. Allocates no device memory.
+int plan_instance(nss_hip_bm3d* h, const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device) {
     if (info->sample_type != 1 || info->bits_per_sample != 32 || info->width <= 0 || info->height <= 0 ||
         info->num_planes < 1 || info->num_planes > 3)
         return fail(NSS_HIP_INVALID_INPUT, "nss_hip.BM3D: constant Gray/YUV/RGB 32-bit float required");
@@ -1238,13 +1249,6 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
         return fail(NSS_HIP_UNSUPPORTED, "nss_hip: radius > 0 supports planes up to 8191 x 8191");
     if (params->radius != 0 && info->num_frames < 1)
         return fail(NSS_HIP_INVALID_INPUT, "nss_hip: radius > 0 needs the clip length");
-    DeviceGuard guard(device);
-    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
-    int rc = ensure_tables(device);
-    if (rc != NSS_HIP_OK) return rc;
-
-    auto* h = new (std::nothrow) nss_hip_bm3d();
-    if (!h) return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: out of host memory");
     h->p = *params;
     h->vi = *info;
     h->device = device;
@@ -1271,17 +1275,11 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
     for (int s = 0; s < h->nstages; ++s) {
         for (int i = 0; i < h->nplanes; ++i) {
             if (h->stages[s].plane[i].sigma == 0.f) continue;
-            rc = check_stage_plane(h->stages[s].plane[i], h->pw[i], h->ph[i]);
-            if (rc != NSS_HIP_OK) {
-                delete h;
-                return rc;
-            }
+            const int rc = check_stage_plane(h->stages[s].plane[i], h->pw[i], h->ph[i]);
+            if (rc != NSS_HIP_OK) return rc;
         }
-        rc = plan_stage(h, h->stages[s]);
-        if (rc != NSS_HIP_OK) {
-            delete h;
-            return rc;
-        }
+        const int rc = plan_stage(h, h->stages[s]);
+        if (rc != NSS_HIP_OK) return rc;
     }
     const bool rolling = p.radius > 0;
     for (int i = 0; i < h->nplanes; ++i) h->part_elems[i] = plane_needs(h, i).part;
@@ -1311,6 +1309,45 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
                 R.part_center[s][i] = centre_partials(g, 2 * r + 1, nullptr);
             }
     }
+    return NSS_HIP_OK;
+}
+
+}  // namespace nsship
+
+extern "C" {
+
+const char* nss_hip_last_error(void) { return g_last_error.c_str(); }
+
+int nss_hip_backend_info(int device, char* text, int len) {
+    if (!text || len < 1) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: no output buffer");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return hip_fail(e, "device count");
+    if (device < 0 || device >= count) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: device index out of range");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return hip_fail(e, "device properties");
+    std::snprintf(text, static_cast<size_t>(len), "nss_hip 1.0 device=%d name=%s arch=%s cus=%d hbm_bytes=%zu",
+                  device, prop.name, prop.gcnArchName, prop.multiProcessorCount, prop.totalGlobalMem);
+    return NSS_HIP_OK;
+}
+
+int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device,
+                        nss_hip_bm3d** out) {
+    if (!params || !info || !out) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    *out = nullptr;
+    DeviceGuard guard(device);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
+    auto* h = new (std::nothrow) nss_hip_bm3d();
+    if (!h) return fail(NSS_HIP_RESOURCE_FAILURE, "nss_hip: out of host memory");
+    int rc = plan_instance(h, params, info, device);
+    if (rc == NSS_HIP_OK) rc = ensure_tables(device);
+    if (rc != NSS_HIP_OK) {
+        delete h;
+        return rc;
+    }
+    const nss_hip_bm3d_params& p = *params;
+    const bool rolling = p.radius > 0;
     // memory_limit_mb caps the instance's device memory; a limit too small
     // for the clip is an error at creation, never while frames are filtered
     // (docs/api/cuda/README.md:141, src/cuda/common/group_driver.cpp:491-498).

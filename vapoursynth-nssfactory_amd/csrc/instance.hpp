@@ -249,6 +249,12 @@ int frame_slot_count(const nss_hip_bm3d* h);
 size_t instance_bytes(const nss_hip_bm3d* h);
 hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb);
 void free_frame_buffers(FrameBuffers& fb);
+// The same buffer set laid out from base (each buffer 256-byte aligned);
+// returns the bytes it spans. base null: the size only.
+size_t carve_frame_buffers(const nss_hip_bm3d* h, char* base, FrameBuffers& fb);
+// The host part of an instance: checks, plane sizes, stage plans, partial
+// sizes and the rolling plan; no device memory.
+int plan_instance(nss_hip_bm3d* h, const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device);
 FrameRef frame_ref(nss_hip_bm3d* h, const float* const* src, const int* src_stride, const float* const* ref,
                    const int* ref_stride, float* const* dst, const int* dst_stride);
 int run_batch(const FrameRef* frames, int nframes, Tracer* tr, hipStream_t s);

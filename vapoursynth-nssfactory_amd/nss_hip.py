@@ -126,7 +126,7 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_bm3d_autotune", "nss_hip_vaggregate", "nss_hip_bm3d_resources", "nss_hip_wnnm_parse",
            "nss_hip_wnnm_create", "nss_hip_wnnm_process", "nss_hip_wnnm_destroy", "nss_hip_ncsr_parse",
            "nss_hip_ncsr_create", "nss_hip_ncsr_process", "nss_hip_ncsr_destroy", "nss_hip_bm3d_set_tiles",
-           "nss_hip_predictive_match")
+           "nss_hip_predictive_match", "nss_hip_bm3d_process_noalloc")
 # Contribution identity of legacy temporal frames (frame properties the
 # VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
 # src/frontend/contribution.hpp:12-27).
@@ -164,6 +164,8 @@ def lib():
         L.nss_hip_bm3d_slots.argtypes = [vp]
         L.nss_hip_bm3d_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), ci, ci, vp]
         L.nss_hip_bm3d_destroy.argtypes = [vp]
+        L.nss_hip_bm3d_process_noalloc.argtypes = [P(Params), P(VideoInfo), ci, P(vp), P(ci), P(vp), P(ci), P(vp),
+                                                   P(ci), vp]
         L.nss_hip_spatial_match.argtypes = [vp, ci, ci, ci, ci, ci, ci, ci, vp, vp, vp]
         L.nss_hip_set_match_path.argtypes = [ci]
         L.nss_hip_predictive_match.argtypes = [vp, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, vp, vp, vp]
