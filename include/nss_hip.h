@@ -150,7 +150,11 @@ int nss_hip_bm3d_slots(const nss_hip_bm3d* handle);
  * bm3dGetFrame, both stages when final=1 (the basic estimate stays on the
  * device). src[p] / ref[p] / dst[p]: device pointers of plane p (ref may be
  * NULL when no ref clip was given), strides in floats. frame / frame_count are
- * informational. Asynchronous on stream. */
+ * informational. Asynchronous on stream. Thread-safe: concurrent calls on
+ * one handle each lease one of the instance's num_streams frame slots (the
+ * reference's SlotPool::acquire, src/cuda/runtime/stream_pool.hpp:1-4),
+ * sleeping while all are leased; a call's work is ordered after the slot's
+ * previous work, whatever stream that was queued on. */
 int nss_hip_bm3d_process(nss_hip_bm3d* handle, const float* const* src, const int* src_stride,
                          const float* const* ref, const int* ref_stride, float* const* dst, const int* dst_stride,
                          int frame, int frame_count, void* stream);
