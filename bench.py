@@ -77,6 +77,25 @@ def cpu_baseline(noisy, params, threads):
     return threads * w0 * h0 / dt / 1e6, outs[0], dt
 
 
+def host_cpu():
+    """The host CPU the baseline ran on: model name, logical CPUs (and how
+    many this process may use), and the SIMD level of the oracle build."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable,
+            "isa": "x86-64-v3 build (AVX2, FMA; -O3, no fast-math); host avx512f: "
+                   + ("yes" if "avx512f" in flags else "no")}
+
+
 FORMS = {"fp32": 0, "bf16x3": 1, "fp16": 2}
 # HBM bytes per launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch, averaged over the basic and Wiener launches)
@@ -92,61 +111,80 @@ FORM_DTYPE = {"fp32": "f32", "bf16x3": "bf16x3 (f32 accumulate)", "fp16": "f16 (
 PEAK_MFMA_TFLOPS = 2500.0  # dense bf16 / fp16
 
 
-def measure(nss_hip, dev, form, steps, warmup, dims):
-    """Times `steps` frames of one transform form; per-kernel averages from
-    the library's event tracing on the launch stream."""
+def measure_forms(nss_hip, dev, steps, warmup, dims, rounds=4):
+    """Times `steps` frames of every transform form, interleaved: the forms
+    take turns in `rounds` rounds (the order rotating each round), so no form
+    always runs first on a cold device. Per-kernel averages come from the
+    library's event tracing on the launch stream, in a separate pass."""
     import numpy as np
     import torch
-    inst = nss_hip.BM3D(FMT, W, H, sigma=SIGMA, final=1, transform=form)
-    out = [torch.empty_like(p) for p in dev]
+    names = list(FORMS)
+    insts = {n: nss_hip.BM3D(FMT, W, H, sigma=SIGMA, final=1, transform=FORMS[n]) for n in names}
+    outs = {n: [torch.empty_like(p) for p in dev] for n in names}
     stream = torch.cuda.current_stream()
-    for _ in range(warmup):
-        inst(dev, out=out)
+    rounds = max(1, min(rounds, steps))
+    for r in range(rounds):
+        for n in names:
+            for _ in range(warmup // rounds + (1 if r < warmup % rounds else 0)):
+                insts[n](dev, out=outs[n])
     torch.cuda.synchronize()
-    # Timed pass without per-kernel events (they would sit between the
+    # Timed passes without per-kernel events (they would sit between the
     # launches); per-frame events only, for the latency percentiles.
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(steps):
-        starts[i].record(stream)
-        inst(dev, out=out)
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
-    # Traced pass over the same frames for the per-kernel averages.
-    inst.set_tracing(True)
-    inst.kernel_times(reset=True)
-    for i in range(max(steps // 2, 1)):
-        inst(dev, out=out)
-    torch.cuda.synchronize()
-    kt = inst.kernel_times(reset=True)
-    inst.set_tracing(False)
-    params = inst.params
-    inst.close()
-    sw_b, tr_b, _ = stage_work(params, dims, True)
-    sw_f, tr_f, _ = stage_work(params, dims, False)
-    kernels = {}
-    for kind, (n, ms) in kt.items():
-        if n:
-            kernels[kind] = {"launches": n, "avg_us": ms / n * 1e3}
-    # Per launch of a kind: one stage of the frame (basic, then Wiener, alternating).
-    filter_peak = PEAK_FP32_TFLOPS if form == 0 else PEAK_MFMA_TFLOPS
-    work = {"match": ((sw_b + sw_f) / 2, "TFLOP/s", PEAK_FP32_TFLOPS),
-            "filter": ((tr_b + tr_f) / 2, "TFLOP/s", filter_peak),
-            "finish": (W * H * 1.5 * 16 / 1, "GB/s", PEAK_HBM_GBS)}
-    for kind, (amount, unit, peak) in work.items():
-        if kind in kernels:
-            avg_s = kernels[kind]["avg_us"] * 1e-6
-            ach = amount / avg_s / (1e12 if unit == "TFLOP/s" else 1e9)
-            kernels[kind].update({"achieved": ach, "unit": unit, "peak": peak, "frac": ach / peak,
-                                  "algorithmic_per_launch": amount})
-    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
-    return {"value": round(W * H * steps / wall / 1e6, 3), "ms_per_step": round(wall / steps * 1e3, 4),
-            "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
-            "kernels": kernels, "dominant": dom, "params": params, "out": [o.cpu().numpy() for o in out]}
+    wall = {n: 0.0 for n in names}
+    lat = {n: [] for n in names}
+    for r in range(rounds):
+        k = steps // rounds + (1 if r < steps % rounds else 0)
+        for n in names[r % len(names):] + names[:r % len(names)]:
+            starts = [torch.cuda.Event(enable_timing=True) for _ in range(k)]
+            ends = [torch.cuda.Event(enable_timing=True) for _ in range(k)]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(k):
+                starts[i].record(stream)
+                insts[n](dev, out=outs[n])
+                ends[i].record(stream)
+            torch.cuda.synchronize()
+            wall[n] += time.perf_counter() - t0
+            lat[n] += [st.elapsed_time(e) for st, e in zip(starts, ends)]
+    res = {}
+    for n in names:
+        inst = insts[n]
+        # Traced pass over the same frames for the per-kernel averages.
+        inst.set_tracing(True)
+        inst.kernel_times(reset=True)
+        for i in range(max(steps // 2, 1)):
+            inst(dev, out=outs[n])
+        torch.cuda.synchronize()
+        kt = inst.kernel_times(reset=True)
+        inst.set_tracing(False)
+        params = inst.params
+        inst.close()
+        sw_b, tr_b, _ = stage_work(params, dims, True)
+        sw_f, tr_f, _ = stage_work(params, dims, False)
+        kernels = {}
+        for kind, (cnt, ms) in kt.items():
+            if cnt:
+                kernels[kind] = {"launches": cnt, "avg_us": ms / cnt * 1e3}
+        # Per launch of a kind: one stage of the frame (basic, then Wiener,
+        # alternating). The matcher's SSD and the FP32 filter run on the FP32
+        # vector units; the MFMA forms' transforms on the matrix cores.
+        form = FORMS[n]
+        work = {"match": ((sw_b + sw_f) / 2, "TFLOP/s", PEAK_FP32_TFLOPS, "valu"),
+                "filter": ((tr_b + tr_f) / 2, "TFLOP/s", PEAK_FP32_TFLOPS if form == 0 else PEAK_MFMA_TFLOPS,
+                           "valu" if form == 0 else "mfma"),
+                "finish": (W * H * 1.5 * 16 / 1, "GB/s", PEAK_HBM_GBS, "hbm")}
+        for kind, (amount, unit, peak, bound) in work.items():
+            if kind in kernels:
+                avg_s = kernels[kind]["avg_us"] * 1e-6
+                ach = amount / avg_s / (1e12 if unit == "TFLOP/s" else 1e9)
+                kernels[kind].update({"achieved": ach, "unit": unit, "peak": peak, "frac": ach / peak,
+                                      "bound": bound, "algorithmic_per_launch": amount})
+        dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+        la = np.array(lat[n])
+        res[n] = {"value": round(W * H * steps / wall[n] / 1e6, 3), "ms_per_step": round(wall[n] / steps * 1e3, 4),
+                  "p50_ms": round(float(np.percentile(la, 50)), 4), "p99_ms": round(float(np.percentile(la, 99)), 4),
+                  "kernels": kernels, "dominant": dom, "params": params, "out": [o.cpu().numpy() for o in outs[n]]}
+    return res
 
 
 def temporal_workload(args):
@@ -800,15 +838,14 @@ def main():
     clean, noisy = synth.noisy_frame(FMT, W, H, SIGMA, seed=0)
     dev = [torch.from_numpy(p).cuda() for p in noisy]
     dims = [tuple(p.shape) for p in noisy]
-    forms = {}
-    for name, code in FORMS.items():
-        forms[name] = measure(nss_hip, dev, code, args.steps, args.warmup, dims)
+    forms = measure_forms(nss_hip, dev, args.steps, args.warmup, dims)
+    for name in FORMS:
         print(f"# {name}: {forms[name]['ms_per_step']:.4f} ms/frame, kernels "
               f"{ {k: round(v['avg_us'], 2) for k, v in forms[name]['kernels'].items()} }", file=sys.stderr, flush=True)
     head = forms[args.form]
     dk = head["kernels"][head["dominant"]]
     tk = TRAFFIC_KIB.get((head["dominant"], args.form))
-    roofline = {"kernel": head["dominant"], "bound": "hbm" if dk.get("unit") == "GB/s" else "mfma",
+    roofline = {"kernel": head["dominant"], "bound": dk.get("bound"),
                 "achieved": dk.get("achieved"), "peak": dk.get("peak"), "unit": dk.get("unit"),
                 "frac": dk.get("frac"),
                 "traffic": None if tk is None else round((tk[0] * FETCH_SCALE + tk[1] * WRITE_SCALE) * 1024)}
@@ -831,13 +868,17 @@ def main():
         import oracle
         oracle.lib()
         cpu_value, want, dt = cpu_baseline(noisy, head["params"], threads)
+        t0 = time.perf_counter()
+        oracle.bm3d_frame(noisy, head["params"])
+        single_ms = (time.perf_counter() - t0) * 1e3
         res["psnr_vs_cpu_min_plane"] = round(min(oracle.psnr(g, w_) for g, w_ in zip(head["out"], want)), 3)
         for k, v in forms.items():
             res["forms"][k]["psnr_vs_cpu_min_plane"] = round(min(oracle.psnr(g, w_)
                                                                  for g, w_ in zip(v["out"], want)), 3)
         res["cpu_baseline"] = {"value": round(cpu_value, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
                                "sample": f"{threads} copies of the same 1920x1080 YUV420PS frame, one per thread, "
-                                         f"two stages ({dt:.1f} s wall)"}
+                                         f"two stages ({dt:.1f} s wall)",
+                               "single_thread_ms_per_frame": round(single_ms, 1), "host": host_cpu()}
     print(json.dumps(res), flush=True)
 
 

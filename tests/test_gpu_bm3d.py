@@ -130,6 +130,22 @@ def test_two_stage(gpu, oracle, fmt):
         assert p >= MIN_PSNR
 
 
+def test_headline_1080p_yuv420_two_stage(gpu, oracle):
+    """The headline workload at its full size: one 1920x1080 YUV420PS frame
+    (the sinusoid-and-rectangles luma, smooth chroma), sigma 10, both stages
+    at the default arguments, >= 60 dB per plane against the oracle."""
+    import synth
+    _, noisy = synth.noisy_frame("YUV420PS", 1920, 1080, 10, seed=0)
+    kw = dict(sigma=10, final=1)
+    got = _run(gpu, "YUV420PS", noisy, **kw)
+    want = _oracle_frame(oracle, gpu, "YUV420PS", noisy, **kw)
+    for i, (g, w_) in enumerate(zip(got, want)):
+        p = oracle.psnr(g, w_)
+        d = np.abs(g.astype(np.float64) - w_)
+        print(f"1080p YUV420 plane {i}: PSNR vs oracle {p:.2f} dB, max {d.max():.3e}", flush=True)
+        assert np.isfinite(g).all() and p >= MIN_PSNR
+
+
 @pytest.mark.parametrize("block", [1, 2, 4, 8, 12, 16, 32])
 def test_shapes_hard_and_wiener(gpu, oracle, block):
     """tests/cuda/test_cuda_bm3d_live.py:33-50: every block x group, basic and

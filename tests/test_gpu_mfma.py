@@ -86,6 +86,18 @@ def test_mfma_wiener_with_ref_group16(gpu, oracle):
                 sigma=20, group_size=16, block_step=4, bm_range=7)
 
 
+def test_mfma_precision_workload_4k_gray_sigma20(gpu, oracle):
+    """The precision workload at its full size: 3840x2160 GRAYS, sigma 20,
+    both stages (default block 8, group 8), the FP32 form and both MFMA
+    forms, each >= 60 dB per plane against the FP32 oracle, with the max and
+    RMS error against the FP64 oracle printed. At sigma 20 the hard
+    threshold sits where many coefficients lie, so this is where fp16's
+    rounding would flip threshold decisions without the FP32 re-decision."""
+    import synth
+    _, noisy = synth.noisy_frame("GRAYS", 3840, 2160, 20, seed=0)
+    _check_form(gpu, oracle, "GRAYS", noisy, "4k gray sigma20", sigma=20, final=1)
+
+
 def test_mfma_large_values_fp16_scaling(gpu, oracle):
     """Pixels in [0, 1000]: the fp16 form's per-group power-of-two scaling
     keeps the products in range (unscaled, 64-term sums would overflow fp16)."""

@@ -64,6 +64,58 @@ def test_mix_equals_alone(gpu, use_graphs):
     srv.close()
 
 
+def test_full_size_streams_mix_equals_alone(gpu):
+    """The serving workload's three streams at their full sizes (720x480
+    YUV420PS at radius 1, 1920x1080 YUV420PS, 3840x2160 RGBS; two stages,
+    per-request sigma from {3, 5, 10, 20}), frames queued at once so the
+    radius-0 frames share batches; every result is bitwise the one its
+    stream's own instance gives alone."""
+    import torch
+    import synth
+    rng = np.random.default_rng(8)
+    srv = gpu.Server(max_batch_frames=4, max_inflight=2)
+    n_t = 4
+    _, clip = synth.shifted_sequence("YUV420PS", 720, 480, n_t, 5, seed=1, max_step=4)
+    tid = srv.open_stream("YUV420PS", 720, 480, n_t, sigma=5, radius=1, final=1)
+    hd = srv.open_stream("YUV420PS", 1920, 1080, sigma=10, final=1)
+    uhd = srv.open_stream("RGBS", 3840, 2160, sigma=10, final=1)
+    work = []
+    for n in range(n_t):
+        src = [_dev(p) for p in clip[n]]
+        work.append(("t", n, tid, src, [torch.empty_like(p) for p in src], 0.0))
+    # one generated frame per stream (the 4K RGB generator is slow on the
+    # host), submitted three times with different sigmas
+    frames = {fmt: [_dev(p) for p in synth.noisy_frame(fmt, w, h, 10, seed=50)[1]]
+              for fmt, w, h in (("YUV420PS", 1920, 1080), ("RGBS", 3840, 2160))}
+    for k in range(3):
+        for sid, fmt in ((hd, "YUV420PS"), (uhd, "RGBS")):
+            src = frames[fmt]
+            sig = float(rng.choice([3, 5, 10, 20]))
+            work.append((fmt, k, sid, src, [torch.empty_like(p) for p in src], sig))
+    torch.cuda.synchronize()
+    rids = [srv.submit(sid, src, out, sigma=sig) for _, _, sid, src, out, sig in work]
+    srv.drain()
+    st = srv.stats()
+    print(f"full-size mix stats {st}", flush=True)
+    assert all(srv.query(r) is not None for r in rids)
+    inst = gpu.BM3D("YUV420PS", 720, 480, n_t, sigma=5, radius=1, final=1)
+    want_t = [[p.cpu().numpy() for p in f] for f in inst.filter_clip([[_dev(p) for p in f] for f in clip])]
+    inst.close()
+    for kind, n, sid, src, out, sig in work:
+        if kind == "t":
+            want = want_t[n]
+        else:
+            h, w = src[0].shape
+            inst = gpu.BM3D(kind, w, h, sigma=sig, final=1)
+            res = inst(src)
+            torch.cuda.synchronize()
+            want = [o.cpu().numpy() for o in res]
+            inst.close()
+        for i, (g, w_) in enumerate(zip(out, want)):
+            assert np.array_equal(g.cpu().numpy(), w_), (kind, n, i, sig)
+    srv.close()
+
+
 def test_graph_replays_match_eager(gpu):
     """The same composition replayed several times, with eager work and
     synchronisations between the replays; every replay equals the eager

@@ -534,6 +534,36 @@ __device__ inline float wave_max_f32(float v) {
     return v;
 }
 
+// v of the lane DPP control CTRL selects (controls that stay inside a row).
+template <int CTRL>
+__device__ inline float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// The value of lane l (uniform).
+__device__ inline float lane_f32(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Sum and maximum over the wave in a fixed order, the result in every lane:
+// each 16-lane row by DPP (quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror,
+// row_mirror), then the four rows' results. All 64 lanes must call.
+__device__ inline float wave_sum_dpp(float v) {
+    v += dpp_f32<0xb1>(v);
+    v += dpp_f32<0x4e>(v);
+    v += dpp_f32<0x141>(v);
+    v += dpp_f32<0x140>(v);
+    return (lane_f32(v, 0) + lane_f32(v, 16)) + (lane_f32(v, 32) + lane_f32(v, 48));
+}
+
+__device__ inline float wave_max_dpp(float v) {
+    v = fmaxf(v, dpp_f32<0xb1>(v));
+    v = fmaxf(v, dpp_f32<0x4e>(v));
+    v = fmaxf(v, dpp_f32<0x141>(v));
+    v = fmaxf(v, dpp_f32<0x140>(v));
+    return fmaxf(fmaxf(lane_f32(v, 0), lane_f32(v, 16)), fmaxf(lane_f32(v, 32), lane_f32(v, 48)));
+}
+
 // The FP32 key of the other matchers for B = 8: rows, then columns, one fmaf
 // per pixel.
 __device__ inline uint32_t mm_exact(const float* plane, size_t stride, int rx, int ry, int x, int y) {
@@ -1780,15 +1810,21 @@ __device__ inline int mfma_slot(int coef, int patch) { return coef * 20 + (patch
 // Forward 3-D transform of a unit. x[q] = lane's pixels of tile q in the A
 // layout, already scaled. Out: raw Haar products out[t][j] = coefficient
 // 16 t + (lane & 15) of Haar row 4 (lane >> 4) + j (not yet normalised).
+// fp16: dlo holds the DCT matrix's fp16 rounding residual, so the matrix
+// enters as hi + lo and only the data operand is rounded.
 template <int FORM>
-__device__ inline void forward_unit(const f32x4 (&x)[4], const Operand<FORM>& dct, const Operand<FORM>& haar,
-                                    float* T, int lane, f32x4 (&out)[4]) {
+__device__ inline void forward_unit(const f32x4 (&x)[4], const Operand<FORM>& dct, const Operand<FORM>& dlo,
+                                    const Operand<FORM>& haar, float* T, int lane, f32x4 (&out)[4]) {
     const f32x4 z0 = {0.f, 0.f, 0.f, 0.f};
     const int h = (lane >> 4) & 1, kx = lane & 7, pc = (lane >> 3) & 1, pr = lane >> 5;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const f32x4 y = mma<FORM>(operand<FORM>(x[q]), dct, z0);  // Q bd(D^T)
-        const f32x4 z = mma<FORM>(dct, operand<FORM>(y), z0);     // bd(D) Q bd(D^T)
+        const Operand<FORM> xq = operand<FORM>(x[q]);
+        f32x4 y = mma<FORM>(xq, dct, z0);  // Q bd(D^T)
+        if constexpr (FORM == kFormF16) y = mma<FORM>(xq, dlo, y);
+        const Operand<FORM> yq = operand<FORM>(y);
+        f32x4 z = mma<FORM>(dct, yq, z0);  // bd(D) Q bd(D^T)
+        if constexpr (FORM == kFormF16) z = mma<FORM>(dlo, yq, z);
 #pragma unroll
         for (int j = 0; j < 4; ++j) T[mfma_slot(8 * (4 * h + j) + kx, 4 * q + 2 * pr + pc)] = z[j];
     }
@@ -1806,8 +1842,8 @@ __device__ inline void forward_unit(const f32x4 (&x)[4], const Operand<FORM>& dc
 // Out: xo[q][j] = pixel (y = 4 ((lane >> 4) & 1) + j, x = lane & 7) of patch
 // 4 q + 2 (lane >> 5) + ((lane >> 3) & 1).
 template <int FORM>
-__device__ inline void inverse_unit(const f32x4 (&c)[4], const Operand<FORM>& dctt, const Operand<FORM>& haart,
-                                    float* T, int lane, f32x4 (&xo)[4]) {
+__device__ inline void inverse_unit(const f32x4 (&c)[4], const Operand<FORM>& dctt, const Operand<FORM>& dtlo,
+                                    const Operand<FORM>& haart, float* T, int lane, f32x4 (&xo)[4]) {
     const f32x4 z0 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -1821,8 +1857,12 @@ __device__ inline void inverse_unit(const f32x4 (&c)[4], const Operand<FORM>& dc
         f32x4 z;
 #pragma unroll
         for (int j = 0; j < 4; ++j) z[j] = T[mfma_slot(8 * ky + kx0 + j, 4 * q + 2 * pr + pc)];
-        const f32x4 v = mma<FORM>(operand<FORM>(z), dctt, z0);  // Z bd(D)
-        xo[q] = mma<FORM>(dctt, operand<FORM>(v), z0);          // bd(D^T) Z bd(D)
+        const Operand<FORM> zq = operand<FORM>(z);
+        f32x4 v = mma<FORM>(zq, dctt, z0);  // Z bd(D)
+        if constexpr (FORM == kFormF16) v = mma<FORM>(zq, dtlo, v);
+        const Operand<FORM> vq = operand<FORM>(v);
+        xo[q] = mma<FORM>(dctt, vq, z0);  // bd(D^T) Z bd(D)
+        if constexpr (FORM == kFormF16) xo[q] = mma<FORM>(dtlo, vq, xo[q]);
     }
     wave_sync();
 }
@@ -1857,25 +1897,102 @@ __device__ inline float group_scale(float m) {
     return ldexpf(1.f, -__builtin_amdgcn_frexp_expf(m));
 }
 
-// Per-group scales of a unit's pixels (fp16 form); s[0] covers tiles 0-1 and
-// s[1] tiles 2-3 (the two groups of 8), equal for a group of 16.
-template <int G, int FORM>
-__device__ inline void unit_scales(f32x4 (&x)[4], float (&s)[2]) {
-    if constexpr (FORM != kFormF16) {
-        s[0] = s[1] = 1.f;
-        return;
-    } else {
-        float m[2] = {0.f, 0.f};
+// The fp16 rounding residual of four values (the lo half of a split operand).
+__device__ inline f32x4 f16_residual(f32x4 v) {
+    const h16x4 h = __builtin_convertvector(v, h16x4);
+    return v - __builtin_convertvector(h, f32x4);
+}
+
+// sum over g < k of the unnormalised Haar butterfly entry (j, g): the Haar
+// coefficient of a constant over the k real patches of a group, before the
+// orthonormal factor (row 0 sums k; a difference row j with lowest set bit h
+// the part of [j - h, j) below k minus the part of [j, j + h) below k).
+__device__ inline int haar_row_count(int j, int k) {
+    if (j == 0) return k;
+    const int h = j & -j;
+    const int plus = min(max(min(j, k) - (j - h), 0), h);
+    const int minus = min(max(min(j + h, k) - j, 0), h);
+    return plus - minus;
+}
+
+// MFMA forms: each group's pixels are centred on their mean mu over the
+// group's real patches (padding patches stay zero) and scaled by a power of
+// two taking max |x - mu| to [0.5, 1) (fp16's range; exact, undone in FP32),
+// so the low-precision operands carry the texture and the noise, not the
+// group's level; the level's contribution (2-D DC coefficients only) is added
+// back in FP32. m = max |x - mu| (unscaled). Groups: q >> 1 for two groups
+// of 8, one group of 16.
+template <int G>
+__device__ inline void unit_centre(f32x4 (&x)[4], const int* cnt, int ngroups, int unit, int lane, float (&mu)[2],
+                                   float (&s)[2], float (&m)[2]) {
+    const int pr = (lane >> 3) & 1, pc = lane >> 5;
+    bool real[4];
+    float sum[2] = {0.f, 0.f};
+    int k[2] = {0, 0};
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) m[q >> 1] = fmaxf(m[q >> 1], fabsf(x[q][j]));
-        if constexpr (G == 16) m[0] = m[1] = fmaxf(m[0], m[1]);
-        s[0] = group_scale(wave_max_f32(m[0]));
-        s[1] = group_scale(wave_max_f32(m[1]));
-#pragma unroll
-        for (int q = 0; q < 4; ++q) x[q] *= s[q >> 1];
+    for (int q = 0; q < 4; ++q) {
+        const int p = 4 * q + 2 * pr + pc;
+        const int g = G == 8 ? 2 * unit + (p >> 3) : unit, idx = p & (G - 1);
+        real[q] = g < ngroups && idx < cnt[g];
+        if (real[q]) sum[G == 8 ? q >> 1 : 0] += (x[q][0] + x[q][1]) + (x[q][2] + x[q][3]);
     }
+#pragma unroll
+    for (int gs = 0; gs < (G == 8 ? 2 : 1); ++gs) {
+        const int g = G == 8 ? 2 * unit + gs : unit;
+        k[gs] = g < ngroups ? cnt[g] : 0;
+        const float tot = wave_sum_dpp(sum[gs]);
+        mu[gs] = k[gs] > 0 ? tot / static_cast<float>(64 * k[gs]) : 0.f;
+    }
+    if constexpr (G == 16) mu[1] = mu[0];
+    float mx[2] = {0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int gs = G == 8 ? q >> 1 : 0;
+        if (real[q]) x[q] -= mu[gs];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mx[gs] = fmaxf(mx[gs], fabsf(x[q][j]));
+    }
+    m[0] = wave_max_dpp(mx[0]);
+    m[1] = G == 8 ? wave_max_dpp(mx[1]) : m[0];
+    s[0] = group_scale(m[0]);
+    s[1] = group_scale(m[1]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] *= s[G == 8 ? q >> 1 : 0];
+}
+
+// FP32 value, in data units, of coefficient ci = 8 u + v (u along y) of Haar
+// row jr of unit group gs, from the lane-held centred and scaled pixels xc
+// (A layout) and the DCT table d (LDS): sum over the group's patches p of
+// haar(jr, p) D[u][y] D[v][x] xc_p[y][x], times the Haar factor and 1 / s,
+// plus the centre's share corr. Every argument but xc, d and lane is
+// wave-uniform; every lane returns the value.
+template <int G>
+__device__ inline float exact_coef(const f32x4 (&xc)[4], const float* d, int ci, int jr, int gs, float inv_s,
+                                   float corr, int lane) {
+    const int u = ci >> 3, v = ci & 7;
+    const float dy = d[u * 8 + (lane & 7)];
+    const f32x4 dx = *reinterpret_cast<const f32x4*>(d + v * 8 + 4 * ((lane >> 4) & 1));
+    float acc = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (G == 8 && (q >> 1) != gs) continue;
+        const int p = 4 * q + 2 * ((lane >> 3) & 1) + (lane >> 5);
+        const float r = fmaf(dx[3], xc[q][3], fmaf(dx[2], xc[q][2], fmaf(dx[1], xc[q][1], dx[0] * xc[q][0])));
+        acc = fmaf(haar_entry(jr, p & (G - 1)) * dy, r, acc);
+    }
+    return fmaf(wave_sum_dpp(acc) * haar_scale(G, jr), inv_s, corr);
+}
+
+// A hard-threshold decision of an MFMA form is re-made in FP32 when the
+// coefficient lies within band * max |x - mu| of the threshold. fp16: the
+// measured error of a coefficient stays below 5.2 * 2^-11 max |x - mu|
+// over two million coefficients of sigma-20 groups (the data rounding, 2^-12
+// relative, carried through the orthonormal transforms; the DCT matrix
+// enters split hi + lo), band 6 * 2^-11. bf16x3: each operand is hi + lo
+// with a residual near 2^-17 relative, band 8 * 2^-16.
+template <int FORM>
+__device__ constexpr float redecide_band() {
+    return FORM == kFormF16 ? 6.f * (1.f / 2048.f) : 8.f * (1.f / 65536.f);
 }
 
 template <int G, bool WIENER, int FORM>
@@ -1918,10 +2035,16 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     }
     __syncthreads();
 
-    const Operand<FORM> dct = operand<FORM>(dct_bd_entries(false, lane));
-    const Operand<FORM> dctt = operand<FORM>(dct_bd_entries(true, lane));
+    const f32x4 de = dct_bd_entries(false, lane), dte = dct_bd_entries(true, lane);
+    const Operand<FORM> dct = operand<FORM>(de);
+    const Operand<FORM> dctt = operand<FORM>(dte);
+    const Operand<FORM> dlo = operand<FORM>(FORM == kFormF16 ? f16_residual(de) : de);
+    const Operand<FORM> dtlo = operand<FORM>(FORM == kFormF16 ? f16_residual(dte) : dte);
     const Operand<FORM> haar = operand<FORM>(haar_unit_entries<G>(false, lane));
     const Operand<FORM> haart = operand<FORM>(haar_unit_entries<G>(true, lane));
+    // The DCT table for the FP32 re-decisions of the hard threshold.
+    float* dtab = scratch + 4 * kMfmaBuf;
+    if (!WIENER && threadIdx.x < 64) dtab[threadIdx.x] = c_dct[dct_offset<8>() + threadIdx.x];
     const float thr = 2.7f * jb.sigma;  // include/nss/params.hpp:46 kBmHardLambda
     const float sig2 = jb.sigma * jb.sigma;
     // Haar rows of this lane in the coefficient layout: 4 (lane >> 4) + j; for
@@ -1930,6 +2053,7 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     float hs[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) hs[j] = haar_scale(G, (4 * (lane >> 4) + j) & (G - 1));
+    if (!WIENER) __syncthreads();
 
     f32x4 x[4], xr[4];
     int u = wave;
@@ -1938,31 +2062,56 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
         if constexpr (WIENER) load_unit<G>(jb.wref, jb.wref_stride, recs, cnt, ngroups, u, lane, xr);
     }
     for (; u < nunits; u += 4) {
-        float s[2], sr[2];
-        unit_scales<G, FORM>(x, s);
+        float s[2], sr[2], mu[2], mur[2], mx[2], mxr[2];
+        unit_centre<G>(x, cnt, ngroups, u, lane, mu, s, mx);
         f32x4 c[4], r[4];
-        forward_unit<FORM>(x, dct, haar, T, lane, c);
+        forward_unit<FORM>(x, dct, dlo, haar, T, lane, c);
         if constexpr (WIENER) {
-            unit_scales<G, FORM>(xr, sr);
-            forward_unit<FORM>(xr, dct, haar, T, lane, r);
+            unit_centre<G>(xr, cnt, ngroups, u, lane, mur, sr, mxr);
+            forward_unit<FORM>(xr, dct, dlo, haar, T, lane, r);
+        }
+        // Hard threshold: the centred, scaled pixels, kept for the FP32
+        // re-decisions after the next unit's loads replace x.
+        f32x4 xc[4];
+        if constexpr (!WIENER) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xc[q] = x[q];
         }
         const int un = u + 4;
         if (un < nunits) {
             load_unit<G>(jb.src, jb.stride, recs, cnt, ngroups, un, lane, x);
             if constexpr (WIENER) load_unit<G>(jb.wref, jb.wref_stride, recs, cnt, ngroups, un, lane, xr);
         }
+        // The centres' share of each 2-D DC coefficient (coefficient 0 of
+        // every Haar row; zero on the other lanes).
+        float corr[4] = {0.f, 0.f, 0.f, 0.f}, corr_r[4] = {0.f, 0.f, 0.f, 0.f};
+        {
+            if ((lane & 15) == 0) {
+                const int g = G == 8 ? 2 * u + hgroup : u;
+                const int k = g < ngroups ? cnt[g] : 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float e = hs[j] * static_cast<float>(haar_row_count((4 * (lane >> 4) + j) & (G - 1), k)) * 8.f;
+                    corr[j] = e * mu[hgroup];
+                    corr_r[j] = e * mur[hgroup];
+                }
+            }
+        }
         // Shrink in FP32 on the orthonormal coefficients of the unscaled data.
         const float inv_s = 1.f / s[hgroup];
-        int kept = 0;
+        float vv[4][4];
+        bool band[4][4];
         float w2 = 0.f;
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const bool dc = t == 0 && (lane & 15) == 0 && ((4 * (lane >> 4) + j) & (G - 1)) == 0;
-                float v = c[t][j] * hs[j] * inv_s;
+                const float cj = t == 0 ? corr[j] : 0.f;
+                float v = fmaf(c[t][j] * hs[j], inv_s, cj);
+                band[t][j] = false;
                 if constexpr (WIENER) {
-                    const float rv = r[t][j] * hs[j] * (1.f / sr[hgroup]);
+                    const float rv = fmaf(r[t][j] * hs[j], 1.f / sr[hgroup], t == 0 ? corr_r[j] : 0.f);
                     float w = 1.f;
                     if (!dc) {
                         const float r2 = rv * rv;
@@ -1972,14 +2121,51 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
                     v *= w;
                     w2 = fmaf(w, w, w2);
                 } else {
+                    band[t][j] = !dc && fabsf(fabsf(v) - thr) <= redecide_band<FORM>() * mx[hgroup];
+                }
+                vv[t][j] = v;
+            }
+        bool any_band = false;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) any_band |= band[t][j];
+        if (!WIENER && __ballot(any_band)) {
+            // Coefficients whose MFMA value cannot decide the threshold: their
+            // FP32 value, one coefficient at a time by the whole wave.
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    unsigned long long pend = __ballot(band[t][j]);
+                    while (pend) {
+                        const int L = __builtin_ctzll(pend);
+                        pend &= pend - 1;
+                        const int gs = G == 8 ? L >> 5 : 0;
+                        const float cl = t == 0 ? __shfl(corr[j], L, 64) : 0.f;
+                        const float e = exact_coef<G>(xc, dtab, 16 * t + (L & 15), (4 * (L >> 4) + j) & (G - 1), gs,
+                                                      1.f / s[gs], cl, lane);
+                        if (lane == L) vv[t][j] = e;
+                    }
+                }
+        }
+        int kept = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float v = vv[t][j];
+                if constexpr (!WIENER) {
+                    const bool dc = t == 0 && (lane & 15) == 0 && ((4 * (lane >> 4) + j) & (G - 1)) == 0;
                     if (!dc && fabsf(v) < thr)
                         v = 0.f;
                     else
                         ++kept;
                 }
-                // back to the operand scale, butterfly factor applied again
-                // (the inverse orthonormal Haar is H^T diag(factors))
-                c[t][j] = v * hs[j] * s[hgroup];
+                // back to the operand scale without the centre's share,
+                // butterfly factor applied again (the inverse orthonormal
+                // Haar is H^T diag(factors))
+                c[t][j] = (v - (t == 0 ? corr[j] : 0.f)) * hs[j] * s[hgroup];
             }
         float wg[2];
         if constexpr (WIENER) {
@@ -1998,7 +2184,7 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
             wg[1] = 1.f / static_cast<float>(max(__shfl(sum, 32, 64), 1));
         }
         f32x4 xo[4];
-        inverse_unit<FORM>(c, dctt, haart, T, lane, xo);
+        inverse_unit<FORM>(c, dctt, dtlo, haart, T, lane, xo);
         const int pr = lane >> 5, pc = (lane >> 3) & 1, px = lane & 7, y0 = 4 * ((lane >> 4) & 1);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -2010,9 +2196,10 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
                 const int lx = static_cast<int>(xy & 0xffffu) - ax0, ly = static_cast<int>(xy >> 16) - ay0;
                 const float w = wg[gsel];
                 const float wq = w / s[gsel];
+                const float wm = w * mu[gsel];  // the centre, added back
                 unsigned long long* cell = num + (ly + y0) * aw + lx + px;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) lds_add(cell + j * aw, to_fixed(wq * xo[q][j]));
+                for (int j = 0; j < 4; ++j) lds_add(cell + j * aw, to_fixed(fmaf(wq, xo[q][j], wm)));
                 if (px == 0 && y0 == 0) lds_add(corner + ly * cw + lx, to_fixed(w));
             }
         }
@@ -3314,7 +3501,7 @@ size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t) {
     const size_t acc = ((aw * ah + (aw - 7) * (ah - 7) + 1) & ~static_cast<size_t>(1)) * 8;
     const size_t recs = ((static_cast<size_t>(t.tw) * t.th * group + 3) & ~static_cast<size_t>(3)) * 4 +
                         ((static_cast<size_t>(t.tw) * t.th + 3) & ~static_cast<size_t>(3)) * 4;
-    const size_t tbuf = 4 * static_cast<size_t>(kMfmaBuf) * 4;
+    const size_t tbuf = 4 * static_cast<size_t>(kMfmaBuf) * 4 + 64 * 4;  // + the fp16 form's DCT table
     return acc + recs + tbuf;  // the den box sum reuses num's area
 }
 
