@@ -86,6 +86,20 @@ def test_temporal_shapes_and_ref(gpu, oracle):
     print("temporal shapes ok", flush=True)
 
 
+def test_temporal_global_cubes(gpu, oracle):
+    """Block 32 with group 32 in both stages (the Wiener stage's two cubes
+    exceed the LDS and live in global memory) and block 32 with group 64."""
+    clip = _moving_clip("GRAYS", 80, 72, 4, 20, seed=5)
+    for kw in (dict(block_size=32, block_step=16, group_size=32, bm_range=4, final=1),
+               dict(block_size=32, block_step=16, group_size=64, bm_range=6)):
+        kw = dict(kw, sigma=20, radius=1)
+        got = _gpu_clip(gpu, "GRAYS", clip, **kw)
+        want = _oracle_clip(oracle, gpu, "GRAYS", clip, **kw)
+        print(f"temporal {kw}: min PSNR vs oracle "
+              f"{min(oracle.psnr(g, w_) for gf, wf in zip(got, want) for g, w_ in zip(gf, wf)):.2f} dB", flush=True)
+        _check(oracle, got, want, str(kw))
+
+
 def test_one_frame_at_max_radius_equals_spatial(gpu):
     """tests/test_bm3d_semantics.py:125-135 on the GPU."""
     rng = np.random.default_rng(73)

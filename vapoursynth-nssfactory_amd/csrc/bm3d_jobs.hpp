@@ -91,9 +91,16 @@ struct FilterJob {
     unsigned long long* part_den;
 };
 
+// Groups whose cube does not fit the LDS (block 32 with group 64, or group
+// 32 with the Wiener reference) keep it in global memory: kCubeSlots
+// workgroups each walk the groups with a cube of cube_floats of its own.
+inline constexpr int kCubeSlots = 512;
+
 struct FilterTable {
     int n;
     FilterJob job[kMaxJobs];
+    float* cube;      // null: one group per workgroup, the cube in LDS
+    int cube_floats;  // floats per workgroup's cube
 };
 
 struct FinishJob {
@@ -196,6 +203,8 @@ struct TFilterJob {
 struct TFilterTable {
     int n;
     TFilterJob job[kMaxJobs];
+    float* cube;  // as FilterTable::cube
+    int cube_floats;
 };
 
 // Temporal tile path (block 8, radius <= 2): every slot's contributions of a

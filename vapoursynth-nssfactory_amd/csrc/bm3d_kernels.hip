@@ -313,17 +313,24 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     // of the lanes' heads, and only the winning lane advances (rebuilding its
     // four from LDS, where given-away keys are marked taken, after its fourth).
     unsigned long long best4[4];
+    auto insert = [&](uint32_t d, int c) {
+        const unsigned long long x = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
+        if (d != kTaken && x < best4[3]) {
+            best4[3] = x;
+            if (best4[3] < best4[2]) { const unsigned long long t = best4[2]; best4[2] = best4[3]; best4[3] = t; }
+            if (best4[2] < best4[1]) { const unsigned long long t = best4[1]; best4[1] = best4[2]; best4[2] = t; }
+            if (best4[1] < best4[0]) { const unsigned long long t = best4[0]; best4[0] = best4[1]; best4[1] = t; }
+        }
+    };
     auto rebuild = [&]() {
         best4[0] = best4[1] = best4[2] = best4[3] = ~0ull;
-        for (int c = hl; c < ncand; c += 32) {
-            const uint32_t d = dk[c];
-            const unsigned long long x = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
-            if (d != kTaken && x < best4[3]) {
-                best4[3] = x;
-                if (best4[3] < best4[2]) { const unsigned long long t = best4[2]; best4[2] = best4[3]; best4[3] = t; }
-                if (best4[2] < best4[1]) { const unsigned long long t = best4[1]; best4[1] = best4[2]; best4[2] = t; }
-                if (best4[1] < best4[0]) { const unsigned long long t = best4[0]; best4[0] = best4[1]; best4[1] = t; }
-            }
+        // four keys read before any is inserted (their LDS loads overlap)
+        for (int c0 = hl; c0 < ncand; c0 += 128) {
+            uint32_t d[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) d[u] = c0 + 32 * u < ncand ? dk[c0 + 32 * u] : kTaken;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) insert(d[u], c0 + 32 * u);
         }
     };
     if (K > 0) {
@@ -336,14 +343,33 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     int given = 0;
     unsigned long long won = ~0ull;
     for (int k = 0; k < kmax; ++k) {
-        const unsigned long long best = half_min_key(k < K ? best4[0] : ~0ull);
-        const int c = static_cast<int>(static_cast<uint32_t>(best));
-        if (hl == k) won = best;
-        if (k < K && k >= 32 && hl == 0) {
-            const int dy = div_small(c, inv_ww), dx = c - dy * ww;
-            out[1 + k] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(best >> 32)};
+        const bool live = k < K;
+        const unsigned long long head = live ? best4[0] : ~0ull;
+        const uint32_t hd = static_cast<uint32_t>(head >> 32);
+        const uint32_t md = half_min_u32(hd);
+        // The lanes holding the smallest distance: one per half (the usual
+        // case) gives the winner directly; a tie between lanes is broken by
+        // the smallest window index, as match_tie_less does.
+        const unsigned long long at_min = __ballot(live && hd == md);
+        const uint32_t lo_mask = static_cast<uint32_t>(at_min), hi_mask = static_cast<uint32_t>(at_min >> 32);
+        uint32_t mc;
+        if (__popc(lo_mask) <= 1 && __popc(hi_mask) <= 1) {
+            const int hc = static_cast<int>(static_cast<uint32_t>(head));
+            const uint32_t c0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(hc, lo_mask ? __builtin_ctz(lo_mask) : 0));
+            const uint32_t c1 =
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(hc, hi_mask ? 32 + __builtin_ctz(hi_mask) : 32));
+            mc = (threadIdx.x & 32) ? c1 : c0;
+        } else {
+            mc = half_min_u32(hd == md ? static_cast<uint32_t>(head) : 0xffffffffu);
         }
-        if (k < K && hl == (c & 31)) {
+        const unsigned long long best = (static_cast<unsigned long long>(md) << 32) | mc;
+        const int c = static_cast<int>(mc);
+        if (hl == k) won = best;
+        if (live && k >= 32 && hl == 0) {
+            const int dy = div_small(c, inv_ww), dx = c - dy * ww;
+            out[1 + k] = MatchRec{left + dx, top + dy, jb.t0, md};
+        }
+        if (live && hl == (c & 31)) {
             dk[c] = kTaken;
             best4[0] = best4[1];
             best4[1] = best4[2];
@@ -2208,17 +2234,12 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     tile_partials<B>(jb, tile, ax0, num, corner);
 }
 
-// Atomic path: one wave per group, frame-sized fixed-point sums.
+// Atomic path: one wave per group, frame-sized fixed-point sums (den at the
+// patch corner).
 template <int B, bool WIENER>
-__global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
-    extern __shared__ __align__(16) float cube[];
+__device__ inline void atomic_group(const FilterJob& jb, int g, float* cube, int lane) {
     constexpr int A = B * B;
     constexpr int PS = A + 1;
-    const int lane = threadIdx.x;
-    const int lb = xcd_block();
-    const FilterJob& jb = tab.job[find_job(tab, lb)];
-    const int g = lb - jb.wg_begin;
-    if (g >= jb.nref) return;
     const int G = jb.group;
     const int k = jb.counts[g];
     const MatchRec* rec = jb.matches + static_cast<size_t>(g) * G;
@@ -2233,6 +2254,31 @@ __global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
         }
         if (lane == 0) atomicAdd(jb.den + static_cast<size_t>(my) * jb.width + mx, wfix);
     }
+}
+
+template <int B, bool WIENER>
+__global__ __launch_bounds__(64) void filter_atomic_kernel(FilterTable tab) {
+    extern __shared__ __align__(16) float lds_cube[];
+    const int lane = threadIdx.x;
+    if (tab.cube) {
+        // Cubes beyond the LDS: workgroup b takes groups b, b + gridDim.x, ...
+        // with its own cube in global memory (the same arithmetic; a wave's
+        // global stores and loads stay in order on its CU).
+        const FilterJob& last = tab.job[tab.n - 1];
+        const int total = last.wg_begin + last.nref;
+        float* cube = tab.cube + static_cast<size_t>(blockIdx.x) * tab.cube_floats;
+        for (int gi = blockIdx.x; gi < total; gi += gridDim.x) {
+            const FilterJob& jb = tab.job[find_job(tab, gi)];
+            atomic_group<B, WIENER>(jb, gi - jb.wg_begin, cube, lane);
+            wave_sync();
+        }
+        return;
+    }
+    const int lb = xcd_block();
+    const FilterJob& jb = tab.job[find_job(tab, lb)];
+    const int g = lb - jb.wg_begin;
+    if (g >= jb.nref) return;
+    atomic_group<B, WIENER>(jb, g, lds_cube, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -2482,15 +2528,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void pr
 // One wave per group: patches from their slots' frames, sums into the
 // accumulators of those frames.
 template <int B, bool WIENER>
-__global__ __launch_bounds__(64) void filter_temporal_kernel(TFilterTable tab) {
-    extern __shared__ __align__(16) float cube[];
+__device__ inline void temporal_group(const TFilterJob& jb, int g, float* cube, int lane) {
     constexpr int A = B * B;
     constexpr int PS = A + 1;
-    const int lane = threadIdx.x;
-    const int lb = xcd_block();
-    const TFilterJob& jb = tab.job[find_job(tab, lb)];
-    const int g = lb - jb.wg_begin;
-    if (g >= jb.nref) return;
     const int G = jb.group;
     const int k = jb.counts[g];
     const MatchRec* rec = jb.matches + static_cast<size_t>(g) * G;
@@ -2511,6 +2551,29 @@ __global__ __launch_bounds__(64) void filter_temporal_kernel(TFilterTable tab) {
         }
         if (lane == 0) atomicAdd(jb.den + slot + static_cast<size_t>(my) * jb.width + mx, wfix);
     }
+}
+
+template <int B, bool WIENER>
+__global__ __launch_bounds__(64) void filter_temporal_kernel(TFilterTable tab) {
+    extern __shared__ __align__(16) float lds_cube[];
+    const int lane = threadIdx.x;
+    if (tab.cube) {
+        // Cubes beyond the LDS, as in filter_atomic_kernel.
+        const TFilterJob& last = tab.job[tab.n - 1];
+        const int total = last.wg_begin + last.nref;
+        float* cube = tab.cube + static_cast<size_t>(blockIdx.x) * tab.cube_floats;
+        for (int gi = blockIdx.x; gi < total; gi += gridDim.x) {
+            const TFilterJob& jb = tab.job[find_job(tab, gi)];
+            temporal_group<B, WIENER>(jb, gi - jb.wg_begin, cube, lane);
+            wave_sync();
+        }
+        return;
+    }
+    const int lb = xcd_block();
+    const TFilterJob& jb = tab.job[find_job(tab, lb)];
+    const int g = lb - jb.wg_begin;
+    if (g >= jb.nref) return;
+    temporal_group<B, WIENER>(jb, g, lds_cube, lane);
 }
 
 // Block 8, G in {8, 16, 32}: four groups per workgroup, a wave per group with
@@ -3608,12 +3671,17 @@ static hipError_t launch_filter_b(const FilterTable& t, int total_wg, size_t lds
             allow_lds(&filter_tile_kernel<B, false>, lds);
             filter_tile_kernel<B, false><<<total_wg, 256, lds, s>>>(t);
         }
-    } else if (wiener) {
-        allow_lds(&filter_atomic_kernel<B, true>, lds);
-        filter_atomic_kernel<B, true><<<total_wg, 64, lds, s>>>(t);
     } else {
-        allow_lds(&filter_atomic_kernel<B, false>, lds);
-        filter_atomic_kernel<B, false><<<total_wg, 64, lds, s>>>(t);
+        // a table with global cubes runs kCubeSlots workgroups (no LDS cube)
+        const int grid = t.cube ? std::min(total_wg, kCubeSlots) : total_wg;
+        if (t.cube) lds = 0;
+        if (wiener) {
+            allow_lds(&filter_atomic_kernel<B, true>, lds);
+            filter_atomic_kernel<B, true><<<grid, 64, lds, s>>>(t);
+        } else {
+            allow_lds(&filter_atomic_kernel<B, false>, lds);
+            filter_atomic_kernel<B, false><<<grid, 64, lds, s>>>(t);
+        }
     }
     return hipGetLastError();
 }
@@ -3747,12 +3815,14 @@ hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t
 
 template <int B>
 static hipError_t launch_tfilter_b(const TFilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
+    const int grid = t.cube ? std::min(total_wg, kCubeSlots) : total_wg;
+    if (t.cube) lds = 0;
     if (wiener) {
         allow_lds(&filter_temporal_kernel<B, true>, lds);
-        filter_temporal_kernel<B, true><<<total_wg, 64, lds, s>>>(t);
+        filter_temporal_kernel<B, true><<<grid, 64, lds, s>>>(t);
     } else {
         allow_lds(&filter_temporal_kernel<B, false>, lds);
-        filter_temporal_kernel<B, false><<<total_wg, 64, lds, s>>>(t);
+        filter_temporal_kernel<B, false><<<grid, 64, lds, s>>>(t);
     }
     return hipGetLastError();
 }

@@ -225,12 +225,20 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
             if (placed || (placed = apply_tile_shape(h, st, bk, sh[0], sh[1]))) break;
         if (!bk.tiles) {
             bk.fast_group = bk.form = 0;
-            for (int k = 0; k < bk.nplanes; ++k)
+            int gmax = 0;
+            for (int k = 0; k < bk.nplanes; ++k) {
+                gmax = std::max(gmax, st.plane[bk.planes[k]].group);
                 bk.filter_lds =
                     std::max(bk.filter_lds, filter_atomic_lds_bytes(b, st.plane[bk.planes[k]].group, st.wiener));
-            if (bk.filter_lds > kLdsLimit)
-                return fail(NSS_HIP_UNSUPPORTED, "nss_hip: block_size " + std::to_string(b) +
-                                                     " with this group_size exceeds the on-chip group cube");
+            }
+            // A cube beyond the LDS (block 32 with group 64, or group 32 with
+            // the Wiener reference) lives in global memory, as the
+            // reference's device path keeps it (src/cuda/bm3d/kernels.cu:248-343).
+            if (bk.filter_lds > kLdsLimit) {
+                bk.filter_lds = 0;
+                bk.cube_floats = ((st.wiener ? 2 : 1) * gmax * (b * b + 1) + 63) & ~63;
+                h->cube_floats = std::max(h->cube_floats, static_cast<size_t>(bk.cube_floats));
+            }
         }
         st.bucket[st.nbuckets++] = bk;
     }
@@ -304,6 +312,10 @@ PlaneNeeds plane_needs(const nss_hip_bm3d* h, int i) {
 
 namespace nsship {
 
+// The global group cubes of one frame slot (every bucket's launch uses at
+// most kCubeSlots cubes of the instance's largest size).
+size_t cube_bytes(const nss_hip_bm3d* h) { return static_cast<size_t>(kCubeSlots) * h->cube_floats * sizeof(float); }
+
 int frame_slot_count(const nss_hip_bm3d* h) { return h->p.radius > 0 ? 1 : std::max(1, h->p.num_streams); }
 
 size_t frame_buffer_bytes(const nss_hip_bm3d* h) {
@@ -314,7 +326,7 @@ size_t frame_buffer_bytes(const nss_hip_bm3d* h) {
         total += n.recs * sizeof(MatchRec) + n.nref * sizeof(int) + 2 * h->part_elems[i] * sizeof(unsigned long long) +
                  (n.atomic ? 16 * npx : 0) + (n.basic ? 4 * npx : 0);
     }
-    return total;
+    return total + cube_bytes(h);
 }
 
 // Rolling state of plane i (rings and sums), from the plan.
@@ -347,6 +359,7 @@ void free_frame_buffers(FrameBuffers& fb) {
                         static_cast<void*>(fb.basic[i])})
             if (p) (void)hipFree(p);
     }
+    if (fb.cube) (void)hipFree(fb.cube);
     fb = FrameBuffers{};
 }
 
@@ -367,6 +380,7 @@ hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb) {
         if (e == hipSuccess && h->part_elems[i]) e = hipMalloc(&fb.part[i], 2 * h->part_elems[i] * sizeof(unsigned long long));
         if (e == hipSuccess && n.basic) e = hipMalloc(&fb.basic[i], npx * 4);
     }
+    if (e == hipSuccess && h->cube_floats) e = hipMalloc(&fb.cube, cube_bytes(h));
     if (e != hipSuccess) free_frame_buffers(fb);
     return e;
 }
@@ -394,6 +408,7 @@ size_t carve_frame_buffers(const nss_hip_bm3d* h, char* base, FrameBuffers& fb) 
             fb.part[i] = reinterpret_cast<unsigned long long*>(take(2 * h->part_elems[i] * sizeof(unsigned long long)));
         if (n.basic) fb.basic[i] = reinterpret_cast<float*>(take(npx * 4));
     }
+    if (h->cube_floats) fb.cube = reinterpret_cast<float*>(take(cube_bytes(h)));
     return off;
 }
 
@@ -466,9 +481,10 @@ StageIO stage_io(const FrameRef& f, int k) {
 // Jobs of one bucket shape gathered over the frames of a batch, launched as
 // one clear / match / filter / finish sequence.
 struct LaunchGroup {
-    int block = 0, match_kind = 0, fast_group = 0, form = 0;
+    int block = 0, match_kind = 0, fast_group = 0, form = 0, cube_floats = 0;
     bool tiles = false, wiener = false;
     size_t match_lds = 0, filter_lds = 0;
+    float* cube = nullptr;  // global group cubes (the first frame's), with cube_floats
     ZeroTable zt{};
     MatchTable mt{};
     FilterTable ft{};
@@ -477,7 +493,7 @@ struct LaunchGroup {
 
     bool fits(const Bucket& bk, bool w) const {
         return block == bk.block && match_kind == bk.match_kind && fast_group == bk.fast_group && form == bk.form &&
-               tiles == bk.tiles && wiener == w;
+               tiles == bk.tiles && wiener == w && cube_floats == bk.cube_floats;
     }
     bool empty() const { return ft.n == 0 && mt.n == 0; }
 };
@@ -495,6 +511,8 @@ int flush_group(LaunchGroup& g, Tracer* tr, hipStream_t s) {
         if (e != hipSuccess) return hip_fail(e, "match launch");
     }
     if (g.ft.n) {
+        g.ft.cube = g.cube_floats ? g.cube : nullptr;
+        g.ft.cube_floats = g.cube_floats;
         e = traced(tr, 2, s, [&] {
             return launch_filter(b, g.ft, g.fwg, g.filter_lds, g.wiener, g.tiles, g.fast_group, g.form, s);
         });
@@ -509,6 +527,7 @@ int flush_group(LaunchGroup& g, Tracer* tr, hipStream_t s) {
     g.fn = fresh.fn;
     g.zwg = g.mwg = g.fwg = g.nwg = 0;
     g.match_lds = g.filter_lds = 0;
+    g.cube = nullptr;
     return NSS_HIP_OK;
 }
 
@@ -519,6 +538,7 @@ void add_bucket_jobs(LaunchGroup& g, const FrameRef& f, const Stage& st, const B
     const int b = bk.block;
     g.match_lds = std::max(g.match_lds, bk.match_lds);
     g.filter_lds = std::max(g.filter_lds, bk.filter_lds);
+    if (bk.cube_floats && !g.cube) g.cube = f.buf.cube;
     // CBM3D (chroma=1): the groups are matched once, on plane 0 of the match
     // planes, and every filtered plane uses them (src/host/filter_bm3d.cpp:451-486).
     int mplanes[3], nmp = 0;
@@ -646,6 +666,7 @@ int run_batch(const FrameRef* frames, int nframes, Tracer* tr, hipStream_t s) {
                     g->form = bk.form;
                     g->tiles = bk.tiles;
                     g->wiener = st.wiener;
+                    g->cube_floats = bk.cube_floats;
                 }
                 const int nmp = h->p.chroma ? 1 : bk.nplanes;
                 if (g->mt.n + nmp > kMaxJobs || g->ft.n + bk.nplanes > kMaxJobs) {
@@ -859,6 +880,10 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
             fwg += fast ? (f.nref + 3) / 4 : f.nref;
             flds = std::max(flds, fast ? filter_temporal_b8_lds_bytes(sp.group)
                                        : filter_atomic_lds_bytes(b, sp.group, stg.wiener));
+        }
+        if (bk.cube_floats && !fast) {
+            ft.cube = h->slots[0].buf.cube;
+            ft.cube_floats = bk.cube_floats;
         }
         hipError_t e = traced(h, 1, st, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, st); });
         if (e != hipSuccess) return hip_fail(e, "match launch");

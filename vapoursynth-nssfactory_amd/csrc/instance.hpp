@@ -111,6 +111,7 @@ struct Bucket {
     size_t filter_lds = 0;
     int fast_group = 0;  // block 8 with one group size in {8, 16, 32}: the register kernel
     int form = 0;        // 1 bf16x3, 2 fp16: the MFMA kernel (block 8, groups of 8 or 16)
+    int cube_floats = 0;  // atomic path with the group cube in global memory (floats per cube); 0: LDS
 };
 
 struct Stage {
@@ -157,6 +158,7 @@ struct FrameBuffers {
     MatchRec* matches[3]{};
     int* counts[3]{};
     float* basic[3]{};
+    float* cube = nullptr;  // kCubeSlots global group cubes (the instance's cube_floats each)
 };
 
 // One of an instance's num_streams frame slots: a frame's buffers and the
@@ -181,6 +183,7 @@ struct nss_hip_bm3d {
     nsship::Stage stages[2]{};
     int nstages = 0;
     size_t part_elems[3]{};  // per half (num or den) of a plane's tile partials, int64 entries
+    size_t cube_floats = 0;  // largest global group cube of any bucket (0: every cube fits the LDS)
     // num_streams frame slots (radius 0; one for temporal instances). A
     // process call leases the slot free the longest for the time it takes to
     // queue the frame, sleeping while all are leased
