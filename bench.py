@@ -101,11 +101,11 @@ FORMS = {"fp32": 0, "bf16x3": 1, "fp16": 2}
 # WRITE_SIZE (KiB per dispatch, averaged over the basic and Wiener launches)
 # scaled by what the counters read for a 256 MiB elementwise copy in the same
 # process (FETCH x2, WRITE x1); profiles/bm3d_1080p_yuv420_two_stage_kernels.md.
-TRAFFIC_KIB = {("match", "fp32"): (6562, 6280), ("match", "bf16x3"): (6562, 6280), ("match", "fp16"): (6562, 6280),
-               ("filter", "fp32"): ((9649 + 16064) / 2, (71536 + 66940) / 2),
-               ("filter", "bf16x3"): ((9696 + 16149) / 2, 52599), ("filter", "fp16"): ((9695 + 16138) / 2, 52601),
-               ("finish", "fp32"): (28788, 12374), ("finish", "bf16x3"): (28788, 12374),
-               ("finish", "fp16"): (28788, 12374)}
+TRAFFIC_KIB = {("match", "fp32"): (6583, 6280), ("match", "bf16x3"): (6583, 6280), ("match", "fp16"): (6583, 6280),
+               ("filter", "fp32"): ((10014 + 16402) / 2, (143055 + 133875) / 2),
+               ("filter", "bf16x3"): ((10225 + 16654) / 2, 105188), ("filter", "fp16"): ((10149 + 16840) / 2, 105188),
+               ("finish", "fp32"): (57579, 13327), ("finish", "bf16x3"): (57579, 13327),
+               ("finish", "fp16"): (57579, 13327)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 FORM_DTYPE = {"fp32": "f32", "bf16x3": "bf16x3 (f32 accumulate)", "fp16": "f16 (f32 accumulate)"}
 PEAK_MFMA_TFLOPS = 2500.0  # dense bf16 / fp16
