@@ -139,7 +139,15 @@ int nss_hip_bm3d_output_info(const nss_hip_bm3d_params* params, const nss_hip_vi
 
 /* Filter instance for frames of one fixed format and size on one device
  * (the node creation of src/host/filter_bm3d.cpp:1143-1163). Device buffers
- * are sized here, from the parameters. */
+ * are sized here, from the parameters. When memory_limit_mb is below what
+ * the frame buffers of the num_streams slots need (radius 0), the instance
+ * batches the work to fit, as the reference does
+ * (docs/api/cuda/README.md:141, src/cuda/common/group_driver.cpp:481-520):
+ * every stage adds into frame-sized int64 sums, matching and filtering the
+ * reference grid in batches of rows (and running fewer groups at once where
+ * group cubes live in global memory); the output bits do not change. The
+ * MFMA transform forms have no batched plan. A limit too small for one batch
+ * is an INVALID_INPUT error with the reference's message. */
 int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device,
                         nss_hip_bm3d** out);
 
@@ -165,10 +173,14 @@ int nss_hip_bm3d_process(nss_hip_bm3d* handle, const float* const* src, const in
  * the resolved parameters and format instead of an instance and allocates
  * nothing: the frame's working buffers (match lists, tile partials, the basic
  * estimate) live in one fixed device array of 192 MiB per GPU, held from the
- * library's load until the process ends. A frame that needs more (a
- * 1920 x 1080 YUV420 two-stage frame at the default arguments needs about
- * 158 MiB; 3840 x 2160 does not fit) fails with NSS_HIP_UNSUPPORTED before
- * any work is queued. Radius 0 only. Every call on a GPU shares that GPU's
+ * library's load until the process ends. A frame whose buffers do not fit
+ * (a 1920 x 1080 YUV420 two-stage frame at the default arguments needs
+ * 158 MiB) takes the memory_limit_mb plan batched over grid rows, which
+ * needs about 20 bytes per pixel of a two-stage frame (a 3840 x 2160 gray
+ * frame fits, 3840 x 2160 RGB does not); a frame that fits neither way
+ * fails with NSS_HIP_UNSUPPORTED before any work is queued. A
+ * memory_limit_mb in the parameters lowers the cap further. Radius 0 only.
+ * The MFMA transform forms have no batched plan. Every call on a GPU shares that GPU's
  * array and keeps nothing between calls: calls that overlap on one GPU
  * (different streams, or threads without ordering) race. The first call on
  * a device also loads the DCT tables there (a constant-memory copy), so make

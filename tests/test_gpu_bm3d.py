@@ -259,8 +259,9 @@ def test_memory_limit_enforced_and_reported(gpu):
     assert r["_NSSResourcePeak"] == sum(r["_NSSResourceBytes"]) > (1 << 20)
     assert r["_NSSResourceBytes"][1:] == [0, 0, 0, 0, 0]
     need = (r["_NSSResourcePeak"] >> 20) + 1
+    # below one batch of grid rows (the frame-sized sums alone are 5.5 MB)
     with pytest.raises(gpu.Error) as e:
-        gpu.BM3D("YUV420PS", w, h, sigma=10, final=1, memory_limit_mb=need - 1)
+        gpu.BM3D("YUV420PS", w, h, sigma=10, final=1, memory_limit_mb=1)
     assert "memory_limit_mb is too small for this clip" in str(e.value)
     inst = gpu.BM3D("YUV420PS", w, h, sigma=10, final=1, memory_limit_mb=need)
     r2 = inst.resources()
@@ -286,3 +287,65 @@ def test_memory_limit_does_not_change_output(gpu):
     b = _run(gpu, "YUV420PS", noisy, sigma=10, final=1, memory_limit_mb=1)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+LIMIT_CASES = [
+    ("YUV420PS", 480, 272, dict(sigma=10, final=1)),                      # block-8 register buckets
+    ("YUV444PS", 320, 224, dict(sigma=10, final=1, chroma=1)),            # CBM3D: groups of plane 0
+    ("GRAYS", 400, 240, dict(sigma=20, block_size=4, group_size=16, block_step=2, bm_range=6)),  # atomic kernel
+    ("GRAYS", 320, 224, dict(sigma=20, final=1, group_size=32, group_size_basic=16, block_step=4)),
+    ("RGBS", 256, 192, dict(sigma=10, final=1, block_size=12, group_size=8, block_step=6)),
+    ("GRAYS", 288, 224, dict(sigma=20, final=1, block_size=32, group_size=32, block_step=16, bm_range=8)),  # global cubes
+]
+
+
+@pytest.mark.parametrize("case", range(len(LIMIT_CASES)))
+def test_memory_limit_batches_give_the_same_bits(gpu, case):
+    """docs/api/cuda/README.md:141: under a memory_limit_mb below what the
+    frame buffers need, the work is batched to fit and the output does not
+    change. The smallest limit that creates (found by bisection) batches the
+    grid rows the most (or, with group cubes in global memory, runs the
+    fewest cubes at once); its bits equal the unlimited instance's, its
+    buffers stay within the limit, and the matcher runs once per batch."""
+    import torch
+    import synth
+    fmt, w, h, kw = LIMIT_CASES[case]
+    _, noisy = synth.noisy_frame(fmt, w, h, kw["sigma"], seed=60 + case)
+    src = [_dev(p) for p in noisy]
+    free = gpu.BM3D(fmt, w, h, **kw)
+    want = [_host(o) for o in free(src)]
+    need = (free.resources()["_NSSResourcePeak"] >> 20) + 1
+    free.close()
+
+    def creates(mb):
+        try:
+            gpu.BM3D(fmt, w, h, memory_limit_mb=mb, **kw).close()
+            return True
+        except gpu.Error as e:
+            assert "memory_limit_mb is too small for this clip" in str(e)
+            return False
+    lo, hi = 1, need  # creates(hi); find the smallest limit that creates
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if creates(mid):
+            hi = mid
+        else:
+            lo = mid + 1
+    inst = gpu.BM3D(fmt, w, h, memory_limit_mb=lo, **kw)
+    r = inst.resources()
+    inst.set_tracing(True)
+    inst.kernel_times(reset=True)
+    got = [_host(o) for o in inst(src)]
+    torch.cuda.synchronize()
+    kt = inst.kernel_times(reset=True)
+    inst.close()
+    print(f"case {case}: unlimited {need} MiB, smallest limit {lo} MiB, owned {sum(r['_NSSResourceBytes']) >> 20} MiB, "
+          f"launches {kt}", flush=True)
+    assert lo < need and sum(r["_NSSResourceBytes"]) <= lo << 20
+    nstages = 2 if kw.get("final") else 1
+    if kw.get("block_size", 8) < 32:
+        assert kt["match"][0] >= 2 * nstages  # at least two batches per stage
+    # (global group cubes: the cube count shrinks first, since the cubes, not
+    # the rows' records, dominate the buffers)
+    for g, w_ in zip(got, want):
+        assert np.array_equal(g, w_)

@@ -47,6 +47,8 @@ def _instance(gpu, fmt, w, h, planes, **kw):
 # Largest first: every later call runs on a working array that still holds
 # the earlier, larger frame's records, partials and basic estimate.
 CASES = [
+    # beyond the array on the tile plan: batched over grid rows in it
+    ("GRAYS", 3840, 2160, dict(sigma=20, final=1)),
     ("YUV420PS", 1920, 1080, dict(sigma=10, final=1)),
     ("GRAYS", 256, 256, dict(sigma=5, block_step=3, group_size=16, bm_range=16)),
     ("RGBS", 160, 120, dict(sigma=10, final=1, chroma=0, block_size=16, group_size=16, block_step=8, bm_range=12)),
@@ -94,4 +96,4 @@ def test_noalloc_refuses_frames_beyond_its_array(gpu):
                                                 None, ptrs, strides, None)
     torch.cuda.synchronize()
     assert rc == gpu.UNSUPPORTED
-    assert "MiB of working memory" in gpu.last_error()
+    assert "MiB of working memory even in batches" in gpu.last_error()

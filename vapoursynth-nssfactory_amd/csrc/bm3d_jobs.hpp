@@ -28,6 +28,7 @@ struct MatchRec {
 // (width - block, height - block) (src/gpuhost/geometry.hpp:17-26).
 struct Grid {
     int nx, ny, step;
+    int row0 = 0;  // first grid row (a batch of rows of a larger grid: nx x ny references from row row0)
 };
 
 struct MatchJob {
@@ -89,6 +90,10 @@ struct FilterJob {
     TileGeom tiles;
     unsigned long long* part_num;
     unsigned long long* part_den;
+    // A batch of tile rows (memory-limited plan): workgroup w filters tile
+    // tile0 + w of the frame and writes partials [w]; the records hold the
+    // grid rows from grid.row0 on. Both 0 for a whole frame.
+    int tile0;
 };
 
 // Groups whose cube does not fit the LDS (block 32 with group 64, or group
@@ -101,6 +106,7 @@ struct FilterTable {
     FilterJob job[kMaxJobs];
     float* cube;      // null: one group per workgroup, the cube in LDS
     int cube_floats;  // floats per workgroup's cube
+    int cube_slots;   // cubes at `cube` (at most kCubeSlots workgroups)
 };
 
 struct FinishJob {
@@ -120,6 +126,13 @@ struct FinishJob {
     int width, height, sstride, dstride;
     int tiles_x;        // output blocks per row (64 x 16 tile path, 16 x 16 atomic path)
     int wg_begin;
+    // A batch of tile rows [ty0, ty1) (memory-limited plan): the exact
+    // partials of its tiles (tile ty at index ty - ty0) are added into the
+    // frame sums acc_num / acc_den over pixel rows from py0 on, and no output
+    // is written. acc_num null: a whole frame.
+    long long* acc_num;
+    long long* acc_den;
+    int ty0, ty1, py0;
 };
 
 struct FinishTable {
@@ -205,6 +218,7 @@ struct TFilterTable {
     TFilterJob job[kMaxJobs];
     float* cube;  // as FilterTable::cube
     int cube_floats;
+    int cube_slots;
 };
 
 // Temporal tile path (block 8, radius <= 2): every slot's contributions of a

@@ -183,7 +183,9 @@ __device__ inline int div_small(int c, float inv) {
 }
 
 __device__ inline int ref_x(const Grid& g, int i, int width, int block) { return min(i * g.step, width - block); }
-__device__ inline int ref_y(const Grid& g, int j, int height, int block) { return min(j * g.step, height - block); }
+__device__ inline int ref_y(const Grid& g, int j, int height, int block) {
+    return min((j + g.row0) * g.step, height - block);
+}
 
 // ---------------------------------------------------------------------------
 // Block matching (src/cpu/bm/matcher.hpp:247-283 collect_spatial_coords:
@@ -1377,8 +1379,8 @@ __global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
     const int lb = xcd_block();
     const FilterJob& jb = tab.job[find_job(tab, lb)];
     const TileGeom& tg = jb.tiles;
-    const int tile = lb - jb.wg_begin;
-    const int tx = tile % tg.ntx, ty = tile / tg.ntx;
+    const int tile = lb - jb.wg_begin;  // partials index
+    const int tx = (tile + jb.tile0) % tg.ntx, ty = (tile + jb.tile0) / tg.ntx;
     const int aw = tg.aw, ah = tg.ah;
     const int cw = aw - B + 1, ch = ah - B + 1;
     const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
@@ -1398,7 +1400,7 @@ __global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
     const int ni = min(tg.tw, jb.grid.nx - i_begin), nj = min(tg.th, jb.grid.ny - j_begin);
     for (int g = wave; g < ni * nj; g += 4) {
         const int gj = g / ni, gi = g - gj * ni;
-        const int rid = (j_begin + gj) * jb.grid.nx + i_begin + gi;
+        const int rid = (j_begin + gj - jb.grid.row0) * jb.grid.nx + i_begin + gi;
         const int k = jb.counts[rid];
         const MatchRec* rec = jb.matches + static_cast<size_t>(rid) * G;
         const float w = filter_group_wave<B, WIENER>(jb, rec, k, cube, rc, lane);
@@ -1587,8 +1589,8 @@ __global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
     const int lb = xcd_block();
     const FilterJob& jb = tab.job[find_job(tab, lb)];
     const TileGeom& tg = jb.tiles;
-    const int tile = lb - jb.wg_begin;
-    const int tx = tile % tg.ntx, ty = tile / tg.ntx;
+    const int tile = lb - jb.wg_begin;  // partials index
+    const int tx = (tile + jb.tile0) % tg.ntx, ty = (tile + jb.tile0) / tg.ntx;
     const int aw = tg.aw, ah = tg.ah;
     const int cw = aw - B + 1, ch = ah - B + 1;
     const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
@@ -1608,12 +1610,12 @@ __global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
     for (int e = threadIdx.x; e < ngroups * G; e += 256) {
         const int g = e / G, j = e - g * G;
         const int gj = g / ni, gi = g - gj * ni;
-        const int rid = (j_begin + gj) * jb.grid.nx + i_begin + gi;
+        const int rid = (j_begin + gj - jb.grid.row0) * jb.grid.nx + i_begin + gi;
         recs[e] = jb.matches[static_cast<size_t>(rid) * G + j];
     }
     for (int g = threadIdx.x; g < ngroups; g += 256) {
         const int gj = g / ni, gi = g - gj * ni;
-        cnt[g] = jb.counts[(j_begin + gj) * jb.grid.nx + i_begin + gi];
+        cnt[g] = jb.counts[(j_begin + gj - jb.grid.row0) * jb.grid.nx + i_begin + gi];
     }
     __syncthreads();
 
@@ -2029,8 +2031,8 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     const int lb = xcd_block();
     const FilterJob& jb = tab.job[find_job(tab, lb)];
     const TileGeom& tg = jb.tiles;
-    const int tile = lb - jb.wg_begin;
-    const int tx = tile % tg.ntx, ty = tile / tg.ntx;
+    const int tile = lb - jb.wg_begin;  // partials index
+    const int tx = (tile + jb.tile0) % tg.ntx, ty = (tile + jb.tile0) / tg.ntx;
     const int aw = tg.aw, ah = tg.ah;
     const int cw = aw - B + 1, ch = ah - B + 1;
     const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
@@ -2052,12 +2054,13 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     for (int e = threadIdx.x; e < ngroups * G; e += 256) {
         const int g = e / G, j = e - g * G;
         const int gj = g / ni, gi = g - gj * ni;
-        const MatchRec r = jb.matches[static_cast<size_t>((j_begin + gj) * jb.grid.nx + i_begin + gi) * G + j];
+        const MatchRec r =
+            jb.matches[static_cast<size_t>((j_begin + gj - jb.grid.row0) * jb.grid.nx + i_begin + gi) * G + j];
         recs[e] = static_cast<uint32_t>(r.x) | (static_cast<uint32_t>(r.y) << 16);
     }
     for (int g = threadIdx.x; g < ngroups; g += 256) {
         const int gj = g / ni, gi = g - gj * ni;
-        cnt[g] = jb.counts[(j_begin + gj) * jb.grid.nx + i_begin + gi];
+        cnt[g] = jb.counts[(j_begin + gj - jb.grid.row0) * jb.grid.nx + i_begin + gi];
     }
     __syncthreads();
 
@@ -2969,7 +2972,7 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
     const FinishJob& jb = tab.job[find_job(tab, lb)];
     const int t = lb - jb.wg_begin;
     const int x0 = (t % jb.tiles_x) * 64 + (threadIdx.x & 15) * 4;
-    const int y = (t / jb.tiles_x) * 16 + (threadIdx.x >> 4);
+    const int y = jb.py0 + (t / jb.tiles_x) * 16 + (threadIdx.x >> 4);
     if (x0 >= jb.width || y >= jb.height) return;
     const TileGeom& tg = jb.tiles;
     const int B = jb.block, s = jb.step, r = jb.range;
@@ -2978,18 +2981,22 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
     while (txh + 1 < tg.ntx && tg.ax0(txh + 1, jb.width, B, s, r) <= xr) ++txh;
     int tyh = min(tg.nty - 1, (y + r) / (tg.th * s));
     while (tyh + 1 < tg.nty && tg.ay0(tyh + 1, jb.height, B, s, r) <= y) ++tyh;
+    // a batch of tile rows: only its tiles, at partials index ty - ty0
+    const bool batch = EXACT && jb.acc_num;
+    const int ty_lo = batch ? jb.ty0 : 0;
+    if (batch) tyh = min(tyh, jb.ty1 - 1);
     using Acc = typename std::conditional<EXACT, long long, float>::type;
     Acc n[4] = {0, 0, 0, 0}, d[4] = {0, 0, 0, 0};
     const size_t area = static_cast<size_t>(tg.pitch) * tg.ah;
-    for (int ty = tyh; ty >= 0; --ty) {
+    for (int ty = tyh; ty >= ty_lo; --ty) {
         const int oy = y - tg.ay0(ty, jb.height, B, s, r);
         if (oy >= tg.ah) break;
         for (int tx = txh; tx >= 0; --tx) {
             const int ax0 = tg.ax0(tx, jb.width, B, s, r);
             if (ax0 + tg.aw <= x0) break;
             // column of x0: a multiple of 4 in [0, pitch - 4]
-            const size_t o = (static_cast<size_t>(ty) * tg.ntx + tx) * area + static_cast<size_t>(oy) * tg.pitch +
-                             (x0 - ax0 + (ax0 & 3));
+            const size_t o = (static_cast<size_t>(ty - ty_lo) * tg.ntx + tx) * area +
+                             static_cast<size_t>(oy) * tg.pitch + (x0 - ax0 + (ax0 & 3));
             Acc pn[4], pd[4];
             if constexpr (EXACT) {
                 const longlong2* qn = reinterpret_cast<const longlong2*>(jb.exact_num + o);
@@ -3009,6 +3016,19 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
                 n[i] += in ? pn[i] : Acc(0);
                 d[i] += in ? pd[i] : Acc(0);
             }
+        }
+    }
+    if constexpr (EXACT) {
+        if (batch) {
+            long long* an = jb.acc_num + static_cast<size_t>(y) * jb.width;
+            long long* ad = jb.acc_den + static_cast<size_t>(y) * jb.width;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (x0 + i < jb.width) {
+                    an[x0 + i] += n[i];
+                    ad[x0 + i] += d[i];
+                }
+            return;
         }
     }
     float* dst = jb.dst + static_cast<size_t>(y) * jb.dstride;
@@ -3086,6 +3106,24 @@ __global__ __launch_bounds__(256) void finish_corner_kernel(FinishTable tab) {
             dd > 0 ? static_cast<float>(static_cast<double>(n) / static_cast<double>(dd))
                    : jb.src[static_cast<size_t>(y) * jb.sstride + x];
     }
+}
+
+// The output from frame-sized exact sums with the den per pixel (the
+// memory-limited plan's tile buckets, whose batches added their partials
+// into acc_num / acc_den): the division of finish_tiles_kernel. 64 x 4
+// pixels per block.
+__global__ __launch_bounds__(256) void finish_sums_kernel(FinishTable tab) {
+    const int lb = xcd_block();
+    const FinishJob& jb = tab.job[find_job(tab, lb)];
+    const int t = lb - jb.wg_begin;
+    const int x = (t % jb.tiles_x) * 64 + (threadIdx.x & 63);
+    const int y = (t / jb.tiles_x) * 4 + (threadIdx.x >> 6);
+    if (x >= jb.width || y >= jb.height) return;
+    const size_t o = static_cast<size_t>(y) * jb.width + x;
+    const long long n = jb.acc_num[o], d = jb.acc_den[o];
+    jb.dst[static_cast<size_t>(y) * jb.dstride + x] =
+        d > 0 ? static_cast<float>(static_cast<double>(n) / static_cast<double>(d))
+              : jb.src[static_cast<size_t>(y) * jb.sstride + x];
 }
 
 // nss.VAggregate's reduction of one plane (src/cpu/bm/agg.cpp:159-201
@@ -3673,7 +3711,7 @@ static hipError_t launch_filter_b(const FilterTable& t, int total_wg, size_t lds
         }
     } else {
         // a table with global cubes runs kCubeSlots workgroups (no LDS cube)
-        const int grid = t.cube ? std::min(total_wg, kCubeSlots) : total_wg;
+        const int grid = t.cube ? std::min(total_wg, t.cube_slots) : total_wg;
         if (t.cube) lds = 0;
         if (wiener) {
             allow_lds(&filter_atomic_kernel<B, true>, lds);
@@ -3739,6 +3777,11 @@ hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t l
         case 32: return launch_filter_b<32>(t, total_wg, lds, wiener, tiles, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_finish_sums(const FinishTable& t, int total_wg, hipStream_t s) {
+    finish_sums_kernel<<<total_wg, 256, 0, s>>>(t);
+    return hipGetLastError();
 }
 
 hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s) {
@@ -3815,7 +3858,7 @@ hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t
 
 template <int B>
 static hipError_t launch_tfilter_b(const TFilterTable& t, int total_wg, size_t lds, bool wiener, hipStream_t s) {
-    const int grid = t.cube ? std::min(total_wg, kCubeSlots) : total_wg;
+    const int grid = t.cube ? std::min(total_wg, t.cube_slots) : total_wg;
     if (t.cube) lds = 0;
     if (wiener) {
         allow_lds(&filter_temporal_kernel<B, true>, lds);

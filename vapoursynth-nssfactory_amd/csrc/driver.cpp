@@ -275,6 +275,22 @@ struct PlaneNeeds {
     bool atomic = false, basic = false;
 };
 
+// Grid rows one batch of a memory-limited plan matches on plane i in stage
+// st: the rows of a batch of tile rows for a tile bucket (plane i's own
+// geometry, or plane 0's with chroma, where plane 0 holds the groups),
+// ceil(ny / batches) otherwise.
+int batch_grid_rows(const nss_hip_bm3d* h, const Stage& st, int i, const Grid& g) {
+    for (int bi = 0; bi < st.nbuckets; ++bi) {
+        const Bucket& bk = st.bucket[bi];
+        for (int k = 0; k < bk.nplanes; ++k)
+            if (bk.planes[k] == i && bk.tiles) {
+                const TileGeom& tg = bk.geom[k];
+                return std::min(g.ny, (tg.nty + h->batches - 1) / h->batches * tg.th);
+            }
+    }
+    return (g.ny + h->batches - 1) / h->batches;
+}
+
 PlaneNeeds plane_needs(const nss_hip_bm3d* h, int i) {
     PlaneNeeds n;
     const nss_hip_bm3d_params& p = h->p;
@@ -288,7 +304,9 @@ PlaneNeeds plane_needs(const nss_hip_bm3d* h, int i) {
         const bool matched = p.chroma ? (i == 0 && any) : sp.sigma != 0.f;
         if (matched) {
             const Grid g = make_grid(h->pw[i], h->ph[i], sp.block, sp.step);
-            const size_t nref = static_cast<size_t>(g.nx) * g.ny;
+            // a memory-limited plan holds the records of one batch of grid rows
+            const int rows = h->batches > 0 ? batch_grid_rows(h, st, i, g) : g.ny;
+            const size_t nref = static_cast<size_t>(g.nx) * rows;
             n.recs = std::max(n.recs, nref * sp.group);
             n.nref = std::max(n.nref, nref);
         }
@@ -297,8 +315,12 @@ PlaneNeeds plane_needs(const nss_hip_bm3d* h, int i) {
             for (int k = 0; k < st.bucket[bi].nplanes; ++k) {
                 if (st.bucket[bi].planes[k] != i) continue;
                 if (st.bucket[bi].tiles) {
+                    // a memory-limited plan: one batch of tile rows, and the
+                    // frame sums its batches add into
                     const TileGeom& tg = st.bucket[bi].geom[k];
-                    n.part = std::max(n.part, static_cast<size_t>(tg.ntx) * tg.nty * tg.pitch * tg.ah);
+                    const int rows = h->batches > 0 ? (tg.nty + h->batches - 1) / h->batches : tg.nty;
+                    n.part = std::max(n.part, static_cast<size_t>(tg.ntx) * rows * tg.pitch * tg.ah);
+                    if (h->batches > 0) n.atomic = true;
                 } else {
                     n.atomic = true;
                 }
@@ -313,8 +335,8 @@ PlaneNeeds plane_needs(const nss_hip_bm3d* h, int i) {
 namespace nsship {
 
 // The global group cubes of one frame slot (every bucket's launch uses at
-// most kCubeSlots cubes of the instance's largest size).
-size_t cube_bytes(const nss_hip_bm3d* h) { return static_cast<size_t>(kCubeSlots) * h->cube_floats * sizeof(float); }
+// most cube_slots cubes of the instance's largest size).
+size_t cube_bytes(const nss_hip_bm3d* h) { return static_cast<size_t>(h->cube_slots) * h->cube_floats * sizeof(float); }
 
 int frame_slot_count(const nss_hip_bm3d* h) { return h->p.radius > 0 ? 1 : std::max(1, h->p.num_streams); }
 
@@ -412,6 +434,39 @@ size_t carve_frame_buffers(const nss_hip_bm3d* h, char* base, FrameBuffers& fb) 
     return off;
 }
 
+bool fit_frame_buffers(nss_hip_bm3d* h, size_t bytes) {
+    if (frame_buffer_bytes(h) <= bytes) return true;
+    if (h->p.radius > 0) return false;
+    // The batched plan (src/cuda/common/group_driver.cpp:481-520 batches the
+    // work to fit the limit): the same kernels, over batches of tile rows (of
+    // grid rows on the atomic path), every batch adding its exact sums into
+    // frame-sized int64 sums, so the finish sees the sums of the whole-frame
+    // plan.
+    int max_rows = 1;
+    for (int s = 0; s < h->nstages; ++s)
+        for (int bi = 0; bi < h->stages[s].nbuckets; ++bi) {
+            const Bucket& bk = h->stages[s].bucket[bi];
+            for (int k = 0; k < bk.nplanes; ++k) {
+                const int i = bk.planes[k];
+                max_rows = std::max(max_rows, bk.tiles ? bk.geom[k].nty
+                                                      : make_grid(h->pw[i], h->ph[i], bk.block,
+                                                                  h->stages[s].plane[i].step).ny);
+            }
+        }
+    // the global group cubes (if any) in halving counts once batching alone
+    // does not fit: the slots only set how many groups run at once
+    for (int slots = kCubeSlots; slots >= 1; slots /= 2) {
+        h->cube_slots = slots;
+        for (int nb = 1; nb <= max_rows; ++nb) {
+            h->batches = nb;
+            for (int i = 0; i < h->nplanes; ++i) h->part_elems[i] = plane_needs(h, i).part;
+            if (frame_buffer_bytes(h) <= bytes) return true;
+        }
+        if (!h->cube_floats) break;
+    }
+    return false;
+}
+
 }  // namespace nsship
 
 namespace {
@@ -485,6 +540,7 @@ struct LaunchGroup {
     bool tiles = false, wiener = false;
     size_t match_lds = 0, filter_lds = 0;
     float* cube = nullptr;  // global group cubes (the first frame's), with cube_floats
+    int cube_slots = 0;
     ZeroTable zt{};
     MatchTable mt{};
     FilterTable ft{};
@@ -513,6 +569,7 @@ int flush_group(LaunchGroup& g, Tracer* tr, hipStream_t s) {
     if (g.ft.n) {
         g.ft.cube = g.cube_floats ? g.cube : nullptr;
         g.ft.cube_floats = g.cube_floats;
+        g.ft.cube_slots = g.cube_slots;
         e = traced(tr, 2, s, [&] {
             return launch_filter(b, g.ft, g.fwg, g.filter_lds, g.wiener, g.tiles, g.fast_group, g.form, s);
         });
@@ -538,7 +595,10 @@ void add_bucket_jobs(LaunchGroup& g, const FrameRef& f, const Stage& st, const B
     const int b = bk.block;
     g.match_lds = std::max(g.match_lds, bk.match_lds);
     g.filter_lds = std::max(g.filter_lds, bk.filter_lds);
-    if (bk.cube_floats && !g.cube) g.cube = f.buf.cube;
+    if (bk.cube_floats && !g.cube) {
+        g.cube = f.buf.cube;
+        g.cube_slots = h->cube_slots;
+    }
     // CBM3D (chroma=1): the groups are matched once, on plane 0 of the match
     // planes, and every filtered plane uses them (src/host/filter_bm3d.cpp:451-486).
     int mplanes[3], nmp = 0;
@@ -629,10 +689,310 @@ void add_bucket_jobs(LaunchGroup& g, const FrameRef& f, const Stage& st, const B
 
 namespace nsship {
 
+// A tile bucket of a memory-limited instance: per batch of tile rows, the
+// grid rows of those tiles are matched (records of the batch only), the
+// tiles filtered (the kernels of the whole-frame plan, partials of the batch
+// only), and the batch's partials added into the frame sums; then the output
+// from the sums. The sums equal the whole-frame finish's, so do the bits.
+int run_limited_tiles(const FrameRef& f, const Stage& st, const Bucket& bk, const StageIO& io, const float* sigma,
+                      Tracer* tr, hipStream_t s) {
+    nss_hip_bm3d* h = f.h;
+    const int nb = h->batches, b = bk.block;
+    // the tile rows of batch `batch` of bucket plane q: [tr0, tr1)
+    auto rows_of = [&](int q, int batch, int* tr0, int* tr1) {
+        const TileGeom& tg = bk.geom[q];
+        const int t = (tg.nty + nb - 1) / nb;
+        *tr0 = std::min(tg.nty, batch * t);
+        *tr1 = std::min(tg.nty, *tr0 + t);
+    };
+    for (int batch = 0; batch < nb; ++batch) {
+        MatchTable mt{};
+        int mwg = 0;
+        for (int q = 0; q < bk.nplanes; ++q) {
+            const int i = bk.planes[q];
+            // with chroma the groups are matched once, on plane 0 (the same
+            // grid and tiles as every plane of a 4:4:4 clip)
+            if (h->p.chroma && q > 0) break;
+            const int mi = h->p.chroma ? 0 : i;
+            const StagePlane& sp = st.plane[mi];
+            int tr0, tr1;
+            rows_of(q, batch, &tr0, &tr1);
+            if (tr0 >= tr1) continue;
+            Grid g = make_grid(h->pw[mi], h->ph[mi], b, sp.step);
+            g.row0 = tr0 * bk.geom[q].th;
+            g.ny = std::min(g.ny, tr1 * bk.geom[q].th) - g.row0;
+            MatchJob& m = mt.job[mt.n++];
+            m.plane = io.match[mi];
+            m.width = h->pw[mi];
+            m.height = h->ph[mi];
+            m.stride = io.match_stride[mi];
+            m.grid = g;
+            m.range = sp.range;
+            m.group = sp.group;
+            m.t0 = 0;
+            m.wg_begin = mwg;
+            m.out = f.buf.matches[mi];
+            m.counts = f.buf.counts[mi];
+            mwg += match_workgroups(bk.match_kind, g);
+        }
+        if (!mt.n) continue;
+        hipError_t e = traced(tr, 1, s, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, s); });
+        if (e != hipSuccess) return hip_fail(e, "match launch");
+        FilterTable ft{};
+        FinishTable fn{};
+        int fwg = 0, nwg = 0;
+        for (int q = 0; q < bk.nplanes; ++q) {
+            const int i = bk.planes[q];
+            const int mi = h->p.chroma ? 0 : i;
+            const StagePlane& sp = st.plane[i];
+            const TileGeom& tg = bk.geom[q];
+            int tr0, tr1;
+            rows_of(q, batch, &tr0, &tr1);
+            if (tr0 >= tr1) continue;
+            Grid g = make_grid(h->pw[i], h->ph[i], b, sp.step);
+            g.row0 = tr0 * tg.th;
+            FilterJob& fj = ft.job[ft.n++];
+            fj.src = io.src[i];
+            fj.wref = st.wiener ? io.wref[i] : nullptr;
+            fj.width = h->pw[i];
+            fj.height = h->ph[i];
+            fj.stride = io.src_stride[i];
+            fj.wref_stride = st.wiener ? io.wref_stride[i] : 0;
+            fj.grid = g;
+            fj.range = sp.range;
+            fj.nref = g.nx * g.ny;
+            fj.group = sp.group;
+            fj.sigma = sigma[i];
+            fj.wg_begin = fwg;
+            fj.matches = f.buf.matches[mi];
+            fj.counts = f.buf.counts[mi];
+            fj.tiles = tg;
+            fj.part_num = f.buf.part[i];
+            fj.part_den = f.buf.part[i] + h->part_elems[i];
+            fj.tile0 = tr0 * tg.ntx;
+            fwg += (tr1 - tr0) * tg.ntx;
+            // the pixel rows the batch's tiles reach
+            const int py0 = std::max(0, tg.ay0(tr0, h->ph[i], b, sp.step, sp.range));
+            const int py1 = std::min(h->ph[i], tg.ay0(tr1 - 1, h->ph[i], b, sp.step, sp.range) + tg.ah);
+            FinishJob& n = fn.job[fn.n++];
+            n.exact_num = reinterpret_cast<const long long*>(fj.part_num);
+            n.exact_den = reinterpret_cast<const long long*>(fj.part_den);
+            n.tiles = tg;
+            n.block = b;
+            n.step = sp.step;
+            n.range = sp.range;
+            n.src = io.src[i];
+            n.dst = io.dst[i];
+            n.width = h->pw[i];
+            n.height = h->ph[i];
+            n.sstride = io.src_stride[i];
+            n.dstride = io.dst_stride[i];
+            n.acc_num = reinterpret_cast<long long*>(f.buf.num[i]);
+            n.acc_den = reinterpret_cast<long long*>(f.buf.den[i]);
+            n.ty0 = tr0;
+            n.ty1 = tr1;
+            n.py0 = py0;
+            n.tiles_x = (h->pw[i] + 63) / 64;
+            n.wg_begin = nwg;
+            nwg += n.tiles_x * ((py1 - py0 + 15) / 16);
+        }
+        if (!ft.n) continue;
+        e = traced(tr, 2, s, [&] {
+            return launch_filter(b, ft, fwg, bk.filter_lds, st.wiener, true, bk.fast_group, bk.form, s);
+        });
+        if (e != hipSuccess) return hip_fail(e, "filter launch");
+        e = traced(tr, 3, s, [&] { return launch_finish(b, fn, nwg, true, s); });
+        if (e != hipSuccess) return hip_fail(e, "finish launch");
+    }
+    FinishTable fn{};
+    int nwg = 0;
+    for (int q = 0; q < bk.nplanes; ++q) {
+        const int i = bk.planes[q];
+        FinishJob& n = fn.job[fn.n++];
+        n.src = io.src[i];
+        n.dst = io.dst[i];
+        n.width = h->pw[i];
+        n.height = h->ph[i];
+        n.sstride = io.src_stride[i];
+        n.dstride = io.dst_stride[i];
+        n.acc_num = reinterpret_cast<long long*>(f.buf.num[i]);
+        n.acc_den = reinterpret_cast<long long*>(f.buf.den[i]);
+        n.tiles_x = (h->pw[i] + 63) / 64;
+        n.wg_begin = nwg;
+        nwg += n.tiles_x * ((h->ph[i] + 3) / 4);
+    }
+    const hipError_t e = traced(tr, 3, s, [&] { return launch_finish_sums(fn, nwg, s); });
+    return e == hipSuccess ? NSS_HIP_OK : hip_fail(e, "finish launch");
+}
+
+// One frame of a memory-limited instance (h->batches > 0): per stage, the
+// filtered planes' frame sums are cleared, then every bucket runs in
+// batches (tile rows on the tile path, grid rows on the atomic path), each
+// batch's references matched with the batch's records only and their groups
+// filtered into the sums; the finish runs once per bucket.
+int run_limited(const FrameRef& f, Tracer* tr, hipStream_t s) {
+    nss_hip_bm3d* h = f.h;
+    const int nb = h->batches;
+    for (int k = 0; k < h->nstages; ++k) {
+        const Stage& st = h->stages[k];
+        const StageIO io = stage_io(f, k);
+        ZeroTable zt{};
+        int zwg = 0;
+        for (int i = 0; i < h->nplanes; ++i) {
+            if (st.plane[i].sigma == 0.f) {
+                const hipError_t e =
+                    hipMemcpy2DAsync(io.dst[i], static_cast<size_t>(io.dst_stride[i]) * 4, io.src[i],
+                                     static_cast<size_t>(io.src_stride[i]) * 4, static_cast<size_t>(h->pw[i]) * 4,
+                                     h->ph[i], hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) return hip_fail(e, "plane copy");
+                continue;
+            }
+            const long long npx = static_cast<long long>(h->pw[i]) * h->ph[i];
+            for (unsigned long long* buf : {f.buf.num[i], f.buf.den[i]}) {
+                ZeroJob& z = zt.job[zt.n++];
+                z.p = buf;
+                z.count = npx;
+                z.wg_begin = zwg;
+                zwg += static_cast<int>((npx + 1023) / 1024);
+            }
+        }
+        if (zt.n) {
+            const hipError_t e = traced(tr, 0, s, [&] { return launch_zero(zt, zwg, s); });
+            if (e != hipSuccess) return hip_fail(e, "clear launch");
+        }
+        for (int bi = 0; bi < st.nbuckets; ++bi) {
+            const Bucket& bk = st.bucket[bi];
+            const int b = bk.block;
+            if (bk.tiles) {
+                const int rc = run_limited_tiles(f, st, bk, io, f.sigma[k], tr, s);
+                if (rc != NSS_HIP_OK) return rc;
+                continue;
+            }
+            int mplanes[3], nmp = 0;
+            if (h->p.chroma)
+                mplanes[nmp++] = 0;
+            else
+                for (int q = 0; q < bk.nplanes; ++q) mplanes[nmp++] = bk.planes[q];
+            for (int batch = 0; batch < nb; ++batch) {
+                // The batch's grid rows of each plane: [batch R, batch R + R)
+                // clipped, R = ceil(ny / nb); records indexed within the batch.
+                auto batch_grid = [&](int i) {
+                    Grid g = make_grid(h->pw[i], h->ph[i], b, st.plane[i].step);
+                    const int rows = (g.ny + nb - 1) / nb;
+                    g.row0 = std::min(g.ny, batch * rows);
+                    g.ny = std::min(g.ny, g.row0 + rows) - g.row0;
+                    return g;
+                };
+                MatchTable mt{};
+                int mwg = 0;
+                for (int q = 0; q < nmp; ++q) {
+                    const int i = mplanes[q];
+                    const StagePlane& sp = st.plane[i];
+                    const Grid g = batch_grid(i);
+                    if (g.ny <= 0) continue;
+                    MatchJob& m = mt.job[mt.n++];
+                    m.plane = io.match[i];
+                    m.width = h->pw[i];
+                    m.height = h->ph[i];
+                    m.stride = io.match_stride[i];
+                    m.grid = g;
+                    m.range = sp.range;
+                    m.group = sp.group;
+                    m.t0 = 0;
+                    m.wg_begin = mwg;
+                    m.out = f.buf.matches[i];
+                    m.counts = f.buf.counts[i];
+                    mwg += match_workgroups(bk.match_kind, g);
+                }
+                if (!mt.n) continue;
+                hipError_t e = traced(tr, 1, s, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, s); });
+                if (e != hipSuccess) return hip_fail(e, "match launch");
+                int fwg = 0;
+                {
+                    FilterTable ft{};
+                    for (int q = 0; q < bk.nplanes; ++q) {
+                        const int i = bk.planes[q];
+                        const int mi = h->p.chroma ? 0 : i;
+                        const StagePlane& sp = st.plane[i];
+                        const Grid g = batch_grid(i);
+                        if (g.ny <= 0) continue;
+                        FilterJob& fj = ft.job[ft.n++];
+                        fj.src = io.src[i];
+                        fj.wref = st.wiener ? io.wref[i] : nullptr;
+                        fj.width = h->pw[i];
+                        fj.height = h->ph[i];
+                        fj.stride = io.src_stride[i];
+                        fj.wref_stride = st.wiener ? io.wref_stride[i] : 0;
+                        fj.grid = g;
+                        fj.range = sp.range;
+                        fj.nref = g.nx * g.ny;
+                        fj.group = sp.group;
+                        fj.sigma = f.sigma[k][i];
+                        fj.wg_begin = fwg;
+                        fj.matches = f.buf.matches[mi];
+                        fj.counts = f.buf.counts[mi];
+                        fj.num = f.buf.num[i];
+                        fj.den = f.buf.den[i];
+                        fwg += fj.nref;
+                    }
+                    if (!ft.n) continue;
+                    if (bk.cube_floats) {
+                        ft.cube = f.buf.cube;
+                        ft.cube_floats = bk.cube_floats;
+                        ft.cube_slots = h->cube_slots;
+                    }
+                    e = traced(tr, 2, s, [&] {
+                        return launch_filter(b, ft, fwg, bk.filter_lds, st.wiener, false, 0, 0, s);
+                    });
+                }
+                if (e != hipSuccess) return hip_fail(e, "filter launch");
+            }
+            FinishTable fn{};
+            int nwg = 0;
+            for (int q = 0; q < bk.nplanes; ++q) {
+                const int i = bk.planes[q];
+                const StagePlane& sp = st.plane[i];
+                FinishJob& n = fn.job[fn.n++];
+                n.num = f.buf.num[i];
+                n.den = f.buf.den[i];
+                n.block = b;
+                n.step = sp.step;
+                n.range = sp.range;
+                n.src = io.src[i];
+                n.dst = io.dst[i];
+                n.width = h->pw[i];
+                n.height = h->ph[i];
+                n.sstride = io.src_stride[i];
+                n.dstride = io.dst_stride[i];
+                n.tiles_x = (h->pw[i] + 15) / 16;
+                n.wg_begin = nwg;
+                nwg += n.tiles_x * ((h->ph[i] + 15) / 16);
+            }
+            const hipError_t e = traced(tr, 3, s, [&] { return launch_finish(b, fn, nwg, false, s); });
+            if (e != hipSuccess) return hip_fail(e, "finish launch");
+        }
+    }
+    return NSS_HIP_OK;
+}
+
 // Both stages of a batch of radius-0 frames: per stage round, every frame's
 // planes with sigma 0 are copied and the jobs of equal bucket shape share one
-// launch sequence (at most kMaxJobs planes per table, else several).
+// launch sequence (at most kMaxJobs planes per table, else several). Frames
+// of memory-limited instances run on their own, batch by batch.
 int run_batch(const FrameRef* frames, int nframes, Tracer* tr, hipStream_t s) {
+    std::vector<FrameRef> plain;
+    for (int q = 0; q < nframes; ++q) {
+        if (frames[q].h->batches > 0) {
+            const int rc = run_limited(frames[q], tr, s);
+            if (rc != NSS_HIP_OK) return rc;
+        } else {
+            plain.push_back(frames[q]);
+        }
+    }
+    if (plain.size() != static_cast<size_t>(nframes)) {
+        frames = plain.data();
+        nframes = static_cast<int>(plain.size());
+    }
     int rounds = 0;
     for (int q = 0; q < nframes; ++q) rounds = std::max(rounds, frames[q].h->nstages);
     std::vector<LaunchGroup> groups;
@@ -884,6 +1244,7 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
         if (bk.cube_floats && !fast) {
             ft.cube = h->slots[0].buf.cube;
             ft.cube_floats = bk.cube_floats;
+            ft.cube_slots = h->cube_slots;
         }
         hipError_t e = traced(h, 1, st, [&] { return launch_match(b, mt, mwg, bk.match_lds, bk.match_kind, st); });
         if (e != hipSuccess) return hip_fail(e, "match launch");
@@ -1377,7 +1738,19 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
     // for the clip is an error at creation, never while frames are filtered
     // (docs/api/cuda/README.md:141, src/cuda/common/group_driver.cpp:491-498).
     // Checked before anything is allocated.
-    if (p.memory_limit_mb > 0) {
+    if (p.memory_limit_mb > 0 && !rolling) {
+        // Radius 0: batch the work to fit the limit (docs/api/cuda/README.md:141);
+        // an error only when one batch of grid rows does not fit.
+        const int n = frame_slot_count(h);
+        if (!fit_frame_buffers(h, (static_cast<size_t>(p.memory_limit_mb) << 20) / static_cast<size_t>(n))) {
+            const size_t each = frame_buffer_bytes(h);
+            delete h;
+            return fail(NSS_HIP_INVALID_INPUT, "nss_hip.BM3D: memory_limit_mb is too small for this clip: each of the " +
+                                                   std::to_string(n) + " stream(s) needs at least " +
+                                                   std::to_string((each >> 20) + 1) + " MiB");
+        }
+    }
+    if (p.memory_limit_mb > 0 && rolling) {
         const size_t need = instance_bytes(h);
         if (need > (static_cast<size_t>(p.memory_limit_mb) << 20)) {
             const int n = frame_slot_count(h);
@@ -1566,7 +1939,7 @@ int nss_hip_bm3d_process(nss_hip_bm3d* h, const float* const* src, const int* sr
 int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* best_ms) {
     if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null handle");
     if (repeats < 1 || repeats > 100) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: repeats must be in [1, 100]");
-    if (h->p.radius > 0 || (h->p.has_ref && !h->p.final_stage)) return NSS_HIP_OK;  // nothing to tune
+    if (h->p.radius > 0 || (h->p.has_ref && !h->p.final_stage) || h->batches > 0) return NSS_HIP_OK;  // nothing to tune
     DeviceGuard guard(h->device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1667,6 +2040,8 @@ int nss_hip_bm3d_set_tiles(nss_hip_bm3d* h, int stage, int tw, int th) {
     if (stage < 0 || stage >= h->nstages) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: no such stage");
     if (tw < 1 || th < 1 || tw * th > 64) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: tile of 1 to 64 references");
     if (h->p.radius > 0) return fail(NSS_HIP_UNSUPPORTED, "nss_hip: temporal instances keep their tile plan");
+    if (h->batches > 0)
+        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: an instance batched to fit memory_limit_mb has no tile plan");
     DeviceGuard guard(h->device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     AllSlotsLease all(h);

@@ -21,6 +21,9 @@ hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds
 hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
                          int fast_group, int form, hipStream_t s);
 hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s);
+// Output from frame-sized exact sums with per-pixel den (64 x 4 pixels per
+// workgroup; tiles_x = ceil(width / 64)).
+hipError_t launch_finish_sums(const FinishTable& t, int total_wg, hipStream_t s);
 hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s);
 hipError_t upload_dct_tables(const float* host, size_t count);
 size_t match_wave_lds_bytes(int block, int range, bool stage);
@@ -158,7 +161,7 @@ struct FrameBuffers {
     MatchRec* matches[3]{};
     int* counts[3]{};
     float* basic[3]{};
-    float* cube = nullptr;  // kCubeSlots global group cubes (the instance's cube_floats each)
+    float* cube = nullptr;  // cube_slots global group cubes (the instance's cube_floats each)
 };
 
 // One of an instance's num_streams frame slots: a frame's buffers and the
@@ -184,6 +187,11 @@ struct nss_hip_bm3d {
     int nstages = 0;
     size_t part_elems[3]{};  // per half (num or den) of a plane's tile partials, int64 entries
     size_t cube_floats = 0;  // largest global group cube of any bucket (0: every cube fits the LDS)
+    int cube_slots = nsship::kCubeSlots;  // global cubes per frame slot (fewer under a memory limit)
+    // memory_limit_mb below the normal plan's need (radius 0): every stage
+    // runs on frame-sized int64 sums, matching and filtering the grid rows in
+    // this many batches (0: the normal plan)
+    int batches = 0;
     // num_streams frame slots (radius 0; one for temporal instances). A
     // process call leases the slot free the longest for the time it takes to
     // queue the frame, sleeping while all are leased
@@ -258,6 +266,11 @@ size_t carve_frame_buffers(const nss_hip_bm3d* h, char* base, FrameBuffers& fb);
 // The host part of an instance: checks, plane sizes, stage plans, partial
 // sizes and the rolling plan; no device memory.
 int plan_instance(nss_hip_bm3d* h, const nss_hip_bm3d_params* params, const nss_hip_video_info* info, int device);
+// Fits a radius-0 instance's frame buffers into `bytes` per frame slot: the
+// normal plan when it fits, else the batched plan with the fewest batches
+// that fits (same output bits). False when none does (the MFMA forms have
+// no batched plan).
+bool fit_frame_buffers(nss_hip_bm3d* h, size_t bytes);
 FrameRef frame_ref(nss_hip_bm3d* h, const float* const* src, const int* src_stride, const float* const* ref,
                    const int* ref_stride, float* const* dst, const int* dst_stride);
 int run_batch(const FrameRef* frames, int nframes, Tracer* tr, hipStream_t s);

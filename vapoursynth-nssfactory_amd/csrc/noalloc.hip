@@ -8,13 +8,16 @@
 // several threads without ordering) race on it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 
 #include "instance.hpp"
 
 // 192 MiB per device: a 1920 x 1080 YUV420 two-stage frame at the default
-// arguments needs about 170 MiB (the int64 tile partials dominate).
+// arguments needs 158 MiB on the tile plan (the int64 tile partials
+// dominate); larger frames take the plan batched over grid rows, whose
+// frame-sized sums need 20 bytes per pixel of a two-stage frame.
 constexpr size_t kScratchBytes = static_cast<size_t>(192) << 20;
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -66,12 +69,19 @@ extern "C" int nss_hip_bm3d_process_noalloc(const nss_hip_bm3d_params* params, c
             if (!ref[i] || ref_stride[i] < h.pw[i])
                 return fail(NSS_HIP_INVALID_INPUT, "nss_hip: missing ref plane or stride below width");
     }
+    // The frame's buffers in the array: the normal plan, or the plan batched
+    // over grid rows (the memory_limit_mb plan, same output bits) when the
+    // array, or a smaller memory_limit_mb, is too small for it.
+    // (64 KiB held back for the buffers' 256-byte alignment)
     FrameBuffers fb;
-    const size_t need = carve_frame_buffers(&h, nullptr, fb);
-    if (need > kScratchBytes)
-        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: this frame needs " + std::to_string((need >> 20) + 1) +
-                                             " MiB of working memory; the no-allocation entry point holds " +
-                                             std::to_string(kScratchBytes >> 20) + " MiB");
+    size_t cap = kScratchBytes;
+    if (h.p.memory_limit_mb > 0) cap = std::min(cap, static_cast<size_t>(h.p.memory_limit_mb) << 20);
+    if (!fit_frame_buffers(&h, cap - (static_cast<size_t>(64) << 10)) ||
+        carve_frame_buffers(&h, nullptr, fb) > kScratchBytes)
+        return fail(NSS_HIP_UNSUPPORTED, "nss_hip: this frame needs " +
+                                             std::to_string((frame_buffer_bytes(&h) >> 20) + 1) +
+                                             " MiB of working memory even in batches; the no-allocation entry point "
+                                             "holds " + std::to_string(cap >> 20) + " MiB");
     DeviceGuard guard(device);
     if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
     char* base = nullptr;
