@@ -73,10 +73,12 @@ def _compare_matches(gpu, oracle, plane, block, step, rng_, group, flat, label):
 @pytest.mark.parametrize("flat", [False, True])
 @pytest.mark.parametrize("block,group,step,rng_", [(8, 16, 3, 16), (8, 8, 8, 9), (4, 16, 2, 7), (12, 8, 6, 5),
                                                    (16, 4, 8, 6), (2, 32, 1, 3), (1, 8, 1, 2), (32, 2, 16, 4),
-                                                   (8, 64, 3, 30), (16, 8, 8, 20), (32, 4, 16, 64)])
+                                                   (8, 64, 3, 30), (16, 8, 8, 20), (32, 4, 16, 64),
+                                                   (8, 64, 8, 9), (12, 64, 6, 5), (8, 32, 4, 9)])
 def test_match_lists(gpu, oracle, block, group, step, rng_, flat):
-    """Covers the run matcher (B <= 12, small windows), the wave matcher with
-    the window in LDS, and with the window read from global (B 32, range 64)."""
+    """Covers the run matcher (B <= 12, small windows; groups up to 32 by the
+    threshold selection, 64 by the rounds), the wave matcher with the window
+    in LDS, and with the window read from global (B 32, range 64)."""
     print(f"match b{block} g{group} s{step} r{rng_} flat={flat}", flush=True)
     plane = _test_plane(77 if rng_ < 30 else 150, 61 if rng_ < 30 else 140, 7 + block + group, flat)
     flips = _compare_matches(gpu, oracle, plane, block, step, rng_, group, flat, f"b{block}g{group}")

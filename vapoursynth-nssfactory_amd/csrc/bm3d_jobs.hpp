@@ -15,6 +15,7 @@ inline constexpr int kMaxGroup = 64;   // include/nss/params.hpp:39 kBmMaxGroup
 inline constexpr int kMaxRadius = 16;  // include/nss/params.hpp:44 kBmMaxRadius
 inline constexpr int kMaxSlots = 2 * kMaxRadius + 1;
 inline constexpr int kRunRefs = 8;     // references per workgroup of the run matcher
+inline constexpr int kRunGather = 128; // keys at or below a reference's threshold held for ranking
 
 // A match record: x, y, temporal slot, and the distance's bits (the float
 // SSD; non-finite distances collapse to +inf bits, so they rank equal and

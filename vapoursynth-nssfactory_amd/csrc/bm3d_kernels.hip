@@ -193,6 +193,111 @@ __device__ inline int ref_y(const Grid& g, int j, int height, int block) {
 // reference first; order src/cpu/bm/matcher.hpp:27-57 match_less).
 // ---------------------------------------------------------------------------
 
+// The value of lane (lane ^ J) within each group of 32 lanes: DPP quad
+// permutes for J = 1, 2, a bit-mode ds_swizzle for J = 4, 8, 16.
+template <int J>
+__device__ inline uint32_t xor_lane(uint32_t v) {
+    if constexpr (J == 1) {
+        return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xb1, 0xf, 0xf, false));
+    } else if constexpr (J == 2) {
+        return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4e, 0xf, 0xf, false));
+    } else {
+        return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x1f | (J << 10)));
+    }
+}
+
+// One compare-exchange stage (block K, distance J) of a bitonic sort that
+// orders each 32-lane half ascending: a lane keeps the smaller value where
+// its J bit equals its K bit (K = 32: the whole half ascends).
+// One compare-exchange stage (block K, distance J) of a bitonic sort that
+// orders each 32-lane half ascending: a lane keeps the smaller value where
+// its J bit equals its K bit (K = 32: the whole half ascends).
+template <int K, int J>
+__device__ inline unsigned long long bitonic_u64(unsigned long long v) {
+    const int lane = threadIdx.x & 31;
+    const uint32_t plo = xor_lane<J>(static_cast<uint32_t>(v));
+    const uint32_t phi = xor_lane<J>(static_cast<uint32_t>(v >> 32));
+    const unsigned long long p = (static_cast<unsigned long long>(phi) << 32) | plo;
+    const bool keep_min = ((lane & J) == 0) == (K == 32 || (lane & K) == 0);
+    return keep_min == (p < v) ? p : v;
+}
+
+// Ascending sort of the 32 keys of each half-wave (lane i of the half holds
+// the i-th smallest). All 64 lanes must call.
+__device__ inline unsigned long long sort32_u64(unsigned long long v) {
+    v = bitonic_u64<2, 1>(v);
+    v = bitonic_u64<4, 2>(v);
+    v = bitonic_u64<4, 1>(v);
+    v = bitonic_u64<8, 4>(v);
+    v = bitonic_u64<8, 2>(v);
+    v = bitonic_u64<8, 1>(v);
+    v = bitonic_u64<16, 8>(v);
+    v = bitonic_u64<16, 4>(v);
+    v = bitonic_u64<16, 2>(v);
+    v = bitonic_u64<16, 1>(v);
+    v = bitonic_u64<32, 16>(v);
+    v = bitonic_u64<32, 8>(v);
+    v = bitonic_u64<32, 4>(v);
+    v = bitonic_u64<32, 2>(v);
+    return bitonic_u64<32, 1>(v);
+}
+
+// Windows whose 2 x 7 lane tiles fit one per lane of a half-wave (a side of
+// up to 20: ranges up to 9) keep each lane's keys in registers through a
+// workgroup barrier; the window keys and the gathered keys then overlay the
+// staged region in LDS. Larger windows keep both areas.
+__host__ __device__ constexpr bool run_keys_in_regs(int range) {
+    return ((2 * range + 1 + 6) / 7) * ((2 * range + 1 + 1) / 2) <= 32;
+}
+
+// Floats of the staged region the run matcher's LDS layout reserves: the 8
+// windows' span + 6 padding columns with the pitch rounded up to 2 mod 4, by
+// the windows' height + 1 row (every workgroup's region fits).
+__host__ __device__ constexpr int run_region_floats(int block, int step, int range) {
+    return (((kRunRefs - 1) * step + 2 * range + block + 9) * (2 * range + block + 1) + 3) & ~3;
+}
+
+// SSD of the 2 x 7 candidates of lane tile (dy0, dx0) against the reference
+// patch at rp, in block-row order (one fmaf per pixel, the order every
+// matcher uses): region rows dy0 + py (cur) and dy0 + py + 1 (nxt), columns
+// dx0 .. dx0 + B + 5. Row dy0 + py + 1 serves block rows py and py + 1, so it
+// is read once; the 14 differences of a reference pixel are formed before
+// their products so that the accumulations do not wait on each other.
+template <int B>
+__device__ inline void run_tile_ssd(const float* rp, const float* base, int RW, float (&acc)[2][7]) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 7; ++j) acc[i][j] = 0.f;
+    float cur[B + 6];
+#pragma unroll
+    for (int c = 0; c < B + 6; ++c) cur[c] = base[c];
+#pragma unroll 2
+    for (int py = 0; py < B; ++py) {
+        float rrow[B], nxt[B + 6];
+#pragma unroll
+        for (int px = 0; px < B; ++px) rrow[px] = rp[py * RW + px];
+#pragma unroll
+        for (int c = 0; c < B + 6; ++c) nxt[c] = base[(py + 1) * RW + c];
+#pragma unroll
+        for (int px = 0; px < B; ++px) {
+            float d0[7], d1[7];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                d0[j] = rrow[px] - cur[j + px];
+                d1[j] = rrow[px] - nxt[j + px];
+            }
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                acc[0][j] = fmaf(d0[j], d0[j], acc[0][j]);
+                acc[1][j] = fmaf(d1[j], d1[j], acc[1][j]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < B + 6; ++c) cur[c] = nxt[c];
+    }
+}
+
 // Run matcher: references i0 .. i0+7 of grid row j share one LDS region.
 template <int B>
 __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
@@ -217,31 +322,45 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int RW = need + (6 - need % 4) % 4;
     const int RH = bottom - top + B + 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // Rows wave, wave + 4, ...: four rows by two 64-column strips per step,
-    // the eight loads in flight before their stores.
-    for (int y4 = wave; y4 < RH; y4 += 16)
-        for (int c0 = 0; c0 < RW; c0 += 128) {
-            float v[4][2];
+    // Rows wave, wave + 4, ... by 64-column strips: the row address is
+    // wave-uniform and each lane's column is clamped once per strip; cells
+    // outside the plane repeat its edge (only discarded candidates of
+    // overhanging tiles read them).
+    for (int c0 = 0; c0 < RW; c0 += 64) {
+        const int xx = c0 + lane;
+        if (xx < RW) {
+            const int gx = min(rx0 + xx, jb.width - 1);
+            int yy = wave;
+            for (; yy + 12 < RH; yy += 16) {
+                float v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const int yy = y4 + 4 * q, xx = c0 + 64 * c + lane;
-                    const int gy = top + yy, gx = rx0 + xx;
-                    v[q][c] = (yy < RH && xx < RW && gy < jb.height && gx >= 0 && gx < jb.width)
-                                  ? jb.plane[static_cast<size_t>(gy) * jb.stride + gx]
-                                  : 0.f;
+                for (int q = 0; q < 4; ++q) {
+                    const int gy = min(top + yy + 4 * q, jb.height - 1);
+                    v[q] = jb.plane[static_cast<size_t>(gy) * jb.stride + gx];
                 }
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const int yy = y4 + 4 * q, xx = c0 + 64 * c + lane;
-                    if (yy < RH && xx < RW) lds[yy * RW + xx] = v[q][c];
-                }
+                for (int q = 0; q < 4; ++q) lds[(yy + 4 * q) * RW + xx] = v[q];
+            }
+            for (; yy < RH; yy += 4) {
+                const int gy = min(top + yy, jb.height - 1);
+                lds[yy * RW + xx] = jb.plane[static_cast<size_t>(gy) * jb.stride + gx];
+            }
         }
+    }
     const int wmax = 2 * r + 1;
-    uint32_t* dist = reinterpret_cast<uint32_t*>(lds + ((RW * RH + 3) & ~3));
+    const bool regs = run_keys_in_regs(r);
+    // Window keys (wmax^2 per reference) and up to kRunGather gathered keys
+    // per reference. Keys waiting in registers: both over the region. Else
+    // the window keys after the region and the gathered keys over it (or
+    // after the window keys when the region is smaller).
+    // (The offsets follow the reserved region size, the same in every
+    // workgroup of the launch.)
+    const int region = run_region_floats(B, jb.grid.step, r);
+    uint32_t* dist = reinterpret_cast<uint32_t*>(lds + (regs ? 0 : region));
+    unsigned long long* surv =
+        regs || region * 4 < kRunRefs * kRunGather * 8
+            ? reinterpret_cast<unsigned long long*>(dist + kRunRefs * wmax * wmax)
+            : reinterpret_cast<unsigned long long*>(lds);
     __syncthreads();
 
     const int half = threadIdx.x >> 5, hl = threadIdx.x & 31;
@@ -250,57 +369,51 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int ww = right - left + 1, wh = bottom - top + 1;
     const int ncand = ww * wh;
     uint32_t* dk = dist + half * wmax * wmax;
-    if (half < nrun) {
-        const int ox = left - rx0;
-        const float* rp = lds + (cy - top) * RW + (cx - rx0);
-        const int tw = (ww + 6) / 7, th = (wh + 1) >> 1;
-        const int self = (cy - top) * ww + (cx - left);
-        for (int t = hl; t < tw * th; t += 32) {
-            const int ty = t / tw, tx = t - ty * tw;
-            const int dy0 = ty * 2, dx0 = tx * 7;
-            float acc[2][7];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 7; ++j) acc[i][j] = 0.f;
-            const float* base = lds + dy0 * RW + ox + dx0;
-            // Block row py of the reference against block row py of the 2 x 7
-            // candidates: region rows dy0 + py + i, columns dx0 .. dx0 + B + 5.
-            for (int py = 0; py < B; ++py) {
-                float rrow[B];
-#pragma unroll
-                for (int px = 0; px < B; ++px) rrow[px] = rp[py * RW + px];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    const float* rowp = base + (py + i) * RW;
-                    float v[B + 6];
-#pragma unroll
-                    for (int c = 0; c < B + 6; ++c) v[c] = rowp[c];
-#pragma unroll
-                    for (int j = 0; j < 7; ++j)
-#pragma unroll
-                        for (int px = 0; px < B; ++px) {
-                            const float d = rrow[px] - v[j + px];
-                            acc[i][j] = fmaf(d, d, acc[i][j]);
-                        }
-                }
-            }
+    const int tw = (ww + 6) / 7, th = (wh + 1) >> 1;
+    const int self = (cy - top) * ww + (cx - left);
+    const float* rp = lds + (cy - top) * RW + (cx - rx0);
+    const float* wbase = lds + (left - rx0);
+    if (regs) {
+        // one tile per lane: its keys wait in registers for the barrier,
+        // then go to the key area over the region
+        const int ty = hl / tw, tx = hl - ty * tw;
+        const int dy0 = ty * 2, dx0 = tx * 7;
+        const bool mine = half < nrun && hl < tw * th;
+        float acc[2][7];
+        if (mine) run_tile_ssd<B>(rp, wbase + dy0 * RW + dx0, RW, acc);
+        __syncthreads();
+        if (mine) {
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 7; ++j) {
-                    const int dy = dy0 + i, dx = dx0 + j;
-                    if (dy < wh && dx < ww) {
-                        const int c = dy * ww + dx;
-                        dk[c] = c == self ? kTaken : dist_key(acc[i][j]);
-                    }
+                    const int dy = dy0 + i, dx = dx0 + j, c = dy * ww + dx;
+                    if (dy < wh && dx < ww) dk[c] = c == self ? kTaken : dist_key(acc[i][j]);
                 }
         }
+    } else {
+        if (half < nrun) {
+            for (int t = hl; t < tw * th; t += 32) {
+                const int ty = t / tw, tx = t - ty * tw;
+                const int dy0 = ty * 2, dx0 = tx * 7;
+                float acc[2][7];
+                run_tile_ssd<B>(rp, wbase + dy0 * RW + dx0, RW, acc);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) {
+                        const int dy = dy0 + i, dx = dx0 + j, c = dy * ww + dx;
+                        if (dy < wh && dx < ww) dk[c] = c == self ? kTaken : dist_key(acc[i][j]);
+                    }
+            }
+        }
+        __syncthreads();  // the region is read; gathered keys may overlay it
     }
     wave_sync();
-    // Selection, wave-uniform: both half-waves run max(K) rounds (a half with
-    // fewer candidates, or no reference, idles) so the DPP reduction always
-    // sees all 64 lanes.
+    // Selection, wave-uniform: both half-waves take part (a half with fewer
+    // candidates, or no reference, idles) so that cross-lane steps see all
+    // 64 lanes. Keys order as (dist, window index); lane hl owns the keys
+    // c = hl mod 32.
     const int K = half < nrun ? min(jb.group - 1, ncand - 1) : 0;
     const int kmax = max(__builtin_amdgcn_readlane(K, 0), __builtin_amdgcn_readlane(K, 32));
     const float inv_ww = 1.f / static_cast<float>(ww);
@@ -309,6 +422,91 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     if (half < nrun && hl == 0) {
         out[0] = MatchRec{cx, cy, jb.t0, 0u};
         jb.counts[ref_id] = 1 + K;
+    }
+    // Threshold selection (K <= 31). T = the K-th smallest of the half's 32
+    // lane minima: distinct candidates, so at least K keys are <= T and the
+    // K winners all are. The keys <= T (self excluded) are gathered in LDS
+    // and each is ranked against the others; rank k is record 1 + k. When
+    // more than kRunGather keys are <= T in either half the wave takes the
+    // rounds below instead.
+    if (kmax <= 31) {
+        const int nc = K > 0 ? ncand : 0;
+        const int nc_max = max(__builtin_amdgcn_readlane(nc, 0), __builtin_amdgcn_readlane(nc, 32));
+        auto key_at = [&](int c) -> unsigned long long {
+            const uint32_t d = c < nc ? dk[c] : kTaken;
+            return d == kTaken ? ~0ull : (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(c);
+        };
+        unsigned long long lmin = ~0ull;
+        for (int c0 = 0; c0 < nc_max; c0 += 128) {
+            unsigned long long k[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) k[u] = key_at(c0 + 32 * u + hl);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) lmin = k[u] < lmin ? k[u] : lmin;
+        }
+        const unsigned long long sorted = sort32_u64(lmin);
+        const int k0 = __builtin_amdgcn_readlane(K, 0), k1 = __builtin_amdgcn_readlane(K, 32);
+        const int s0 = max(k0 - 1, 0), s1 = 32 + max(k1 - 1, 0);
+        const unsigned long long t0 =
+            (static_cast<unsigned long long>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sorted >> 32), s0))) << 32) |
+            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sorted), s0));
+        const unsigned long long t1 =
+            (static_cast<unsigned long long>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sorted >> 32), s1))) << 32) |
+            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sorted), s1));
+        // ~0 (self, no candidate) is the largest key: clamping T below it
+        // leaves it out of the gather
+        const unsigned long long T = min((threadIdx.x & 32) ? t1 : t0, ~0ull - 1);
+        const uint32_t below = (1u << hl) - 1u;  // lanes of this half below this one
+        unsigned long long* sv = surv + half * kRunGather;
+        int n = 0;  // keys gathered so far by this half
+        for (int c0 = 0; c0 < nc_max; c0 += 128) {
+            unsigned long long k[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) k[u] = key_at(c0 + 32 * u + hl);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool take = k[u] <= T;
+                const unsigned long long m = __ballot(take);
+                if (m) {
+                    const uint32_t mh = (threadIdx.x & 32) ? static_cast<uint32_t>(m >> 32) : static_cast<uint32_t>(m);
+                    const int slot = n + __popc(mh & below);
+                    if (take && slot < kRunGather) sv[slot] = k[u];
+                    n += __popc(mh);
+                }
+            }
+        }
+        const int n0 = __builtin_amdgcn_readlane(n, 0), n1 = __builtin_amdgcn_readlane(n, 32);
+        if (n0 <= kRunGather && n1 <= kRunGather) {
+            wave_sync();
+            // lane hl ranks the gathered keys hl, hl + 32, ... of its half
+            const int nmax = max(n0, n1), qn = (nmax + 31) >> 5;
+            unsigned long long own[kRunGather / 32];
+            int rank[kRunGather / 32];
+#pragma unroll
+            for (int q = 0; q < kRunGather / 32; ++q) {
+                own[q] = q < qn && hl + 32 * q < n ? sv[hl + 32 * q] : ~0ull;
+                rank[q] = 0;
+            }
+            for (int j0 = 0; j0 < nmax; j0 += 8) {
+                unsigned long long o[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) o[u] = j0 + u < n ? sv[j0 + u] : ~0ull;
+#pragma unroll
+                for (int q = 0; q < kRunGather / 32; ++q)
+                    if (q < qn) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) rank[q] += o[u] < own[q] ? 1 : 0;
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < kRunGather / 32; ++q)
+                if (hl + 32 * q < n && rank[q] < K) {
+                    const int c = static_cast<int>(static_cast<uint32_t>(own[q]));
+                    const int dy = div_small(c, inv_ww), dx = c - dy * ww;
+                    out[1 + rank[q]] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(own[q] >> 32)};
+                }
+            return;
+        }
     }
     // Each lane keeps the four smallest of its keys (window indices c = hl
     // mod 32) sorted in registers; a round's winner is the half-wave minimum
@@ -3577,11 +3775,13 @@ int match_workgroups(int kind, const Grid& g) {
 
 size_t match_run_lds_bytes(int block, int step, int range) {
     const size_t w = 2 * static_cast<size_t>(range) + 1;
-    // windows' span + 6 padding columns, rounded up to a pitch of 2 mod 4;
-    // one row below the windows
-    const size_t rw = static_cast<size_t>(kRunRefs - 1) * step + 2 * range + block + 6 + 3;
-    const size_t rh = 2 * static_cast<size_t>(range) + block + 1;
-    return (((rw * rh) + 3) & ~static_cast<size_t>(3)) * 4 + kRunRefs * w * w * 4;
+    const size_t region = static_cast<size_t>(run_region_floats(block, step, range)) * 4;
+    // keys of the 8 windows and kRunGather gathered keys per reference: both
+    // over the region when the keys wait in registers; else the window keys
+    // after the region and the gathered keys over it (after the window keys
+    // when the region is smaller)
+    const size_t keys = kRunRefs * w * w * 4, gather = kRunRefs * kRunGather * 8;
+    return run_keys_in_regs(range) ? std::max(region, keys + gather) : region + keys + (region < gather ? gather : 0);
 }
 
 size_t filter_atomic_lds_bytes(int block, int group, bool wiener) {
