@@ -259,47 +259,72 @@ __host__ __device__ constexpr int run_region_floats(int block, int step, int ran
 
 // SSD of the 2 x 7 candidates of lane tile (dy0, dx0) against the reference
 // patch at rp, in block-row order (one fmaf per pixel, the order every
-// matcher uses): region rows dy0 + py (cur) and dy0 + py + 1 (nxt), columns
-// dx0 .. dx0 + B + 5. Row dy0 + py + 1 serves block rows py and py + 1, so it
-// is read once; the 14 differences of a reference pixel are formed before
-// their products so that the accumulations do not wait on each other.
-template <int B>
+// matcher uses): region rows dy0 + py + i, columns dx0 .. dx0 + B + 5.
+// REUSE (large windows, several tiles per lane): row dy0 + py + 1 serves
+// block rows py and py + 1 and is read once, and the 14 differences of a
+// reference pixel are formed before their products; it costs registers, so
+// the small-window form (one tile per lane, 8 waves per SIMD) reads each
+// row pair per block row instead.
+template <int B, bool REUSE>
 __device__ inline void run_tile_ssd(const float* rp, const float* base, int RW, float (&acc)[2][7]) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 7; ++j) acc[i][j] = 0.f;
-    float cur[B + 6];
+    if constexpr (!REUSE) {
+        for (int py = 0; py < B; ++py) {
+            float rrow[B];
 #pragma unroll
-    for (int c = 0; c < B + 6; ++c) cur[c] = base[c];
-#pragma unroll 2
-    for (int py = 0; py < B; ++py) {
-        float rrow[B], nxt[B + 6];
+            for (int px = 0; px < B; ++px) rrow[px] = rp[py * RW + px];
 #pragma unroll
-        for (int px = 0; px < B; ++px) rrow[px] = rp[py * RW + px];
+            for (int i = 0; i < 2; ++i) {
+                const float* rowp = base + (py + i) * RW;
+                float v[B + 6];
 #pragma unroll
-        for (int c = 0; c < B + 6; ++c) nxt[c] = base[(py + 1) * RW + c];
+                for (int c = 0; c < B + 6; ++c) v[c] = rowp[c];
 #pragma unroll
-        for (int px = 0; px < B; ++px) {
-            float d0[7], d1[7];
+                for (int j = 0; j < 7; ++j)
 #pragma unroll
-            for (int j = 0; j < 7; ++j) {
-                d0[j] = rrow[px] - cur[j + px];
-                d1[j] = rrow[px] - nxt[j + px];
-            }
-#pragma unroll
-            for (int j = 0; j < 7; ++j) {
-                acc[0][j] = fmaf(d0[j], d0[j], acc[0][j]);
-                acc[1][j] = fmaf(d1[j], d1[j], acc[1][j]);
+                    for (int px = 0; px < B; ++px) {
+                        const float d = rrow[px] - v[j + px];
+                        acc[i][j] = fmaf(d, d, acc[i][j]);
+                    }
             }
         }
+    } else {
+        float cur[B + 6];
 #pragma unroll
-        for (int c = 0; c < B + 6; ++c) cur[c] = nxt[c];
+        for (int c = 0; c < B + 6; ++c) cur[c] = base[c];
+#pragma unroll 2
+        for (int py = 0; py < B; ++py) {
+            float rrow[B], nxt[B + 6];
+#pragma unroll
+            for (int px = 0; px < B; ++px) rrow[px] = rp[py * RW + px];
+#pragma unroll
+            for (int c = 0; c < B + 6; ++c) nxt[c] = base[(py + 1) * RW + c];
+#pragma unroll
+            for (int px = 0; px < B; ++px) {
+                float d0[7], d1[7];
+#pragma unroll
+                for (int j = 0; j < 7; ++j) {
+                    d0[j] = rrow[px] - cur[j + px];
+                    d1[j] = rrow[px] - nxt[j + px];
+                }
+#pragma unroll
+                for (int j = 0; j < 7; ++j) {
+                    acc[0][j] = fmaf(d0[j], d0[j], acc[0][j]);
+                    acc[1][j] = fmaf(d1[j], d1[j], acc[1][j]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < B + 6; ++c) cur[c] = nxt[c];
+        }
     }
 }
 
 // Run matcher: references i0 .. i0+7 of grid row j share one LDS region.
-template <int B>
+// SMALL: every job's window takes one lane tile per lane (run_keys_in_regs).
+template <int B, bool SMALL>
 __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     extern __shared__ __align__(16) float lds[];
     const int lb = xcd_block();
@@ -348,7 +373,7 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
         }
     }
     const int wmax = 2 * r + 1;
-    const bool regs = run_keys_in_regs(r);
+    constexpr bool regs = SMALL;
     // Window keys (wmax^2 per reference) and up to kRunGather gathered keys
     // per reference. Keys waiting in registers: both over the region. Else
     // the window keys after the region and the gathered keys over it (or
@@ -380,7 +405,7 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
         const int dy0 = ty * 2, dx0 = tx * 7;
         const bool mine = half < nrun && hl < tw * th;
         float acc[2][7];
-        if (mine) run_tile_ssd<B>(rp, wbase + dy0 * RW + dx0, RW, acc);
+        if (mine) run_tile_ssd<B, !SMALL>(rp, wbase + dy0 * RW + dx0, RW, acc);
         __syncthreads();
         if (mine) {
 #pragma unroll
@@ -397,7 +422,7 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
                 const int ty = t / tw, tx = t - ty * tw;
                 const int dy0 = ty * 2, dx0 = tx * 7;
                 float acc[2][7];
-                run_tile_ssd<B>(rp, wbase + dy0 * RW + dx0, RW, acc);
+                run_tile_ssd<B, !SMALL>(rp, wbase + dy0 * RW + dx0, RW, acc);
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -3773,6 +3798,15 @@ int match_workgroups(int kind, const Grid& g) {
     return g.nx * g.ny;
 }
 
+// LDS of the run matcher in the small-window layout (keys in registers,
+// both key areas over the region) or the other.
+static size_t run_lds_layout(int block, int step, int range, bool small) {
+    const size_t w = 2 * static_cast<size_t>(range) + 1;
+    const size_t region = static_cast<size_t>(run_region_floats(block, step, range)) * 4;
+    const size_t keys = kRunRefs * w * w * 4, gather = kRunRefs * kRunGather * 8;
+    return small ? std::max(region, keys + gather) : region + keys + (region < gather ? gather : 0);
+}
+
 size_t match_run_lds_bytes(int block, int step, int range) {
     const size_t w = 2 * static_cast<size_t>(range) + 1;
     const size_t region = static_cast<size_t>(run_region_floats(block, step, range)) * 4;
@@ -3842,8 +3876,20 @@ static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, 
         }
     } else if (kind == 2) {
         if constexpr (B <= 12) {
-            allow_lds(&match_run_kernel<B>, lds);
-            match_run_kernel<B><<<total_wg, 256, lds, s>>>(t);
+            // the small-window layout when every job's window takes it; the
+            // LDS for the layout launched
+            bool small = true;
+            for (int j = 0; j < t.n; ++j) small = small && run_keys_in_regs(t.job[j].range);
+            lds = 0;
+            for (int j = 0; j < t.n; ++j)
+                lds = std::max(lds, run_lds_layout(B, t.job[j].grid.step, t.job[j].range, small));
+            if (small) {
+                allow_lds(&match_run_kernel<B, true>, lds);
+                match_run_kernel<B, true><<<total_wg, 256, lds, s>>>(t);
+            } else {
+                allow_lds(&match_run_kernel<B, false>, lds);
+                match_run_kernel<B, false><<<total_wg, 256, lds, s>>>(t);
+            }
         } else {
             return hipErrorInvalidValue;
         }
