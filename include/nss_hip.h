@@ -199,7 +199,10 @@ int nss_hip_bm3d_process_noalloc(const nss_hip_bm3d_params* params, const nss_hi
  * none is ready. Pop after every push (a push fails with
  * NSS_HIP_RESOURCE_FAILURE while too many finished frames wait). After the
  * last frame (num_frames of the create call) every remaining frame becomes
- * ready. Use one stream for an instance's calls (or order them). */
+ * ready. Use one stream for an instance's calls (or order them). A push that
+ * fails part-way (a device error while it runs centres) leaves the rolling
+ * sums inconsistent: later pushes return NSS_HIP_RESOURCE_FAILURE, frames
+ * finished before it still pop; create a new instance to go on. */
 int nss_hip_bm3d_push(nss_hip_bm3d* handle, const float* const* src, const int* src_stride, const float* const* ref,
                       const int* ref_stride, void* stream);
 int nss_hip_bm3d_pop(nss_hip_bm3d* handle, float* const* dst, const int* dst_stride, int* frame, void* stream);
@@ -441,6 +444,20 @@ int nss_hip_server_query(nss_hip_server* server, long long request_id, double* l
 int nss_hip_server_drain(nss_hip_server* server);
 int nss_hip_server_get_stats(nss_hip_server* server, nss_hip_server_stats* out);
 int nss_hip_server_destroy(nss_hip_server* server);
+
+/* Error-path hooks for tests. No counterpart in the reference's C++ API; they
+ * stand in for the fake allocator and launch-error hook its tests put behind
+ * their device layer, so that every caller's clean-up and error report can be
+ * checked on a healthy device. Process-wide; not for production use.
+ * nss_hip_debug_fail_alloc(n): the n-th next device allocation the library
+ * makes (n = 0: the next one) fails with hipErrorOutOfMemory; n < 0 disarms.
+ * nss_hip_debug_fail_launch(n): the n-th next kernel launch is refused
+ * (hipErrorLaunchFailure, nothing launched); n < 0 disarms.
+ * nss_hip_debug_live_allocations: the device allocations the library holds
+ * now (count, bytes). */
+int nss_hip_debug_fail_alloc(int n);
+int nss_hip_debug_fail_launch(int n);
+int nss_hip_debug_live_allocations(long long* count, long long* bytes);
 
 #ifdef __cplusplus
 }

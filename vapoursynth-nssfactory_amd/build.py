@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnss_hip.so")
 ARCH = os.environ.get("NSS_HIP_ARCH", "gfx950")
 
-SOURCES = ["bm3d_kernels.hip", "driver.cpp", "bm3d_args.cpp", "server.cpp", "wnnm.cpp", "noalloc.hip"]
+SOURCES = ["bm3d_kernels.hip", "driver.cpp", "bm3d_args.cpp", "server.cpp", "wnnm.cpp", "noalloc.hip", "device_mem.cpp"]
 
 HIPFLAGS = [
     f"--offload-arch={ARCH}",

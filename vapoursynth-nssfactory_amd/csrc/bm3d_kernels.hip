@@ -37,6 +37,10 @@
 
 namespace nsship {
 
+// True when an armed test hook refuses this launch (device_mem.cpp); the
+// launch wrappers then return hipErrorLaunchFailure without launching.
+bool launch_fault();
+
 // Orthonormal DCT-II matrices (row = frequency) for B in {2,4,8,12,16,32},
 // concatenated; filled from the host (double cosines rounded to float, as
 // src/cpu/bm/dct8.cpp:71-89 DctTable).
@@ -3367,6 +3371,7 @@ __global__ __launch_bounds__(256) void vaggregate_kernel(VAggJob jb) {
 }
 
 hipError_t launch_vaggregate(const VAggJob& jb, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     const dim3 grid((jb.width + 63) / 64, (jb.height + 3) / 4);
     vaggregate_kernel<<<grid, 256, 0, s>>>(jb);
     return hipGetLastError();
@@ -3922,6 +3927,7 @@ static hipError_t launch_match_wave(const MatchTable& t, int total_wg, size_t ld
 // kind: 3 MFMA matcher (B = 8), 2 run matcher, 1 wave matcher with the
 // window in LDS, 0 from global.
 hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     switch (block) {
         case 3: return launch_match_wave<3>(t, total_wg, lds, kind, s);
         case 5: return launch_match_wave<5>(t, total_wg, lds, kind, s);
@@ -3998,6 +4004,7 @@ static hipError_t launch_filter_mfma(const FilterTable& t, int total_wg, size_t 
 // form 1 (bf16x3) or 2 (fp16) with fast_group 8 or 16 the MFMA kernel.
 hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
                          int fast_group, int form, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     if (tiles && block == 8 && form) {
         if (fast_group == 8) return form == 1 ? launch_filter_mfma<8, kFormBf16x3>(t, total_wg, lds, wiener, s)
                                               : launch_filter_mfma<8, kFormF16>(t, total_wg, lds, wiener, s);
@@ -4026,11 +4033,13 @@ hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t l
 }
 
 hipError_t launch_finish_sums(const FinishTable& t, int total_wg, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     finish_sums_kernel<<<total_wg, 256, 0, s>>>(t);
     return hipGetLastError();
 }
 
 hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     if (tiles) {
         if (t.n && t.job[0].exact_num)
             finish_tiles_kernel<true><<<total_wg, 256, 0, s>>>(t);
@@ -4062,6 +4071,7 @@ hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool til
 }
 
 hipError_t launch_zero(const ZeroTable& t, int total_wg, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     zero_kernel<<<total_wg, 256, 0, s>>>(t);
     return hipGetLastError();
 }
@@ -4089,6 +4099,7 @@ size_t predictive_table_lds(int block, PredTable& t, int radius) {
 }
 
 hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t lds, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     switch (block) {
         case 1: allow_lds(&predictive_kernel<1>, lds); predictive_kernel<1><<<total_wg, 64, lds, s>>>(t); break;
         case 2: allow_lds(&predictive_kernel<2>, lds); predictive_kernel<2><<<total_wg, 64, lds, s>>>(t); break;
@@ -4156,6 +4167,7 @@ static hipError_t launch_ttile(const TTileTable& t, int tiles, size_t lds, bool 
 
 hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int total_wg, size_t lds, bool wiener,
                                         hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     switch (group) {
         case 8: return launch_ttile<8>(t, total_wg, lds, wiener, s);
         case 16: return launch_ttile<16>(t, total_wg, lds, wiener, s);
@@ -4165,6 +4177,7 @@ hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int tota
 }
 
 hipError_t launch_finish_temporal_tiles(const TFinishTable& t, int total_wg, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     finish_temporal_tiles_kernel<<<total_wg, 256, 0, s>>>(t);
     return hipGetLastError();
 }
@@ -4184,6 +4197,7 @@ static hipError_t launch_tfilter_b8(const TFilterTable& t, int total_wg, size_t 
 // fast_group 8, 16 or 32 (block 8): four groups per workgroup, register cube.
 hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
                                   int fast_group, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     if (block == 8 && fast_group) {
         switch (fast_group) {
             case 8: return launch_tfilter_b8<8>(t, total_wg, lds, wiener, s);
@@ -4213,6 +4227,7 @@ size_t wnnm_lds_bytes(int block, int group) {
 
 hipError_t launch_regularize(float* est, int est_stride, const float* noisy, int noisy_stride, int width, int height,
                              float delta, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     const dim3 grid((width + 63) / 64, (height + 3) / 4);
     regularize_kernel<<<grid, 256, 0, s>>>(est, est_stride, noisy, noisy_stride, width, height, delta);
     return hipGetLastError();
@@ -4231,6 +4246,7 @@ static hipError_t launch_gram_groups(const WnnmTable& t, int total_wg, size_t ld
 }
 
 hipError_t launch_wnnm(const WnnmTable& t, int total_wg, bool ncsr, hipStream_t s) {
+    if (launch_fault()) return hipErrorLaunchFailure;
     int gmax = 0, bmax = 0;
     for (int j = 0; j < t.n; ++j) {
         if (t.job[j].group < 1 || t.job[j].group > 32 || t.job[j].block < 1 || t.job[j].block > 16)

@@ -260,7 +260,7 @@ void free_buffers(nss_hip_bm3d* h) {
                         static_cast<void*>(R.out[i]), static_cast<void*>(R.num[0][i]), static_cast<void*>(R.den[0][i]),
                         static_cast<void*>(R.num[1][i]), static_cast<void*>(R.den[1][i]), static_cast<void*>(R.part[0][i]),
                         static_cast<void*>(R.part[1][i])})
-            if (p) (void)hipFree(p);
+            if (p) dev_free(p);
         R.src[i] = R.ref[i] = R.basic[i] = R.out[i] = nullptr;
         R.num[0][i] = R.den[0][i] = R.num[1][i] = R.den[1][i] = nullptr;
         R.part[0][i] = R.part[1][i] = nullptr;
@@ -379,9 +379,9 @@ void free_frame_buffers(FrameBuffers& fb) {
         for (void* p : {static_cast<void*>(fb.num[i]), static_cast<void*>(fb.den[i]), static_cast<void*>(fb.part[i]),
                         static_cast<void*>(fb.matches[i]), static_cast<void*>(fb.counts[i]),
                         static_cast<void*>(fb.basic[i])})
-            if (p) (void)hipFree(p);
+            if (p) dev_free(p);
     }
-    if (fb.cube) (void)hipFree(fb.cube);
+    if (fb.cube) dev_free(fb.cube);
     fb = FrameBuffers{};
 }
 
@@ -392,17 +392,17 @@ hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb) {
         const PlaneNeeds n = plane_needs(h, i);
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
         if (n.recs) {
-            e = hipMalloc(&fb.matches[i], n.recs * sizeof(MatchRec));
-            if (e == hipSuccess) e = hipMalloc(&fb.counts[i], n.nref * sizeof(int));
+            e = dev_alloc(&fb.matches[i], n.recs * sizeof(MatchRec));
+            if (e == hipSuccess) e = dev_alloc(&fb.counts[i], n.nref * sizeof(int));
         }
         if (e == hipSuccess && n.atomic) {
-            e = hipMalloc(&fb.num[i], npx * 8);
-            if (e == hipSuccess) e = hipMalloc(&fb.den[i], npx * 8);
+            e = dev_alloc(&fb.num[i], npx * 8);
+            if (e == hipSuccess) e = dev_alloc(&fb.den[i], npx * 8);
         }
-        if (e == hipSuccess && h->part_elems[i]) e = hipMalloc(&fb.part[i], 2 * h->part_elems[i] * sizeof(unsigned long long));
-        if (e == hipSuccess && n.basic) e = hipMalloc(&fb.basic[i], npx * 4);
+        if (e == hipSuccess && h->part_elems[i]) e = dev_alloc(&fb.part[i], 2 * h->part_elems[i] * sizeof(unsigned long long));
+        if (e == hipSuccess && n.basic) e = dev_alloc(&fb.basic[i], npx * 4);
     }
-    if (e == hipSuccess && h->cube_floats) e = hipMalloc(&fb.cube, cube_bytes(h));
+    if (e == hipSuccess && h->cube_floats) e = dev_alloc(&fb.cube, cube_bytes(h));
     if (e != hipSuccess) free_frame_buffers(fb);
     return e;
 }
@@ -1606,15 +1606,29 @@ int grow_partials(nss_hip_bm3d* h, const size_t* need) {
     for (auto& sl : h->slots)
         if (e == hipSuccess && sl.used) e = hipEventSynchronize(sl.done);
     if (e != hipSuccess) return hip_fail(e, "synchronize");
+    // All new buffers first; the old ones (and the plan that uses them) stay
+    // when any allocation fails.
+    std::vector<unsigned long long*> fresh;
+    for (int i = 0; i < h->nplanes && e == hipSuccess; ++i) {
+        if (need[i] <= h->part_elems[i]) continue;
+        for (size_t k = 0; k < h->slots.size() && e == hipSuccess; ++k) {
+            unsigned long long* p = nullptr;
+            e = dev_alloc(&p, 2 * need[i] * sizeof(unsigned long long));
+            if (e == hipSuccess) fresh.push_back(p);
+        }
+    }
+    if (e != hipSuccess) {
+        for (auto* p : fresh) dev_free(p);
+        return hip_fail(e, "device allocation");
+    }
+    size_t next = 0;
     for (int i = 0; i < h->nplanes; ++i) {
         if (need[i] <= h->part_elems[i]) continue;
-        h->part_elems[i] = need[i];
         for (auto& sl : h->slots) {
-            if (sl.buf.part[i]) (void)hipFree(sl.buf.part[i]);
-            sl.buf.part[i] = nullptr;
-            e = hipMalloc(&sl.buf.part[i], 2 * need[i] * sizeof(unsigned long long));
-            if (e != hipSuccess) return hip_fail(e, "device allocation");
+            dev_free(sl.buf.part[i]);
+            sl.buf.part[i] = fresh[next++];
         }
+        h->part_elems[i] = need[i];
     }
     return NSS_HIP_OK;
 }
@@ -1777,18 +1791,18 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
         hipError_t e = hipEventCreateWithFlags(&h->roll_done, hipEventDisableTiming);
         for (int i = 0; i < h->nplanes && e == hipSuccess; ++i) {
             const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-            e = hipMalloc(&R.src[i], npx * 4 * R.src_n);
-            if (e == hipSuccess && R.ref_n) e = hipMalloc(&R.ref[i], npx * 4 * R.ref_n);
-            if (e == hipSuccess && R.basic_n) e = hipMalloc(&R.basic[i], npx * 4 * R.basic_n);
-            if (e == hipSuccess) e = hipMalloc(&R.out[i], npx * 4 * R.out_n * R.out_rows);
+            e = dev_alloc(&R.src[i], npx * 4 * R.src_n);
+            if (e == hipSuccess && R.ref_n) e = dev_alloc(&R.ref[i], npx * 4 * R.ref_n);
+            if (e == hipSuccess && R.basic_n) e = dev_alloc(&R.basic[i], npx * 4 * R.basic_n);
+            if (e == hipSuccess) e = dev_alloc(&R.out[i], npx * 4 * R.out_n * R.out_rows);
             for (int s = 0; s < h->nstages && e == hipSuccess; ++s) {
                 if (h->stages[s].plane[i].sigma == 0.f) continue;
                 if (R.tiles[s]) {
-                    e = hipMalloc(&R.part[s][i], R.part_center[s][i] * (2 * R.r + 1) * sizeof(float));
+                    e = dev_alloc(&R.part[s][i], R.part_center[s][i] * (2 * R.r + 1) * sizeof(float));
                     continue;
                 }
-                e = hipMalloc(&R.num[s][i], npx * 8 * R.acc_n);
-                if (e == hipSuccess) e = hipMalloc(&R.den[s][i], npx * 8 * R.acc_n);
+                e = dev_alloc(&R.num[s][i], npx * 8 * R.acc_n);
+                if (e == hipSuccess) e = dev_alloc(&R.den[s][i], npx * 8 * R.acc_n);
                 if (e == hipSuccess) e = hipMemset(R.num[s][i], 0, npx * 8 * R.acc_n);
                 if (e == hipSuccess) e = hipMemset(R.den[s][i], 0, npx * 8 * R.acc_n);
             }
@@ -1836,6 +1850,10 @@ int nss_hip_bm3d_push(nss_hip_bm3d* h, const float* const* src, const int* src_s
     hipStream_t s = static_cast<hipStream_t>(stream);
     RollTurn turn(h, s);
     if (turn.err != hipSuccess) return hip_fail(turn.err, "stream wait");
+    if (R.failed)
+        return fail(NSS_HIP_RESOURCE_FAILURE,
+                    "nss_hip: an earlier push failed part-way; the instance's rolling state is not consistent "
+                    "(create a new instance)");
     if (R.pushed >= R.N) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: every frame of the clip was already pushed");
     const int last = h->nstages - 1;
     const int pending = R.finished[last] - R.out_head;
@@ -1855,7 +1873,9 @@ int nss_hip_bm3d_push(nss_hip_bm3d* h, const float* const* src, const int* src_s
         if (e != hipSuccess) return hip_fail(e, "frame upload");
     }
     R.pushed = f + 1;
-    return temporal_advance(h, s);
+    const int rc = temporal_advance(h, s);
+    if (rc != NSS_HIP_OK) R.failed = true;
+    return rc;
 }
 
 int nss_hip_bm3d_pop(nss_hip_bm3d* h, float* const* dst, const int* dst_stride, int* frame, void* stream) {
@@ -1978,8 +1998,8 @@ int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* be
             host[o] = 0.5f + 0.25f * std::sin(0.05f * x) * std::cos(0.07f * y) + noise;
         }
         stride[i] = h->pw[i];
-        e = hipMalloc(&src[i], npx * 4);
-        if (e == hipSuccess) e = hipMalloc(&dst[i], npx * 4);
+        e = dev_alloc(&src[i], npx * 4);
+        if (e == hipSuccess) e = dev_alloc(&dst[i], npx * 4);
         if (e == hipSuccess) e = hipMemcpy(src[i], host.data(), npx * 4, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = hip_fail(e, "autotune buffers");
     }
@@ -2028,8 +2048,8 @@ int nss_hip_bm3d_autotune(nss_hip_bm3d* h, int repeats, void* stream, double* be
         if (x) (void)hipEventDestroy(x);
     (void)hipStreamSynchronize(s);
     for (int i = 0; i < 3; ++i) {
-        if (src[i]) (void)hipFree(src[i]);
-        if (dst[i]) (void)hipFree(dst[i]);
+        if (src[i]) dev_free(src[i]);
+        if (dst[i]) dev_free(dst[i]);
     }
     if (rc == NSS_HIP_OK && best_ms) *best_ms = best;
     return rc;

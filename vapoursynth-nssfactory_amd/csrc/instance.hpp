@@ -17,6 +17,22 @@
 
 namespace nsship {
 
+// Device memory of the library (device_mem.cpp): hipMalloc / hipFree with a
+// count of the live allocations; an armed test hook makes the n-th next
+// allocation fail with hipErrorOutOfMemory.
+hipError_t dev_alloc(void** p, size_t bytes);
+template <typename T>
+hipError_t dev_alloc(T** p, size_t bytes) {
+    void* v = nullptr;
+    const hipError_t e = dev_alloc(&v, bytes);
+    *p = static_cast<T*>(v);
+    return e;
+}
+void dev_free(void* p);
+// True when an armed test hook refuses this kernel launch (the launch
+// wrappers then return hipErrorLaunchFailure without launching).
+bool launch_fault();
+
 hipError_t launch_match(int block, const MatchTable& t, int total_wg, size_t lds, int kind, hipStream_t s);
 hipError_t launch_filter(int block, const FilterTable& t, int total_wg, size_t lds, bool wiener, bool tiles,
                          int fast_group, int form, hipStream_t s);
@@ -220,6 +236,10 @@ struct nss_hip_bm3d {
         int next_center[2]{};
         int finished[2]{};
         int out_head = 0;
+        // A push failed part-way: a centre's contributions may be in the sums
+        // already, so no later push is served (frames finished before it
+        // still pop).
+        bool failed = false;
         // Legacy temporal output: the last stage emits, per centre, its 2r+1
         // num/den slice pairs (out_rows = 2 (2r + 1) plane heights per output
         // frame) instead of finished frames (out_rows = 1).

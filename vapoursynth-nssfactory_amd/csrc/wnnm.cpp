@@ -45,11 +45,11 @@ namespace {
 
 void release(nss_hip_wnnm* h) {
     for (auto& q : h->pl) {
-        if (q.recs) (void)hipFree(q.recs);
-        if (q.counts) (void)hipFree(q.counts);
-        if (q.num) (void)hipFree(q.num);
-        if (q.den) (void)hipFree(q.den);
-        if (q.est) (void)hipFree(q.est);
+        if (q.recs) dev_free(q.recs);
+        if (q.counts) dev_free(q.counts);
+        if (q.num) dev_free(q.num);
+        if (q.den) dev_free(q.den);
+        if (q.est) dev_free(q.est);
         q.est = nullptr;
         q.recs = nullptr;
         q.counts = nullptr;
@@ -94,9 +94,9 @@ int init_instance(nss_hip_wnnm* h, const nss_hip_wnnm_params& p, const nss_hip_v
         const size_t sum_bytes = pixels * sizeof(unsigned long long);
         const size_t est_bytes = h->ncsr ? pixels * sizeof(float) : 0;
         h->bytes += static_cast<long long>(rec_bytes + cnt_bytes + 2 * sum_bytes + est_bytes);
-        if ((e = hipMalloc(&q.recs, rec_bytes)) != hipSuccess || (e = hipMalloc(&q.counts, cnt_bytes)) != hipSuccess ||
-            (e = hipMalloc(&q.num, sum_bytes)) != hipSuccess || (e = hipMalloc(&q.den, sum_bytes)) != hipSuccess ||
-            (h->ncsr && (e = hipMalloc(&q.est, est_bytes)) != hipSuccess)) {
+        if ((e = dev_alloc(&q.recs, rec_bytes)) != hipSuccess || (e = dev_alloc(&q.counts, cnt_bytes)) != hipSuccess ||
+            (e = dev_alloc(&q.num, sum_bytes)) != hipSuccess || (e = dev_alloc(&q.den, sum_bytes)) != hipSuccess ||
+            (h->ncsr && (e = dev_alloc(&q.est, est_bytes)) != hipSuccess)) {
             release(h);
             return hip_fail(e, "group filter buffers");
         }

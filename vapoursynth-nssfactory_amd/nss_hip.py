@@ -126,7 +126,8 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_bm3d_autotune", "nss_hip_vaggregate", "nss_hip_bm3d_resources", "nss_hip_wnnm_parse",
            "nss_hip_wnnm_create", "nss_hip_wnnm_process", "nss_hip_wnnm_destroy", "nss_hip_ncsr_parse",
            "nss_hip_ncsr_create", "nss_hip_ncsr_process", "nss_hip_ncsr_destroy", "nss_hip_bm3d_set_tiles",
-           "nss_hip_predictive_match", "nss_hip_bm3d_process_noalloc")
+           "nss_hip_predictive_match", "nss_hip_bm3d_process_noalloc", "nss_hip_debug_fail_alloc",
+           "nss_hip_debug_fail_launch", "nss_hip_debug_live_allocations")
 # Contribution identity of legacy temporal frames (frame properties the
 # VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
 # src/frontend/contribution.hpp:12-27).
@@ -194,6 +195,9 @@ def lib():
         L.nss_hip_ncsr_create.argtypes = [P(NcsrParams), P(VideoInfo), ci, P(vp)]
         L.nss_hip_ncsr_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_ncsr_destroy.argtypes = [vp]
+        L.nss_hip_debug_fail_alloc.argtypes = [ci]
+        L.nss_hip_debug_fail_launch.argtypes = [ci]
+        L.nss_hip_debug_live_allocations.argtypes = [P(ctypes.c_longlong), P(ctypes.c_longlong)]
         for name in EXPORTS[1:]:
             getattr(L, name).restype = ci
         _lib = L
@@ -202,6 +206,24 @@ def lib():
 
 def last_error() -> str:
     return lib().nss_hip_last_error().decode()
+
+
+def debug_fail_alloc(n: int) -> None:
+    """Test hook: the n-th next device allocation of the library fails
+    (0: the next one); n < 0 disarms."""
+    _check(lib().nss_hip_debug_fail_alloc(int(n)))
+
+
+def debug_fail_launch(n: int) -> None:
+    """Test hook: the n-th next kernel launch is refused; n < 0 disarms."""
+    _check(lib().nss_hip_debug_fail_launch(int(n)))
+
+
+def debug_live_allocations():
+    """(count, bytes) of the device allocations the library holds now."""
+    c, b = ctypes.c_longlong(), ctypes.c_longlong()
+    _check(lib().nss_hip_debug_live_allocations(ctypes.byref(c), ctypes.byref(b)))
+    return c.value, b.value
 
 
 def _check(rc):
