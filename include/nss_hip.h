@@ -207,6 +207,24 @@ int nss_hip_bm3d_push(nss_hip_bm3d* handle, const float* const* src, const int* 
                       const int* ref_stride, void* stream);
 int nss_hip_bm3d_pop(nss_hip_bm3d* handle, float* const* dst, const int* dst_stride, int* frame, void* stream);
 
+/* Random access to a temporal clip (the reference's rollingGetFrame over
+ * chunks, src/host/filter_bm3d.cpp:985-1072; rolling_chunk_start :548-550).
+ * nss_hip_bm3d_chunk: frame n's chunk [start, start + count) of rolling_chunk
+ * frames, and the source frames [first, last] its computation reads (every
+ * centre whose groups reach the chunk's frames, in every stage: the
+ * reference's dependency window). nss_hip_bm3d_seek: makes frame n's chunk
+ * the next one pop returns. When it is the next chunk in order nothing is
+ * reset (the state carries on, as the reference's carry does); otherwise the
+ * instance restarts at the chunk's first source frame. *next_push = the frame
+ * to push next: push frames from it through `last` (popping as push asks) and
+ * pop until frame start + count - 1 has come out; frames behind the chunk may
+ * come out too, in order. The output of a frame does not depend on the order
+ * in which chunks are asked for. A seek also ends the failed state of an
+ * earlier push. The glue keeps finished chunks (the reference's
+ * rolling_cache_chunks / rolling_cache_limit LRU) as output frames. */
+int nss_hip_bm3d_chunk(nss_hip_bm3d* handle, int n, int* start, int* count, int* first, int* last);
+int nss_hip_bm3d_seek(nss_hip_bm3d* handle, int n, int* next_push, void* stream);
+
 /* Legacy temporal output (radius > 0, temporal_mode "legacy"): push as
  * above; pop returns, per centre frame c in order, its contribution frame
  * (src/host/filter_bm3d.cpp:322-334): per plane 2r+1 slice pairs of plane

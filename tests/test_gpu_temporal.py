@@ -308,3 +308,42 @@ def test_legacy_with_ref_and_identity(gpu, oracle):
                 continue
             peak = float(np.abs(b).max())
             assert oracle.psnr(a / peak, b / peak) >= MIN_PSNR, (c, s)
+
+
+@pytest.mark.parametrize("r,final,chunk,legacy", [(1, 0, 2, False), (1, 1, 3, False), (2, 1, 2, False),
+                                                  (2, 0, 1, False), (1, 1, 2, True), (2, 0, 3, True)])
+def test_random_access_chunks_equal_the_sequential_clip(gpu, r, final, chunk, legacy):
+    """src/host/filter_bm3d.cpp:985-1072 rollingGetFrame: a frame asked for
+    out of order is computed with its chunk of rolling_chunk frames from the
+    chunk's dependency window; the output does not depend on the order. Chunks
+    asked for shuffled, again, and in order (the carried state) are bitwise
+    the sequential clip's frames."""
+    import random
+    import synth
+    n_frames = 11
+    _, seq = synth.shifted_sequence("GRAYS", 64, 48, n_frames, 10, seed=50 + r + 3 * final + chunk)
+    frames = [[_dev(p) for p in f] for f in seq]
+    kw = dict(sigma=10, final=final, radius=r, rolling_chunk=chunk)
+    if legacy:
+        kw["temporal_mode"] = "legacy"
+    inst = gpu.BM3D("GRAYS", 64, 48, frames=n_frames, **kw)
+    want = [_host(f) for f in inst.filter_clip(frames)]
+    inst.close()
+    assert len(want) == n_frames
+    inst = gpu.BM3D("GRAYS", 64, 48, frames=n_frames, **kw)
+    margin = r * ((1 if legacy else 2) + (2 if final else 0))
+    if legacy:
+        chunk = 1  # legacy output takes no rolling arguments: one centre per request
+    starts = list(range(0, n_frames, chunk))
+    order = starts[:]
+    random.Random(7).shuffle(order)
+    order += [order[0], starts[0], starts[min(1, len(starts) - 1)]]  # a repeat, then two chunks in order
+    for n in order:
+        start, count, first, last = inst.chunk(n)
+        assert (start, count) == (n, min(chunk, n_frames - n))
+        assert (first, last) == (max(0, n - margin), min(n_frames - 1, n + count - 1 + margin))
+        got = inst.get_chunk(n, lambda i: frames[i])
+        for i in range(start, start + count):
+            for g, w_ in zip(_host(got[i]), want[i]):
+                assert np.array_equal(g, w_), (n, i)
+    inst.close()

@@ -1878,6 +1878,79 @@ int nss_hip_bm3d_push(nss_hip_bm3d* h, const float* const* src, const int* src_s
     return rc;
 }
 
+// Random access (rollingGetFrame, src/host/filter_bm3d.cpp:985-1072): frame
+// n's chunk of rolling_chunk frames (rolling_chunk_start, :548-550) and the
+// source frames its centres read. Rolling output: last-stage centres
+// [start - r, end + r], which read frames r further; a first stage adds the
+// basic frames those read, 2r further again. Legacy output: the chunk's own
+// centres.
+static void chunk_window(const nss_hip_bm3d* h, int n, int* start, int* count, int* first, int* last,
+                         int* centre0) {
+    const auto& R = h->roll;
+    const int chunk = std::max(1, h->p.rolling_chunk);
+    const int st = (n / chunk) * chunk, cnt = std::min(chunk, R.N - st);
+    const int two = h->nstages == 2 ? 1 : 0;
+    const int margin = R.r * ((R.legacy ? 1 : 2) + 2 * two);
+    *start = st;
+    *count = cnt;
+    *first = std::max(0, st - margin);
+    *last = std::min(R.N - 1, st + cnt - 1 + margin);
+    // first centre of each stage (last stage, then the basic one)
+    const int c_last = R.legacy ? st : st - R.r;
+    centre0[h->nstages - 1] = std::max(0, c_last);
+    if (two) centre0[0] = std::max(0, c_last - 2 * R.r);
+}
+
+int nss_hip_bm3d_chunk(nss_hip_bm3d* h, int n, int* start, int* count, int* first, int* last) {
+    if (!h || !start || !count || !first || !last) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    if (h->p.radius == 0) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: chunks serve radius > 0");
+    if (n < 0 || n >= h->roll.N) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: frame index outside the clip");
+    int c0[2];
+    chunk_window(h, n, start, count, first, last, c0);
+    return NSS_HIP_OK;
+}
+
+int nss_hip_bm3d_seek(nss_hip_bm3d* h, int n, int* next_push, void* stream) {
+    if (!h || !next_push) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
+    auto& R = h->roll;
+    if (h->p.radius == 0) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: seek serves radius > 0");
+    if (n < 0 || n >= R.N) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: frame index outside the clip");
+    int start, count, first, last, c0[2];
+    chunk_window(h, n, &start, &count, &first, &last, c0);
+    DeviceGuard guard(h->device);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "set device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    RollTurn turn(h, s);
+    if (turn.err != hipSuccess) return hip_fail(turn.err, "stream wait");
+    // The next chunk in order carries on from the state as it is.
+    if (!R.failed && start == R.out_head && R.pushed >= first) {
+        *next_push = R.pushed;
+        return NSS_HIP_OK;
+    }
+    // Else restart at the chunk's first source frame: every centre from the
+    // stage's first one runs again. Frames before a stage's first complete
+    // one (first centre + r) are still finished, so that their sums are
+    // cleared before the ring reuses them, but nothing reads them.
+    for (int st = 0; st < h->nstages; ++st)
+        for (int i = 0; i < h->nplanes; ++i) {
+            const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
+            for (unsigned long long* p : {R.num[st][i], R.den[st][i]})
+                if (p) {
+                    const hipError_t e = hipMemsetAsync(p, 0, npx * 8 * R.acc_n, s);
+                    if (e != hipSuccess) return hip_fail(e, "clear");
+                }
+        }
+    R.pushed = first;
+    for (int st = 0; st < h->nstages; ++st) {
+        R.next_center[st] = c0[st];
+        R.finished[st] = c0[st];
+    }
+    R.out_head = start;
+    R.failed = false;
+    *next_push = first;
+    return NSS_HIP_OK;
+}
+
 int nss_hip_bm3d_pop(nss_hip_bm3d* h, float* const* dst, const int* dst_stride, int* frame, void* stream) {
     if (!h || !dst || !dst_stride) return fail(NSS_HIP_INVALID_INPUT, "nss_hip: null argument");
     auto& R = h->roll;

@@ -127,7 +127,7 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_wnnm_create", "nss_hip_wnnm_process", "nss_hip_wnnm_destroy", "nss_hip_ncsr_parse",
            "nss_hip_ncsr_create", "nss_hip_ncsr_process", "nss_hip_ncsr_destroy", "nss_hip_bm3d_set_tiles",
            "nss_hip_predictive_match", "nss_hip_bm3d_process_noalloc", "nss_hip_debug_fail_alloc",
-           "nss_hip_debug_fail_launch", "nss_hip_debug_live_allocations")
+           "nss_hip_debug_fail_launch", "nss_hip_debug_live_allocations", "nss_hip_bm3d_chunk", "nss_hip_bm3d_seek")
 # Contribution identity of legacy temporal frames (frame properties the
 # VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
 # src/frontend/contribution.hpp:12-27).
@@ -195,6 +195,8 @@ def lib():
         L.nss_hip_ncsr_create.argtypes = [P(NcsrParams), P(VideoInfo), ci, P(vp)]
         L.nss_hip_ncsr_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_ncsr_destroy.argtypes = [vp]
+        L.nss_hip_bm3d_chunk.argtypes = [vp, ci, P(ci), P(ci), P(ci), P(ci)]
+        L.nss_hip_bm3d_seek.argtypes = [vp, ci, P(ci), vp]
         L.nss_hip_debug_fail_alloc.argtypes = [ci]
         L.nss_hip_debug_fail_launch.argtypes = [ci]
         L.nss_hip_debug_live_allocations.argtypes = [P(ctypes.c_longlong), P(ctypes.c_longlong)]
@@ -442,6 +444,44 @@ class BM3D:
             return None
         _check(rc)
         return frame.value, out
+
+    def chunk(self, n):
+        """radius > 0: (start, count, first, last) of frame n's chunk and the
+        source frames its computation reads."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().nss_hip_bm3d_chunk(self._h, int(n), *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def seek(self, n, stream=None):
+        """radius > 0: makes frame n's chunk the next one popped; returns the
+        frame to push next."""
+        nxt = ctypes.c_int(-1)
+        _check(lib().nss_hip_bm3d_seek(self._h, int(n), ctypes.byref(nxt), _stream_handle(stream)))
+        return nxt.value
+
+    def get_chunk(self, n, source, ref_source=None, stream=None):
+        """radius > 0, random access: the finished frames of frame n's chunk
+        as {index: planes}, pushing source(i) (and ref_source(i)) for the
+        frames the chunk reads (what a glue's getFrame does on a chunk-cache
+        miss). Frames behind the chunk that come out are included."""
+        start, count, first, last = self.chunk(n)
+        out = {}
+
+        def drain():
+            while True:
+                r = self.pop(stream=stream)
+                if r is None:
+                    return
+                out[r[0]] = r[1]
+
+        for i in range(self.seek(n, stream=stream), last + 1):
+            self.push(source(i), None if ref_source is None else ref_source(i), stream=stream)
+            drain()
+        drain()
+        missing = [i for i in range(start, start + count) if i not in out]
+        if missing:
+            raise Error(f"{NAMESPACE}.BM3D: chunk frames {missing} did not come out")
+        return out
 
     def filter_clip(self, frames, refs=None, stream=None):
         """radius > 0: every frame of a clip (list of per-frame plane lists)
