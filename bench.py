@@ -473,8 +473,10 @@ def wnnm_workload(args):
     (block 8, step 8, group 8, range 7; include/nss/params.hpp:61-75), and a
     denser form (step 4, group 16, range 9, residual 1) beside it, and NCSR
     at its defaults (the same shape, 2 rounds, delta 0.1;
-    include/nss/params.hpp:101-107). The oracles (float64 SVD, one thread)
-    run a 640x360 crop as the CPU baseline and the parity check."""
+    include/nss/params.hpp:101-107), and both at radius 1: one centre
+    frame's contribution per step. The oracles (float64 SVD, one thread) run
+    a 640x360 crop as the CPU baseline and the parity check of the spatial
+    forms."""
     import numpy as np
     import torch
 
@@ -524,6 +526,37 @@ def wnnm_workload(args):
                          "run_to_run_identical": same,
                          "psnr_vs_clean": round(oracle.psnr(got, clean[0]), 3)}
         print(f"# wnnm {name}: {results[name]['ms_per_step']} ms/frame", file=sys.stderr, flush=True)
+    # Temporal forms (radius 1): one centre frame's contribution (every slot's
+    # num / den) per step, centres cycling inside a 5-frame shifted clip.
+    _, seq = synth.shifted_sequence("GRAYS", w, h, 5, 10, seed=1)
+    clip = [torch.from_numpy(f[0]).cuda() for f in seq]
+    for name, cls in (("wnnm_temporal_r1", nss_hip.WNNM), ("ncsr_temporal_r1", nss_hip.NCSR)):
+        inst = cls("GRAYS", w, h, frames=5, sigma=[10.0], radius=1)
+        windows = [[[clip[f]] for f in inst.slot_frames(c)] for c in (1, 2, 3)]
+        out = [torch.empty((6 * h, w), dtype=torch.float32, device="cuda")]
+        for i in range(args.warmup):
+            inst.contribution(1 + i % 3, windows[i % 3], out=out)
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            starts[i].record(stream)
+            inst.contribution(1 + i % 3, windows[i % 3], out=out)
+            ends[i].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
+        first = inst.contribution(2, windows[1])[0].cpu().numpy()
+        again = inst.contribution(2, windows[1])[0].cpu().numpy()
+        inst.close()
+        results[name] = {"ms_per_step": round(wall / args.steps * 1e3, 4),
+                         "value": round(w * h * args.steps / wall / 1e6, 3),
+                         "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                         "run_to_run_identical": bool(np.array_equal(first.view(np.uint32), again.view(np.uint32)))}
+        print(f"# {name}: {results[name]['ms_per_step']} ms per centre frame", file=sys.stderr, flush=True)
+    del clip
     # Parity and the CPU leg on a crop.
     cw, ch = 640, 360
     crop = np.ascontiguousarray(noisy[0][:ch, :cw])

@@ -348,6 +348,7 @@ typedef struct nss_hip_wnnm_params {
     int has_rclip;       /* an rclip was given: matching runs on it */
     int memory_limit_mb; /* 0 = not given */
     int device_id;
+    int temporal_mode;   /* radius > 0: 0 legacy, 1 rolling (the default) */
 } nss_hip_wnnm_params;
 
 typedef struct nss_hip_wnnm nss_hip_wnnm;
@@ -370,6 +371,24 @@ int nss_hip_wnnm_process(nss_hip_wnnm* handle, const float* const* src, const in
                          const float* const* rclip, const int* rclip_stride, float* const* dst,
                          const int* dst_stride, void* stream);
 int nss_hip_wnnm_destroy(nss_hip_wnnm* handle);
+
+/* nss.WNNM at radius > 0 (wnnmGetFrame with its fat output,
+ * src/host/filter_wnnm.cpp:180-260, process_plane_batched :44-177): the
+ * contribution of centre frame `center` (0 <= center < num_frames of the
+ * create call). src[t * num_planes + p] (rclip likewise, or NULL): device
+ * planes of slot t = 0 .. 2 radius, the frame center + clamp(t - radius,
+ * -center, num_frames - 1 - center), strides in floats per entry. Each
+ * reference is matched on rclip (or src) by the predictive search over the
+ * slots inside the clip, its group shrunk, and its patches added into the
+ * num / den sums of their slot. dst[p] (2 (2 radius + 1) plane heights): slot
+ * t's num at rows 2t h, den at (2t + 1) h, as nss_hip_bm3d's legacy output;
+ * planes with sigma 0 carry the source in the centre slice with weight 1.
+ * Rolling output (temporal_mode "rolling") is nss_hip_vaggregate over the
+ * contributions of centres n - radius .. n + radius, which the glue keeps.
+ * Asynchronous on stream; calls on one handle are serialised. */
+int nss_hip_wnnm_process_temporal(nss_hip_wnnm* handle, int center, const float* const* src, const int* src_stride,
+                                  const float* const* rclip, const int* rclip_stride, float* const* dst,
+                                  const int* dst_stride, void* stream);
 
 /* ---- NCSR ---------------------------------------------------------------
  * nss.NCSR (src/host/filter_ncsr.cpp; the device form src/cuda/ncsr/filter.cpp:16-80,
@@ -394,13 +413,14 @@ typedef struct nss_hip_ncsr_params {
     int has_rclip;
     int memory_limit_mb;
     int device_id;
+    int temporal_mode;   /* radius > 0: 0 legacy, 1 rolling (the default) */
 } nss_hip_ncsr_params;
 
 typedef struct nss_hip_ncsr nss_hip_ncsr;
 
 /* <ns>.NCSR's argument resolution (src/frontend/ncsr_args.cpp:10-32 parse_ncsr
  * after parse_temporal_mode and the memory limit; rclip and plane checks of
- * src/cuda/ncsr/filter.cpp:26-34). Radius 0, as for WNNM. */
+ * src/cuda/ncsr/filter.cpp:26-34). */
 int nss_hip_ncsr_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
                        const nss_hip_arg* args, int nargs, const char* ns, nss_hip_ncsr_params* out);
 int nss_hip_ncsr_create(const nss_hip_ncsr_params* params, const nss_hip_video_info* info, int device,
@@ -411,6 +431,19 @@ int nss_hip_ncsr_process(nss_hip_ncsr* handle, const float* const* src, const in
                          const float* const* rclip, const int* rclip_stride, float* const* dst,
                          const int* dst_stride, void* stream);
 int nss_hip_ncsr_destroy(nss_hip_ncsr* handle);
+
+/* nss.NCSR at radius > 0 (ncsrGetFrame with its fat output,
+ * src/host/filter_ncsr.cpp:34-190): the contribution of centre frame
+ * `center`, with nss_hip_wnnm_process_temporal's arguments and output layout.
+ * The iterations run on every slot's estimate: round i > 0 relaxes each slot
+ * inside the clip toward its noisy frame by delta, every round matches on the
+ * estimates (rclip's slots in round 0) by the predictive search, and each
+ * round but the last finishes every slot's estimate (num / den, or the
+ * estimate where no patch landed); the last round's sums are the output.
+ * Asynchronous on stream; calls on one handle are serialised. */
+int nss_hip_ncsr_process_temporal(nss_hip_ncsr* handle, int center, const float* const* src, const int* src_stride,
+                                  const float* const* rclip, const int* rclip_stride, float* const* dst,
+                                  const int* dst_stride, void* stream);
 
 /* ---- Serving loop ------------------------------------------------------
  * No counterpart in the reference: its filters answer one VapourSynth frame

@@ -478,6 +478,38 @@ def wnnm_plane(src, sigma_user, block=8, step=8, group=8, bm_range=7, residual=0
     return (out, xy, counts) if want_matches else out
 
 
+def wnnm_contribution(src_slots, center, n_frames, sigma_user, block=8, step=8, group=8, bm_range=7, ps_num=2,
+                      ps_range=4, residual=0, adaptive=1, match_slots=None):
+    """One plane of nss.WNNM's fat output at radius r > 0 for centre frame
+    `center` of an n_frames clip (src/host/filter_wnnm.cpp:44-177
+    process_plane_batched with emit_fat): src_slots / match_slots are the
+    planes of slots 0 .. 2r (temporal_slot_frame). Each raster reference is
+    matched on the centre of match_slots (or src_slots) and predictively over
+    the slots inside the clip (predictive_match_batch, :99-102), its k patches
+    packed from their slot's frame (:116-121), shrunk, and added with the
+    group weight into the num / den of their slot (:140-160). Returns an
+    array (2r + 1, 2, h, w): slot t's num and den."""
+    ns = len(src_slots)
+    r = (ns - 1) // 2
+    srcs = [np.ascontiguousarray(p, np.float32) for p in src_slots]
+    mpl = srcs if match_slots is None else [np.ascontiguousarray(p, np.float32) for p in match_slots]
+    h, w = srcs[0].shape
+    sigma = float(np.float32(sigma_user) / np.float32(255.0))
+    vb = max(0, r - center)
+    ve = r + min(r + 1, n_frames - center)
+    out = np.zeros((ns, 2, h, w), np.float64)
+    dy, dx = np.divmod(np.arange(block * block), block)
+    s64 = [p.astype(np.float64) for p in srcs]
+    for bx, by in raster_grid(w, h, block, step):
+        m = predictive_match(mpl, bx, by, r, block, group, bm_range, ps_num, ps_range, r, vb, ve)
+        cols = [s64[t][y + dy, x + dx] for x, y, t, _ in m]
+        X, wt = wnnm_group(np.stack(cols, axis=1), sigma, residual, adaptive)
+        for j, (x, y, t, _) in enumerate(m):
+            out[t, 0, y + dy, x + dx] += wt * X[:, j]
+            out[t, 1, y:y + block, x:x + block] += wt
+    return out
+
+
 def wnnm_frame(planes, sigma, block=8, step=8, group=8, bm_range=7, residual=0, adaptive=1, rclip=None):
     """All planes of one frame; a plane with sigma 0 is copied
     (src/host/filter_wnnm.cpp:200-260)."""
@@ -566,3 +598,44 @@ def ncsr_plane(src, sigma_user, block=8, step=8, group=8, bm_range=7, iters=2, d
                 den[xy[r, j, 1]:xy[r, j, 1] + block, xy[r, j, 0]:xy[r, j, 0] + block] += 1.0
         est = np.where(den > 1e-12, num / np.where(den > 1e-12, den, 1.0), est)
     return est.astype(np.float32)
+
+
+def ncsr_contribution(src_slots, center, n_frames, sigma_user, block=8, step=8, group=8, bm_range=7, ps_num=2,
+                      ps_range=4, iters=2, delta=0.1, match_slots=None):
+    """One plane of nss.NCSR's fat output at radius r > 0 for centre frame
+    `center` of an n_frames clip (src/host/filter_ncsr.cpp:34-190): every
+    slot t keeps an estimate and its noisy plane (:124-133); round i > 0
+    relaxes each estimate toward its noisy plane (:154-159); each round
+    matches every raster reference on match_slots in round 0 (else the
+    estimates) by the predictive search over the slots inside the clip,
+    packs the patches from their slot's estimate (ncsr_run_groups,
+    src/cpu/ncsr/centralize.cpp:423-500), filters with the match distances
+    and adds each patch with weight 1 into its slot's num / den (:162-167);
+    every round but the last finishes each estimate (:171-175). Returns the
+    last round's sums as an array (2r + 1, 2, h, w)."""
+    ns = len(src_slots)
+    r = (ns - 1) // 2
+    noisy = [np.ascontiguousarray(p, np.float32).astype(np.float64) for p in src_slots]
+    h, w = noisy[0].shape
+    sigma = float(np.float32(sigma_user) / np.float32(255.0))
+    vb = max(0, r - center)
+    ve = r + min(r + 1, n_frames - center)
+    est = [p.copy() for p in noisy]
+    dy, dx = np.divmod(np.arange(block * block), block)
+    for it in range(iters):
+        if it > 0:
+            est = [e + delta * (y - e) for e, y in zip(est, noisy)]
+        mpl = match_slots if (it == 0 and match_slots is not None) else est
+        mpl = [np.asarray(p, np.float32) for p in mpl]
+        out = np.zeros((ns, 2, h, w), np.float64)
+        for bx, by in raster_grid(w, h, block, step):
+            m = predictive_match(mpl, bx, by, r, block, group, bm_range, ps_num, ps_range, r, vb, ve)
+            cols = [est[t][y + dy, x + dx] for x, y, t, _ in m]
+            X = ncsr_group(np.stack(cols, axis=1), np.array([d for *_, d in m], np.float64), sigma)
+            for j, (x, y, t, _) in enumerate(m):
+                out[t, 0, y + dy, x + dx] += X[:, j]
+                out[t, 1, y:y + block, x:x + block] += 1.0
+        if it + 1 == iters:
+            return out
+        est = [np.where(out[t, 1] > 1e-12, out[t, 0] / np.where(out[t, 1] > 1e-12, out[t, 1], 1.0), est[t])
+               for t in range(ns)]

@@ -101,8 +101,9 @@ def test_wnnm_parse_messages_and_null_checks():
         assert str(e.value) == text
     with pytest.raises(nss_hip.Error, match="selected plane is smaller than block_size"):
         nss_hip.parse_wnnm("GRAYS", 6, 6)
-    with pytest.raises(nss_hip.Error, match="not available in nss_hip"):
-        nss_hip.parse_wnnm("GRAYS", 64, 48, radius=1)
+    p = nss_hip.parse_wnnm("GRAYS", 64, 48, radius=1)
+    assert (p.radius, p.temporal_mode) == (1, 1)
+    assert nss_hip.parse_wnnm("GRAYS", 64, 48, radius=2, temporal_mode="legacy").temporal_mode == 0
     assert lib.nss_hip_wnnm_create(None, None, 0, None) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_wnnm_process(None, None, None, None, None, None, None, None) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_wnnm_destroy(None) == nss_hip.OK
@@ -127,6 +128,13 @@ def test_ncsr_parse_messages():
         with pytest.raises(nss_hip.Error) as e:
             nss_hip.parse_ncsr("GRAYS", 64, 48, **kw)
         assert str(e.value) == text
+    p = nss_hip.parse_ncsr("GRAYS", 64, 48, radius=2)
+    assert (p.radius, p.temporal_mode) == (2, 1)
+    assert nss_hip.parse_ncsr("GRAYS", 64, 48, radius=1, temporal_mode="legacy").temporal_mode == 0
+    with pytest.raises(nss_hip.Error) as e:
+        nss_hip.parse_ncsr("GRAYS", 64, 48, radius=1, temporal_mode="fused")
+    assert str(e.value) == "nss_hip.NCSR: temporal_mode=fused is not supported; use rolling or legacy"
     assert lib.nss_hip_ncsr_create(None, None, 0, None) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_ncsr_process(None, None, None, None, None, None, None, None) == nss_hip.INVALID_INPUT
+    assert lib.nss_hip_ncsr_process_temporal(None, 0, None, None, None, None, None, None, None) == nss_hip.INVALID_INPUT
     assert lib.nss_hip_ncsr_destroy(None) == nss_hip.OK

@@ -40,10 +40,13 @@ def _cases(gpu):
         "rolling": lambda: gpu.BM3D("GRAYS", 64, 48, frames=4, sigma=10, final=1, radius=1),
         "legacy": lambda: gpu.BM3D("GRAYS", 64, 48, frames=4, sigma=10, radius=1, temporal_mode="legacy"),
         "wnnm": lambda: gpu.WNNM("GRAYS", 64, 48, sigma=10),
+        "wnnm-temporal": lambda: gpu.WNNM("YUV420PS", 64, 48, frames=4, sigma=[10, 0, 10], radius=1),
+        "ncsr-temporal": lambda: gpu.NCSR("GRAYS", 64, 48, frames=4, sigma=10, radius=1, rclip=True),
     }
 
 
-CASE_NAMES = ["two-stage", "two-slots", "memory-limit", "global-cubes", "rolling", "legacy", "wnnm"]
+CASE_NAMES = ["two-stage", "two-slots", "memory-limit", "global-cubes", "rolling", "legacy", "wnnm", "wnnm-temporal",
+              "ncsr-temporal"]
 
 
 def _live(gpu):
@@ -222,3 +225,21 @@ def test_refused_launch_stops_a_rolling_instance(gpu):
     for got, exp in zip(again, want):
         for g, w_ in zip(got, exp):
             assert np.array_equal(g, w_)
+
+
+@pytest.mark.parametrize("kind", ["WNNM", "NCSR"])
+def test_refused_launch_in_a_temporal_contribution_reports_and_the_next_is_exact(gpu, kind):
+    """nss_hip_wnnm_process_temporal and nss_hip_ncsr_process_temporal keep
+    no state between calls: after a refused launch the same centre's
+    contribution is exact again."""
+    import synth
+    _, seq = synth.shifted_sequence("GRAYS", 64, 48, 4, 10, seed=44)
+    inst = getattr(gpu, kind)("GRAYS", 64, 48, frames=4, sigma=10, radius=1)
+    frames = [[_dev(seq[f][0])] for f in inst.slot_frames(1)]
+    want = _host(inst.contribution(1, frames)[0])
+    refused = 0
+    for k in _refused_launches(lambda: inst.contribution(1, frames)):
+        refused += 1
+        assert np.array_equal(_host(inst.contribution(1, frames)[0]), want), k
+    inst.close()
+    assert refused >= 4

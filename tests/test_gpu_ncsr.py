@@ -115,3 +115,152 @@ def test_ncsr_1080p_defaults_against_oracle(gpu, oracle):
     assert p >= PSNR_MIN
     assert np.array_equal(out, again)
     assert oracle.psnr(out, clean) > oracle.psnr(src, clean)
+
+
+TEMPORAL_CASES = [
+    dict(radius=1),
+    dict(radius=2, block_step=4, iters=3, delta=0.2),
+    dict(radius=1, block_size=4, block_step=2, group_size=16, bm_range=5, ps_num=1, ps_range=3, iters=1),
+]
+
+
+def _slot_estimate(num, den):
+    ok = den > 1e-12
+    return ok, np.where(ok, num / np.where(ok, den, 1.0), np.nan)
+
+
+def _compare_contribution(fat, ref, ns, h, tag):
+    """Per slot: the estimates (num / den) within 60 dB where both reached
+    a pixel, the reached pixels equal to 1 %."""
+    for t in range(ns):
+        ok_g, est_g = _slot_estimate(fat[2 * t * h:(2 * t + 1) * h], fat[(2 * t + 1) * h:(2 * t + 2) * h])
+        ok_w, est_w = _slot_estimate(ref[t, 0], ref[t, 1])
+        assert (ok_g != ok_w).mean() < 0.01, (tag, t)
+        both = ok_g & ok_w
+        if not both.any():
+            assert not fat[2 * t * h:(2 * t + 2) * h].any(), (tag, t)
+            continue
+        mse = float(np.mean((est_g[both] - est_w[both]) ** 2))
+        psnr = 10 * np.log10(1.0 / max(mse, 1e-30))
+        print(f"{tag} slot {t}: {psnr:.1f} dB over {both.sum()} px", flush=True)
+        assert psnr >= PSNR_MIN, (tag, t, psnr)
+
+
+@pytest.mark.parametrize("kw", TEMPORAL_CASES, ids=[",".join(f"{k}={v}" for k, v in c.items()) for c in TEMPORAL_CASES])
+def test_ncsr_temporal_contribution_against_oracle(gpu, oracle, kw):
+    """src/host/filter_ncsr.cpp:34-190 at radius > 0: a centre frame's fat
+    output (every slot's num / den after the iterations on the slot
+    estimates) against the oracle, for centres at the clip's ends and inside
+    it; slots outside the clip stay empty."""
+    import torch
+    import synth
+    n_frames, h, w = 5, 48, 64
+    _, seq = synth.shifted_sequence("GRAYS", w, h, n_frames, 10, seed=91 + kw["radius"])
+    inst = gpu.NCSR("GRAYS", w, h, frames=n_frames, sigma=[10.0], **kw)
+    r = kw["radius"]
+    ns = 2 * r + 1
+    for center in (0, 2, n_frames - 1):
+        slots = inst.slot_frames(center)
+        out = inst.contribution(center, [[torch.from_numpy(seq[f][0]).cuda()] for f in slots])[0]
+        torch.cuda.synchronize()
+        fat = out.cpu().numpy().astype(np.float64)
+        ref = oracle.ncsr_contribution([seq[f][0] for f in slots], center, n_frames, 10.0,
+                                       block=kw.get("block_size", 8), step=kw.get("block_step", 8),
+                                       group=kw.get("group_size", 8), bm_range=kw.get("bm_range", 7),
+                                       ps_num=kw.get("ps_num", 2), ps_range=kw.get("ps_range", 4),
+                                       iters=kw.get("iters", 2), delta=kw.get("delta", 0.1))
+        _compare_contribution(fat, ref, ns, h, f"centre {center}")
+        vb, ve = max(0, r - center), r + min(r + 1, n_frames - center)
+        for t in list(range(vb)) + list(range(ve, ns)):
+            assert not fat[2 * t * h:(2 * t + 2) * h].any(), (center, t)
+    inst.close()
+
+
+def test_ncsr_temporal_rclip_guides_round_zero(gpu, oracle):
+    """rclip's slot frames guide round 0's matching only; later rounds match
+    on the estimates (src/host/filter_ncsr.cpp:163-165)."""
+    import torch
+    import synth
+    n_frames, h, w, r = 4, 48, 64, 1
+    clean, seq = synth.shifted_sequence("GRAYS", w, h, n_frames, 10, seed=95)
+    inst = gpu.NCSR("GRAYS", w, h, frames=n_frames, rclip=True, sigma=[10.0], radius=r, iters=2)
+    center = 1
+    slots = inst.slot_frames(center)
+    src = [[torch.from_numpy(seq[f][0]).cuda()] for f in slots]
+    guide = [[torch.from_numpy(np.ascontiguousarray(clean[f][0])).cuda()] for f in slots]
+    fat = inst.contribution(center, src, rclip=guide)[0]
+    torch.cuda.synchronize()
+    fat = fat.cpu().numpy().astype(np.float64)
+    inst.close()
+    ref = oracle.ncsr_contribution([seq[f][0] for f in slots], center, n_frames, 10.0,
+                                   match_slots=[clean[f][0] for f in slots])
+    _compare_contribution(fat, ref, 2 * r + 1, h, "rclip")
+
+
+def test_ncsr_temporal_single_frame_equals_spatial_and_repeats(gpu):
+    """A one-frame clip at radius 1: only the centre slot is inside the clip,
+    so its num / den is the spatial NCSR output; a sigma-0 plane carries the
+    identity contribution; runs repeat bitwise."""
+    import torch
+    import synth
+    h, w = 48, 64
+    _, noisy = synth.noisy_frame("YUV444PS", w, h, 10, seed=97)
+    planes = [torch.from_numpy(p).cuda() for p in noisy]
+    sig = [10.0, 0.0, 8.0]
+    spatial = gpu.NCSR("YUV444PS", w, h, sigma=sig, iters=3)
+    want = [o.cpu().numpy() for o in spatial(planes)]
+    spatial.close()
+    inst = gpu.NCSR("YUV444PS", w, h, frames=1, sigma=sig, radius=1, iters=3)
+    fats = [inst.contribution(0, [planes] * 3) for _ in range(2)]
+    torch.cuda.synchronize()
+    a = [o.cpu().numpy() for o in fats[0]]
+    b = [o.cpu().numpy() for o in fats[1]]
+    inst.close()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    for i in (0, 2):
+        num, den = a[i][2 * h:3 * h].astype(np.float64), a[i][3 * h:4 * h].astype(np.float64)
+        est = np.where(den > 1e-12, num / np.where(den > 1e-12, den, 1.0), noisy[i])
+        assert np.abs(est - want[i]).max() <= 2e-6 * (1 + np.abs(want[i]).max()), i
+        assert not np.concatenate([a[i][:2 * h], a[i][4 * h:]]).any()
+    ident = a[1]
+    assert np.array_equal(ident[2 * h:3 * h], noisy[1]) and (ident[3 * h:4 * h] == 1.0).all()
+
+
+def test_ncsr_temporal_clip_through_vaggregate_against_oracle(gpu, oracle):
+    """nss.NCSR radius > 0 then nss.VAggregate on a 4-frame YUV420 clip:
+    every output frame within 60 dB of the oracle's contributions aggregated
+    by the oracle, the sigma-0 plane the source."""
+    import torch
+    import synth
+    n_frames, h, w, r = 4, 48, 64, 1
+    _, seq = synth.shifted_sequence("YUV420PS", w, h, n_frames, 10, seed=98)
+    sig = [10.0, 0.0, 12.0]
+    inst = gpu.NCSR("YUV420PS", w, h, frames=n_frames, sigma=sig, radius=r)
+    dev = [[torch.from_numpy(p).cuda() for p in f] for f in seq]
+    fats = [inst.contribution(c, [dev[f] for f in inst.slot_frames(c)]) for c in range(n_frames)]
+    got = gpu.VAggregate(fats, dev, r)
+    torch.cuda.synchronize()
+    got = [[p.cpu().numpy() for p in f] for f in got]
+    ref_fat = []
+    for c in range(n_frames):
+        slots = inst.slot_frames(c)
+        frame = []
+        for i in range(3):
+            ph, pw = seq[0][i].shape
+            if sig[i] == 0:
+                f = np.zeros((2 * (2 * r + 1) * ph, pw), np.float32)
+                f[2 * r * ph:(2 * r + 1) * ph] = seq[c][i]
+                f[(2 * r + 1) * ph:(2 * r + 2) * ph] = 1.0
+            else:
+                con = oracle.ncsr_contribution([seq[s][i] for s in slots], c, n_frames, sig[i])
+                f = con.reshape(2 * (2 * r + 1) * ph, pw).astype(np.float32)
+            frame.append(f)
+        ref_fat.append(frame)
+    inst.close()
+    want = oracle.vaggregate(ref_fat, seq, r)
+    for n in range(n_frames):
+        assert np.array_equal(got[n][1], seq[n][1])
+        for i in (0, 2):
+            p = oracle.psnr(got[n][i], want[n][i])
+            assert p >= PSNR_MIN, (n, i, p)

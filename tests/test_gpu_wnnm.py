@@ -162,3 +162,135 @@ def test_wnnm_group_capacity_changes_no_bit(gpu):
         ref = _run(gpu, "GRAYS", [src], group_size=9, **kw)[0]
         for capacity in (16, 32):
             assert np.array_equal(_run(gpu, "GRAYS", [src], group_size=capacity, **kw)[0], ref)
+
+
+TEMPORAL_CASES = [
+    dict(radius=1),
+    dict(radius=2, group_size=16, block_step=4, residual=1),
+    dict(radius=1, block_size=4, block_step=2, group_size=8, bm_range=5, ps_num=1, ps_range=3, adaptive_aggregation=0),
+]
+
+
+def _slot_estimates(fat, ns, h):
+    """Per slot: num / den where den > 1e-12 (NaN elsewhere)."""
+    out = []
+    for t in range(ns):
+        num, den = fat[2 * t * h:(2 * t + 1) * h], fat[(2 * t + 1) * h:(2 * t + 2) * h]
+        out.append(np.where(den > 1e-12, num / np.where(den > 1e-12, den, 1.0), np.nan))
+    return out
+
+
+@pytest.mark.parametrize("kw", TEMPORAL_CASES, ids=[",".join(f"{k}={v}" for k, v in c.items()) for c in TEMPORAL_CASES])
+def test_wnnm_temporal_contribution_against_oracle(gpu, oracle, kw):
+    """src/host/filter_wnnm.cpp:44-260 at radius > 0: a centre frame's fat
+    output (each slot's num / den) against the oracle, for centres at the
+    clip's ends and inside it (slots outside the clip stay empty), the
+    estimate of every slot within 60 dB and the slots each reaches equal."""
+    import torch
+    import synth
+    n_frames, h, w = 5, 48, 64
+    _, seq = synth.shifted_sequence("GRAYS", w, h, n_frames, 10, seed=61 + kw["radius"])
+    inst = gpu.WNNM("GRAYS", w, h, frames=n_frames, sigma=[10.0], **kw)
+    r = kw["radius"]
+    ns = 2 * r + 1
+    for center in (0, 2, n_frames - 1):
+        slots = inst.slot_frames(center)
+        frames = [[torch.from_numpy(seq[f][0]).cuda()] for f in slots]
+        out = inst.contribution(center, frames)[0]
+        torch.cuda.synchronize()
+        fat = out.cpu().numpy().astype(np.float64)
+        ref = oracle.wnnm_contribution([seq[f][0] for f in slots], center, n_frames, 10.0,
+                                       block=kw.get("block_size", 8), step=kw.get("block_step", 8),
+                                       group=kw.get("group_size", 8), bm_range=kw.get("bm_range", 7),
+                                       ps_num=kw.get("ps_num", 2), ps_range=kw.get("ps_range", 4),
+                                       residual=kw.get("residual", 0), adaptive=kw.get("adaptive_aggregation", 1))
+        got_est = _slot_estimates(fat, ns, h)
+        for t in range(ns):
+            want = np.where(ref[t, 1] > 1e-12, ref[t, 0] / np.where(ref[t, 1] > 1e-12, ref[t, 1], 1.0), np.nan)
+            reached_got, reached_want = np.isfinite(got_est[t]), np.isfinite(want)
+            assert (reached_got != reached_want).mean() < 0.01, (center, t)
+            both = reached_got & reached_want
+            if not both.any():
+                continue
+            mse = float(np.mean((got_est[t][both] - want[both]) ** 2))
+            psnr = 10 * np.log10(1.0 / max(mse, 1e-30))
+            print(f"centre {center} slot {t}: {psnr:.1f} dB over {both.sum()} px", flush=True)
+            assert psnr >= PSNR_MIN, (center, t, psnr)
+        # slots outside the clip: no patch, zero num and den
+        vb, ve = max(0, r - center), r + min(r + 1, n_frames - center)
+        for t in list(range(vb)) + list(range(ve, ns)):
+            assert not fat[2 * t * h:(2 * t + 2) * h].any(), (center, t)
+    inst.close()
+
+
+def test_wnnm_temporal_single_frame_equals_spatial_and_identity_planes(gpu):
+    """A one-frame clip at radius 2: only the centre slot is inside the clip,
+    so the centre slice finished (num / den) is the spatial WNNM's output;
+    a sigma-0 plane carries the source with weight 1 in the centre slice and
+    zeros elsewhere (src/frontend/temporal.hpp:23-36); runs repeat bitwise."""
+    import torch
+    import synth
+    h, w = 48, 64
+    _, noisy = synth.noisy_frame("YUV444PS", w, h, 10, seed=70)
+    planes = [torch.from_numpy(p).cuda() for p in noisy]
+    sig = [10.0, 0.0, 8.0]
+    spatial = gpu.WNNM("YUV444PS", w, h, sigma=sig)
+    want = [o.cpu().numpy() for o in spatial(planes)]
+    spatial.close()
+    inst = gpu.WNNM("YUV444PS", w, h, frames=1, sigma=sig, radius=2)
+    fats = [inst.contribution(0, [planes] * 5) for _ in range(2)]
+    torch.cuda.synchronize()
+    a = [o.cpu().numpy() for o in fats[0]]
+    b = [o.cpu().numpy() for o in fats[1]]
+    inst.close()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    for i in (0, 2):
+        num, den = a[i][4 * h:5 * h].astype(np.float64), a[i][5 * h:6 * h].astype(np.float64)
+        est = np.where(den > 1e-12, num / np.where(den > 1e-12, den, 1.0), noisy[i])
+        assert np.abs(est - want[i]).max() <= 2e-6 * (1 + np.abs(want[i]).max()), i
+        other = np.concatenate([a[i][:4 * h], a[i][6 * h:]])
+        assert not other.any()
+    ident = a[1]
+    assert np.array_equal(ident[4 * h:5 * h], noisy[1]) and (ident[5 * h:6 * h] == 1.0).all()
+    assert not np.concatenate([ident[:4 * h], ident[6 * h:]]).any()
+
+
+def test_wnnm_temporal_clip_through_vaggregate_against_oracle(gpu, oracle):
+    """The temporal WNNM workflow (nss.WNNM radius > 0 then nss.VAggregate,
+    src/host/filter_vaggregate.cpp:29-100) on a 4-frame YUV420 clip with a
+    sigma-0 chroma plane: every output frame within 60 dB of the oracle's
+    contributions aggregated by the oracle, the sigma-0 plane the source."""
+    import torch
+    import synth
+    n_frames, h, w, r = 4, 48, 64, 1
+    _, seq = synth.shifted_sequence("YUV420PS", w, h, n_frames, 10, seed=80)
+    sig = [10.0, 0.0, 12.0]
+    inst = gpu.WNNM("YUV420PS", w, h, frames=n_frames, sigma=sig, radius=r)
+    dev = [[torch.from_numpy(p).cuda() for p in f] for f in seq]
+    fats = [inst.contribution(c, [dev[f] for f in inst.slot_frames(c)]) for c in range(n_frames)]
+    got = gpu.VAggregate(fats, dev, r)
+    torch.cuda.synchronize()
+    got = [[p.cpu().numpy() for p in f] for f in got]
+    ref_fat = []
+    for c in range(n_frames):
+        slots = inst.slot_frames(c)
+        frame = []
+        for i in range(3):
+            ph, pw = seq[0][i].shape
+            if sig[i] == 0:
+                f = np.zeros((2 * (2 * r + 1) * ph, pw), np.float32)
+                f[2 * r * ph:(2 * r + 1) * ph] = seq[c][i]
+                f[(2 * r + 1) * ph:(2 * r + 2) * ph] = 1.0
+            else:
+                con = oracle.wnnm_contribution([seq[s][i] for s in slots], c, n_frames, sig[i])
+                f = con.reshape(2 * (2 * r + 1) * ph, pw).astype(np.float32)
+            frame.append(f)
+        ref_fat.append(frame)
+    inst.close()
+    want = oracle.vaggregate(ref_fat, seq, r)
+    for n in range(n_frames):
+        assert np.array_equal(got[n][1], seq[n][1])
+        for i in (0, 2):
+            p = oracle.psnr(got[n][i], want[n][i])
+            assert p >= PSNR_MIN, (n, i, p)

@@ -501,8 +501,7 @@ int nss_hip_bm3d_output_info(const nss_hip_bm3d_params* p, const nss_hip_video_i
 // <ns>.WNNM's create-time checks in the reference's order: numeric
 // representability and memory_limit_mb (src/frontend/validate.hpp:33-45),
 // temporal_mode (src/frontend/temporal_args.cpp:11-28), parse_wnnm, rclip
-// against clip, validate_group_planes (src/frontend/validate.hpp:19-25); then
-// nss_hip's own limit (radius 0).
+// against clip, validate_group_planes (src/frontend/validate.hpp:19-25).
 int nss_hip_wnnm_parse(const nss_hip_video_info* clip_info, const nss_hip_video_info* rclip_info,
                        const nss_hip_arg* args, int nargs, const char* ns_c, nss_hip_wnnm_params* out) {
     const std::string ns = ns_c ? ns_c : "nss_hip";
@@ -536,7 +535,7 @@ int nss_hip_wnnm_parse(const nss_hip_video_info* clip_info, const nss_hip_video_
                 (plane_width(*clip_info, i) < p.block_size || plane_height(*clip_info, i) < p.block_size))
                 throw Invalid("nss: selected plane is smaller than block_size");
         }
-        if (p.radius > 0) fail_wnnm(ns, "radius > 0 (temporal WNNM) is not available in nss_hip; use radius=0");
+        p.temporal_mode = m == "legacy" ? 0 : 1;
         const std::int64_t device = in.get_int("device_id", 0, &err);
         if (!err && (device < 0 || device > INT32_MAX)) fail_wnnm(ns, "device_id must be >= 0");
         p.device_id = err ? 0 : static_cast<int>(device);
@@ -583,7 +582,7 @@ int nss_hip_ncsr_parse(const nss_hip_video_info* clip_info, const nss_hip_video_
                 (plane_width(*clip_info, i) < p.block_size || plane_height(*clip_info, i) < p.block_size))
                 throw Invalid("nss: selected plane is smaller than block_size");
         }
-        if (p.radius > 0) fail_ncsr(ns, "radius > 0 (temporal NCSR) is not available in nss_hip; use radius=0");
+        p.temporal_mode = m == "legacy" ? 0 : 1;
         const std::int64_t device = in.get_int("device_id", 0, &err);
         if (!err && (device < 0 || device > INT32_MAX)) fail_ncsr(ns, "device_id must be >= 0");
         p.device_id = err ? 0 : static_cast<int>(device);

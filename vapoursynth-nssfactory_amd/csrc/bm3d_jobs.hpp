@@ -301,6 +301,12 @@ struct WnnmJob {
     const int* counts;
     unsigned long long* num;
     unsigned long long* den;
+    // Temporal (radius > 0): patch j comes from the frame of its slot t =
+    // rec.t, ring.frame(slots.frame(t)), and adds into num / den + t * sum_slot.
+    int temporal;
+    FrameRing ring;
+    Slots slots;
+    long long sum_slot;
 };
 
 struct WnnmTable {

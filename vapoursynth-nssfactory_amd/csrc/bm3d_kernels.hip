@@ -3458,7 +3458,8 @@ __device__ __forceinline__ void gram_group_body(const WnnmTable& tab) {
     }
     for (int j = 0; j < N; ++j) {
         if (j < k) {
-            const float* base = jb.src + static_cast<size_t>(rec[j].y) * jb.stride + rec[j].x;
+            const float* plane = jb.temporal ? jb.ring.frame(jb.slots.frame(rec[j].t)) : jb.src;
+            const float* base = plane + static_cast<size_t>(rec[j].y) * jb.stride + rec[j].x;
             float v[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) v[q] = lane + 64 * q < m ? base[soff[q]] : 0.f;
@@ -3730,7 +3731,8 @@ __device__ __forceinline__ void gram_group_body(const WnnmTable& tab) {
     const float wt = !NCSR && jb.adaptive && kept > 0 ? 1.f / static_cast<float>(kept) : 1.f;
     const unsigned long long wfix = to_fixed(wt);
     for (int j = 0; j < k; ++j) {
-        unsigned long long* cell = jb.num + static_cast<size_t>(rec[j].y) * jb.width + rec[j].x;
+        const size_t slice = jb.temporal ? static_cast<size_t>(rec[j].t) * jb.sum_slot : 0;
+        unsigned long long* cell = jb.num + slice + static_cast<size_t>(rec[j].y) * jb.width + rec[j].x;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int e = lane + 64 * q;
@@ -3740,7 +3742,7 @@ __device__ __forceinline__ void gram_group_body(const WnnmTable& tab) {
                 atomicAdd(cell + noff[q], to_fixed(wt * (x + mean[q])));
             }
         }
-        if (lane == 0) atomicAdd(jb.den + static_cast<size_t>(rec[j].y) * jb.width + rec[j].x, wfix);
+        if (lane == 0) atomicAdd(jb.den + slice + static_cast<size_t>(rec[j].y) * jb.width + rec[j].x, wfix);
     }
 }
 

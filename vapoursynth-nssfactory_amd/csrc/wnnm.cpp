@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -31,11 +32,16 @@ struct nss_hip_wnnm {
         bool active = false;
         MatchRec* recs = nullptr;
         int* counts = nullptr;
-        unsigned long long* num = nullptr;
+        unsigned long long* num = nullptr;  // 2 radius + 1 slices when temporal
         unsigned long long* den = nullptr;
-        float* est = nullptr;  // NCSR estimate, width floats per row
+        float* est = nullptr;  // NCSR estimate, width floats per row; a ring like win when temporal
+        // Temporal: the window's frames (frame f at (f % (2 radius + 1)) * pixels),
+        // and rclip's when one is given.
+        float* win = nullptr;
+        float* mwin = nullptr;
     } pl[3];
     long long bytes = 0;
+    std::mutex mu;  // temporal calls: one at a time (the window and sums are shared)
 };
 
 // The NCSR instance is the same state with the NCSR fields set.
@@ -50,6 +56,9 @@ void release(nss_hip_wnnm* h) {
         if (q.num) dev_free(q.num);
         if (q.den) dev_free(q.den);
         if (q.est) dev_free(q.est);
+        dev_free(q.win);
+        dev_free(q.mwin);
+        q.win = q.mwin = nullptr;
         q.est = nullptr;
         q.recs = nullptr;
         q.counts = nullptr;
@@ -64,9 +73,17 @@ int init_instance(nss_hip_wnnm* h, const nss_hip_wnnm_params& p, const nss_hip_v
     const std::string ns = std::string("nss_hip.") + name;
     if (info->num_planes < 1 || info->num_planes > 3 || info->width < 1 || info->height < 1)
         return fail(NSS_HIP_INVALID_INPUT, ns + ": invalid video info");
-    if (p.radius != 0 || p.block_size < 1 || p.block_size > 16 || p.group_size < 1 || p.group_size > 32 ||
-        p.block_step < 1 || p.block_step > p.block_size || p.bm_range < 1 || p.bm_range > 64)
+    if (p.radius < 0 || p.radius > kMaxRadius || p.block_size < 1 ||
+        p.block_size > 16 || p.group_size < 1 || p.group_size > 32 || p.block_step < 1 ||
+        p.block_step > p.block_size || p.bm_range < 1 || p.bm_range > 64 ||
+        (p.radius > 0 && (p.ps_num < 1 || p.ps_num > p.group_size || p.ps_range < 1 || p.ps_range > 64)))
         return fail(NSS_HIP_INVALID_INPUT, ns + ": parameters not resolved by the parse call");
+    if (p.radius > 0 && (info->num_frames < 1 || info->width > 8191 || info->height > 8191))
+        return fail(NSS_HIP_INVALID_INPUT, ns + ": radius > 0 needs num_frames and planes up to 8191 x 8191");
+    if (p.radius > 0 && p.block_size != 1 && p.block_size != 2 && p.block_size != 4 && p.block_size != 8 &&
+        p.block_size != 12 && p.block_size != 16)
+        return fail(NSS_HIP_UNSUPPORTED, ns + ": radius > 0 takes block_size 1, 2, 4, 8, 12 or 16 in nss_hip "
+                                              "(the predictive search's sizes)");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess) return hip_fail(e, "device count");
@@ -91,12 +108,17 @@ int init_instance(nss_hip_wnnm* h, const nss_hip_wnnm_params& p, const nss_hip_v
         const size_t pixels = static_cast<size_t>(q.width) * q.height;
         const size_t rec_bytes = static_cast<size_t>(q.nref) * p.group_size * sizeof(MatchRec);
         const size_t cnt_bytes = static_cast<size_t>(q.nref) * sizeof(int);
-        const size_t sum_bytes = pixels * sizeof(unsigned long long);
-        const size_t est_bytes = h->ncsr ? pixels * sizeof(float) : 0;
-        h->bytes += static_cast<long long>(rec_bytes + cnt_bytes + 2 * sum_bytes + est_bytes);
+        const size_t slices = 2 * static_cast<size_t>(p.radius) + 1;
+        const size_t sum_bytes = slices * pixels * sizeof(unsigned long long);
+        const size_t est_bytes = h->ncsr ? slices * pixels * sizeof(float) : 0;
+        const size_t win_bytes = p.radius > 0 ? slices * pixels * sizeof(float) : 0;
+        const size_t mwin_bytes = p.radius > 0 && p.has_rclip ? win_bytes : 0;
+        h->bytes += static_cast<long long>(rec_bytes + cnt_bytes + 2 * sum_bytes + est_bytes + win_bytes + mwin_bytes);
         if ((e = dev_alloc(&q.recs, rec_bytes)) != hipSuccess || (e = dev_alloc(&q.counts, cnt_bytes)) != hipSuccess ||
             (e = dev_alloc(&q.num, sum_bytes)) != hipSuccess || (e = dev_alloc(&q.den, sum_bytes)) != hipSuccess ||
-            (h->ncsr && (e = dev_alloc(&q.est, est_bytes)) != hipSuccess)) {
+            (h->ncsr && (e = dev_alloc(&q.est, est_bytes)) != hipSuccess) ||
+            (win_bytes && (e = dev_alloc(&q.win, win_bytes)) != hipSuccess) ||
+            (mwin_bytes && (e = dev_alloc(&q.mwin, mwin_bytes)) != hipSuccess)) {
             release(h);
             return hip_fail(e, "group filter buffers");
         }
@@ -221,6 +243,264 @@ int copy_inactive(const nss_hip_wnnm* h, const float* const* src, const int* src
     return NSS_HIP_OK;
 }
 
+// Every slot's num / den of one contribution as float planes, slot t at
+// rows 2t h (num) and (2t + 1) h (den, the corner box sum, no division).
+int finish_slices(nss_hip_wnnm* h, float* const* dst, const int* dst_stride, hipStream_t s) {
+    const nss_hip_wnnm_params& p = h->p;
+    const int ns = 2 * p.radius + 1;
+    FinishTable ft{};
+    int finish_wg = 0;
+    hipError_t e;
+    for (int i = 0; i < h->vi.num_planes; ++i) {
+        const auto& q = h->pl[i];
+        if (!q.active) continue;
+        const size_t npx = static_cast<size_t>(q.width) * q.height, rows = static_cast<size_t>(q.height) * dst_stride[i];
+        for (int t = 0; t < ns; ++t) {
+            if (ft.n == kMaxJobs) {
+                if ((e = launch_finish(p.block_size, ft, finish_wg, false, s)) != hipSuccess)
+                    return hip_fail(e, "slice launch");
+                ft = FinishTable{};
+                finish_wg = 0;
+            }
+            FinishJob& f = ft.job[ft.n++];
+            f.num = q.num + t * npx;
+            f.den = q.den + t * npx;
+            f.block = p.block_size;
+            f.src = nullptr;
+            f.dst = dst[i] + 2 * t * rows;
+            f.dst_den = dst[i] + (2 * t + 1) * rows;
+            f.width = q.width;
+            f.height = q.height;
+            f.sstride = q.width;
+            f.dstride = dst_stride[i];
+            f.tiles_x = (q.width + 15) / 16;
+            f.wg_begin = finish_wg;
+            finish_wg += f.tiles_x * ((q.height + 15) / 16);
+        }
+    }
+    if (ft.n && (e = launch_finish(p.block_size, ft, finish_wg, false, s)) != hipSuccess)
+        return hip_fail(e, "slice launch");
+    return NSS_HIP_OK;
+}
+
+// NCSR rounds before the last: the estimate of every slot inside the clip
+// becomes num / den, or stays where no patch landed (in place).
+int finish_estimates(nss_hip_wnnm* h, const Slots& slots, hipStream_t s) {
+    const nss_hip_wnnm_params& p = h->p;
+    const int ns = 2 * p.radius + 1;
+    FinishTable ft{};
+    int finish_wg = 0;
+    hipError_t e;
+    for (int i = 0; i < h->vi.num_planes; ++i) {
+        const auto& q = h->pl[i];
+        if (!q.active) continue;
+        const size_t npx = static_cast<size_t>(q.width) * q.height;
+        for (int t = slots.valid_begin(); t < slots.valid_end(); ++t) {
+            if (ft.n == kMaxJobs) {
+                if ((e = launch_finish(p.block_size, ft, finish_wg, false, s)) != hipSuccess)
+                    return hip_fail(e, "NCSR estimate launch");
+                ft = FinishTable{};
+                finish_wg = 0;
+            }
+            float* est = q.est + static_cast<size_t>(slots.frame(t) % ns) * npx;
+            FinishJob& f = ft.job[ft.n++];
+            f.num = q.num + t * npx;
+            f.den = q.den + t * npx;
+            f.block = p.block_size;
+            f.src = est;
+            f.dst = est;
+            f.dst_den = nullptr;
+            f.width = q.width;
+            f.height = q.height;
+            f.sstride = q.width;
+            f.dstride = q.width;
+            f.tiles_x = (q.width + 15) / 16;
+            f.wg_begin = finish_wg;
+            finish_wg += f.tiles_x * ((q.height + 15) / 16);
+        }
+    }
+    if (ft.n && (e = launch_finish(p.block_size, ft, finish_wg, false, s)) != hipSuccess)
+        return hip_fail(e, "NCSR estimate launch");
+    return NSS_HIP_OK;
+}
+
+// The contribution of centre frame `center`: the fat output of wnnmGetFrame
+// (src/host/filter_wnnm.cpp:180-260, process_plane_batched :44-177 with
+// emit_fat) and of ncsrGetFrame (src/host/filter_ncsr.cpp:34-190 at radius
+// > 0). Per active plane the window's frames go into the instance's ring
+// (frame f at f % (2r + 1); distinct frames of one window never share an
+// entry); each reference is matched on the centre of the matching ring and
+// predictively over the slots inside the clip (collect_temporal, as
+// nss.BM3D), its group shrunk from the slot frames and its patches added into
+// their slot's sums. NCSR runs its rounds on an estimate ring that starts as
+// the window: round i > 0 relaxes every slot inside the clip toward its
+// noisy frame, each round matches on the estimate (on rclip in round 0) and,
+// but the last, finishes every such slot's estimate in place. The last
+// round's sums are written as every slot's num / den; sigma-0 planes carry
+// the identity contribution (src/frontend/temporal.hpp:23-36).
+int temporal_contribution(nss_hip_wnnm* h, const char* name, int center, const float* const* src,
+                          const int* src_stride, const float* const* rclip, const int* rclip_stride,
+                          float* const* dst, const int* dst_stride, void* stream) {
+    const std::string ns_name = std::string("nss_hip.") + name;
+    if (!src || !src_stride || !dst || !dst_stride || (rclip && !rclip_stride))
+        return fail(NSS_HIP_INVALID_INPUT, ns_name + ": null argument");
+    const nss_hip_wnnm_params& p = h->p;
+    if (p.radius == 0) return fail(NSS_HIP_INVALID_INPUT, ns_name + ": the temporal call serves radius > 0");
+    const int r = p.radius, ns = 2 * r + 1, np = h->vi.num_planes, nframes = h->vi.num_frames;
+    if (center < 0 || center >= nframes) return fail(NSS_HIP_INVALID_INPUT, ns_name + ": centre outside the clip");
+    if (p.has_rclip && !rclip) return fail(NSS_HIP_INVALID_INPUT, ns_name + ": rclip planes required");
+    if (rclip && !p.has_rclip)
+        return fail(NSS_HIP_INVALID_INPUT, ns_name + ": this instance was made without rclip");
+    for (int t = 0; t < ns; ++t)
+        for (int i = 0; i < np; ++i) {
+            const int k = t * np + i;
+            if (!src[k] || src_stride[k] < h->pl[i].width || (rclip && (!rclip[k] || rclip_stride[k] < h->pl[i].width)))
+                return fail(NSS_HIP_INVALID_INPUT, ns_name + ": missing plane or stride below width");
+        }
+    for (int i = 0; i < np; ++i)
+        if (!dst[i] || dst_stride[i] < h->pl[i].width)
+            return fail(NSS_HIP_INVALID_INPUT, ns_name + ": missing output plane or stride below width");
+    std::lock_guard<std::mutex> lock(h->mu);
+    DeviceGuard dg(h->device);
+    if (dg.err != hipSuccess) return hip_fail(dg.err, "set device");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const Slots slots{center, nframes, r};
+    const int vb = slots.valid_begin(), ve = slots.valid_end();
+    hipError_t e = hipSuccess;
+    bool any = false;
+    for (int i = 0; i < np && e == hipSuccess; ++i) {
+        const auto& q = h->pl[i];
+        const size_t w = static_cast<size_t>(q.width), ph = static_cast<size_t>(q.height), npx = w * ph;
+        float* fat = dst[i];
+        const size_t dpitch = static_cast<size_t>(dst_stride[i]) * 4, rows = ph * dst_stride[i];
+        if (!q.active) {
+            // identity contribution: the source in the centre slice, weight 1
+            e = hipMemset2DAsync(fat, dpitch, 0, w * 4, 2 * ns * ph, s);
+            if (e == hipSuccess)
+                e = hipMemcpy2DAsync(fat + 2 * r * rows, dpitch, src[r * np + i], static_cast<size_t>(src_stride[r * np + i]) * 4,
+                                     w * 4, ph, hipMemcpyDeviceToDevice, s);
+            for (size_t y = 0; y < ph && e == hipSuccess; ++y)
+                e = hipMemsetD32Async(fat + (2 * r + 1) * rows + y * dst_stride[i], 0x3f800000u, w, s);
+            continue;
+        }
+        any = true;
+        for (int t = 0; t < ns && e == hipSuccess; ++t) {
+            const size_t at = static_cast<size_t>(slots.frame(t) % ns) * npx;
+            const int k = t * np + i;
+            e = hipMemcpy2DAsync(q.win + at, w * 4, src[k], static_cast<size_t>(src_stride[k]) * 4, w * 4, ph,
+                                 hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess && rclip)
+                e = hipMemcpy2DAsync(q.mwin + at, w * 4, rclip[k], static_cast<size_t>(rclip_stride[k]) * 4, w * 4,
+                                     ph, hipMemcpyDeviceToDevice, s);
+        }
+        for (int t = vb; h->ncsr && t < ve && e == hipSuccess; ++t) {
+            const size_t at = static_cast<size_t>(slots.frame(t) % ns) * npx;
+            e = hipMemcpyAsync(q.est + at, q.win + at, npx * 4, hipMemcpyDeviceToDevice, s);
+        }
+    }
+    if (e != hipSuccess) return hip_fail(e, "window copy");
+    if (!any) return NSS_HIP_OK;
+    const int rounds = h->ncsr ? h->iters : 1;
+    for (int it = 0; it < rounds; ++it) {
+        for (int i = 0; it > 0 && i < np; ++i) {
+            const auto& q = h->pl[i];
+            if (!q.active) continue;
+            const size_t npx = static_cast<size_t>(q.width) * q.height;
+            for (int t = vb; t < ve; ++t) {
+                const size_t at = static_cast<size_t>(slots.frame(t) % ns) * npx;
+                if ((e = launch_regularize(q.est + at, q.width, q.win + at, q.width, q.width, q.height, h->delta, s)) !=
+                    hipSuccess)
+                    return hip_fail(e, "NCSR regularize launch");
+            }
+        }
+        ZeroTable zt{};
+        WnnmTable wt{};
+        int zero_wg = 0, wnnm_wg = 0;
+        for (int i = 0; i < np; ++i) {
+            const auto& q = h->pl[i];
+            if (!q.active) continue;
+            const size_t npx = static_cast<size_t>(q.width) * q.height;
+            for (unsigned long long* buf : {q.num, q.den}) {
+                ZeroJob& z = zt.job[zt.n++];
+                z.p = buf;
+                z.count = static_cast<long long>(ns * npx);
+                z.wg_begin = zero_wg;
+                zero_wg += static_cast<int>((ns * npx + 1023) / 1024);
+            }
+            WnnmJob& j = wt.job[wt.n++];
+            j.src = nullptr;
+            j.width = q.width;
+            j.height = q.height;
+            j.stride = q.width;
+            j.nref = q.nref;
+            j.group = p.group_size;
+            j.block = p.block_size;
+            j.sigma = p.sigma[i] / 255.f;
+            j.residual = h->ncsr ? 1 : p.residual;
+            j.adaptive = p.adaptive;
+            j.wg_begin = wnnm_wg;
+            j.matches = q.recs;
+            j.counts = q.counts;
+            j.num = q.num;
+            j.den = q.den;
+            j.temporal = 1;
+            j.ring = FrameRing{h->ncsr ? q.est : q.win, static_cast<long long>(npx), ns};
+            j.slots = slots;
+            j.sum_slot = static_cast<long long>(npx);
+            wnnm_wg += q.nref;
+        }
+        if ((e = launch_zero(zt, zero_wg, s)) != hipSuccess) return hip_fail(e, "group filter clear");
+        // spatial top on the centre of the matching ring, then the predictive
+        // layers (nss_hip_predictive_match's two launches)
+        for (int i = 0; i < np; ++i) {
+            const auto& q = h->pl[i];
+            if (!q.active) continue;
+            const float* match = it == 0 && rclip ? q.mwin : h->ncsr ? q.est : q.win;
+            const FrameRing mring{match, static_cast<long long>(q.width) * q.height, ns};
+            MatchTable mt{};
+            MatchJob& m = mt.job[mt.n++];
+            m.plane = mring.frame(slots.frame(r));
+            m.stride = q.width;
+            m.width = q.width;
+            m.height = q.height;
+            m.grid = q.grid;
+            m.range = p.bm_range;
+            m.group = p.group_size;
+            m.t0 = r;
+            m.wg_begin = 0;
+            m.out = q.recs;
+            m.counts = q.counts;
+            size_t lds = 0;
+            const int kind =
+                choose_match_kind(p.block_size, p.block_step, p.bm_range, p.group_size, q.width, q.height, q.nref, &lds);
+            if ((e = launch_match(p.block_size, mt, match_workgroups(kind, q.grid), lds, kind, s)) != hipSuccess)
+                return hip_fail(e, "group filter match launch");
+            if (p.group_size == 1) continue;  // the reference alone (src/cpu/bm/matcher.hpp:323)
+            PredTable pt{};
+            PredJob& pj = pt.job[pt.n++];
+            pj.match = mring;
+            pj.width = q.width;
+            pj.height = q.height;
+            pj.stride = q.width;
+            pj.nref = q.nref;
+            pj.group = p.group_size;
+            pj.ps_num = p.ps_num;
+            pj.ps_range = p.ps_range;
+            pj.slots = slots;
+            pj.wg_begin = 0;
+            pj.recs = q.recs;
+            pj.counts = q.counts;
+            const size_t plds = predictive_table_lds(p.block_size, pt, r);
+            if ((e = launch_predictive(p.block_size, pt, q.nref, plds, s)) != hipSuccess)
+                return hip_fail(e, "group filter predictive launch");
+        }
+        if ((e = launch_wnnm(wt, wnnm_wg, h->ncsr, s)) != hipSuccess) return hip_fail(e, "group filter launch");
+        const int rc = it + 1 == rounds ? finish_slices(h, dst, dst_stride, s) : finish_estimates(h, slots, s);
+        if (rc != NSS_HIP_OK) return rc;
+    }
+    return NSS_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -249,9 +529,20 @@ int nss_hip_wnnm_process(nss_hip_wnnm* h, const float* const* src, const int* sr
     DeviceGuard dg(h->device);
     if (dg.err != hipSuccess) return hip_fail(dg.err, "set device");
     const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->p.radius > 0)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: radius > 0 filters through nss_hip_wnnm_process_temporal");
     if ((rc = copy_inactive(h, src, src_stride, dst, dst_stride, s)) != NSS_HIP_OK) return rc;
     return group_round(h, rclip ? rclip : src, rclip ? rclip_stride : src_stride, src, src_stride, src, src_stride,
                        dst, dst_stride, s);
+}
+
+// src/host/filter_wnnm.cpp:180-260 at radius > 0: the contribution of one
+// centre frame (temporal_contribution).
+int nss_hip_wnnm_process_temporal(nss_hip_wnnm* h, int center, const float* const* src, const int* src_stride,
+                                  const float* const* rclip, const int* rclip_stride, float* const* dst,
+                                  const int* dst_stride, void* stream) {
+    if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip.WNNM: null argument");
+    return temporal_contribution(h, "WNNM", center, src, src_stride, rclip, rclip_stride, dst, dst_stride, stream);
 }
 
 int nss_hip_wnnm_destroy(nss_hip_wnnm* h) {
@@ -280,6 +571,7 @@ int nss_hip_ncsr_create(const nss_hip_ncsr_params* params, const nss_hip_video_i
     p.group_size = params->group_size;
     p.bm_range = params->bm_range;
     p.radius = params->radius;
+    p.temporal_mode = params->temporal_mode;
     p.ps_num = params->ps_num;
     p.ps_range = params->ps_range;
     p.residual = 1;
@@ -313,6 +605,8 @@ int nss_hip_ncsr_process(nss_hip_ncsr* h, const float* const* src, const int* sr
     DeviceGuard dg(h->device);
     if (dg.err != hipSuccess) return hip_fail(dg.err, "set device");
     const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->p.radius > 0)
+        return fail(NSS_HIP_INVALID_INPUT, "nss_hip.NCSR: radius > 0 filters through nss_hip_ncsr_process_temporal");
     if ((rc = copy_inactive(h, src, src_stride, dst, dst_stride, s)) != NSS_HIP_OK) return rc;
     const float* est_c[3] = {};
     float* est_m[3] = {};
@@ -344,6 +638,15 @@ int nss_hip_ncsr_process(nss_hip_ncsr* h, const float* const* src, const int* sr
         if (rc != NSS_HIP_OK) return rc;
     }
     return NSS_HIP_OK;
+}
+
+// src/host/filter_ncsr.cpp:34-190 at radius > 0: the contribution of one
+// centre frame (temporal_contribution).
+int nss_hip_ncsr_process_temporal(nss_hip_ncsr* h, int center, const float* const* src, const int* src_stride,
+                                  const float* const* rclip, const int* rclip_stride, float* const* dst,
+                                  const int* dst_stride, void* stream) {
+    if (!h) return fail(NSS_HIP_INVALID_INPUT, "nss_hip.NCSR: null argument");
+    return temporal_contribution(h, "NCSR", center, src, src_stride, rclip, rclip_stride, dst, dst_stride, stream);
 }
 
 int nss_hip_ncsr_destroy(nss_hip_ncsr* h) {

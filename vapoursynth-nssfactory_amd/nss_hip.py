@@ -96,7 +96,7 @@ class WnnmParams(ctypes.Structure):
                 ("group_size", ctypes.c_int), ("bm_range", ctypes.c_int), ("radius", ctypes.c_int),
                 ("ps_num", ctypes.c_int), ("ps_range", ctypes.c_int), ("residual", ctypes.c_int),
                 ("adaptive", ctypes.c_int), ("has_rclip", ctypes.c_int), ("memory_limit_mb", ctypes.c_int),
-                ("device_id", ctypes.c_int)]
+                ("device_id", ctypes.c_int), ("temporal_mode", ctypes.c_int)]
 
     def as_dict(self):
         out = {}
@@ -111,7 +111,8 @@ class NcsrParams(ctypes.Structure):
                 ("group_size", ctypes.c_int), ("bm_range", ctypes.c_int), ("radius", ctypes.c_int),
                 ("ps_num", ctypes.c_int), ("ps_range", ctypes.c_int), ("iters", ctypes.c_int),
                 ("delta", ctypes.c_float), ("has_rclip", ctypes.c_int), ("memory_limit_mb", ctypes.c_int),
-                ("device_id", ctypes.c_int)]
+                ("device_id", ctypes.c_int),
+                ("temporal_mode", ctypes.c_int)]
 
     as_dict = WnnmParams.as_dict
 
@@ -127,7 +128,8 @@ EXPORTS = ("nss_hip_last_error", "nss_hip_backend_info", "nss_hip_bm3d_parse", "
            "nss_hip_wnnm_create", "nss_hip_wnnm_process", "nss_hip_wnnm_destroy", "nss_hip_ncsr_parse",
            "nss_hip_ncsr_create", "nss_hip_ncsr_process", "nss_hip_ncsr_destroy", "nss_hip_bm3d_set_tiles",
            "nss_hip_predictive_match", "nss_hip_bm3d_process_noalloc", "nss_hip_debug_fail_alloc",
-           "nss_hip_debug_fail_launch", "nss_hip_debug_live_allocations", "nss_hip_bm3d_chunk", "nss_hip_bm3d_seek")
+           "nss_hip_debug_fail_launch", "nss_hip_debug_live_allocations", "nss_hip_bm3d_chunk", "nss_hip_bm3d_seek",
+           "nss_hip_wnnm_process_temporal", "nss_hip_ncsr_process_temporal")
 # Contribution identity of legacy temporal frames (frame properties the
 # VapourSynth glue stamps; include/nss/contracts.hpp:7-20,
 # src/frontend/contribution.hpp:12-27).
@@ -191,6 +193,8 @@ def lib():
         L.nss_hip_wnnm_create.argtypes = [P(WnnmParams), P(VideoInfo), ci, P(vp)]
         L.nss_hip_wnnm_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_wnnm_destroy.argtypes = [vp]
+        for fn in (L.nss_hip_wnnm_process_temporal, L.nss_hip_ncsr_process_temporal):
+            fn.argtypes = [vp, ci, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), vp]
         L.nss_hip_ncsr_parse.argtypes = [P(VideoInfo), P(VideoInfo), P(Arg), ci, ctypes.c_char_p, P(NcsrParams)]
         L.nss_hip_ncsr_create.argtypes = [P(NcsrParams), P(VideoInfo), ci, P(vp)]
         L.nss_hip_ncsr_process.argtypes = [vp, P(vp), P(ci), P(vp), P(ci), P(vp), P(ci), vp]
@@ -561,7 +565,8 @@ class WNNM:
     """nss.WNNM on the device (src/cuda/wnnm/filter.cpp:16-70): one instance
     per format and size. Calling it filters one frame: ``planes`` is a list
     of 2-D float32 CUDA tensors; ``rclip`` (instances made with
-    ``rclip=True``) guides the matching only. Radius 0 (spatial)."""
+    ``rclip=True``) guides the matching only. At radius > 0, ``contribution``
+    gives a centre frame's num/den slices (the fat output)."""
 
     _NAME = "WNNM"
 
@@ -614,6 +619,44 @@ class WNNM:
         dp = VP(*[p.data_ptr() for p in out])
         dst = IP(*[p.stride(0) for p in out])
         _check(self._call("process", self._h, src, sst, rp, rst, dp, dst, _stream_handle(stream)))
+        return out
+
+    def slot_frames(self, center):
+        """Clip frames of slots 0 .. 2 radius around `center`
+        (src/frontend/temporal.hpp:19-21)."""
+        r, n = self.params.radius, self.info.num_frames
+        return [center + min(max(t - r, -center), n - 1 - center) for t in range(2 * r + 1)]
+
+    def contribution(self, center, frames, rclip=None, out=None, stream=None):
+        """radius > 0: centre frame `center`'s contribution from `frames`, the
+        planes of its 2 radius + 1 slot frames (slot_frames order), matched
+        on `rclip`'s likewise: per plane, 2 (2 radius + 1) plane heights, slot
+        t's num at rows 2t h and den at (2t + 1) h."""
+        import torch
+        r = self.params.radius
+        ns, n = 2 * r + 1, len(self.plane_dims)
+        if len(frames) != ns or (rclip is not None and len(rclip) != ns):
+            raise Error(f"{NAMESPACE}.{self._NAME}: expected {ns} slot frames")
+        if rclip is not None and not self.params.has_rclip:
+            raise Error(f"{NAMESPACE}.{self._NAME}: this instance was made without rclip")
+        for f in frames:
+            self._check_planes(f)
+        if rclip is not None:
+            for f in rclip:
+                self._check_planes(f, "rclip plane")
+        dims = [(2 * ns * hh, ww) for hh, ww in self.plane_dims]
+        if out is None:
+            out = [torch.empty(d, dtype=torch.float32, device=frames[0][0].device) for d in dims]
+        _check_out(self._NAME, out, dims)
+        VP, IP = ctypes.c_void_p, ctypes.c_int
+        src = (VP * (ns * n))(*[p.data_ptr() for f in frames for p in f])
+        sst = (IP * (ns * n))(*[p.stride(0) for f in frames for p in f])
+        rp = (VP * (ns * n))(*[p.data_ptr() for f in rclip for p in f]) if rclip is not None else None
+        rst = (IP * (ns * n))(*[p.stride(0) for f in rclip for p in f]) if rclip is not None else None
+        dp = (VP * n)(*[p.data_ptr() for p in out])
+        dst = (IP * n)(*[p.stride(0) for p in out])
+        _check(self._call("process_temporal", self._h, int(center), src, sst, rp, rst, dp, dst,
+                          _stream_handle(stream)))
         return out
 
     def close(self):
