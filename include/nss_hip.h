@@ -452,8 +452,10 @@ int nss_hip_ncsr_process_temporal(nss_hip_ncsr* handle, int center, const float*
  * thread admits queued frames into device frame slots under an HBM budget,
  * packs the radius-0 frames admitted so far into one launch sequence per
  * stage round (planes of several clips, sizes and sigmas in the same
- * kernels) while earlier batches still run, and replays a captured hipGraph
- * for a batch whose composition was seen before. Radius > 0 clips keep their
+ * kernels) while earlier batches still run, and runs a batch whose shape
+ * (instances and filtered planes, in order) was seen before as a hipGraph:
+ * instantiated once per shape, then updated with each batch's buffers and
+ * sigmas (hipGraphExecUpdate) and replayed. Radius > 0 clips keep their
  * rolling state on the device and run on a HIP stream of their own. */
 typedef struct nss_hip_server nss_hip_server;
 
@@ -462,11 +464,13 @@ typedef struct nss_hip_server_config {
     long long hbm_budget_bytes; /* frame slots + rolling state; 0 = a quarter of the device */
     int max_batch_frames;       /* radius-0 frames per launch sequence, 1..16 */
     int max_inflight;           /* batches on the device at once, 1..4 */
-    int use_graphs;             /* capture / replay hipGraphs per batch composition */
+    int use_graphs;             /* run repeated batch shapes as hipGraphs */
     int autotune;               /* run nss_hip_bm3d_autotune when a radius-0 stream opens */
 } nss_hip_server_config;
 
 typedef struct nss_hip_server_stats {
+    /* graph_captures: graphs instantiated; graph_replays: batches run on an
+     * existing executable graph after updating its arguments */
     long long submitted, completed, batches, batched_frames, graph_replays, graph_captures;
     long long bytes_in_use, peak_bytes, admissions_deferred;
     long long slots_released; /* idle frame slots released to admit another stream's frame */

@@ -117,7 +117,8 @@ def test_full_size_streams_mix_equals_alone(gpu):
 
 
 def test_graph_replays_match_eager(gpu):
-    """The same composition replayed several times, with eager work and
+    """One batch shape replayed several times with alternating buffers and
+    sigmas (each replay updates the executable graph's arguments), with eager work and
     synchronisations between the replays; every replay equals the eager
     (no-graph) result."""
     import torch
@@ -136,7 +137,7 @@ def test_graph_replays_match_eager(gpu):
                 for p in f:
                     p.fill_(-1.0)
             torch.cuda.synchronize()
-            rids = [srv.submit(sid, src[k], out[k], sigma=5.0) for k in range(2)]
+            rids = [srv.submit(sid, src[k], out[k], sigma=(5.0, 10.0)[(rep + k) % 2]) for k in range(2)]
             srv.drain()
             assert all(srv.query(r) is not None for r in rids)
             got.append([[p.cpu().numpy() for p in f] for f in out])
@@ -147,8 +148,8 @@ def test_graph_replays_match_eager(gpu):
         st = srv.stats()
         print(f"use_graphs={use_graphs} stats {st}", flush=True)
         if use_graphs:
-            # each composition: direct, captured, then replayed twice
-            assert st["graph_captures"] == 2 and st["graph_replays"] == 4
+            # the one shape: direct, captured once, then six updated replays
+            assert st["graph_captures"] == 1 and st["graph_replays"] == 6
         srv.close()
     for rep in range(4):
         for fe, fg in zip(outs[False][rep], outs[True][rep]):

@@ -12,9 +12,10 @@
 //   launch  the admitted frames as one batch (run_batch: per stage round one
 //           launch sequence over the planes of every frame, sizes and sigmas
 //           mixed) on one of max_inflight HIP streams, so a batch starts while
-//           the previous ones still run; a batch whose composition (frames,
-//           planes, slots, sigmas) is seen a second time is captured as a
-//           hipGraph, and replayed from then on.
+//           the previous ones still run; from the second batch of one shape
+//           (the same instances and filtered planes, in order) on, the batch
+//           is captured as a hipGraph whose executable form is instantiated
+//           once per shape and updated with each batch's buffers and sigmas.
 // Radius > 0 streams keep their rolling state in their own instance and run
 // push / pop on a HIP stream of their own, in frame order.
 #include <hip/hip_runtime.h>
@@ -237,30 +238,29 @@ struct nss_hip_server {
         return true;
     }
 
-    // Everything a batch's kernel arguments depend on, field by field.
+    // A batch's shape: what its launch sequence depends on (the instances of
+    // its frames, in order, and which planes of each stage filter). Buffers,
+    // strides and sigma values are kernel arguments only.
     static std::string graph_key(const std::vector<FrameRef>& frames) {
         std::string key;
-        auto put = [&key](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
         for (const FrameRef& f : frames) {
-            put(&f.h, sizeof(f.h));
+            key.append(reinterpret_cast<const char*>(&f.h), sizeof(f.h));
+            char flags = 0;
             for (int i = 0; i < 3; ++i) {
-                put(&f.src[i], sizeof(f.src[i]));
-                put(&f.src_stride[i], sizeof(int));
-                put(&f.dst[i], sizeof(f.dst[i]));
-                put(&f.dst_stride[i], sizeof(int));
-                for (const void* b : {static_cast<const void*>(f.buf.num[i]), static_cast<const void*>(f.buf.den[i]),
-                                      static_cast<const void*>(f.buf.part[i]), static_cast<const void*>(f.buf.matches[i]),
-                                      static_cast<const void*>(f.buf.counts[i]), static_cast<const void*>(f.buf.basic[i])})
-                    put(&b, sizeof(b));
-                put(&f.sigma[0][i], sizeof(float));
-                put(&f.sigma[1][i], sizeof(float));
+                flags = static_cast<char>(flags | (f.sigma[0][i] != 0.f ? 1 << i : 0) |
+                                          (f.sigma[1][i] != 0.f ? 8 << i : 0));
             }
+            key.push_back(flags);
+            key.push_back(f.ref[0] || f.ref[1] || f.ref[2] ? 1 : 0);
         }
         return key;
     }
 
-    // A composition runs direct the first time it is seen and is captured the
-    // second time; later sightings replay the graph.
+    // A shape runs direct the first time it is seen. From the second time on
+    // the batch is captured into a graph: the first capture is instantiated,
+    // later ones hand their arguments (buffers, strides, sigmas) to that
+    // executable graph through hipGraphExecUpdate and replay it; an update
+    // the runtime refuses (a different launch sequence) re-instantiates.
     int run_graph(const std::vector<FrameRef>& frames, hipStream_t s) {
         std::string key = graph_key(frames);
         auto it = graphs.find(key);
@@ -271,35 +271,53 @@ struct nss_hip_server {
                 seen_once.insert(key);
                 return run_batch(frames.data(), static_cast<int>(frames.size()), nullptr, s);
             }
-            seen_once.erase(seen);
+        }
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) return hip_fail(e, "graph capture");
+        const int rc = run_batch(frames.data(), static_cast<int>(frames.size()), nullptr, s);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(s, &g);
+        if (rc != NSS_HIP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "graph capture");
+        bool fresh = it == graphs.end();
+        if (!fresh) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            e = hipGraphExecUpdate(it->second.exec, g, &bad, &res);
+            if (e != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();  // the refusal is not a failure of the batch
+                (void)hipGraphExecDestroy(it->second.exec);
+                graphs.erase(it);
+                it = graphs.end();
+                fresh = true;
+            }
+        }
+        if (fresh) {
+            seen_once.erase(key);
             if (graphs.size() >= kGraphCacheLimit) {
                 auto lru = graphs.begin();
-                for (auto g = graphs.begin(); g != graphs.end(); ++g)
-                    if (g->second.last_use < lru->second.last_use) lru = g;
+                for (auto q = graphs.begin(); q != graphs.end(); ++q)
+                    if (q->second.last_use < lru->second.last_use) lru = q;
                 (void)hipGraphExecDestroy(lru->second.exec);
                 graphs.erase(lru);
             }
-            hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
-            if (e != hipSuccess) return hip_fail(e, "graph capture");
-            const int rc = run_batch(frames.data(), static_cast<int>(frames.size()), nullptr, s);
-            hipGraph_t g = nullptr;
-            e = hipStreamEndCapture(s, &g);
-            if (rc != NSS_HIP_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            if (e != hipSuccess) return hip_fail(e, "graph capture");
             GraphEntry ge;
             e = hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return hip_fail(e, "graph instantiate");
+            }
             it = graphs.emplace(std::move(key), ge).first;
             ++st.graph_captures;
         } else {
             ++st.graph_replays;
         }
+        (void)hipGraphDestroy(g);
         it->second.last_use = ++use_clock;
-        const hipError_t e = hipGraphLaunch(it->second.exec, s);
+        e = hipGraphLaunch(it->second.exec, s);
         return e == hipSuccess ? NSS_HIP_OK : hip_fail(e, "graph launch");
     }
 
