@@ -3215,35 +3215,57 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
     using Acc = typename std::conditional<EXACT, long long, float>::type;
     Acc n[4] = {0, 0, 0, 0}, d[4] = {0, 0, 0, 0};
     const size_t area = static_cast<size_t>(tg.pitch) * tg.ah;
-    for (int ty = tyh; ty >= ty_lo; --ty) {
-        const int oy = y - tg.ay0(ty, jb.height, B, s, r);
-        if (oy >= tg.ah) break;
-        for (int tx = txh; tx >= 0; --tx) {
-            const int ax0 = tg.ax0(tx, jb.width, B, s, r);
-            if (ax0 + tg.aw <= x0) break;
-            // column of x0: a multiple of 4 in [0, pitch - 4]
-            const size_t o = (static_cast<size_t>(ty - ty_lo) * tg.ntx + tx) * area +
-                             static_cast<size_t>(oy) * tg.pitch + (x0 - ax0 + (ax0 & 3));
-            Acc pn[4], pd[4];
-            if constexpr (EXACT) {
-                const longlong2* qn = reinterpret_cast<const longlong2*>(jb.exact_num + o);
-                const longlong2* qd = reinterpret_cast<const longlong2*>(jb.exact_den + o);
-                const longlong2 n0 = qn[0], n1 = qn[1], d0 = qd[0], d1 = qd[1];
-                pn[0] = n0.x, pn[1] = n0.y, pn[2] = n1.x, pn[3] = n1.y;
-                pd[0] = d0.x, pd[1] = d0.y, pd[2] = d1.x, pd[3] = d1.y;
-            } else {
-                const f32x4 vn = *reinterpret_cast<const f32x4*>(jb.part_num + o);
-                const f32x4 vd = *reinterpret_cast<const f32x4*>(jb.part_den + o);
+    // The covering tiles are the rectangle [tyl, tyh] x [txl, txh]; walked
+    // in descending (ty, tx) order, two tiles' partials are loaded before
+    // either is added, so their memory latencies overlap (four at a time
+    // need 106 VGPRs and ran slower).
+    int tyl = tyh + 1;
+    while (tyl - 1 >= ty_lo && y - tg.ay0(tyl - 1, jb.height, B, s, r) < tg.ah) --tyl;
+    int txl = txh + 1;
+    while (txl - 1 >= 0 && tg.ax0(txl - 1, jb.width, B, s, r) + tg.aw > x0) --txl;
+    const int nx = txh - txl + 1;
+    const int count = tyl <= tyh ? (tyh - tyl + 1) * nx : 0;
+    for (int k0 = 0; k0 < count; k0 += 2) {
+        Acc pn[2][4], pd[2][4];
+        int lo[2];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) pn[i] = vn[i], pd[i] = vd[i];
-            }
+        for (int u = 0; u < 2; ++u) {
+            const int k = k0 + u;
+            lo[u] = x0 + 4;  // no pixel of the four inside
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const bool in = static_cast<unsigned>(x0 + i - ax0) < static_cast<unsigned>(tg.aw);
-                n[i] += in ? pn[i] : Acc(0);
-                d[i] += in ? pd[i] : Acc(0);
+            for (int i = 0; i < 4; ++i) pn[u][i] = pd[u][i] = Acc(0);
+            if (k < count) {
+                const int ty = tyh - k / nx, tx = txh - k % nx;
+                const int oy = y - tg.ay0(ty, jb.height, B, s, r);
+                const int ax0 = tg.ax0(tx, jb.width, B, s, r);
+                lo[u] = ax0;
+                // column of x0: a multiple of 4 in [0, pitch - 4]
+                const size_t o = (static_cast<size_t>(ty - ty_lo) * tg.ntx + tx) * area +
+                                 static_cast<size_t>(oy) * tg.pitch + (x0 - ax0 + (ax0 & 3));
+                if constexpr (EXACT) {
+                    const longlong2* qn =
+                        static_cast<const longlong2*>(__builtin_assume_aligned(jb.exact_num + o, 32));
+                    const longlong2* qd =
+                        static_cast<const longlong2*>(__builtin_assume_aligned(jb.exact_den + o, 32));
+                    const longlong2 n0 = qn[0], n1 = qn[1], d0 = qd[0], d1 = qd[1];
+                    pn[u][0] = n0.x, pn[u][1] = n0.y, pn[u][2] = n1.x, pn[u][3] = n1.y;
+                    pd[u][0] = d0.x, pd[u][1] = d0.y, pd[u][2] = d1.x, pd[u][3] = d1.y;
+                } else {
+                    const f32x4 vn = *reinterpret_cast<const f32x4*>(jb.part_num + o);
+                    const f32x4 vd = *reinterpret_cast<const f32x4*>(jb.part_den + o);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) pn[u][i] = vn[i], pd[u][i] = vd[i];
+                }
             }
         }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bool in = k0 + u < count && static_cast<unsigned>(x0 + i - lo[u]) < static_cast<unsigned>(tg.aw);
+                n[i] += in ? pn[u][i] : Acc(0);
+                d[i] += in ? pd[u][i] : Acc(0);
+            }
     }
     if constexpr (EXACT) {
         if (batch) {
