@@ -86,11 +86,13 @@ struct FilterJob {
     // pixel, weight at each patch's top-left corner, 2^32 units.
     unsigned long long* num;
     unsigned long long* den;
-    // Per-tile partial sums (the tile path): [tile][ah][pitch] int64 each,
-    // 2^32 units, exact (the finish adds them exactly).
+    // Per-tile partial numerator sums (the tile path): [tile][ah][pitch]
+    // int64, 2^32 units, exact (the finish adds them exactly), and each
+    // group's weight (2^32 units) at its record index, from which the finish
+    // rebuilds the weight sums.
     TileGeom tiles;
     unsigned long long* part_num;
-    unsigned long long* part_den;
+    unsigned long long* gweight;
     // A batch of tile rows (memory-limited plan): workgroup w filters tile
     // tile0 + w of the frame and writes partials [w]; the records hold the
     // grid rows from grid.row0 on. Both 0 for a whole frame.
@@ -113,8 +115,16 @@ struct FilterTable {
 struct FinishJob {
     const unsigned long long* num;  // atomic path
     const unsigned long long* den;
-    const long long* exact_num;     // spatial tile path: exact int64 partials
-    const long long* exact_den;
+    const long long* exact_num;     // spatial tile path: exact int64 numerator partials
+    // ... and the weights: every record j < count of the references of
+    // `grid` (rows grid.row0 .. grid.row0 + grid.ny - 1, records at
+    // ((row - grid.row0) * grid.nx + column) * group) adds gweight of its
+    // reference at the patch's top-left corner, box-summed block x block.
+    const MatchRec* matches;
+    const int* counts;
+    const unsigned long long* gweight;
+    Grid grid;
+    int group;
     const float* part_num;          // temporal tile path: float slot partials
     const float* part_den;
     TileGeom tiles;

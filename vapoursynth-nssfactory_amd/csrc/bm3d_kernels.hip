@@ -1485,56 +1485,18 @@ __device__ inline void lds_add(unsigned long long* p, unsigned long long v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Tile epilogue: the tile's exact int64 sums as its partials (2^32 units), so
-// that the finish adds every pixel's sums exactly and the output does not
-// depend on how the grid was cut into tiles. num goes out first; its LDS area
-// then holds the horizontal B-wide sums of the corner weights, and den is
-// their vertical B-high sum. pn / pd point at column ax0 & 3 of the tile's
-// first row (rows of `pitch` entries). Called by every thread of the block
-// after the accumulation's barrier.
-template <int B>
-__device__ inline void partials_out(int aw, int ah, int pitch, unsigned long long* num,
-                                    const unsigned long long* corner, unsigned long long* pn,
-                                    unsigned long long* pd) {
-    const int cw = aw - B + 1, ch = ah - B + 1;
+// Tile epilogue: the tile's exact int64 numerator sums as its partials (2^32
+// units), so that the finish adds every pixel's sums exactly and the output
+// does not depend on how the grid was cut into tiles. The weights are not
+// written per pixel: each group's weight goes out once (FilterJob::gweight)
+// and the finish box-sums them from the records. pn points at column ax0 & 3
+// of the tile's first row (rows of `pitch` entries). Called by every thread
+// of the block after the accumulation's barrier.
+__device__ inline void partials_out(int aw, int ah, int pitch, const unsigned long long* num, unsigned long long* pn) {
     const int nt = static_cast<int>(blockDim.x);
     for (int e = threadIdx.x; e < aw * ah; e += nt) {
         const int y = e / aw, x = e - y * aw;
         pn[y * pitch + x] = num[e];
-    }
-    __syncthreads();
-    unsigned long long* hsum = num;  // ch rows of aw
-    // Horizontal: a thread slides the B-wide window over 8 outputs of a row.
-    const int segs = (aw + 7) >> 3;
-    for (int e = threadIdx.x; e < ch * segs; e += nt) {
-        const int y = e / segs, x0 = (e - y * segs) * 8;
-        const unsigned long long* row = corner + y * cw;
-        unsigned long long w[B + 7];  // corners x0 - B + 1 .. x0 + 7
-#pragma unroll
-        for (int i = 0; i < B + 7; ++i) {
-            const int cx = x0 - B + 1 + i;
-            w[i] = (cx >= 0 && cx < cw) ? row[cx] : 0ull;
-        }
-        unsigned long long sum = 0;
-#pragma unroll
-        for (int i = 0; i < B - 1; ++i) sum += w[i];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            sum += w[B - 1 + j];
-            if (x0 + j < aw) hsum[y * aw + x0 + j] = sum;
-            sum -= w[j];
-        }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < aw * ah; e += nt) {
-        const int y = e / aw, x = e - y * aw;
-        unsigned long long d = 0;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int cy = y - i;
-            if (cy >= 0 && cy < ch) d += hsum[cy * aw + x];
-        }
-        pd[y * pitch + x] = d;
     }
 }
 
@@ -1588,16 +1550,13 @@ __device__ inline void partials_out_float(int aw, int ah, int pitch, const unsig
     }
 }
 
-template <int B>
-__device__ inline void tile_partials(const FilterJob& jb, int tile, int ax0, unsigned long long* num,
-                                     const unsigned long long* corner) {
+__device__ inline void tile_partials(const FilterJob& jb, int tile, int ax0, const unsigned long long* num) {
     const size_t base = static_cast<size_t>(tile) * jb.tiles.pitch * jb.tiles.ah + (ax0 & 3);
-    partials_out<B>(jb.tiles.aw, jb.tiles.ah, jb.tiles.pitch, num, corner, jb.part_num + base, jb.part_den + base);
+    partials_out(jb.tiles.aw, jb.tiles.ah, jb.tiles.pitch, num, jb.part_num + base);
 }
 
-// Tile path: sums of a tile of groups in LDS, then exact partials.
-// LDS: num [ah][aw] int64 | den corners [ah-B+1][aw-B+1] int64 | per-wave
-// cubes (reused after filtering for the den box sum).
+// Tile path: numerator sums of a tile of groups in LDS, then exact partials;
+// each group's weight to gweight. LDS: num [ah][aw] int64 | per-wave cubes.
 template <int B, bool WIENER>
 __global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
     extern __shared__ __align__(16) unsigned long long acc[];
@@ -1609,14 +1568,12 @@ __global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
     const int tile = lb - jb.wg_begin;  // partials index
     const int tx = (tile + jb.tile0) % tg.ntx, ty = (tile + jb.tile0) / tg.ntx;
     const int aw = tg.aw, ah = tg.ah;
-    const int cw = aw - B + 1, ch = ah - B + 1;
     const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
     const int ay0 = tg.ay0(ty, jb.height, B, jb.grid.step, jb.range);
     unsigned long long* num = acc;
-    unsigned long long* corner = acc + aw * ah;
-    float* scratch = reinterpret_cast<float*>(corner + cw * ch);
+    float* scratch = reinterpret_cast<float*>(acc + aw * ah);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int e = threadIdx.x; e < aw * ah + cw * ch; e += 256) acc[e] = 0ull;
+    for (int e = threadIdx.x; e < aw * ah; e += 256) acc[e] = 0ull;
     __syncthreads();
 
     const int G = jb.group;
@@ -1638,12 +1595,12 @@ __global__ __launch_bounds__(256) void filter_tile_kernel(FilterTable tab) {
                 const int py = e / B, px = e - (e / B) * B;
                 lds_add(num + (ly + py) * aw + lx + px, to_fixed(w * cube[j * PS + e]));
             }
-            if (lane == 0) lds_add(corner + ly * cw + lx, wfix);
         }
+        if (lane == 0) jb.gweight[rid] = wfix;
         wave_sync();
     }
     __syncthreads();
-    tile_partials<B>(jb, tile, ax0, num, corner);
+    tile_partials(jb, tile, ax0, num);
 }
 
 // ---------------------------------------------------------------------------
@@ -1805,9 +1762,8 @@ __device__ inline void gather_rows_b8(const float* plane, int stride, const Matc
     }
 }
 
-// Tile kernel for block 8: LDS = num [ah][aw] int64 | corner [ch][cw] int64 |
-// match records of the tile | per-wave transpose buffers (reused for the
-// den box sum).
+// Tile kernel for block 8: LDS = num [ah][aw] int64 | match records of the
+// tile | per-wave transpose buffers.
 template <int G, bool WIENER>
 __global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
     extern __shared__ __align__(16) unsigned long long acc[];
@@ -1819,20 +1775,18 @@ __global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
     const int tile = lb - jb.wg_begin;  // partials index
     const int tx = (tile + jb.tile0) % tg.ntx, ty = (tile + jb.tile0) / tg.ntx;
     const int aw = tg.aw, ah = tg.ah;
-    const int cw = aw - B + 1, ch = ah - B + 1;
     const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
     const int ay0 = tg.ay0(ty, jb.height, B, jb.grid.step, jb.range);
     const int i_begin = tx * tg.tw, j_begin = ty * tg.th;
     const int ni = min(tg.tw, jb.grid.nx - i_begin), nj = min(tg.th, jb.grid.ny - j_begin);
     const int ngroups = ni * nj;
     unsigned long long* num = acc;
-    unsigned long long* corner = acc + aw * ah;
-    MatchRec* recs = reinterpret_cast<MatchRec*>(corner + cw * ch);
+    MatchRec* recs = reinterpret_cast<MatchRec*>(acc + aw * ah);
     int* cnt = reinterpret_cast<int*>(recs + tg.tw * tg.th * G);
     float* scratch = reinterpret_cast<float*>(cnt + ((tg.tw * tg.th + 3) & ~3));
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* T = scratch + wave * (G * kTs);
-    for (int e = threadIdx.x; e < aw * ah + cw * ch; e += 256) acc[e] = 0ull;
+    for (int e = threadIdx.x; e < aw * ah; e += 256) acc[e] = 0ull;
     // The tile's match records and counts, row by row of the grid.
     for (int e = threadIdx.x; e < ngroups * G; e += 256) {
         const int g = e / G, j = e - g * G;
@@ -1911,12 +1865,15 @@ __global__ __launch_bounds__(256) void filter_tile_b8_kernel(FilterTable tab) {
                 unsigned long long* row = num + (ly + y) * aw + lx;
 #pragma unroll
                 for (int x = 0; x < 8; ++x) lds_add(row + x, to_fixed(w * u[m][x]));
-                if (y == 0) lds_add(corner + ly * cw + lx, to_fixed(w));
             }
+        }
+        if (lane == 0) {
+            const int gj = g / ni, gi = g - gj * ni;
+            jb.gweight[(j_begin + gj - jb.grid.row0) * jb.grid.nx + i_begin + gi] = to_fixed(w);
         }
     }
     __syncthreads();
-    tile_partials<B>(jb, tile, ax0, num, corner);
+    tile_partials(jb, tile, ax0, num);
 }
 
 // ---------------------------------------------------------------------------
@@ -2261,7 +2218,6 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     const int tile = lb - jb.wg_begin;  // partials index
     const int tx = (tile + jb.tile0) % tg.ntx, ty = (tile + jb.tile0) / tg.ntx;
     const int aw = tg.aw, ah = tg.ah;
-    const int cw = aw - B + 1, ch = ah - B + 1;
     const int ax0 = tg.ax0(tx, jb.width, B, jb.grid.step, jb.range);
     const int ay0 = tg.ay0(ty, jb.height, B, jb.grid.step, jb.range);
     const int i_begin = tx * tg.tw, j_begin = ty * tg.th;
@@ -2269,15 +2225,14 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
     const int ngroups = ni * nj;
     const int nunits = G == 8 ? (ngroups + 1) / 2 : ngroups;
     unsigned long long* num = acc;
-    unsigned long long* corner = acc + aw * ah;
     // Patch positions (x | y << 16) and counts of the tile's groups, then the
     // transpose buffers, 16-byte aligned.
-    uint32_t* recs = reinterpret_cast<uint32_t*>(acc + ((aw * ah + cw * ch + 1) & ~1));
+    uint32_t* recs = reinterpret_cast<uint32_t*>(acc + ((aw * ah + 1) & ~1));
     int* cnt = reinterpret_cast<int*>(recs + ((tg.tw * tg.th * G + 3) & ~3));
     float* scratch = reinterpret_cast<float*>(cnt + ((tg.tw * tg.th + 3) & ~3));
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* T = scratch + wave * kMfmaBuf;
-    for (int e = threadIdx.x; e < aw * ah + cw * ch; e += 256) acc[e] = 0ull;
+    for (int e = threadIdx.x; e < aw * ah; e += 256) acc[e] = 0ull;
     for (int e = threadIdx.x; e < ngroups * G; e += 256) {
         const int g = e / G, j = e - g * G;
         const int gj = g / ni, gi = g - gj * ni;
@@ -2456,12 +2411,19 @@ __global__ __launch_bounds__(256) void filter_tile_mfma_kernel(FilterTable tab) 
                 unsigned long long* cell = num + (ly + y0) * aw + lx + px;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) lds_add(cell + j * aw, to_fixed(fmaf(wq, xo[q][j], wm)));
-                if (px == 0 && y0 == 0) lds_add(corner + ly * cw + lx, to_fixed(w));
+            }
+        }
+        // the unit's group weights (two groups of 8 or one of 16)
+        if (lane < (G == 8 ? 2 : 1)) {
+            const int g = G == 8 ? 2 * u + lane : u;
+            if (g < ngroups) {
+                const int gj = g / ni, gi = g - gj * ni;
+                jb.gweight[(j_begin + gj - jb.grid.row0) * jb.grid.nx + i_begin + gi] = to_fixed(lane ? wg[1] : wg[0]);
             }
         }
     }
     __syncthreads();
-    tile_partials<B>(jb, tile, ax0, num, corner);
+    tile_partials(jb, tile, ax0, num);
 }
 
 // Atomic path: one wave per group, frame-sized fixed-point sums (den at the
@@ -3187,22 +3149,81 @@ __global__ __launch_bounds__(256) void finish_temporal_tiles_kernel(TFinishTable
 // Tile path: 64 x 16 pixels per block, four consecutive pixels of a row per
 // thread; per pixel, the partials of every tile whose rectangle covers it
 // (tiles in descending (ty, tx) order), read as aligned vectors per tile for
-// the four pixels. EXACT: the spatial path's int64 partials, summed exactly
-// and converted once, num / den in double rounded to float (the atomic path's
-// formula, finish_corner_kernel); otherwise the temporal slot partials
+// the four pixels. EXACT: the spatial path's int64 numerator partials, summed
+// exactly, and the weights from the records: every patch of a reference
+// whose window reaches the block adds its group's weight (gweight) at its
+// top-left corner in an LDS corner tile, which is box-summed B x B; then one
+// conversion and num / den in double rounded to float (the atomic path's
+// formula, finish_corner_kernel). Otherwise the temporal slot partials
 // (floats) in slice mode.
 template <bool EXACT>
 __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
+    extern __shared__ __align__(16) unsigned long long fsm[];
     // XCD-contiguous blocks: neighbouring blocks read the same tile partial
     // lines through one L2.
     const int lb = xcd_block();
     const FinishJob& jb = tab.job[find_job(tab, lb)];
     const int t = lb - jb.wg_begin;
-    const int x0 = (t % jb.tiles_x) * 64 + (threadIdx.x & 15) * 4;
-    const int y = jb.py0 + (t / jb.tiles_x) * 16 + (threadIdx.x >> 4);
+    const int bx0 = (t % jb.tiles_x) * 64, by0 = jb.py0 + (t / jb.tiles_x) * 16;
+    const int x0 = bx0 + (threadIdx.x & 15) * 4;
+    const int y = by0 + (threadIdx.x >> 4);
+    const int B = jb.block, s = jb.step, r = jb.range;
+    long long dw[4] = {0, 0, 0, 0};  // EXACT: the weights' sums
+    if constexpr (EXACT) {
+        const int CW = 64 + B - 1, CH = 16 + B - 1;
+        const int cx0 = bx0 - B + 1, cy0 = by0 - B + 1;  // corner tile origin
+        unsigned long long* ct = fsm;                     // [CH][CW] corner weights
+        unsigned long long* hs = fsm + CH * CW;           // [CH][64] horizontal B-wide sums
+        for (int e = threadIdx.x; e < CH * CW; e += 256) ct[e] = 0ull;
+        __syncthreads();
+        // The references whose windows (reference +- range, clamped) reach
+        // the corner tile, of the grid rows the records hold.
+        const Grid& g = jb.grid;
+        const int lx = cx0 - r, ux = bx0 + 63 + r, ly = cy0 - r, uy = by0 + 15 + r;
+        const int i_lo = min(max(lx, 0) / s, g.nx - 1);
+        const int i_hi = jb.width - B <= ux ? g.nx - 1 : min(g.nx - 1, ux / s);
+        const int a_lo = max(max(ly, 0) / s, g.row0);
+        const int a_hi = min(jb.height - B <= uy ? g.row0 + g.ny - 1 : uy / s, g.row0 + g.ny - 1);
+        const int ni = i_hi - i_lo + 1, nj = a_hi - a_lo + 1;
+        const int G = jb.group;
+        const int pairs = ni > 0 && nj > 0 ? ni * nj * G : 0;
+        for (int e = threadIdx.x; e < pairs; e += 256) {
+            const int ref = e / G, k = e - ref * G;
+            const int rj = ref / ni, ri = ref - rj * ni;
+            const int rid = (a_lo - g.row0 + rj) * g.nx + i_lo + ri;
+            if (k < jb.counts[rid]) {
+                const MatchRec m = jb.matches[static_cast<size_t>(rid) * G + k];
+                const int ox = m.x - cx0, oy = m.y - cy0;
+                if (static_cast<unsigned>(ox) < static_cast<unsigned>(CW) &&
+                    static_cast<unsigned>(oy) < static_cast<unsigned>(CH))
+                    lds_add(ct + oy * CW + ox, jb.gweight[rid]);
+            }
+        }
+        __syncthreads();
+        // hs[row][x] = sum of ct[row][x .. x + B - 1], a thread per 8 outputs
+        for (int e = threadIdx.x; e < CH * 8; e += 256) {
+            const int row = e >> 3, xs = (e & 7) * 8;
+            const unsigned long long* cr = ct + row * CW + xs;
+            unsigned long long sum = 0;
+            for (int b = 0; b < B - 1; ++b) sum += cr[b];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                sum += cr[q + B - 1];
+                hs[row * 64 + xs + q] = sum;
+                sum -= cr[q];
+            }
+        }
+        __syncthreads();
+        // the weight at (x, y): corners (x - B + 1 .. x, y - B + 1 .. y)
+        const int lyy = y - by0, lxx = x0 - bx0;
+        for (int a = 0; a < B; ++a) {
+            const unsigned long long* hr = hs + (lyy + a) * 64 + lxx;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dw[q] += static_cast<long long>(hr[q]);
+        }
+    }
     if (x0 >= jb.width || y >= jb.height) return;
     const TileGeom& tg = jb.tiles;
-    const int B = jb.block, s = jb.step, r = jb.range;
     const int xr = min(x0 + 3, jb.width - 1);
     int txh = min(tg.ntx - 1, (xr + r) / (tg.tw * s));
     while (txh + 1 < tg.ntx && tg.ax0(txh + 1, jb.width, B, s, r) <= xr) ++txh;
@@ -3245,11 +3266,8 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
                 if constexpr (EXACT) {
                     const longlong2* qn =
                         static_cast<const longlong2*>(__builtin_assume_aligned(jb.exact_num + o, 32));
-                    const longlong2* qd =
-                        static_cast<const longlong2*>(__builtin_assume_aligned(jb.exact_den + o, 32));
-                    const longlong2 n0 = qn[0], n1 = qn[1], d0 = qd[0], d1 = qd[1];
+                    const longlong2 n0 = qn[0], n1 = qn[1];
                     pn[u][0] = n0.x, pn[u][1] = n0.y, pn[u][2] = n1.x, pn[u][3] = n1.y;
-                    pd[u][0] = d0.x, pd[u][1] = d0.y, pd[u][2] = d1.x, pd[u][3] = d1.y;
                 } else {
                     const f32x4 vn = *reinterpret_cast<const f32x4*>(jb.part_num + o);
                     const f32x4 vd = *reinterpret_cast<const f32x4*>(jb.part_den + o);
@@ -3268,6 +3286,8 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
             }
     }
     if constexpr (EXACT) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = dw[i];
         if (batch) {
             long long* an = jb.acc_num + static_cast<size_t>(y) * jb.width;
             long long* ad = jb.acc_den + static_cast<size_t>(y) * jb.width;
@@ -3853,26 +3873,26 @@ size_t filter_atomic_lds_bytes(int block, int group, bool wiener) {
 
 size_t filter_tile_b8_lds_bytes(int group, const TileGeom& t) {
     const size_t aw = t.aw, ah = t.ah;
-    const size_t acc = aw * ah * 8 + (aw - 7) * (ah - 7) * 8;
+    const size_t acc = aw * ah * 8;
     const size_t recs = static_cast<size_t>(t.tw) * t.th * group * sizeof(MatchRec) +
                         ((static_cast<size_t>(t.tw) * t.th + 3) & ~static_cast<size_t>(3)) * 4;
     const size_t tbuf = 4 * static_cast<size_t>(group) * kTs * 4;
-    return acc + recs + tbuf;  // the den box sum reuses num's area
+    return acc + recs + tbuf;
 }
 
 size_t filter_tile_mfma_lds_bytes(int group, const TileGeom& t) {
     const size_t aw = t.aw, ah = t.ah;
-    const size_t acc = ((aw * ah + (aw - 7) * (ah - 7) + 1) & ~static_cast<size_t>(1)) * 8;
+    const size_t acc = ((aw * ah + 1) & ~static_cast<size_t>(1)) * 8;
     const size_t recs = ((static_cast<size_t>(t.tw) * t.th * group + 3) & ~static_cast<size_t>(3)) * 4 +
                         ((static_cast<size_t>(t.tw) * t.th + 3) & ~static_cast<size_t>(3)) * 4;
     const size_t tbuf = 4 * static_cast<size_t>(kMfmaBuf) * 4 + 64 * 4;  // + the fp16 form's DCT table
-    return acc + recs + tbuf;  // the den box sum reuses num's area
+    return acc + recs + tbuf;
 }
 
 size_t filter_tile_lds_bytes(int block, int group, bool wiener, const TileGeom& t) {
     const size_t aw = t.aw, ah = t.ah;
-    const size_t acc = aw * ah * 8 + (aw - block + 1) * (ah - block + 1) * 8;
-    return acc + 4 * filter_atomic_lds_bytes(block, group, wiener);  // the den box sum reuses num's area
+    const size_t acc = aw * ah * 8;
+    return acc + 4 * filter_atomic_lds_bytes(block, group, wiener);
 }
 
 template <typename K>
@@ -4062,13 +4082,24 @@ hipError_t launch_finish_sums(const FinishTable& t, int total_wg, hipStream_t s)
     return hipGetLastError();
 }
 
+size_t finish_tiles_lds_bytes(int block) {
+    const size_t ch = 16 + block - 1, cw = 64 + block - 1;
+    return (ch * cw + ch * 64) * 8;
+}
+
 hipError_t launch_finish(int block, const FinishTable& t, int total_wg, bool tiles, hipStream_t s) {
     if (launch_fault()) return hipErrorLaunchFailure;
     if (tiles) {
-        if (t.n && t.job[0].exact_num)
-            finish_tiles_kernel<true><<<total_wg, 256, 0, s>>>(t);
-        else
+        if (t.n && t.job[0].exact_num) {
+            // the corner tile and its horizontal sums of the largest block
+            int bmax = 1;
+            for (int k = 0; k < t.n; ++k) bmax = std::max(bmax, t.job[k].block);
+            const size_t lds = finish_tiles_lds_bytes(bmax);
+            allow_lds(&finish_tiles_kernel<true>, lds);
+            finish_tiles_kernel<true><<<total_wg, 256, lds, s>>>(t);
+        } else {
             finish_tiles_kernel<false><<<total_wg, 256, 0, s>>>(t);
+        }
         return hipGetLastError();
     }
     switch (block) {

@@ -660,14 +660,19 @@ void add_bucket_jobs(LaunchGroup& g, const FrameRef& f, const Stage& st, const B
         fj.den = f.buf.den[i];
         fj.tiles = bk.geom[k];
         fj.part_num = f.buf.part[i];
-        fj.part_den = f.buf.part[i] ? f.buf.part[i] + h->part_elems[i] : nullptr;
+        // the group weights after the numerator partials
+        fj.gweight = f.buf.part[i] ? f.buf.part[i] + h->part_elems[i] : nullptr;
         g.fwg += bk.tiles ? bk.geom[k].ntx * bk.geom[k].nty : nref;
 
         FinishJob& n = g.fn.job[g.fn.n++];
         n.num = f.buf.num[i];
         n.den = f.buf.den[i];
         n.exact_num = reinterpret_cast<const long long*>(fj.part_num);
-        n.exact_den = reinterpret_cast<const long long*>(fj.part_den);
+        n.matches = fj.matches;
+        n.counts = fj.counts;
+        n.gweight = fj.gweight;
+        n.grid = gr;
+        n.group = sp.group;
         n.tiles = bk.geom[k];
         n.block = b;
         n.step = sp.step;
@@ -768,7 +773,7 @@ int run_limited_tiles(const FrameRef& f, const Stage& st, const Bucket& bk, cons
             fj.counts = f.buf.counts[mi];
             fj.tiles = tg;
             fj.part_num = f.buf.part[i];
-            fj.part_den = f.buf.part[i] + h->part_elems[i];
+            fj.gweight = f.buf.part[i] + h->part_elems[i];
             fj.tile0 = tr0 * tg.ntx;
             fwg += (tr1 - tr0) * tg.ntx;
             // the pixel rows the batch's tiles reach
@@ -776,7 +781,13 @@ int run_limited_tiles(const FrameRef& f, const Stage& st, const Bucket& bk, cons
             const int py1 = std::min(h->ph[i], tg.ay0(tr1 - 1, h->ph[i], b, sp.step, sp.range) + tg.ah);
             FinishJob& n = fn.job[fn.n++];
             n.exact_num = reinterpret_cast<const long long*>(fj.part_num);
-            n.exact_den = reinterpret_cast<const long long*>(fj.part_den);
+            // the weights of the batch's references (grid rows of its tiles)
+            n.matches = fj.matches;
+            n.counts = fj.counts;
+            n.gweight = fj.gweight;
+            n.grid = g;
+            n.grid.ny = std::min(g.ny, tr1 * tg.th) - g.row0;
+            n.group = sp.group;
             n.tiles = tg;
             n.block = b;
             n.step = sp.step;
