@@ -3187,17 +3187,30 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
         const int ni = i_hi - i_lo + 1, nj = a_hi - a_lo + 1;
         const int G = jb.group;
         const int pairs = ni > 0 && nj > 0 ? ni * nj * G : 0;
-        for (int e = threadIdx.x; e < pairs; e += 256) {
-            const int ref = e / G, k = e - ref * G;
-            const int rj = ref / ni, ri = ref - rj * ni;
-            const int rid = (a_lo - g.row0 + rj) * g.nx + i_lo + ri;
-            if (k < jb.counts[rid]) {
+        // two (reference, record) pairs per thread and step, every load of
+        // both issued before any is used (a reference holds G records; the
+        // ones at or beyond its count are read and not used)
+        for (int e0 = threadIdx.x; e0 < pairs; e0 += 512) {
+            int cnt[2], kk[2], ox[2], oy[2];
+            unsigned long long wgt[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int e = min(e0 + 256 * u, pairs - 1);
+                const int ref = e / G, k = e - ref * G;
+                const int rj = ref / ni, ri = ref - rj * ni;
+                const int rid = (a_lo - g.row0 + rj) * g.nx + i_lo + ri;
+                cnt[u] = jb.counts[rid];
                 const MatchRec m = jb.matches[static_cast<size_t>(rid) * G + k];
-                const int ox = m.x - cx0, oy = m.y - cy0;
-                if (static_cast<unsigned>(ox) < static_cast<unsigned>(CW) &&
-                    static_cast<unsigned>(oy) < static_cast<unsigned>(CH))
-                    lds_add(ct + oy * CW + ox, jb.gweight[rid]);
+                wgt[u] = jb.gweight[rid];
+                kk[u] = e0 + 256 * u < pairs ? k : G;
+                ox[u] = m.x - cx0;
+                oy[u] = m.y - cy0;
             }
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if (kk[u] < cnt[u] && static_cast<unsigned>(ox[u]) < static_cast<unsigned>(CW) &&
+                    static_cast<unsigned>(oy[u]) < static_cast<unsigned>(CH))
+                    lds_add(ct + oy[u] * CW + ox[u], wgt[u]);
         }
         __syncthreads();
         // hs[row][x] = sum of ct[row][x .. x + B - 1], a thread per 8 outputs

@@ -215,12 +215,13 @@ int plan_stage(nss_hip_bm3d* h, Stage& st) {
         const int form = ((fast == 8 || fast == 16) && small) ? h->p.transform : 0;
         bk.fast_group = fast;
         bk.form = form;
-        // The FP32 register filter is latency bound at two workgroups per CU,
-        // so it starts from smaller tiles (more workgroups per CU): on 1080p
-        // YUV420 (B8, G8) 8 x 2 references filter fastest for the hard
-        // threshold and 4 x 4 for Wiener. The autotuner refines per input.
+        // The FP32 register filter is latency bound, so it starts from small
+        // tiles (more workgroups per CU): on 1080p YUV420 (B8, G8) 4 x 4
+        // references filter fastest in both stages (hard threshold 4 x 4
+        // 46.5, 4 x 2 47.1, 8 x 2 48.0, 8 x 4 49.3 us; stage averages with
+        // the numerator-only partials). The autotuner refines per input.
         bool placed = false;
-        if (fast && !form) placed = apply_tile_shape(h, st, bk, st.wiener ? 4 : 8, st.wiener ? 4 : 2);
+        if (fast && !form) placed = apply_tile_shape(h, st, bk, 4, 4);
         for (const auto& sh : kTileShapes)
             if (placed || (placed = apply_tile_shape(h, st, bk, sh[0], sh[1]))) break;
         if (!bk.tiles) {
