@@ -3213,27 +3213,29 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
                     lds_add(ct + oy[u] * CW + ox[u], wgt[u]);
         }
         __syncthreads();
-        // hs[row][x] = sum of ct[row][x .. x + B - 1], a thread per 8 outputs
-        for (int e = threadIdx.x; e < CH * 8; e += 256) {
-            const int row = e >> 3, xs = (e & 7) * 8;
-            const unsigned long long* cr = ct + row * CW + xs;
+        // hs[row][x] = sum of ct[row][x .. x + B - 1], then the B-high sums
+        // into the corner tile's area, den[yr][x]: a wave's lanes take 64
+        // consecutive x of one row (bank-conflict-free reads and writes)
+        for (int e = threadIdx.x; e < CH * 64; e += 256) {
+            const int row = e >> 6, x = e & 63;
+            const unsigned long long* cr = ct + row * CW + x;
             unsigned long long sum = 0;
-            for (int b = 0; b < B - 1; ++b) sum += cr[b];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                sum += cr[q + B - 1];
-                hs[row * 64 + xs + q] = sum;
-                sum -= cr[q];
-            }
+            for (int b = 0; b < B; ++b) sum += cr[b];
+            hs[e] = sum;
+        }
+        __syncthreads();
+        unsigned long long* den = ct;  // [16][64]
+        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+            const unsigned long long* hc = hs + e;
+            unsigned long long sum = 0;
+            for (int a = 0; a < B; ++a) sum += hc[a * 64];
+            den[e] = sum;
         }
         __syncthreads();
         // the weight at (x, y): corners (x - B + 1 .. x, y - B + 1 .. y)
-        const int lyy = y - by0, lxx = x0 - bx0;
-        for (int a = 0; a < B; ++a) {
-            const unsigned long long* hr = hs + (lyy + a) * 64 + lxx;
+        const unsigned long long* dr = den + (y - by0) * 64 + (x0 - bx0);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) dw[q] += static_cast<long long>(hr[q]);
-        }
+        for (int q = 0; q < 4; ++q) dw[q] = static_cast<long long>(dr[q]);
     }
     if (x0 >= jb.width || y >= jb.height) return;
     const TileGeom& tg = jb.tiles;

@@ -269,10 +269,11 @@ void free_buffers(nss_hip_bm3d* h) {
 }
 
 // Per-plane buffer needs of an instance: match records and counts (the
-// largest stage), tile partial floats per half, frame-sized sums (atomic
-// path), basic estimate.
+// largest stage), tile numerator partials (int64 entries) and the group
+// weights after them (one per reference of the plane's grid), frame-sized
+// sums (atomic path), basic estimate.
 struct PlaneNeeds {
-    size_t recs = 0, nref = 0, part = 0;
+    size_t recs = 0, nref = 0, part = 0, gw = 0;
     bool atomic = false, basic = false;
 };
 
@@ -319,6 +320,8 @@ PlaneNeeds plane_needs(const nss_hip_bm3d* h, int i) {
                     // a memory-limited plan: one batch of tile rows, and the
                     // frame sums its batches add into
                     const TileGeom& tg = st.bucket[bi].geom[k];
+                    const Grid g = make_grid(h->pw[i], h->ph[i], sp.block, sp.step);
+                    n.gw = std::max(n.gw, static_cast<size_t>(g.nx) * g.ny);
                     const int rows = h->batches > 0 ? (tg.nty + h->batches - 1) / h->batches : tg.nty;
                     n.part = std::max(n.part, static_cast<size_t>(tg.ntx) * rows * tg.pitch * tg.ah);
                     if (h->batches > 0) n.atomic = true;
@@ -346,7 +349,8 @@ size_t frame_buffer_bytes(const nss_hip_bm3d* h) {
     for (int i = 0; i < h->nplanes; ++i) {
         const PlaneNeeds n = plane_needs(h, i);
         const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
-        total += n.recs * sizeof(MatchRec) + n.nref * sizeof(int) + 2 * h->part_elems[i] * sizeof(unsigned long long) +
+        const size_t part = h->part_elems[i] ? h->part_elems[i] + n.gw : 0;
+        total += n.recs * sizeof(MatchRec) + n.nref * sizeof(int) + part * sizeof(unsigned long long) +
                  (n.atomic ? 16 * npx : 0) + (n.basic ? 4 * npx : 0);
     }
     return total + cube_bytes(h);
@@ -400,7 +404,8 @@ hipError_t alloc_frame_buffers(const nss_hip_bm3d* h, FrameBuffers& fb) {
             e = dev_alloc(&fb.num[i], npx * 8);
             if (e == hipSuccess) e = dev_alloc(&fb.den[i], npx * 8);
         }
-        if (e == hipSuccess && h->part_elems[i]) e = dev_alloc(&fb.part[i], 2 * h->part_elems[i] * sizeof(unsigned long long));
+        if (e == hipSuccess && h->part_elems[i])
+            e = dev_alloc(&fb.part[i], (h->part_elems[i] + n.gw) * sizeof(unsigned long long));
         if (e == hipSuccess && n.basic) e = dev_alloc(&fb.basic[i], npx * 4);
     }
     if (e == hipSuccess && h->cube_floats) e = dev_alloc(&fb.cube, cube_bytes(h));
@@ -428,7 +433,7 @@ size_t carve_frame_buffers(const nss_hip_bm3d* h, char* base, FrameBuffers& fb) 
             fb.den[i] = reinterpret_cast<unsigned long long*>(take(npx * 8));
         }
         if (h->part_elems[i])
-            fb.part[i] = reinterpret_cast<unsigned long long*>(take(2 * h->part_elems[i] * sizeof(unsigned long long)));
+            fb.part[i] = reinterpret_cast<unsigned long long*>(take((h->part_elems[i] + n.gw) * sizeof(unsigned long long)));
         if (n.basic) fb.basic[i] = reinterpret_cast<float*>(take(npx * 4));
     }
     if (h->cube_floats) fb.cube = reinterpret_cast<float*>(take(cube_bytes(h)));
@@ -1623,9 +1628,10 @@ int grow_partials(nss_hip_bm3d* h, const size_t* need) {
     std::vector<unsigned long long*> fresh;
     for (int i = 0; i < h->nplanes && e == hipSuccess; ++i) {
         if (need[i] <= h->part_elems[i]) continue;
+        const size_t gw = plane_needs(h, i).gw;
         for (size_t k = 0; k < h->slots.size() && e == hipSuccess; ++k) {
             unsigned long long* p = nullptr;
-            e = dev_alloc(&p, 2 * need[i] * sizeof(unsigned long long));
+            e = dev_alloc(&p, (need[i] + gw) * sizeof(unsigned long long));
             if (e == hipSuccess) fresh.push_back(p);
         }
     }

@@ -201,7 +201,7 @@ struct nss_hip_bm3d {
     int pw[3]{}, ph[3]{};
     nsship::Stage stages[2]{};
     int nstages = 0;
-    size_t part_elems[3]{};  // per half (num or den) of a plane's tile partials, int64 entries
+    size_t part_elems[3]{};  // a plane's tile numerator partials, int64 entries (the group weights follow)
     size_t cube_floats = 0;  // largest global group cube of any bucket (0: every cube fits the LDS)
     int cube_slots = nsship::kCubeSlots;  // global cubes per frame slot (fewer under a memory limit)
     // memory_limit_mb below the normal plan's need (radius 0): every stage
