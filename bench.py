@@ -101,11 +101,11 @@ FORMS = {"fp32": 0, "bf16x3": 1, "fp16": 2}
 # WRITE_SIZE (KiB per dispatch, averaged over the basic and Wiener launches)
 # scaled by what the counters read for a 256 MiB elementwise copy in the same
 # process (FETCH x2, WRITE x1); profiles/bm3d_1080p_yuv420_two_stage_kernels.md.
-TRAFFIC_KIB = {("match", "fp32"): (6583, 6280), ("match", "bf16x3"): (6583, 6280), ("match", "fp16"): (6583, 6280),
-               ("filter", "fp32"): ((10014 + 16402) / 2, (143055 + 133875) / 2),
-               ("filter", "bf16x3"): ((10225 + 16654) / 2, 105188), ("filter", "fp16"): ((10149 + 16840) / 2, 105188),
-               ("finish", "fp32"): (57579, 13327), ("finish", "bf16x3"): (57579, 13327),
-               ("finish", "fp16"): (57579, 13327)}
+TRAFFIC_KIB = {("match", "fp32"): (6594, 6280), ("match", "bf16x3"): (6594, 6280), ("match", "fp16"): (6594, 6280),
+               ("filter", "fp32"): ((9629 + 16047) / 2, 67318),
+               ("filter", "bf16x3"): ((9792 + 16155) / 2, 52976), ("filter", "fp16"): ((9778 + 16135) / 2, 52976),
+               ("finish", "fp32"): (32048, 12151), ("finish", "bf16x3"): (32048, 12151),
+               ("finish", "fp16"): (32048, 12151)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 FORM_DTYPE = {"fp32": "f32", "bf16x3": "bf16x3 (f32 accumulate)", "fp16": "f16 (f32 accumulate)"}
 PEAK_MFMA_TFLOPS = 2500.0  # dense bf16 / fp16
@@ -355,7 +355,7 @@ def single_stage_256_workload(args):
            "config": {"workload": "one-stage BM3D, sigma 5, block 8, step 3, group 16, range 16, one 256x256 GRAYS "
                                   "frame per step"},
            "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
-           "roofline": {"kernel": dom, "bound": "hbm" if kernels[dom]["unit"] == "GB/s" else "mfma",
+           "roofline": {"kernel": dom, "bound": "hbm" if kernels[dom]["unit"] == "GB/s" else "valu",
                         "achieved": kernels[dom]["achieved"], "peak": kernels[dom]["peak"],
                         "unit": kernels[dom]["unit"], "frac": kernels[dom]["frac"], "traffic": None},
            "kernels": kernels,
