@@ -247,6 +247,12 @@ struct TTileJob {
     int range[kMaxTileSlots];
     float* part;                    // this centre's partials
     long long slot_off[kMaxTileSlots];  // floats: slot t's num at part + slot_off[t], den after it
+    // Rolling output: each group's weight goes, by an int64 device atomic,
+    // to the top-left corner of each of its patches in the frame-sized corner
+    // plane of the patch's frame (frame f at corners + (f % acc_ring) W H),
+    // and the den partials are not written. Null: den partials (legacy).
+    unsigned long long* corners;
+    int acc_ring;
     int nref, group;
     float sigma;
     Slots slots;
@@ -272,6 +278,9 @@ struct TFinishJob {
     TSource src[kMaxTileSlots];
     int nsrc;
     int block, step;
+    // The frame's corner plane of group weights (rolling output): den is its
+    // B x B box sum, exact, instead of the sources' den partials.
+    const unsigned long long* corners;
     const float* fallback;  // the source frame
     float* dst;
     int width, height;

@@ -1512,6 +1512,14 @@ __device__ inline void partials_out_float(int aw, int ah, int pitch, const unsig
                                     const unsigned long long* corner, float* scratch, float* pn, float* pd) {
     const int cw = aw - B + 1, ch = ah - B + 1;
     const int nt = static_cast<int>(blockDim.x);
+    if (!pd) {  // numerator only (the weights went to the frame's corner plane)
+        for (int e = threadIdx.x; e < aw * ah; e += nt) {
+            const int y = e / aw, x = e - (e / aw) * aw;
+            pn[y * pitch + x] =
+                static_cast<float>(static_cast<double>(static_cast<long long>(num[e])) * (1.0 / 4294967296.0));
+        }
+        return;
+    }
     // Horizontal: a thread slides the B-wide window over 8 outputs of a row
     // (exact int64 sums, so the order of the adds does not matter).
     const int segs = (aw + 7) >> 3;
@@ -3068,7 +3076,16 @@ __global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable ta
                         unsigned long long* row = acc + q.off + (ly + y) * q.aw + lx;
 #pragma unroll
                         for (int x = 0; x < 8; ++x) lds_add(row + x, to_fixed(w * u[m][x]));
-                        if (y == 0) lds_add(acc + q.off + q.aw * q.ah + ly * (q.aw - B + 1) + lx, to_fixed(w));
+                        if (y == 0) {
+                            if (jb.corners)
+                                atomicAdd(jb.corners +
+                                              static_cast<size_t>(jb.slots.frame(rr.t) % jb.acc_ring) * jb.width *
+                                                  jb.height +
+                                              static_cast<size_t>(rr.y) * jb.width + rr.x,
+                                          to_fixed(w));
+                            else
+                                lds_add(acc + q.off + q.aw * q.ah + ly * (q.aw - B + 1) + lx, to_fixed(w));
+                        }
                     }
                 }
             }
@@ -3084,7 +3101,7 @@ __global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable ta
             const TileGeom& g = jb.tiles[t];
             const size_t area = static_cast<size_t>(g.pitch) * g.ah;
             float* pn = jb.part + jb.slot_off[t] + static_cast<size_t>(cur.tile) * area + (q.ax0 & 3);
-            float* pd = pn + static_cast<size_t>(g.ntx) * g.nty * area;
+            float* pd = jb.corners ? nullptr : pn + static_cast<size_t>(g.ntx) * g.nty * area;
             partials_out_float<B>(q.aw, q.ah, g.pitch, acc + q.off, acc + q.off + q.aw * q.ah, scratch, pn, pd);
             if (t == 0 && more) publish(nxt, nf);
             __syncthreads();
@@ -3098,13 +3115,51 @@ __global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable ta
 // it (sources in ascending centre order), 64 x 16 pixels per block, four per
 // thread, as finish_tiles_kernel.
 __global__ __launch_bounds__(256) void finish_temporal_tiles_kernel(TFinishTable tab) {
+    extern __shared__ __align__(16) unsigned long long tsm[];
     const int lb = xcd_block();
     const TFinishJob& jb = tab.job[find_job(tab, lb)];
     const int t = lb - jb.wg_begin;
-    const int x0 = (t % jb.tiles_x) * 64 + (threadIdx.x & 15) * 4;
-    const int y = (t / jb.tiles_x) * 16 + (threadIdx.x >> 4);
-    if (x0 >= jb.width || y >= jb.height) return;
+    const int bx0 = (t % jb.tiles_x) * 64, by0 = (t / jb.tiles_x) * 16;
+    const int x0 = bx0 + (threadIdx.x & 15) * 4;
+    const int y = by0 + (threadIdx.x >> 4);
     const int B = jb.block, s = jb.step;
+    long long dw[4] = {0, 0, 0, 0};
+    if (jb.corners) {
+        // den: the B x B box sum of the frame's corner plane (corners
+        // (x - B + 1 .. x, y - B + 1 .. y)), staged in LDS, exact
+        const int CW = 64 + B - 1, CH = 16 + B - 1;
+        const int cx0 = bx0 - B + 1, cy0 = by0 - B + 1;
+        unsigned long long* ct = tsm;            // [CH][CW]
+        unsigned long long* hs = tsm + CH * CW;  // [CH][64]
+        for (int e = threadIdx.x; e < CH * CW; e += 256) {
+            const int r = e / CW, c = e - r * CW;
+            const int py = cy0 + r, px = cx0 + c;
+            ct[e] = py >= 0 && px >= 0 && py < jb.height && px < jb.width
+                        ? jb.corners[static_cast<size_t>(py) * jb.width + px]
+                        : 0ull;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < CH * 64; e += 256) {
+            const int row = e >> 6, x = e & 63;
+            const unsigned long long* cr = ct + row * CW + x;
+            unsigned long long sum = 0;
+            for (int b = 0; b < B; ++b) sum += cr[b];
+            hs[e] = sum;
+        }
+        __syncthreads();
+        unsigned long long* den = ct;  // [16][64]
+        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+            const unsigned long long* hc = hs + e;
+            unsigned long long sum = 0;
+            for (int a = 0; a < B; ++a) sum += hc[a * 64];
+            den[e] = sum;
+        }
+        __syncthreads();
+        const unsigned long long* dr = den + (y - by0) * 64 + (x0 - bx0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dw[q] = static_cast<long long>(dr[q]);
+    }
+    if (x0 >= jb.width || y >= jb.height) return;
     const int xr = min(x0 + 3, jb.width - 1);
     f32x4 n = {0.f, 0.f, 0.f, 0.f}, d = {0.f, 0.f, 0.f, 0.f};
     for (int q = 0; q < jb.nsrc; ++q) {
@@ -3125,7 +3180,7 @@ __global__ __launch_bounds__(256) void finish_temporal_tiles_kernel(TFinishTable
                 const size_t o = (static_cast<size_t>(ty) * tg.ntx + tx) * area + static_cast<size_t>(oy) * tg.pitch +
                                  (x0 - ax0 + (ax0 & 3));
                 const f32x4 pn = *reinterpret_cast<const f32x4*>(src.num + o);
-                const f32x4 pd = *reinterpret_cast<const f32x4*>(src.den + o);
+                const f32x4 pd = jb.corners ? f32x4{0.f, 0.f, 0.f, 0.f} : *reinterpret_cast<const f32x4*>(src.den + o);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const bool in = static_cast<unsigned>(x0 + i - ax0) < static_cast<unsigned>(tg.aw);
@@ -3135,6 +3190,9 @@ __global__ __launch_bounds__(256) void finish_temporal_tiles_kernel(TFinishTable
             }
         }
     }
+    if (jb.corners)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = static_cast<float>(static_cast<double>(dw[i]) * (1.0 / 4294967296.0));
     float* dst = jb.dst + static_cast<size_t>(y) * jb.width;
     const float* fb = jb.fallback + static_cast<size_t>(y) * jb.width;
 #pragma unroll
@@ -3187,14 +3245,14 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
         const int ni = i_hi - i_lo + 1, nj = a_hi - a_lo + 1;
         const int G = jb.group;
         const int pairs = ni > 0 && nj > 0 ? ni * nj * G : 0;
-        // two (reference, record) pairs per thread and step, every load of
-        // both issued before any is used (a reference holds G records; the
+        // four (reference, record) pairs per thread and step, every load of
+        // them issued before any is used (a reference holds G records; the
         // ones at or beyond its count are read and not used)
-        for (int e0 = threadIdx.x; e0 < pairs; e0 += 512) {
-            int cnt[2], kk[2], ox[2], oy[2];
-            unsigned long long wgt[2];
+        for (int e0 = threadIdx.x; e0 < pairs; e0 += 1024) {
+            int cnt[4], kk[4], ox[4], oy[4];
+            unsigned long long wgt[4];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < 4; ++u) {
                 const int e = min(e0 + 256 * u, pairs - 1);
                 const int ref = e / G, k = e - ref * G;
                 const int rj = ref / ni, ri = ref - rj * ni;
@@ -3207,7 +3265,7 @@ __global__ __launch_bounds__(256) void finish_tiles_kernel(FinishTable tab) {
                 oy[u] = m.y - cy0;
             }
 #pragma unroll
-            for (int u = 0; u < 2; ++u)
+            for (int u = 0; u < 4; ++u)
                 if (kk[u] < cnt[u] && static_cast<unsigned>(ox[u]) < static_cast<unsigned>(CW) &&
                     static_cast<unsigned>(oy[u]) < static_cast<unsigned>(CH))
                     lds_add(ct + oy[u] * CW + ox[u], wgt[u]);
@@ -4248,6 +4306,16 @@ hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int tota
 
 hipError_t launch_finish_temporal_tiles(const TFinishTable& t, int total_wg, hipStream_t s) {
     if (launch_fault()) return hipErrorLaunchFailure;
+    // the corner tile and its horizontal sums when the weights come from corner planes
+    int bmax = 0;
+    for (int k = 0; k < t.n; ++k)
+        if (t.job[k].corners) bmax = std::max(bmax, t.job[k].block);
+    const size_t lds = bmax ? finish_tiles_lds_bytes(bmax) : 0;
+    if (lds) {
+        allow_lds(&finish_temporal_tiles_kernel, lds);
+        finish_temporal_tiles_kernel<<<total_wg, 256, lds, s>>>(t);
+        return hipGetLastError();
+    }
     finish_temporal_tiles_kernel<<<total_wg, 256, 0, s>>>(t);
     return hipGetLastError();
 }

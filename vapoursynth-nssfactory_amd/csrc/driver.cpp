@@ -365,7 +365,10 @@ size_t rolling_bytes(const nss_hip_bm3d* h, int i, size_t* cached, size_t* outpu
     const size_t o = npx * 4 * static_cast<size_t>(R.out_n) * R.out_rows;
     for (int s = 0; s < h->nstages; ++s) {
         if (h->stages[s].plane[i].sigma == 0.f) continue;
-        c += R.tiles[s] ? R.part_center[s][i] * (2 * static_cast<size_t>(R.r) + 1) * sizeof(float)
+        // tile path: the centres' partials, and for rolling output the
+        // frames' corner planes of group weights
+        c += R.tiles[s] ? R.part_center[s][i] * (2 * static_cast<size_t>(R.r) + 1) * sizeof(float) +
+                              (R.legacy ? 0 : npx * 8 * static_cast<size_t>(R.acc_n))
                         : npx * 16 * static_cast<size_t>(R.acc_n);
     }
     if (cached) *cached += c;
@@ -1289,6 +1292,8 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
                 slot_geoms(h, s, i, R.tw[s], R.th[s], j.tiles, j.range);
                 centre_partials(j.tiles, ns, j.slot_off);
                 j.part = R.part[s][i] + static_cast<size_t>(c % ns) * R.part_center[s][i];
+                j.corners = R.legacy ? nullptr : R.den[s][i];
+                j.acc_ring = R.acc_n;
                 j.nref = f.nref;
                 j.group = f.group;
                 j.sigma = f.sigma;
@@ -1352,6 +1357,7 @@ int temporal_finish(nss_hip_bm3d* h, int s, int n, hipStream_t st) {
             }
             f.block = 8;
             f.step = stg.plane[i].step;
+            f.corners = R.legacy ? nullptr : R.den[s][i] + static_cast<size_t>(n % R.acc_n) * npx;
             f.fallback = R.src[i] + static_cast<size_t>(n % R.src_n) * npx;
             f.dst = to_out ? R.out[i] + static_cast<size_t>(n % R.out_n) * npx
                            : R.basic[i] + static_cast<size_t>(n % R.basic_n) * npx;
@@ -1361,8 +1367,23 @@ int temporal_finish(nss_hip_bm3d* h, int s, int n, hipStream_t st) {
             f.wg_begin = nwg;
             nwg += f.tiles_x * ((h->ph[i] + 15) / 16);
         }
-        const hipError_t e = traced(h, 3, st, [&] { return launch_finish_temporal_tiles(fn, nwg, st); });
-        return e == hipSuccess ? NSS_HIP_OK : hip_fail(e, "finish launch");
+        hipError_t e = traced(h, 3, st, [&] { return launch_finish_temporal_tiles(fn, nwg, st); });
+        if (e != hipSuccess) return hip_fail(e, "finish launch");
+        if (R.legacy) return NSS_HIP_OK;
+        // the frame's corner planes start empty for the frame that reuses them
+        ZeroTable zt{};
+        int zwg = 0;
+        for (int k = 0; k < fn.n; ++k) {
+            const int i = bk.planes[k];
+            const long long npx = static_cast<long long>(h->pw[i]) * h->ph[i];
+            ZeroJob& z = zt.job[zt.n++];
+            z.p = R.den[s][i] + static_cast<size_t>(n % R.acc_n) * npx;
+            z.count = npx;
+            z.wg_begin = zwg;
+            zwg += static_cast<int>((npx + 1023) / 1024);
+        }
+        e = traced(h, 0, st, [&] { return launch_zero(zt, zwg, st); });
+        return e == hipSuccess ? NSS_HIP_OK : hip_fail(e, "clear launch");
     }
     for (int bi = 0; bi < stg.nbuckets; ++bi) {
         const Bucket& bk = stg.bucket[bi];
@@ -1817,6 +1838,10 @@ int nss_hip_bm3d_create(const nss_hip_bm3d_params* params, const nss_hip_video_i
                 if (h->stages[s].plane[i].sigma == 0.f) continue;
                 if (R.tiles[s]) {
                     e = dev_alloc(&R.part[s][i], R.part_center[s][i] * (2 * R.r + 1) * sizeof(float));
+                    if (e == hipSuccess && !R.legacy) {
+                        e = dev_alloc(&R.den[s][i], npx * 8 * R.acc_n);
+                        if (e == hipSuccess) e = hipMemset(R.den[s][i], 0, npx * 8 * R.acc_n);
+                    }
                     continue;
                 }
                 e = dev_alloc(&R.num[s][i], npx * 8 * R.acc_n);
@@ -1946,9 +1971,11 @@ int nss_hip_bm3d_seek(nss_hip_bm3d* h, int n, int* next_push, void* stream) {
         return NSS_HIP_OK;
     }
     // Else restart at the chunk's first source frame: every centre from the
-    // stage's first one runs again. Frames before a stage's first complete
-    // one (first centre + r) are still finished, so that their sums are
-    // cleared before the ring reuses them, but nothing reads them.
+    // stage's first one runs again. Rolling output also finishes the r
+    // frames before a stage's first centre, and every frame before its first
+    // complete one (first centre + r): the centres add into their sums, so
+    // finishing them clears those sums before the ring reuses them; nothing
+    // reads their output.
     for (int st = 0; st < h->nstages; ++st)
         for (int i = 0; i < h->nplanes; ++i) {
             const size_t npx = static_cast<size_t>(h->pw[i]) * h->ph[i];
@@ -1961,7 +1988,7 @@ int nss_hip_bm3d_seek(nss_hip_bm3d* h, int n, int* next_push, void* stream) {
     R.pushed = first;
     for (int st = 0; st < h->nstages; ++st) {
         R.next_center[st] = c0[st];
-        R.finished[st] = c0[st];
+        R.finished[st] = R.legacy ? c0[st] : std::max(0, c0[st] - R.r);
     }
     R.out_head = start;
     R.failed = false;
