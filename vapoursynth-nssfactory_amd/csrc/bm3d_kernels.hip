@@ -2999,16 +2999,18 @@ __global__ __launch_bounds__(512) void filter_temporal_tile_kernel(TTileTable ta
     for (int lt = lo; lt < hi; ++lt) {
         const TTileJob& jb = tab.job[cur.job];
         const int ns = jb.nslots;
+        // per slot: the numerator sums, then (den partials only) the corner sums
+        const int cs = jb.corners ? 0 : 1;
         int total = 0;
         for (int t = 0; t < ns; ++t) {
             const TileGeom& g = jb.tiles[t];
-            total += g.aw * g.ah + (g.aw - B + 1) * (g.ah - B + 1);
+            total += g.aw * g.ah + cs * (g.aw - B + 1) * (g.ah - B + 1);
         }
         if (static_cast<int>(threadIdx.x) < ns) {
             const int t = static_cast<int>(threadIdx.x);
             int o = 0;
             for (int q = 0; q < t; ++q)
-                o += jb.tiles[q].aw * jb.tiles[q].ah + (jb.tiles[q].aw - B + 1) * (jb.tiles[q].ah - B + 1);
+                o += jb.tiles[q].aw * jb.tiles[q].ah + cs * (jb.tiles[q].aw - B + 1) * (jb.tiles[q].ah - B + 1);
             const TileGeom& g = jb.tiles[t];
             sl[t] = SlotLds{o, g.ax0(cur.tx, jb.width, B, jb.grid.step, jb.range[t]),
                             g.ay0(cur.ty, jb.height, B, jb.grid.step, jb.range[t]), g.aw, g.ah};
@@ -4258,10 +4260,12 @@ static hipError_t launch_tfilter_b(const TFilterTable& t, int total_wg, size_t l
 size_t filter_temporal_b8_lds_bytes(int group) { return 4 * static_cast<size_t>(group) * kTs * 4; }
 
 // Dynamic LDS of filter_temporal_tile_kernel for per-slot geometries g[0..ns).
-size_t filter_temporal_tile_lds_bytes(int group, const TileGeom* g, int ns) {
+size_t filter_temporal_tile_lds_bytes(int group, const TileGeom* g, int ns, bool corner_planes) {
     size_t acc = 0, boxsum = 0;
     for (int t = 0; t < ns; ++t) {
-        acc += static_cast<size_t>(g[t].aw) * g[t].ah + static_cast<size_t>(g[t].aw - 7) * (g[t].ah - 7);
+        acc += static_cast<size_t>(g[t].aw) * g[t].ah;
+        if (corner_planes) continue;  // no corner sums, no den box sum
+        acc += static_cast<size_t>(g[t].aw - 7) * (g[t].ah - 7);
         boxsum = std::max(boxsum, static_cast<size_t>(g[t].ah - 7) * g[t].aw * 4);
     }
     acc = ((acc + 1) & ~static_cast<size_t>(1)) * 8;

@@ -1159,7 +1159,7 @@ void plan_temporal_tiles(nss_hip_bm3d* h, int s) {
             TileGeom g[kMaxTileSlots];
             int range[kMaxTileSlots];
             slot_geoms(h, s, bk.planes[k], sh[0], sh[1], g, range);
-            lds = std::max(lds, filter_temporal_tile_lds_bytes(G, g, 2 * R.r + 1));
+            lds = std::max(lds, filter_temporal_tile_lds_bytes(G, g, 2 * R.r + 1, !R.legacy));
         }
         if (lds + 1024 <= kLdsLimit) {
             R.tiles[s] = true;

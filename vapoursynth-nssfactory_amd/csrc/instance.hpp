@@ -68,7 +68,9 @@ hipError_t launch_predictive(int block, const PredTable& t, int total_wg, size_t
 hipError_t launch_filter_temporal(int block, const TFilterTable& t, int total_wg, size_t lds, bool wiener,
                                   int fast_group, hipStream_t s);
 size_t filter_temporal_b8_lds_bytes(int group);
-size_t filter_temporal_tile_lds_bytes(int group, const TileGeom* g, int ns);
+// corner_planes: the weights go to frame corner planes (rolling output), so
+// the workgroup holds no corner sums.
+size_t filter_temporal_tile_lds_bytes(int group, const TileGeom* g, int ns, bool corner_planes);
 hipError_t launch_filter_temporal_tiles(int group, const TTileTable& t, int total_wg, size_t lds, bool wiener,
                                         hipStream_t s);
 hipError_t launch_finish_temporal_tiles(const TFinishTable& t, int total_wg, hipStream_t s);
