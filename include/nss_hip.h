@@ -175,7 +175,7 @@ int nss_hip_bm3d_process(nss_hip_bm3d* handle, const float* const* src, const in
  * estimate) live in one fixed device array of 192 MiB per GPU, held from the
  * library's load until the process ends. A frame whose buffers do not fit
  * (a 1920 x 1080 YUV420 two-stage frame at the default arguments needs
- * 158 MiB) takes the memory_limit_mb plan batched over grid rows, which
+ * about 103 MiB) takes the memory_limit_mb plan batched over grid rows, which
  * needs about 20 bytes per pixel of a two-stage frame (a 3840 x 2160 gray
  * frame fits, 3840 x 2160 RGB does not); a frame that fits neither way
  * fails with NSS_HIP_UNSUPPORTED before any work is queued. A

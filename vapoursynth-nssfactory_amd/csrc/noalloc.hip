@@ -15,8 +15,8 @@
 #include "instance.hpp"
 
 // 192 MiB per device: a 1920 x 1080 YUV420 two-stage frame at the default
-// arguments needs 158 MiB on the tile plan (the int64 tile partials
-// dominate); larger frames take the plan batched over grid rows, whose
+// arguments needs about 103 MiB on the tile plan (the int64 numerator
+// partials dominate); larger frames take the plan batched over grid rows, whose
 // frame-sized sums need 20 bytes per pixel of a two-stage frame.
 constexpr size_t kScratchBytes = static_cast<size_t>(192) << 20;
 
