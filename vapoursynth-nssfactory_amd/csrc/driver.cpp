@@ -1127,13 +1127,14 @@ void slot_geoms(const nss_hip_bm3d* h, int s, int i, int tw, int th, TileGeom* g
     }
 }
 
-// Floats of one centre's partials of plane i (every slot, num and den), and
-// slot offsets.
-size_t centre_partials(const TileGeom* g, int ns, long long* slot_off) {
+// Floats of one centre's partials of plane i (every slot's num, and its den
+// after it for legacy output; rolling output keeps the weights in corner
+// planes), and slot offsets.
+size_t centre_partials(const TileGeom* g, int ns, long long* slot_off, bool den) {
     size_t total = 0;
     for (int t = 0; t < ns; ++t) {
         if (slot_off) slot_off[t] = static_cast<long long>(total);
-        total += 2 * static_cast<size_t>(g[t].ntx) * g[t].nty * g[t].pitch * g[t].ah;
+        total += (den ? 2 : 1) * static_cast<size_t>(g[t].ntx) * g[t].nty * g[t].pitch * g[t].ah;
     }
     return total;
 }
@@ -1290,7 +1291,7 @@ int temporal_center(nss_hip_bm3d* h, int s, int c, hipStream_t st) {
                 j.grid = make_grid(h->pw[i], h->ph[i], 8, stg.plane[i].step);
                 j.nslots = ns;
                 slot_geoms(h, s, i, R.tw[s], R.th[s], j.tiles, j.range);
-                centre_partials(j.tiles, ns, j.slot_off);
+                centre_partials(j.tiles, ns, j.slot_off, R.legacy);
                 j.part = R.part[s][i] + static_cast<size_t>(c % ns) * R.part_center[s][i];
                 j.corners = R.legacy ? nullptr : R.den[s][i];
                 j.acc_ring = R.acc_n;
@@ -1344,14 +1345,15 @@ int temporal_finish(nss_hip_bm3d* h, int s, int n, hipStream_t st) {
             int range[kMaxTileSlots];
             long long off[kMaxTileSlots];
             slot_geoms(h, s, i, R.tw[s], R.th[s], g, range);
-            centre_partials(g, ns, off);
+            centre_partials(g, ns, off, R.legacy);
             f.nsrc = 0;
             for (int c = std::max(0, n - R.r); c <= std::min(R.N - 1, n + R.r); ++c) {
                 const int t = n - c + R.r;
                 const float* base = R.part[s][i] + static_cast<size_t>(c % ns) * R.part_center[s][i] + off[t];
                 TSource& src = f.src[f.nsrc++];
                 src.num = base;
-                src.den = base + static_cast<size_t>(g[t].ntx) * g[t].nty * g[t].pitch * g[t].ah;
+                // rolling output: the weights come from the corner plane
+                src.den = R.legacy ? base + static_cast<size_t>(g[t].ntx) * g[t].nty * g[t].pitch * g[t].ah : nullptr;
                 src.tiles = g[t];
                 src.range = range[t];
             }
@@ -1481,7 +1483,7 @@ int legacy_emit(nss_hip_bm3d* h, int c, hipStream_t st) {
             long long off[kMaxTileSlots];
             if (R.tiles[s]) {
                 slot_geoms(h, s, i, R.tw[s], R.th[s], g, range);
-                centre_partials(g, ns, off);
+                centre_partials(g, ns, off, R.legacy);
             }
             for (int t = 0; t < ns; ++t) {
                 if (!R.tiles[s] && (t < vb || t >= ve)) continue;
@@ -1745,7 +1747,7 @@ int plan_instance(nss_hip_bm3d* h, const nss_hip_bm3d_params* params, const nss_
                 TileGeom g[kMaxTileSlots];
                 int range[kMaxTileSlots];
                 slot_geoms(h, s, i, R.tw[s], R.th[s], g, range);
-                R.part_center[s][i] = centre_partials(g, 2 * r + 1, nullptr);
+                R.part_center[s][i] = centre_partials(g, 2 * r + 1, nullptr, R.legacy);
             }
     }
     return NSS_HIP_OK;
