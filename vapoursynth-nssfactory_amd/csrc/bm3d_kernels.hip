@@ -2,22 +2,25 @@
 // weighted fixed-point aggregation, and finish.
 //
 // Main path (the shapes whose working sets fit the LDS budget):
-//   match_run_kernel<B>        one workgroup per run of 8 references on one
-//                              grid row; the union of their search windows is
+//   match_run_kernel<B, SMALL> one workgroup per run of 8 references on one
+//   match_run_mid_kernel<B>    grid row; the union of their search windows is
 //                              staged in LDS once; each half-wave owns one
-//                              reference (patch in registers) and each lane a
-//                              2 x 7 tile of candidates; then the G-1 smallest
-//                              keys (dist, y, x) by rounds of a half-wave min.
+//                              reference and each lane 2 x 7 tiles of
+//                              candidates (one tile: SMALL; up to three: mid,
+//                              keys held in registers); then the G-1 smallest
+//                              keys (dist, y, x) by a threshold and a rank
+//                              count (rounds of a half-wave min as fallback).
 //   filter_tile_kernel<B, W>   one workgroup per tile of references; each wave
-//                              filters whole groups (LDS cube: 2-D DCT, Haar
-//                              along the group, hard threshold or Wiener,
-//                              inverse) and adds them into the tile's int64
-//                              fixed-point sums in LDS (ds_add_u64: exact and
-//                              order-free); the tile then writes its num and
-//                              box-summed den as float partials.
-//   finish_tiles_kernel        per pixel, the partials of the (at most a few)
-//                              tiles covering it, summed in a fixed order, then
-//                              num / den or the source.
+//                              filters whole groups (2-D DCT, Haar along the
+//                              group, hard threshold or Wiener, inverse) and
+//                              adds them into the tile's int64 fixed-point
+//                              numerator sums in LDS (ds_add_u64: exact and
+//                              order-free); the tile writes those sums as its
+//                              partials and one weight per group.
+//   finish_tiles_kernel        per pixel, the numerator partials of the tiles
+//                              covering it, summed exactly; the weights rebuilt
+//                              from the match records; then num / den or the
+//                              source.
 // General path (any legal shape):
 //   match_wave_kernel<B, S>    one wave per reference, window in LDS (or read
 //                              from global when it does not fit).
@@ -246,12 +249,24 @@ __device__ inline unsigned long long sort32_u64(unsigned long long v) {
     return bitonic_u64<32, 1>(v);
 }
 
-// Windows whose 2 x 7 lane tiles fit one per lane of a half-wave (a side of
-// up to 20: ranges up to 9) keep each lane's keys in registers through a
-// workgroup barrier; the window keys and the gathered keys then overlay the
-// staged region in LDS. Larger windows keep both areas.
-__host__ __device__ constexpr bool run_keys_in_regs(int range) {
-    return ((2 * range + 1 + 6) / 7) * ((2 * range + 1 + 1) / 2) <= 32;
+// Lane tiles (2 x 7 candidates) a half-wave needs per lane for a whole
+// window of the range.
+__host__ __device__ constexpr int run_tiles_per_lane(int range) {
+    return (((2 * range + 1 + 6) / 7) * ((2 * range + 1 + 1) / 2) + 31) / 32;
+}
+
+// Windows whose lane tiles fit one per lane of a half-wave (a side of up to
+// 20: ranges up to 9) keep each lane's keys in registers through a workgroup
+// barrier; the window keys and the gathered keys then overlay the staged
+// region in LDS. Up to kRunMidTiles tiles per lane (ranges up to 16) do the
+// same with the keys of every tile in registers and a smaller gather area
+// (kRunMidGather keys per reference), so that a 33 x 33 window's workgroup
+// fits four to a CU. Larger windows keep both areas.
+__host__ __device__ constexpr bool run_keys_in_regs(int range) { return run_tiles_per_lane(range) <= 1; }
+inline constexpr int kRunMidTiles = 3;
+inline constexpr int kRunMidGather = 64;
+__host__ __device__ constexpr bool run_keys_mid(int range) {
+    return run_tiles_per_lane(range) > 1 && run_tiles_per_lane(range) <= kRunMidTiles;
 }
 
 // Floats of the staged region the run matcher's LDS layout reserves: the 8
@@ -327,9 +342,13 @@ __device__ inline void run_tile_ssd(const float* rp, const float* base, int RW, 
 }
 
 // Run matcher: references i0 .. i0+7 of grid row j share one LDS region.
-// SMALL: every job's window takes one lane tile per lane (run_keys_in_regs).
-template <int B, bool SMALL>
-__global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
+// NT = 1: every job's window takes one lane tile per lane
+// (run_keys_in_regs); NT = kRunMidTiles: up to that many (run_keys_mid),
+// keys in registers; NT = 0: keys in LDS as they are scored.
+template <int B, int NT>
+__device__ __forceinline__ void match_run_body(const MatchTable& tab) {
+    constexpr bool SMALL = NT == 1;
+    constexpr int NG = NT == kRunMidTiles ? kRunMidGather : kRunGather;  // gathered keys per reference
     extern __shared__ __align__(16) float lds[];
     const int lb = xcd_block();
     const MatchJob& jb = tab.job[find_job(tab, lb)];
@@ -377,8 +396,8 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
         }
     }
     const int wmax = 2 * r + 1;
-    constexpr bool regs = SMALL;
-    // Window keys (wmax^2 per reference) and up to kRunGather gathered keys
+    constexpr bool regs = NT > 0;
+    // Window keys (wmax^2 per reference) and up to NG gathered keys
     // per reference. Keys waiting in registers: both over the region. Else
     // the window keys after the region and the gathered keys over it (or
     // after the window keys when the region is smaller).
@@ -387,7 +406,7 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int region = run_region_floats(B, jb.grid.step, r);
     uint32_t* dist = reinterpret_cast<uint32_t*>(lds + (regs ? 0 : region));
     unsigned long long* surv =
-        regs || region * 4 < kRunRefs * kRunGather * 8
+        regs || region * 4 < kRunRefs * NG * 8
             ? reinterpret_cast<unsigned long long*>(dist + kRunRefs * wmax * wmax)
             : reinterpret_cast<unsigned long long*>(lds);
     __syncthreads();
@@ -402,7 +421,40 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     const int self = (cy - top) * ww + (cx - left);
     const float* rp = lds + (cy - top) * RW + (cx - rx0);
     const float* wbase = lds + (left - rx0);
-    if (regs) {
+    if (NT > 1) {
+        // up to NT tiles per lane (t = hl, hl + 32, ...): their keys wait in
+        // registers for the barrier, then go to the key area over the region
+        uint32_t kr[NT > 1 ? NT : 1][2][7];
+        const int nt = half < nrun ? tw * th : 0;
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            const int t = hl + 32 * q;
+            if (t < nt) {
+                const int ty = t / tw, tx = t - ty * tw;
+                float acc[2][7];
+                run_tile_ssd<B, true>(rp, wbase + ty * 2 * RW + tx * 7, RW, acc);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) kr[q][i][j] = dist_key(acc[i][j]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            const int t = hl + 32 * q;
+            if (t < nt) {
+                const int ty = t / tw, tx = t - ty * tw;
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) {
+                        const int dy = ty * 2 + i, dx = tx * 7 + j, c = dy * ww + dx;
+                        if (dy < wh && dx < ww) dk[c] = c == self ? kTaken : kr[q][i][j];
+                    }
+            }
+        }
+    } else if (regs) {
         // one tile per lane: its keys wait in registers for the barrier,
         // then go to the key area over the region
         const int ty = hl / tw, tx = hl - ty * tw;
@@ -456,7 +508,7 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
     // lane minima: distinct candidates, so at least K keys are <= T and the
     // K winners all are. The keys <= T (self excluded) are gathered in LDS
     // and each is ranked against the others; rank k is record 1 + k. When
-    // more than kRunGather keys are <= T in either half the wave takes the
+    // more than NG keys are <= T in either half the wave takes the
     // rounds below instead.
     if (kmax <= 31) {
         const int nc = K > 0 ? ncand : 0;
@@ -486,7 +538,7 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
         // leaves it out of the gather
         const unsigned long long T = min((threadIdx.x & 32) ? t1 : t0, ~0ull - 1);
         const uint32_t below = (1u << hl) - 1u;  // lanes of this half below this one
-        unsigned long long* sv = surv + half * kRunGather;
+        unsigned long long* sv = surv + half * NG;
         int n = 0;  // keys gathered so far by this half
         for (int c0 = 0; c0 < nc_max; c0 += 128) {
             unsigned long long k[4];
@@ -499,20 +551,20 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
                 if (m) {
                     const uint32_t mh = (threadIdx.x & 32) ? static_cast<uint32_t>(m >> 32) : static_cast<uint32_t>(m);
                     const int slot = n + __popc(mh & below);
-                    if (take && slot < kRunGather) sv[slot] = k[u];
+                    if (take && slot < NG) sv[slot] = k[u];
                     n += __popc(mh);
                 }
             }
         }
         const int n0 = __builtin_amdgcn_readlane(n, 0), n1 = __builtin_amdgcn_readlane(n, 32);
-        if (n0 <= kRunGather && n1 <= kRunGather) {
+        if (n0 <= NG && n1 <= NG) {
             wave_sync();
             // lane hl ranks the gathered keys hl, hl + 32, ... of its half
             const int nmax = max(n0, n1), qn = (nmax + 31) >> 5;
-            unsigned long long own[kRunGather / 32];
-            int rank[kRunGather / 32];
+            unsigned long long own[NG / 32];
+            int rank[NG / 32];
 #pragma unroll
-            for (int q = 0; q < kRunGather / 32; ++q) {
+            for (int q = 0; q < NG / 32; ++q) {
                 own[q] = q < qn && hl + 32 * q < n ? sv[hl + 32 * q] : ~0ull;
                 rank[q] = 0;
             }
@@ -521,14 +573,14 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
 #pragma unroll
                 for (int u = 0; u < 8; ++u) o[u] = j0 + u < n ? sv[j0 + u] : ~0ull;
 #pragma unroll
-                for (int q = 0; q < kRunGather / 32; ++q)
+                for (int q = 0; q < NG / 32; ++q)
                     if (q < qn) {
 #pragma unroll
                         for (int u = 0; u < 8; ++u) rank[q] += o[u] < own[q] ? 1 : 0;
                     }
             }
 #pragma unroll
-            for (int q = 0; q < kRunGather / 32; ++q)
+            for (int q = 0; q < NG / 32; ++q)
                 if (hl + 32 * q < n && rank[q] < K) {
                     const int c = static_cast<int>(static_cast<uint32_t>(own[q]));
                     const int dy = div_small(c, inv_ww), dx = c - dy * ww;
@@ -615,6 +667,18 @@ __global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
         const int dy = div_small(c, inv_ww), dx = c - dy * ww;
         out[1 + hl] = MatchRec{left + dx, top + dy, jb.t0, static_cast<uint32_t>(won >> 32)};
     }
+}
+
+// SMALL: the one-tile-per-lane layout; else keys in LDS as they are scored.
+template <int B, bool SMALL>
+__global__ __launch_bounds__(256) void match_run_kernel(MatchTable tab) {
+    match_run_body<B, SMALL ? 1 : 0>(tab);
+}
+
+// Up to kRunMidTiles lane tiles per lane, their keys in registers.
+template <int B>
+__global__ __launch_bounds__(256) void match_run_mid_kernel(MatchTable tab) {
+    match_run_body<B, kRunMidTiles>(tab);
 }
 
 // General matcher: one wave per reference.
@@ -3922,24 +3986,23 @@ int match_workgroups(int kind, const Grid& g) {
     return g.nx * g.ny;
 }
 
-// LDS of the run matcher in the small-window layout (keys in registers,
-// both key areas over the region) or the other.
-static size_t run_lds_layout(int block, int step, int range, bool small) {
+// LDS of the run matcher in a layout: NT = 1 or kRunMidTiles (keys in
+// registers; the window keys and the gathered keys both over the region) or
+// 0 (the window keys after the region, the gathered keys over it, or after
+// the window keys when the region is smaller).
+static size_t run_lds_layout(int block, int step, int range, int nt) {
     const size_t w = 2 * static_cast<size_t>(range) + 1;
     const size_t region = static_cast<size_t>(run_region_floats(block, step, range)) * 4;
-    const size_t keys = kRunRefs * w * w * 4, gather = kRunRefs * kRunGather * 8;
-    return small ? std::max(region, keys + gather) : region + keys + (region < gather ? gather : 0);
+    const size_t keys = kRunRefs * w * w * 4;
+    const size_t gather = kRunRefs * static_cast<size_t>(nt == kRunMidTiles ? kRunMidGather : kRunGather) * 8;
+    return nt > 0 ? std::max(region, keys + gather) : region + keys + (region < gather ? gather : 0);
 }
 
+// The layout a job of this range takes alone.
+static int run_layout(int range) { return run_keys_in_regs(range) ? 1 : run_keys_mid(range) ? kRunMidTiles : 0; }
+
 size_t match_run_lds_bytes(int block, int step, int range) {
-    const size_t w = 2 * static_cast<size_t>(range) + 1;
-    const size_t region = static_cast<size_t>(run_region_floats(block, step, range)) * 4;
-    // keys of the 8 windows and kRunGather gathered keys per reference: both
-    // over the region when the keys wait in registers; else the window keys
-    // after the region and the gathered keys over it (after the window keys
-    // when the region is smaller)
-    const size_t keys = kRunRefs * w * w * 4, gather = kRunRefs * kRunGather * 8;
-    return run_keys_in_regs(range) ? std::max(region, keys + gather) : region + keys + (region < gather ? gather : 0);
+    return run_lds_layout(block, step, range, run_layout(range));
 }
 
 size_t filter_atomic_lds_bytes(int block, int group, bool wiener) {
@@ -4000,16 +4063,22 @@ static hipError_t launch_match_b(const MatchTable& t, int total_wg, size_t lds, 
         }
     } else if (kind == 2) {
         if constexpr (B <= 12) {
-            // the small-window layout when every job's window takes it; the
-            // LDS for the layout launched
-            bool small = true;
-            for (int j = 0; j < t.n; ++j) small = small && run_keys_in_regs(t.job[j].range);
+            // the layout every job's window takes (the one of the largest
+            // range); the LDS for the layout launched
+            int nt = 1;
+            for (int j = 0; j < t.n; ++j) {
+                const int l = run_layout(t.job[j].range);
+                nt = nt == 0 || l == 0 ? 0 : std::max(nt, l);
+            }
             lds = 0;
             for (int j = 0; j < t.n; ++j)
-                lds = std::max(lds, run_lds_layout(B, t.job[j].grid.step, t.job[j].range, small));
-            if (small) {
+                lds = std::max(lds, run_lds_layout(B, t.job[j].grid.step, t.job[j].range, nt));
+            if (nt == 1) {
                 allow_lds(&match_run_kernel<B, true>, lds);
                 match_run_kernel<B, true><<<total_wg, 256, lds, s>>>(t);
+            } else if (nt == kRunMidTiles) {
+                allow_lds(&match_run_mid_kernel<B>, lds);
+                match_run_mid_kernel<B><<<total_wg, 256, lds, s>>>(t);
             } else {
                 allow_lds(&match_run_kernel<B, false>, lds);
                 match_run_kernel<B, false><<<total_wg, 256, lds, s>>>(t);
