@@ -74,11 +74,15 @@ def _compare_matches(gpu, oracle, plane, block, step, rng_, group, flat, label):
 @pytest.mark.parametrize("block,group,step,rng_", [(8, 16, 3, 16), (8, 8, 8, 9), (4, 16, 2, 7), (12, 8, 6, 5),
                                                    (16, 4, 8, 6), (2, 32, 1, 3), (1, 8, 1, 2), (32, 2, 16, 4),
                                                    (8, 64, 3, 30), (16, 8, 8, 20), (32, 4, 16, 64),
-                                                   (8, 64, 8, 9), (12, 64, 6, 5), (8, 32, 4, 9)])
+                                                   (8, 64, 8, 9), (12, 64, 6, 5), (8, 32, 4, 9),
+                                                   (4, 8, 4, 12), (12, 16, 6, 11), (8, 32, 5, 13),
+                                                   (2, 16, 2, 10), (1, 8, 3, 14)])
 def test_match_lists(gpu, oracle, block, group, step, rng_, flat):
-    """Covers the run matcher (B <= 12, small windows; groups up to 32 by the
-    threshold selection, 64 by the rounds), the wave matcher with the window
-    in LDS, and with the window read from global (B 32, range 64)."""
+    """Covers the run matcher (B <= 12: one lane tile per lane up to range 9,
+    two or three with the keys in registers up to range 16, keys in LDS
+    beyond; groups up to 32 by the threshold selection, 64 by the rounds),
+    the wave matcher with the window in LDS, and with the window read from
+    global (B 32, range 64)."""
     print(f"match b{block} g{group} s{step} r{rng_} flat={flat}", flush=True)
     plane = _test_plane(77 if rng_ < 30 else 150, 61 if rng_ < 30 else 140, 7 + block + group, flat)
     flips = _compare_matches(gpu, oracle, plane, block, step, rng_, group, flat, f"b{block}g{group}")
@@ -117,6 +121,26 @@ def test_config0_one_stage(gpu, oracle):
     p = oracle.psnr(got[0], want[0])
     print(f"config0 PSNR vs oracle {p:.2f} dB", flush=True)
     assert p >= MIN_PSNR
+
+
+@pytest.mark.parametrize("ranges", [[9, 14], [14, 20], [16, 12]])
+def test_planes_of_different_matcher_layouts_in_one_launch(gpu, oracle, ranges):
+    """One match launch covers the three planes of a block size, and its
+    layout is the one the largest range needs (one lane tile per lane up to
+    range 9, up to three with the keys in registers up to 16, keys in LDS
+    beyond). Each plane's output must equal that plane run alone, bit for
+    bit, and the oracle's within the PSNR gate."""
+    import synth
+    _, noisy = synth.noisy_frame("YUV420PS", 160, 120, 10, seed=3)
+    kw = dict(sigma=10, block_step=4, group_size=16)
+    got = _run(gpu, "YUV420PS", noisy, bm_range=ranges, **kw)
+    want = _oracle_frame(oracle, gpu, "YUV420PS", noisy, bm_range=ranges, **kw)
+    for i in range(3):
+        r = ranges[min(i, len(ranges) - 1)]
+        alone = _run(gpu, "GRAYS", [noisy[i]], bm_range=r, **kw)[0]
+        print(f"ranges {ranges} plane {i} range {r}", flush=True)
+        assert np.array_equal(got[i].view(np.uint32), alone.view(np.uint32)), (ranges, i)
+        assert oracle.psnr(got[i], want[i]) >= MIN_PSNR
 
 
 @pytest.mark.parametrize("fmt", ["YUV420PS", "RGBS", "GRAYS"])
