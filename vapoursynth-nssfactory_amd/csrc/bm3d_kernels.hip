@@ -374,24 +374,21 @@ __device__ __forceinline__ void match_run_body(const MatchTable& tab) {
     // wave-uniform and each lane's column is clamped once per strip; cells
     // outside the plane repeat its edge (only discarded candidates of
     // overhanging tiles read them).
+    // Each load goes straight to LDS (global_load_lds: lane l's word lands
+    // at the wave-uniform row address + 4 l), so the staging holds no
+    // registers and issues no LDS stores; the barrier below waits for them.
+    // The row (and so its address) is wave-uniform: scalar arithmetic, the
+    // lane's clamped column the only per-lane offset.
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
     for (int c0 = 0; c0 < RW; c0 += 64) {
         const int xx = c0 + lane;
         if (xx < RW) {
             const int gx = min(rx0 + xx, jb.width - 1);
-            int yy = wave;
-            for (; yy + 12 < RH; yy += 16) {
-                float v[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int gy = min(top + yy + 4 * q, jb.height - 1);
-                    v[q] = jb.plane[static_cast<size_t>(gy) * jb.stride + gx];
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) lds[(yy + 4 * q) * RW + xx] = v[q];
-            }
-            for (; yy < RH; yy += 4) {
-                const int gy = min(top + yy, jb.height - 1);
-                lds[yy * RW + xx] = jb.plane[static_cast<size_t>(gy) * jb.stride + gx];
+            for (int yy = wv; yy < RH; yy += 4) {
+                const float* row = jb.plane + static_cast<size_t>(min(top + yy, jb.height - 1)) * jb.stride;
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(row + gx),
+                                                 (__attribute__((address_space(3))) void*)(lds + yy * RW + c0), 4, 0,
+                                                 0);
             }
         }
     }
