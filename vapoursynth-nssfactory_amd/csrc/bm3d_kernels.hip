@@ -2634,8 +2634,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void pr
             const bool staged = cells <= tab.chunk && nc * R * P <= tab.region;
             if (staged) {
                 // Window w's pixels (rows y_w - ps .., columns x_w - ps ..) at
-                // region[w R P + j P + i]; columns from R on and pixels off the
-                // plane hold 0 (read only for discarded candidates).
+                // region[w R P + j P + i]. Columns from R on and pixels off the
+                // plane are read only for discarded candidates (positions off
+                // the plane are not members; a member's patch lies in rows and
+                // columns below R), so the fast form below leaves them as they
+                // are and the general form writes 0.
                 // Lanes cover 64 / P rows of P columns per step (P <= 64), or
                 // one row in steps of 64 columns.
                 const int rows_per = P <= 64 ? 64 / P : 1;
@@ -2646,20 +2649,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void pr
                     float* dst = region + w * (R * P);
                     if (lr >= rows_per) continue;
                     if (P <= 64 && R <= 8 * rows_per) {
-                        // one column per lane, up to 8 rows: every load in
-                        // flight before the first store
-                        float v[8];
+                        // one column per lane, up to 8 rows, each load
+                        // straight to LDS (global_load_lds: lane l's word at
+                        // the wave-uniform address dst + q rows_per P + 4 l,
+                        // which is row j = lr + q rows_per, column li);
+                        // the barrier below waits for them
 #pragma unroll
                         for (int q = 0; q < 8; ++q) {
                             const int j = lr + q * rows_per, y = y0 + j, x = x0 + li;
-                            v[q] = (j < R && li < R && y >= 0 && y < jb.height && x >= 0 && x < jb.width)
-                                       ? pt[static_cast<size_t>(y) * jb.stride + x]
-                                       : 0.f;
-                        }
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            const int j = lr + q * rows_per;
-                            if (j < R) dst[j * P + li] = v[q];
+                            if (j < R && li < R && y >= 0 && y < jb.height && x >= 0 && x < jb.width)
+                                __builtin_amdgcn_global_load_lds(
+                                    (const __attribute__((address_space(1))) void*)(pt + static_cast<size_t>(y) * jb.stride + x),
+                                    (__attribute__((address_space(3))) void*)(dst + q * rows_per * P), 4, 0, 0);
                         }
                         continue;
                     }
