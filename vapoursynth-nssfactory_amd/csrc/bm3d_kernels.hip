@@ -700,10 +700,17 @@ __global__ __launch_bounds__(64) void match_wave_kernel(MatchTable tab) {
     int rs;
     uint32_t* dk;
     if constexpr (STAGE) {
+        // straight to LDS (global_load_lds: lane l's word at the uniform
+        // address lreg + yy rw + x0 + 4 l), every row's loads in flight
+        // together; the barrier below waits for them
         float* lreg = reinterpret_cast<float*>(smem);
         for (int yy = 0; yy < rh; ++yy) {
             const float* row = jb.plane + static_cast<size_t>(top + yy) * jb.stride + left;
-            for (int xx = lane; xx < rw; xx += 64) lreg[yy * rw + xx] = row[xx];
+            for (int x0 = 0; x0 < rw; x0 += 64)
+                if (x0 + lane < rw)
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(row + x0 + lane),
+                                                     (__attribute__((address_space(3))) void*)(lreg + yy * rw + x0), 4,
+                                                     0, 0);
         }
         reg = lreg;
         rs = rw;
