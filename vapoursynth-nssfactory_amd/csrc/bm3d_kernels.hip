@@ -2644,8 +2644,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void pr
                 // region[w R P + j P + i]. Columns from R on and pixels off the
                 // plane are read only for discarded candidates (positions off
                 // the plane are not members; a member's patch lies in rows and
-                // columns below R), so the fast form below leaves them as they
-                // are and the general form writes 0.
+                // columns below R), so the loads leave them as they are.
                 // Lanes cover 64 / P rows of P columns per step (P <= 64), or
                 // one row in steps of 64 columns.
                 const int rows_per = P <= 64 ? 64 / P : 1;
@@ -2671,12 +2670,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void pr
                         }
                         continue;
                     }
+                    // the same loads for any pitch: lane l's word at
+                    // dst + j P + i = (dst + j P + i - l) + 4 l, the base
+                    // alike for every lane of a step
                     for (int j = lr; j < R; j += rows_per)
                         for (int i = li; i < P; i += (P <= 64 ? P : 64)) {
                             const int y = y0 + j, x = x0 + i;
-                            dst[j * P + i] = (i < R && y >= 0 && y < jb.height && x >= 0 && x < jb.width)
-                                                 ? pt[static_cast<size_t>(y) * jb.stride + x]
-                                                 : 0.f;
+                            if (i < R && y >= 0 && y < jb.height && x >= 0 && x < jb.width)
+                                __builtin_amdgcn_global_load_lds(
+                                    (const __attribute__((address_space(1))) void*)(pt + static_cast<size_t>(y) * jb.stride + x),
+                                    (__attribute__((address_space(3))) void*)(dst + j * P + i - lane), 4, 0, 0);
                         }
                 }
                 __syncthreads();
