@@ -3205,12 +3205,21 @@ __global__ __launch_bounds__(256) void finish_temporal_tiles_kernel(TFinishTable
         const int cx0 = bx0 - B + 1, cy0 = by0 - B + 1;
         unsigned long long* ct = tsm;            // [CH][CW]
         unsigned long long* hs = tsm + CH * CW;  // [CH][64]
-        for (int e = threadIdx.x; e < CH * CW; e += 256) {
-            const int r = e / CW, c = e - r * CW;
-            const int py = cy0 + r, px = cx0 + c;
-            ct[e] = py >= 0 && px >= 0 && py < jb.height && px < jb.width
-                        ? jb.corners[static_cast<size_t>(py) * jb.width + px]
-                        : 0ull;
+        // eight loads per thread in flight before their stores
+        for (int e0 = threadIdx.x; e0 < CH * CW; e0 += 8 * 256) {
+            unsigned long long v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + 256 * u;
+                const int r = e / CW, c = e - r * CW;
+                const int py = cy0 + r, px = cx0 + c;
+                v[u] = e < CH * CW && py >= 0 && px >= 0 && py < jb.height && px < jb.width
+                           ? jb.corners[static_cast<size_t>(py) * jb.width + px]
+                           : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e0 + 256 * u < CH * CW) ct[e0 + 256 * u] = v[u];
         }
         __syncthreads();
         for (int e = threadIdx.x; e < CH * 64; e += 256) {
